@@ -1,0 +1,354 @@
+// C-ABI entry points (include/adg.h) over the device context, the built-in
+// point functions of the benchmark configurations, and the whole-run drivers.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "../../include/adg.h"
+#include "context.hpp"
+
+struct adg_ctx {
+  adg::Context* c = nullptr;
+  std::string err;
+};
+
+namespace {
+
+template <class F>
+int guard(adg_ctx* h, F&& f) {
+  if (!h || !h->c) return ADG_E_ARG;
+  try {
+    f(*h->c);
+    return ADG_OK;
+  } catch (const adg::HipError& e) {
+    h->err = e.what();
+    return ADG_E_HIP;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    const std::string w = e.what();
+    if (w.find("missing ancestor") != std::string::npos) return ADG_E_ANCESTOR;
+    if (w.find("non-finite") != std::string::npos) return ADG_E_NONFINITE;
+    return ADG_E_ARG;
+  }
+}
+
+int field_size(const adg::Context& c, int field) {
+  if (field == ADG_FIELD_UCOE || field == ADG_FIELD_RHS || field == ADG_FIELD_TN) return c.K;
+  return c.P;
+}
+
+double* field_ptr(adg::Context& c, int field) {
+  switch (field) {
+    case ADG_FIELD_UCOE: return c.u.p;
+    case ADG_FIELD_RHS: return c.rhs.p;
+    case ADG_FIELD_UP: return c.up.p;
+    case ADG_FIELD_TN: return c.tn.p;
+    case ADG_FIELD_UINTP: return c.uintp.p;
+  }
+  if (field >= ADG_FIELD_FUCOE && field < ADG_FIELD_FUCOE + c.D)
+    return c.fucoe.p + static_cast<int64_t>(c.same_flux ? 0 : field - ADG_FIELD_FUCOE) * c.E() * c.P;
+  throw std::runtime_error("unknown field");
+}
+
+double periodic_dist2(const double* x, const double* c, int dim) {
+  double r2 = 0.;
+  for (int d = 0; d < dim; ++d) {
+    double t = std::abs(x[d] - c[d]);
+    t = std::min(t, 1. - t);
+    r2 += t * t;
+  }
+  return r2;
+}
+
+// one RK3 stage of the Lagrange-interpolated hyperbolic operator
+void hyperbolic_rhs(adg::Context& c, const adg_problem* p) {
+  const double fp = p->flux_param;
+  c.eval_up();
+  c.flux(p->flux_kind, &fp);
+  c.fp_to_coe();
+  c.rhs_zero();
+  c.rhs_lagr(false);
+  c.rhs_lagr(true);
+  c.rhs_penalty(p->alpha);
+}
+
+}  // namespace
+
+extern "C" {
+
+int adg_create(int dim, int k, int m, int nmax, adg_ctx** out) {
+  if (!out) return ADG_E_ARG;
+  *out = nullptr;
+  adg_ctx* h = new adg_ctx();
+  try {
+    h->c = new adg::Context(dim, k, m, nmax);
+  } catch (const adg::HipError& e) {
+    h->err = e.what();
+    *out = h;
+    return ADG_E_HIP;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    *out = h;
+    return ADG_E_ARG;
+  }
+  *out = h;
+  return ADG_OK;
+}
+
+int adg_destroy(adg_ctx* h) {
+  if (!h) return ADG_E_ARG;
+  delete h->c;
+  delete h;
+  return ADG_OK;
+}
+
+const char* adg_last_error(adg_ctx* h) { return h ? h->err.c_str() : "null context"; }
+
+int adg_sync(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.sync(); }); }
+
+int adg_grid_init(adg_ctx* h, int sparse, int level_init) {
+  return guard(h, [&](adg::Context& c) {
+    if (level_init < 0 || level_init > c.nmax) throw std::runtime_error("level_init must be in [0, nmax]");
+    c.set_grid_slots(adg::HostGrid::initial_slots(c.D, level_init, sparse != 0), false);
+    c.sync();
+  });
+}
+
+int adg_grid_set(adg_ctx* h, const int32_t* level, const int32_t* suppt, int64_t n) {
+  return guard(h, [&](adg::Context& c) {
+    std::vector<int32_t> s(static_cast<size_t>(n) * c.D);
+    for (int64_t i = 0; i < n * c.D; ++i) {
+      if (level[i] < 0 || level[i] > c.nmax) throw std::runtime_error("level out of range");
+      const int jmax = level[i] == 0 ? 1 : (1 << level[i]) - 1;
+      if (suppt[i] < 1 || suppt[i] > jmax || suppt[i] % 2 == 0) throw std::runtime_error("support index out of range");
+      s[i] = adg::slot_of(level[i], suppt[i]);
+    }
+    c.set_grid_slots(s, false);
+    c.sync();
+  });
+}
+
+int64_t adg_num_elem(adg_ctx* h) { return (h && h->c) ? h->c->E() : -1; }
+
+int adg_grid_get(adg_ctx* h, int32_t* level, int32_t* suppt, int32_t* hash) {
+  return guard(h, [&](adg::Context& c) {
+    for (int64_t e = 0; e < c.E(); ++e) {
+      for (int d = 0; d < c.D; ++d) {
+        if (level) level[e * c.D + d] = c.grid.level(e, d);
+        if (suppt) suppt[e * c.D + d] = c.grid.suppt(e, d);
+      }
+      if (hash) hash[e] = c.grid.hash[e];
+    }
+  });
+}
+
+int adg_max_level(adg_ctx* h, int32_t* out) {
+  return guard(h, [&](adg::Context& c) {
+    for (int d = 0; d < c.D; ++d) out[d] = 0;
+    for (int64_t e = 0; e < c.E(); ++e)
+      for (int d = 0; d < c.D; ++d) out[d] = std::max<int32_t>(out[d], c.grid.level(e, d));
+  });
+}
+
+int adg_field_set(adg_ctx* h, int field, const double* host) {
+  return guard(h, [&](adg::Context& c) {
+    if (field >= ADG_FIELD_FUCOE) c.same_flux = false;
+    const size_t n = static_cast<size_t>(c.E()) * field_size(c, field);
+    adg::hip_check(hipMemcpyAsync(field_ptr(c, field), host, n * sizeof(double), hipMemcpyHostToDevice, c.stream),
+                   "field upload");
+    c.sync();
+  });
+}
+
+int adg_field_get(adg_ctx* h, int field, double* host) {
+  return guard(h, [&](adg::Context& c) {
+    const size_t n = static_cast<size_t>(c.E()) * field_size(c, field);
+    adg::hip_check(hipMemcpyAsync(host, field_ptr(c, field), n * sizeof(double), hipMemcpyDeviceToHost, c.stream),
+                   "field download");
+    c.sync();
+  });
+}
+
+int adg_sweep(adg_ctx* h, int table, int flx, int lu, int dim, const int32_t* size_in, const int32_t* size_out,
+              double coef, const double* in, double* out) {
+  return guard(h, [&](adg::Context& c) {
+    if (dim < 0 || dim >= c.D) throw std::runtime_error("dimension out of range");
+    if (lu < 0 || lu > 2) throw std::runtime_error("lu must be 0, 1 or 2");
+    const int pf = (table == 0 || table >= 4) ? c.k : c.m;
+    const int pt = (table == 0) ? c.m : c.k;
+    int sin[adg::kMaxDim], sout[adg::kMaxDim];
+    int64_t nin = 1, nout = 1;
+    const int mx = std::max(c.k, c.m) + 1;
+    for (int d = 0; d < c.D; ++d) {
+      sin[d] = size_in[d];
+      sout[d] = size_out[d];
+      if (sin[d] < 1 || sin[d] > mx || sout[d] < 1 || sout[d] > mx) throw std::runtime_error("tensor size out of range");
+      if (d != dim && sin[d] != sout[d]) throw std::runtime_error("sizes may differ only in the swept dimension");
+      nin *= sin[d];
+      nout *= sout[d];
+    }
+    c.tmpA.ensure(std::max<int64_t>(c.E() * nin, 1));
+    c.tmpB.ensure(std::max<int64_t>(c.E() * nout, 1));
+    adg::hip_check(hipMemcpyAsync(c.tmpA.p, in, c.E() * nin * sizeof(double), hipMemcpyHostToDevice, c.stream),
+                   "upload");
+    c.sweep(c.tmpA.p, sin, c.tmpB.p, nullptr, sout, dim, c.table(table), flx != 0, lu, coef, pf, pt);
+    adg::hip_check(hipMemcpyAsync(out, c.tmpB.p, c.E() * nout * sizeof(double), hipMemcpyDeviceToHost, c.stream),
+                   "download");
+    c.sync();
+  });
+}
+
+int adg_eval_up(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.eval_up(); }); }
+int adg_flux(adg_ctx* h, int kind, const double* params) {
+  return guard(h, [&](adg::Context& c) {
+    if (kind < 0 || kind > 2) throw std::runtime_error("unknown flux kind");
+    c.flux(kind, params);
+  });
+}
+int adg_fp_to_coe(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.fp_to_coe(); }); }
+int adg_rhs_zero(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_zero(); }); }
+int adg_rhs_lagr_vol(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_lagr(false); }); }
+int adg_rhs_lagr_flx(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_lagr(true); }); }
+int adg_rhs_penalty(adg_ctx* h, const double* alpha) { return guard(h, [&](adg::Context& c) { c.rhs_penalty(alpha); }); }
+int adg_rhs_linear(adg_ctx* h, const double* coef) { return guard(h, [&](adg::Context& c) { c.rhs_linear(coef); }); }
+int adg_rk_init(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rk_init(); }); }
+int adg_rk_stage(adg_ctx* h, int scheme, int stage, double dt) {
+  return guard(h, [&](adg::Context& c) {
+    if (!((scheme == ADG_RK_EULER && stage == 0) || (scheme == ADG_RK3 && stage >= 0 && stage <= 2)))
+      throw std::runtime_error("invalid RK scheme/stage");
+    c.rk_stage(scheme, stage, dt);
+  });
+}
+int adg_copy_to_predict(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.copy_to_predict(); }); }
+int adg_copy_predict_to_ucoe(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.copy_predict_to_ucoe(); }); }
+int adg_indicator_sumsq(adg_ctx* h, double* out) { return guard(h, [&](adg::Context& c) { c.indicator_sumsq(out); }); }
+int adg_refine(adg_ctx* h, double eps, int64_t* n_added) {
+  return guard(h, [&](adg::Context& c) {
+    const int64_t a = c.refine(eps);
+    if (n_added) *n_added = a;
+  });
+}
+int adg_coarsen(adg_ctx* h, double eta, int64_t* n_removed) {
+  return guard(h, [&](adg::Context& c) {
+    const int64_t r = c.coarsen(eta);
+    if (n_removed) *n_removed = r;
+  });
+}
+int adg_init_adaptive_intp(adg_ctx* h, adg_point_fn f, void* user, double eps) {
+  return guard(h, [&](adg::Context& c) {
+    if (!f) throw std::runtime_error("null point function");
+    c.init_adaptive_intp(f, user, eps);
+  });
+}
+
+double adg_ic_eval(const double* x, int dim, void* user) {
+  const adg_ic_params* p = static_cast<const adg_ic_params*>(user);
+  const double pi = adg::PI;
+  switch (p->kind) {
+    case 0: {
+      double s = 0.;
+      for (int d = 0; d < dim; ++d) s += x[d];
+      return std::sin(2. * pi * (s - p->shift));
+    }
+    case 1: {
+      double v = 1.;
+      for (int d = 0; d < dim; ++d) v *= std::sin(2. * pi * (x[d] - p->shift));
+      return v;
+    }
+    case 2:
+    case 4: {
+      double s = 0.;
+      for (int d = 0; d < dim; ++d) s += x[d];
+      double v = (p->kind == 2) ? p->base + p->ampl * std::sin(2. * pi * s) : 0.;
+      for (int b = 0; b < p->nbump && b < 8; ++b) {
+        const double r2 = periodic_dist2(x, p->bump_center[b], dim);
+        v += p->bump_amp[b] * std::exp(-r2 / (p->bump_width[b] * p->bump_width[b]));
+      }
+      return v;
+    }
+    case 3: {
+      double r2 = 0.;
+      for (int d = 0; d < dim; ++d) {
+        const double X = p->offset[d] + p->scale[d] * x[d] - p->center[d];
+        r2 += X * X;
+      }
+      return (r2 <= p->radius * p->radius) ? p->inside : p->outside;
+    }
+  }
+  return 0.;
+}
+
+int adg_run_linear(adg_ctx* h, const double* coef, double dt, int64_t nsteps, adg_run_stats* st) {
+  return guard(h, [&](adg::Context& c) {
+    c.sync();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int64_t n = 0; n < nsteps; ++n) {
+      c.rk_init();
+      for (int s = 0; s < 3; ++s) {
+        c.rhs_zero();
+        c.rhs_linear(coef);
+        c.rk_stage(ADG_RK3, s, dt);
+      }
+    }
+    c.sync();
+    const auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+      st->steps = nsteps;
+      st->dof_stages = nsteps * 3 * c.E() * c.K;
+      st->max_elem = c.E();
+      st->seconds = std::chrono::duration<double>(t1 - t0).count();
+      st->final_time = dt * nsteps;
+    }
+  });
+}
+
+int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
+  return guard(h, [&](adg::Context& c) {
+    c.sync();
+    const auto t0 = std::chrono::steady_clock::now();
+    double t = 0.;
+    int64_t steps = 0, dof_stages = 0, max_elem = c.E();
+    while (t < p->t_final && (p->max_steps <= 0 || steps < p->max_steps)) {
+      double sum_c_dx = 0.;
+      for (int d = 0; d < c.D; ++d) {
+        int lmax = 0;
+        for (int64_t e = 0; e < c.E(); ++e) lmax = std::max(lmax, c.grid.level(e, d));
+        sum_c_dx += std::abs(p->wave_speed[d]) * std::pow(2., lmax);
+      }
+      double dt = p->cfl / sum_c_dx;
+      dt = std::min(dt, p->t_final - t);
+      if (p->adaptive) {
+        c.copy_to_predict();
+        c.rk_init();
+        hyperbolic_rhs(c, p);
+        c.rk_stage(ADG_RK_EULER, 0, dt);
+        c.refine(p->eps);
+        c.copy_predict_to_ucoe();
+        max_elem = std::max(max_elem, c.E());
+      }
+      c.rk_init();
+      for (int s = 0; s < 3; ++s) {
+        hyperbolic_rhs(c, p);
+        c.rk_stage(ADG_RK3, s, dt);
+      }
+      dof_stages += 3 * c.E() * c.K;
+      if (p->adaptive) c.coarsen(p->eta);
+      t += dt;
+      ++steps;
+    }
+    c.sync();
+    c.check_error_flag();
+    const auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+      st->steps = steps;
+      st->dof_stages = dof_stages;
+      st->max_elem = max_elem;
+      st->seconds = std::chrono::duration<double>(t1 - t0).count();
+      st->final_time = t;
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+// Device context of one adaptive sparse-grid DG solution (internal C++ API
+// behind the C-ABI in include/adg.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "grid.hpp"
+#include "setup.hpp"
+
+namespace adg {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what);
+
+constexpr int ADG_FLUX_VARCOEF_ID = 2;  // ADG_FLUX_VARCOEF of include/adg.h
+
+// Growable device buffer.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  void ensure(size_t n) {
+    if (n <= cap) return;
+    if (p) hip_check(hipFree(p), "hipFree");
+    p = nullptr;
+    size_t c = cap ? cap : 1024;
+    while (c < n) c *= 2;
+    hip_check(hipMalloc(&p, c * sizeof(T)), "hipMalloc");
+    cap = c;
+  }
+};
+
+struct SweepTable {
+  const double* dev = nullptr;
+  int ld = 0;  // columns
+};
+
+class Context {
+ public:
+  Context(int dim, int k, int m, int nmax);
+  ~Context();
+
+  const int D, k, m, nmax;
+  int K, P;  // (k+1)^D, (m+1)^D
+
+  // host state
+  Basis1D alpt, lagr;
+  HashKey hk;
+  HostGrid grid;
+  std::vector<double> up_host;    // point values of the adaptive interpolation
+  std::vector<char> new_add;      // Element::new_add of the adaptive interpolation
+  std::vector<double> lin_coef;   // coefficients of the uploaded linear tables
+  int flux_kind = -1;
+  double flux_param = 0.;
+
+  hipStream_t stream = nullptr;
+
+  // tables
+  DevBuf<double> t_pt, t_la_uv, t_la_uvx, t_la_flx, t_aa[5], t_lin[kMaxDim], t_lmass, t_wt;
+  DevBuf<int32_t> t_anc_slot, t_anc_point;
+  int na = 0, nl = 0;  // 1D basis counts
+
+  // grid arrays
+  DevBuf<int32_t> g_slot[kMaxDim], g_fbase[kMaxDim], g_fmaxl[kMaxDim], g_fmap[kMaxDim];
+  DevBuf<int32_t> g_remap;
+
+  // fields
+  DevBuf<double> u, tn, pred, rhs, up, uintp, fp, fucoe, interA, interB, tmpA, tmpB, vcoef, sumsq, remap_tmp;
+  DevBuf<int32_t> err_flag;
+  bool same_flux = false;  // all flux components equal (Burgers): one surplus transform
+
+  int64_t E() const { return grid.E; }
+
+  void set_grid_slots(const std::vector<int32_t>& slots, bool keep_fields);
+  void upload_grid();
+  void ensure_fields();
+
+  // operators (stream ordered)
+  void sweep(const double* in, const int* size_in, double* out, double* accum, const int* size_out, int dd,
+             SweepTable tab, bool flx, int lu, double coef, int pf, int pt);
+  void partial_sums(const SweepTable* tabs, const int* flx, int pf, int pt, double coef, const double* src, double* dst);
+  void eval_up();
+  void flux(int kind, const double* params);
+  void fp_to_coe();
+  void rhs_zero();
+  void rhs_lagr(bool flux_term);
+  void rhs_single_dim(SweepTable tab, int d, double coef);
+  void rhs_penalty(const double* alpha);
+  void rhs_linear(const double* coef);
+  void rk_init();
+  void rk_stage(int scheme, int stage, double dt);
+  void copy_to_predict();
+  void copy_predict_to_ucoe();
+  void indicator_sumsq(double* host_out);
+  int64_t refine(double eps);
+  int64_t coarsen(double eta);
+  void init_adaptive_intp(double (*f)(const double*, int, void*), void* user, double eps);
+  void check_error_flag();
+
+  SweepTable table(int id);
+  void sync();
+
+ private:
+  void remap_field(DevBuf<double>& f, int per_elem, const std::vector<int32_t>& old_of_new, int64_t old_E);
+  void surplus_transform(const double* in, double* out, double* scratch_a, double* scratch_b, int ncomp);
+  void compute_vcoef();
+};
+
+std::vector<std::vector<int>> transform_orders(int D, std::vector<std::vector<int>>& lus);
+
+}  // namespace adg

@@ -1,0 +1,649 @@
+// HIP kernels of the adaptive sparse-grid DG hot path for gfx950, and the
+// device context that sequences them.
+//
+// FP64 throughout, compiled with -ffp-contract=off so that every product and
+// sum rounds exactly as the reference's x86-64 build does (no FMA).  Each
+// output value accumulates its terms in ascending source-slot order, the
+// order the oracle uses, so results are bit-identical to it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "context.hpp"
+#include "fiber.hpp"
+
+namespace adg {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned nblocks(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+// ---------------------------------------------------------------------------
+// One 1D sweep (FastMultiplyLU::transform_1D, source/FastMultiplyLU.cpp:436-512).
+// Thread per (target element, index over the dimensions other than dd).
+struct SweepArgs {
+  const double* in;
+  double* out;    // written when non-null
+  double* accum;  // accumulated (+=) when non-null
+  const double* M;
+  const int32_t* slot;
+  const int32_t* fbase;
+  const int32_t* fmaxl;
+  const int32_t* fmap;
+  int64_t E;
+  int R, Tin, Tout, sdd_in, sdd_out, ld;
+  int nrest;
+  int rsz[3], rsin[3], rsout[3];
+  int flx, lu;
+  double coef;
+};
+
+template <int PF1, int PT1>
+__global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= a.E * a.R) return;
+  const int64_t e = gid / a.R;
+  int r = static_cast<int>(gid - e * a.R);
+  int bin = 0, bout = 0;
+  for (int i = a.nrest - 1; i >= 0; --i) {
+    const int q = r % a.rsz[i];
+    r /= a.rsz[i];
+    bin += q * a.rsin[i];
+    bout += q * a.rsout[i];
+  }
+  const int st = a.slot[e];
+  const int fb = a.fbase[e];
+  const double* Mcol = a.M + st * PT1;
+  double acc[PT1];
+#pragma unroll
+  for (int i = 0; i < PT1; ++i) acc[i] = 0.;
+  for_each_related(st, a.fmaxl[e], a.flx != 0, a.lu, [&](int s) {
+    const int src = a.fmap[fb + s];
+    if (src < 0) return;
+    const double* x = a.in + static_cast<int64_t>(src) * a.Tin + bin;
+    const double* Mr = Mcol + static_cast<int64_t>(s) * PF1 * a.ld;
+#pragma unroll
+    for (int kk = 0; kk < PF1; ++kk) {
+      const double xv = x[kk * a.sdd_in];
+#pragma unroll
+      for (int i = 0; i < PT1; ++i) acc[i] += xv * Mr[kk * a.ld + i];
+    }
+  });
+  const int64_t ob = e * a.Tout + bout;
+#pragma unroll
+  for (int i = 0; i < PT1; ++i) {
+    const double v = acc[i] * a.coef;
+    if (a.out) a.out[ob + i * a.sdd_out] = v;
+    if (a.accum) a.accum[ob + i * a.sdd_out] += v;
+  }
+}
+
+// Point flux (LagrInterpolation::eval_fp_Lag / eval_coe_u_Lag).
+__global__ __launch_bounds__(kBlock) void k_flux(const double* up, double* fp, const double* vcoef, int64_t n,
+                                                  int kind, double s, int ncomp) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double u = up[i];
+  if (kind == 0) {
+    fp[i] = u * u * 0.5;
+  } else if (kind == 1) {
+    fp[i] = sin(u) * s;
+    fp[n + i] = cos(u) * s;
+  } else {
+    for (int c = 0; c < ncomp; ++c) fp[c * n + i] = vcoef[c * n + i] * u;
+  }
+}
+
+// One dimension of the hierarchical-surplus transform
+// (LagrInterpolation::eval_fp_to_coe_D_Lag, source/Interplation.cpp:1279-1400).
+struct SurplusArgs {
+  const double* in;
+  double* out;
+  const int32_t* slot;
+  const int32_t* fbase;
+  const int32_t* fmap;
+  const int32_t* anc_slot;
+  const int32_t* anc_point;
+  const double* wt;
+  int32_t* err;
+  int64_t E, comp_stride;
+  int P, p1, stride_d, ncomp;
+};
+
+__global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t per = a.E * a.P;
+  if (gid >= per * a.ncomp) return;
+  const int64_t c = gid / per;
+  const int64_t rem = gid - c * per;
+  const int64_t e = rem / a.P;
+  const int q = static_cast<int>(rem - e * a.P);
+  const double* in = a.in + c * a.comp_stride;
+  double coe = in[e * a.P + q];
+  const int s = a.slot[e];
+  if (s != 0) {
+    const int pd = (q / a.stride_d) % a.p1;
+    for (int ic = 0; ic < a.p1; ++ic) {
+      const int anc = a.fmap[a.fbase[e] + a.anc_slot[s * a.p1 + ic]];
+      if (anc < 0) { a.err[0] = 1; continue; }
+      const int mm = q + (a.anc_point[s * a.p1 + ic] - pd) * a.stride_d;
+      coe += a.wt[(s * a.p1 + pd) * a.p1 + ic] * in[static_cast<int64_t>(anc) * a.P + mm];
+    }
+  }
+  a.out[c * a.comp_stride + e * a.P + q] = coe;
+}
+
+// SSP-RK3 / forward Euler stage (source/ODESolver.cpp:209-213,285-301).
+__global__ __launch_bounds__(kBlock) void k_rk(double* u, const double* tn, const double* r, int64_t n, int scheme,
+                                                int stage, double dt) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double t = tn[i], uu = u[i], rr = r[i];
+  double out;
+  if (scheme == 0 || stage == 0) out = t + dt * rr;
+  else if (stage == 1) out = 3. / 4. * t + 1. / 4. * (uu + dt * rr);
+  else out = 1. / 3. * t + 2. / 3. * (uu + dt * rr);
+  u[i] = out;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(double* p, int64_t n, double v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(double* dst, const double* src, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Sum of squares per element (VecMultiD::norm before the final root).
+__global__ __launch_bounds__(kBlock) void k_sumsq(const double* u, double* out, int64_t E, int K) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double s = 0.;
+  for (int i = 0; i < K; ++i) {
+    const double x = u[e * K + i];
+    s += x * x;
+  }
+  out[e] = s;
+}
+
+// Weighted Lagrange surplus sum (DGAdaptIntp::indicator_norm_intp_Lag before the root).
+struct LagrNormArgs {
+  const double* c;
+  const double* lmass;
+  const int32_t* slot[kMaxDim];
+  double* out;
+  int64_t E;
+  int D, P, p1;
+};
+
+__global__ __launch_bounds__(kBlock) void k_lagr_norm(LagrNormArgs a) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= a.E) return;
+  double norm = 0.;
+  for (int q = 0; q < a.P; ++q) {
+    double val = 1.0;
+    int rem = q, stride = a.P;
+    for (int d = 0; d < a.D; ++d) {
+      stride /= a.p1;
+      const int pd = rem / stride;
+      rem -= pd * stride;
+      val *= a.lmass[a.slot[d][e] * a.p1 + pd];
+    }
+    const double x = a.c[e * a.P + q];
+    val *= x * x;
+    norm += val;
+  }
+  a.out[e] = norm;
+}
+
+__global__ __launch_bounds__(kBlock) void k_remap(double* dst, const double* src, const int32_t* old_of_new, int64_t E,
+                                                   int T) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= E * T) return;
+  const int64_t e = i / T;
+  const int t = static_cast<int>(i - e * T);
+  const int32_t o = old_of_new[e];
+  dst[i] = (o >= 0) ? src[static_cast<int64_t>(o) * T + t] : 0.;
+}
+
+template <int PF1, int PT1>
+void launch_sweep_t(const SweepArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_sweep<PF1, PT1>), dim3(nblocks(a.E * a.R)), dim3(kBlock), 0, s, a);
+}
+
+void launch_sweep(const SweepArgs& a, int pf1, int pt1, hipStream_t s) {
+#define ADG_SW(F, T) if (pf1 == F && pt1 == T) { launch_sweep_t<F, T>(a, s); return; }
+  ADG_SW(2, 2) ADG_SW(2, 3) ADG_SW(2, 4) ADG_SW(3, 2) ADG_SW(3, 3) ADG_SW(3, 4)
+  ADG_SW(4, 2) ADG_SW(4, 3) ADG_SW(4, 4)
+#undef ADG_SW
+  throw std::runtime_error("sweep: unsupported basis sizes (degrees 1..3 are provided)");
+}
+
+template <class T>
+void upload(DevBuf<T>& b, const std::vector<T>& h, hipStream_t s) {
+  b.ensure(std::max<size_t>(h.size(), 1));
+  if (!h.empty()) hip_check(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+
+}  // namespace
+
+std::vector<std::vector<int>> transform_orders(int D, std::vector<std::vector<int>>& lus) {
+  // FastMultiplyLU::generate_transform_order (source/FastMultiplyLU.cpp:614-664)
+  std::vector<std::vector<int>> dims;
+  lus.clear();
+  if (D == 1) { dims.push_back({0}); lus.push_back({LU_FULL}); return dims; }
+  const int n = 1 << (D - 1);
+  for (int num = 0; num < n; ++num) {
+    std::vector<int> v(D);
+    for (int b = 0; b < D - 1; ++b) v[b] = 2 * ((num >> (D - 2 - b)) & 1);
+    v[D - 1] = 1;
+    std::vector<int> order(D);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return v[a] < v[b]; });
+    std::vector<int> lu(D);
+    for (int i = 0; i < D; ++i) {
+      const int val = v[order[i]];
+      lu[i] = (val == 0) ? LU_L : (val == 1 ? LU_FULL : LU_U);
+    }
+    dims.push_back(order);
+    lus.push_back(lu);
+  }
+  return dims;
+}
+
+Context::Context(int dim, int k_, int m_, int nmax_)
+    : D(dim), k(k_), m(m_), nmax(nmax_),
+      alpt(BasisKind::Alpert, k_, nmax_), lagr(BasisKind::Lagrange, m_, nmax_) {
+  if (D < 1 || D > kMaxDim) throw std::runtime_error("dimension must be 1..4");
+  if (k < 1 || k > 3 || m < 1 || m > 3) throw std::runtime_error("degrees k and m must be 1..3");
+  if (nmax < 1 || nmax > 13) throw std::runtime_error("maximum level must be 1..13");
+  K = 1; P = 1;
+  for (int d = 0; d < D; ++d) { K *= k + 1; P *= m + 1; }
+  hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  na = alpt.size();
+  nl = lagr.size();
+  OperatorTables aa(alpt, alpt), la(lagr, alpt);
+  upload(t_pt, alpt_at_lagr_points(alpt, lagr), stream);
+  upload(t_la_uv, la.u_v, stream);
+  upload(t_la_uvx, la.u_vx, stream);
+  std::vector<double> flx(la.ulft_vjp.size());
+  for (size_t i = 0; i < flx.size(); ++i) flx[i] = la.ulft_vjp[i] + la.urgt_vjp[i];
+  upload(t_la_flx, flx, stream);
+  upload(t_aa[0], aa.u_v, stream);
+  upload(t_aa[1], aa.u_vx, stream);
+  upload(t_aa[2], aa.ulft_vjp, stream);
+  upload(t_aa[3], aa.urgt_vjp, stream);
+  upload(t_aa[4], aa.ujp_vjp, stream);
+  upload(t_lmass, lagr_mass_diag(lagr), stream);
+  SurplusTables sur(lagr);
+  upload(t_anc_slot, sur.anc_slot, stream);
+  upload(t_anc_point, sur.anc_point, stream);
+  upload(t_wt, sur.wt, stream);
+  err_flag.ensure(1);
+  hip_check(hipMemcpyAsync(err_flag.p, std::vector<int32_t>{0}.data(), sizeof(int32_t), hipMemcpyHostToDevice, stream),
+            "upload");
+  sync();
+}
+
+Context::~Context() {
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Context::sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+SweepTable Context::table(int id) {
+  switch (id) {
+    case 0: return {t_pt.p, nl};
+    case 1: return {t_la_uv.p, na};
+    case 2: return {t_la_uvx.p, na};
+    case 3: return {t_la_flx.p, na};
+    case 4: case 5: case 6: case 7: case 8: return {t_aa[id - 4].p, na};
+  }
+  throw std::runtime_error("unknown table id");
+}
+
+void Context::upload_grid() {
+  for (int d = 0; d < D; ++d) {
+    upload(g_slot[d], grid.dslot[d], stream);
+    upload(g_fbase[d], grid.fbase[d], stream);
+    upload(g_fmaxl[d], grid.fmaxl[d], stream);
+    upload(g_fmap[d], grid.fmap[d], stream);
+  }
+}
+
+void Context::ensure_fields() {
+  const size_t e = static_cast<size_t>(std::max<int64_t>(E(), 1));
+  const int mx = std::max(k, m) + 1;
+  int T = 1;
+  for (int d = 0; d < D; ++d) T *= mx;
+  for (DevBuf<double>* b : {&u, &tn, &pred, &rhs}) b->ensure(e * K);
+  for (DevBuf<double>* b : {&up, &uintp}) b->ensure(e * P);
+  for (DevBuf<double>* b : {&fp, &fucoe, &interA, &interB, &vcoef}) b->ensure(e * P * D);
+  tmpA.ensure(e * T);
+  tmpB.ensure(e * T);
+  sumsq.ensure(e);
+}
+
+void Context::remap_field(DevBuf<double>& f, int per_elem, const std::vector<int32_t>& old_of_new, int64_t old_E) {
+  const int64_t newE = static_cast<int64_t>(old_of_new.size());
+  remap_tmp.ensure(std::max<int64_t>(old_E * per_elem, 1));
+  if (old_E > 0)
+    hipLaunchKernelGGL(k_copy, dim3(nblocks(old_E * per_elem)), dim3(kBlock), 0, stream, remap_tmp.p, f.p,
+                       old_E * per_elem);
+  f.ensure(std::max<int64_t>(newE * per_elem, 1));
+  if (newE > 0)
+    hipLaunchKernelGGL(k_remap, dim3(nblocks(newE * per_elem)), dim3(kBlock), 0, stream, f.p, remap_tmp.p,
+                       g_remap.p, newE, per_elem);
+}
+
+void Context::set_grid_slots(const std::vector<int32_t>& slots, bool keep_fields) {
+  HostGrid ng;
+  ng.build(D, nmax, slots, hk);
+  if (keep_fields && grid.E > 0) {
+    const std::vector<int32_t> m = map_new_to_old(grid, ng);
+    upload(g_remap, m, stream);
+    const int64_t oldE = grid.E;
+    remap_field(u, K, m, oldE);
+    remap_field(pred, K, m, oldE);
+    // host-side interpolation state follows the grid
+    if (!up_host.empty()) {
+      std::vector<double> nu(static_cast<size_t>(ng.E) * P, 0.);
+      std::vector<char> na_(ng.E, 1);
+      for (int64_t i = 0; i < ng.E; ++i)
+        if (m[i] >= 0) {
+          std::copy(&up_host[static_cast<size_t>(m[i]) * P], &up_host[static_cast<size_t>(m[i]) * P] + P, &nu[i * P]);
+          na_[i] = 0;
+        }
+      up_host.swap(nu);
+      new_add.swap(na_);
+    }
+  } else {
+    up_host.clear();
+    new_add.assign(ng.E, 1);
+  }
+  grid = std::move(ng);
+  upload_grid();
+  ensure_fields();
+  if (!keep_fields) {
+    for (DevBuf<double>* b : {&u, &tn, &pred, &rhs})
+      hipLaunchKernelGGL(k_fill, dim3(nblocks(std::max<int64_t>(E() * K, 1))), dim3(kBlock), 0, stream, b->p,
+                         E() * K, 0.);
+  }
+  // variable coefficients depend on the point set
+  if (flux_kind == ADG_FLUX_VARCOEF_ID) compute_vcoef();
+}
+
+void Context::sweep(const double* in, const int* sin, double* out, double* accum, const int* sout, int dd,
+                    SweepTable tab, bool flx, int lu, double coef, int pf, int pt) {
+  if (E() == 0) return;
+  SweepArgs a{};
+  a.in = in; a.out = out; a.accum = accum; a.M = tab.dev; a.ld = tab.ld;
+  a.slot = g_slot[dd].p; a.fbase = g_fbase[dd].p; a.fmaxl = g_fmaxl[dd].p; a.fmap = g_fmap[dd].p;
+  a.E = E();
+  int stin[kMaxDim], stout[kMaxDim];
+  stin[D - 1] = 1; stout[D - 1] = 1;
+  for (int d = D - 2; d >= 0; --d) { stin[d] = stin[d + 1] * sin[d + 1]; stout[d] = stout[d + 1] * sout[d + 1]; }
+  a.Tin = stin[0] * sin[0];
+  a.Tout = stout[0] * sout[0];
+  a.sdd_in = stin[dd];
+  a.sdd_out = stout[dd];
+  a.nrest = 0;
+  a.R = 1;
+  for (int d = 0; d < D; ++d) {
+    if (d == dd) continue;
+    a.rsz[a.nrest] = sin[d];
+    a.rsin[a.nrest] = stin[d];
+    a.rsout[a.nrest] = stout[d];
+    a.R *= sin[d];
+    ++a.nrest;
+  }
+  a.flx = flx ? 1 : 0;
+  a.lu = lu;
+  a.coef = coef;
+  if (sin[dd] != pf + 1 || sout[dd] != pt + 1) throw std::runtime_error("sweep: size mismatch in swept dimension");
+  launch_sweep(a, pf + 1, pt + 1, stream);
+}
+
+void Context::partial_sums(const SweepTable* tabs, const int* flx, int pf, int pt, double coef, const double* src,
+                           double* dst) {
+  std::vector<std::vector<int>> lus;
+  const std::vector<std::vector<int>> dims = transform_orders(D, lus);
+  for (size_t o = 0; o < dims.size(); ++o) {
+    int sz[kMaxDim];
+    for (int d = 0; d < D; ++d) sz[d] = pf + 1;
+    const double* cur = src;
+    for (int s = 0; s < D; ++s) {
+      const int dd = dims[o][s];
+      int sin[kMaxDim], sout[kMaxDim];
+      for (int d = 0; d < D; ++d) sin[d] = sz[d];
+      sz[dd] = pt + 1;
+      for (int d = 0; d < D; ++d) sout[d] = sz[d];
+      const bool last = (s == D - 1);
+      double* out = last ? nullptr : ((s % 2 == 0) ? tmpA.p : tmpB.p);
+      sweep(cur, sin, out, last ? dst : nullptr, sout, dd, tabs[dd], flx[dd] != 0, lus[o][s], s == 0 ? coef : 1.0,
+            pf, pt);
+      cur = out;
+    }
+  }
+}
+
+void Context::eval_up() {
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(E() * P)), dim3(kBlock), 0, stream, up.p, E() * P, 0.);
+  SweepTable tabs[kMaxDim];
+  int flx[kMaxDim];
+  for (int d = 0; d < D; ++d) { tabs[d] = table(0); flx[d] = 0; }
+  partial_sums(tabs, flx, k, m, 1.0, u.p, up.p);
+}
+
+void Context::compute_vcoef() {
+  // a_d(x) = 1 + s*sin(2 pi x_{(d+1) mod D}) at every Lagrange point; the
+  // coefficient is sampled on the host with the same libm as the oracle.
+  std::vector<double> h(static_cast<size_t>(E()) * P * D);
+  for (int64_t e = 0; e < E(); ++e) {
+    for (int q = 0; q < P; ++q) {
+      int idx[kMaxDim];
+      int rem = q;
+      for (int d = D - 1; d >= 0; --d) { idx[d] = rem % (m + 1); rem /= (m + 1); }
+      for (int d = 0; d < D; ++d) {
+        const int dn = (d + 1) % D;
+        const double x = lagr.point(grid.slot[e * D + dn] * (m + 1) + idx[dn]);
+        h[(static_cast<size_t>(d) * E() + e) * P + q] = 1. + flux_param * std::sin(2. * PI * x);
+      }
+    }
+  }
+  upload(vcoef, h, stream);
+  ensure_fields();
+}
+
+void Context::flux(int kind, const double* params) {
+  const double s = params ? params[0] : 0.;
+  if (kind == ADG_FLUX_VARCOEF_ID && (flux_kind != kind || flux_param != s)) {
+    flux_kind = kind;
+    flux_param = s;
+    compute_vcoef();
+  }
+  flux_kind = kind;
+  flux_param = s;
+  same_flux = (kind == 0);
+  if (kind == 1 && D != 2) throw std::runtime_error("sin/cos flux is two-dimensional");
+  const int64_t n = E() * P;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_flux, dim3(nblocks(n)), dim3(kBlock), 0, stream, up.p, fp.p, vcoef.p, n, kind, s, D);
+}
+
+void Context::surplus_transform(const double* in, double* out, double* sa, double* sb, int ncomp) {
+  const int p1 = m + 1;
+  const double* cur = in;
+  int stride = P;
+  for (int d = 0; d < D; ++d) {
+    stride /= p1;
+    double* dst = (d == D - 1) ? out : ((d % 2 == 0) ? sa : sb);
+    SurplusArgs a{};
+    a.in = cur; a.out = dst;
+    a.slot = g_slot[d].p; a.fbase = g_fbase[d].p; a.fmap = g_fmap[d].p;
+    a.anc_slot = t_anc_slot.p; a.anc_point = t_anc_point.p; a.wt = t_wt.p; a.err = err_flag.p;
+    a.E = E(); a.comp_stride = E() * P; a.P = P; a.p1 = p1; a.stride_d = stride; a.ncomp = ncomp;
+    if (E() > 0)
+      hipLaunchKernelGGL(k_surplus, dim3(nblocks(E() * P * ncomp)), dim3(kBlock), 0, stream, a);
+    cur = dst;
+  }
+}
+
+void Context::fp_to_coe() {
+  const int ncomp = same_flux ? 1 : D;
+  surplus_transform(fp.p, fucoe.p, interA.p, interB.p, ncomp);
+}
+
+void Context::rhs_zero() {
+  if (E() > 0) hipLaunchKernelGGL(k_fill, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, rhs.p, E() * K, 0.);
+}
+
+void Context::rhs_lagr(bool flux_term) {
+  for (int d = 0; d < D; ++d) {
+    SweepTable tabs[kMaxDim];
+    int flx[kMaxDim];
+    for (int dd = 0; dd < D; ++dd) {
+      if (dd == d) { tabs[dd] = flux_term ? table(3) : table(2); flx[dd] = flux_term ? 1 : 0; }
+      else { tabs[dd] = table(1); flx[dd] = 0; }
+    }
+    const double* src = fucoe.p + static_cast<int64_t>(same_flux ? 0 : d) * E() * P;
+    partial_sums(tabs, flx, m, k, flux_term ? 0.5 : 1.0, src, rhs.p);
+  }
+}
+
+void Context::rhs_single_dim(SweepTable tab, int d, double coef) {
+  int sz[kMaxDim];
+  for (int i = 0; i < D; ++i) sz[i] = k + 1;
+  sweep(u.p, sz, nullptr, rhs.p, sz, d, tab, true, LU_FULL, coef, k, k);
+}
+
+void Context::rhs_penalty(const double* alpha) {
+  for (int d = 0; d < D; ++d) rhs_single_dim(table(8), d, -alpha[d] / 2.);
+}
+
+void Context::rhs_linear(const double* coef) {
+  if (lin_coef.size() != static_cast<size_t>(D) || !std::equal(lin_coef.begin(), lin_coef.end(), coef)) {
+    OperatorTables aa(alpt, alpt);
+    for (int d = 0; d < D; ++d) upload(t_lin[d], linear_operator_table(aa, coef[d]), stream);
+    lin_coef.assign(coef, coef + D);
+  }
+  for (int d = 0; d < D; ++d) rhs_single_dim({t_lin[d].p, na}, d, 1.0);
+}
+
+void Context::rk_init() {
+  if (E() > 0) hipLaunchKernelGGL(k_copy, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, tn.p, u.p, E() * K);
+}
+
+void Context::rk_stage(int scheme, int stage, double dt) {
+  if (E() > 0)
+    hipLaunchKernelGGL(k_rk, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, tn.p, rhs.p, E() * K, scheme,
+                       stage, dt);
+}
+
+void Context::copy_to_predict() {
+  if (E() > 0) hipLaunchKernelGGL(k_copy, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, pred.p, u.p, E() * K);
+}
+
+void Context::copy_predict_to_ucoe() {
+  if (E() > 0) hipLaunchKernelGGL(k_copy, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, pred.p, E() * K);
+}
+
+void Context::check_error_flag() {
+  int32_t h = 0;
+  hip_check(hipMemcpyAsync(&h, err_flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, stream), "download");
+  sync();
+  if (h != 0) {
+    int32_t z = 0;
+    hip_check(hipMemcpyAsync(err_flag.p, &z, sizeof(int32_t), hipMemcpyHostToDevice, stream), "upload");
+    sync();
+    throw std::runtime_error("surplus transform: missing ancestor element (grid has a hole)");
+  }
+}
+
+void Context::indicator_sumsq(double* host_out) {
+  if (E() == 0) return;
+  hipLaunchKernelGGL(k_sumsq, dim3(nblocks(E())), dim3(kBlock), 0, stream, u.p, sumsq.p, E(), K);
+  hip_check(hipMemcpyAsync(host_out, sumsq.p, E() * sizeof(double), hipMemcpyDeviceToHost, stream), "download");
+  sync();
+}
+
+int64_t Context::refine(double eps) {
+  std::vector<double> s(E());
+  indicator_sumsq(s.data());
+  for (double& v : s) v = std::pow(v, 1. / 2.);
+  int64_t added = 0;
+  std::vector<int32_t> ns = refine_slots(grid, s, eps, added);
+  if (added > 0) set_grid_slots(ns, true);
+  return added;
+}
+
+int64_t Context::coarsen(double eta) {
+  std::vector<double> s(E());
+  indicator_sumsq(s.data());
+  for (double& v : s) v = std::pow(v, 1. / 2.);
+  int64_t removed = 0;
+  std::vector<int32_t> ns = coarsen_slots(grid, s, eta, removed);
+  if (removed > 0) set_grid_slots(ns, true);
+  return removed;
+}
+
+void Context::init_adaptive_intp(double (*f)(const double*, int, void*), void* user, double eps) {
+  if (up_host.size() != static_cast<size_t>(E()) * P) {
+    up_host.assign(static_cast<size_t>(E()) * P, 0.);
+    new_add.assign(E(), 1);
+  }
+  std::vector<double> x(D);
+  auto eval_new = [&]() {
+    for (int64_t e = 0; e < E(); ++e) {
+      if (!new_add[e]) continue;
+      for (int q = 0; q < P; ++q) {
+        int rem = q;
+        int idx[kMaxDim];
+        for (int d = D - 1; d >= 0; --d) { idx[d] = rem % (m + 1); rem /= (m + 1); }
+        for (int d = 0; d < D; ++d) x[d] = lagr.point(grid.slot[e * D + d] * (m + 1) + idx[d]);
+        up_host[e * P + q] = f(x.data(), D, user);
+      }
+    }
+    upload(up, up_host, stream);
+    ensure_fields();
+    surplus_transform(up.p, uintp.p, interA.p, interB.p, 1);
+  };
+  eval_new();
+  while (true) {
+    std::fill(new_add.begin(), new_add.end(), 0);
+    LagrNormArgs a{};
+    a.c = uintp.p; a.lmass = t_lmass.p; a.out = sumsq.p; a.E = E(); a.D = D; a.P = P; a.p1 = m + 1;
+    for (int d = 0; d < D; ++d) a.slot[d] = g_slot[d].p;
+    hipLaunchKernelGGL(k_lagr_norm, dim3(nblocks(E())), dim3(kBlock), 0, stream, a);
+    std::vector<double> s(E());
+    hip_check(hipMemcpyAsync(s.data(), sumsq.p, E() * sizeof(double), hipMemcpyDeviceToHost, stream), "download");
+    sync();
+    check_error_flag();
+    for (double& v : s) v = std::pow(v, 0.5);
+    int64_t added = 0;
+    std::vector<int32_t> ns = refine_slots(grid, s, eps, added);
+    if (added == 0) break;
+    set_grid_slots(ns, true);
+    eval_new();
+  }
+  // FastLagrInit::eval_ucoe_Alpt_Lagr: Lagrange surplus -> Alpert coefficients
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, E() * K, 0.);
+  SweepTable tabs[kMaxDim];
+  int flx[kMaxDim];
+  for (int d = 0; d < D; ++d) { tabs[d] = table(1); flx[d] = 0; }
+  partial_sums(tabs, flx, m, k, 1.0, uintp.p, u.p);
+  sync();
+}
+
+}  // namespace adg

@@ -1,0 +1,71 @@
+// Host-side active-element table of the adaptive sparse grid.
+//
+// Elements are kept as a structure of arrays sorted by the reference hash key
+// (Hash::hash_key, source/Hash.cpp:55-114).  Each element is identified by its
+// per-dimension 1D slots (level-ordered cell index, see setup.hpp).  For every
+// dimension d the table also carries the fiber structure the device kernels
+// use instead of the reference's per-element neighbour pointer sets
+// (Element::ptr_vol_alpt / ptr_flx_alpt, include/Element.h:152-155):
+//   fbase[d][e]  offset of e's d-fiber in fmap[d] (fiber id << nmax)
+//   fmap[d][fbase + s] = element index holding slot s of that fiber, or -1
+//   fmaxl[d][e]  deepest level present in e's d-fiber
+// The refine / coarsen rules are the reference's (DGAdapt::refine,
+// DGAdapt::coarsen, source/DGAdapt.cpp:371-404,675-688,982-1016,1250-1385)
+// expressed as set operations on the table.
+#pragma once
+#include <cstdint>
+#include <unordered_map>
+#include <vector>
+
+#include "setup.hpp"
+
+namespace adg {
+
+constexpr int kMaxDim = 4;
+
+struct HostGrid {
+  int D = 0, nmax = 0;
+  int64_t E = 0;
+  std::vector<int32_t> slot;                // [E * D]
+  std::vector<int32_t> hash;                // [E]
+  std::unordered_map<uint64_t, int32_t> index;
+  std::vector<int32_t> dslot[kMaxDim];      // [E]  slot of element e in dim d
+  std::vector<int32_t> fbase[kMaxDim];      // [E]
+  std::vector<int32_t> fmaxl[kMaxDim];      // [E]
+  std::vector<int32_t> fmap[kMaxDim];       // [nfib << nmax]
+  int32_t nfib[kMaxDim] = {0, 0, 0, 0};
+
+  static uint64_t pack(const int32_t* s, int D) {
+    uint64_t k = 0;
+    for (int d = 0; d < D; ++d) k |= static_cast<uint64_t>(s[d]) << (16 * d);
+    return k;
+  }
+  int level(int64_t e, int d) const { return level_of_slot(slot[e * D + d]); }
+  int suppt(int64_t e, int d) const { return suppt_of_slot(slot[e * D + d]); }
+  int32_t find(const int32_t* s) const {
+    auto it = index.find(pack(s, D));
+    return it == index.end() ? -1 : it->second;
+  }
+  int num_total_children(int64_t e) const;
+
+  // Build the table from a list of element slot vectors (any order, no duplicates).
+  void build(int D, int nmax, const std::vector<int32_t>& slots, const HashKey& hk);
+  // Sparse (level sum <= level_init) or full (every level <= level_init) grid
+  // as the DGSolution constructor enumerates it (source/DGSolution.cpp:10-57).
+  static std::vector<int32_t> initial_slots(int D, int level_init, bool sparse);
+};
+
+// Leaves with indicator above eps get their missing children, then missing
+// parents are added until no hole remains.  `norm` is the per-element
+// indicator (pow(sum of squares, 0.5)).  Returns the new slot list; `added`
+// counts the new elements.
+std::vector<int32_t> refine_slots(const HostGrid& g, const std::vector<double>& norm, double eps, int64_t& added);
+
+// Children-free elements whose indicator is below eta and whose maximum level
+// is positive are removed repeatedly until none qualifies.
+std::vector<int32_t> coarsen_slots(const HostGrid& g, const std::vector<double>& norm, double eta, int64_t& removed);
+
+// old_of_new[i] = index in `from` of element i of `to`, or -1 when new.
+std::vector<int32_t> map_new_to_old(const HostGrid& from, const HostGrid& to);
+
+}  // namespace adg

@@ -1,0 +1,112 @@
+// Host-side setup for the adaptive sparse-grid DG hot path: 1D hierarchical
+// bases, Gauss-Legendre products, level-ordered 1D operator tables, Lagrange
+// point tables, hierarchical-surplus weights and the element hash key.
+//
+// Everything here runs once per problem (or once per grid change for the
+// hash keys) on the host; the results are uploaded to the device as dense
+// tables.  Formulas and their floating-point evaluation order follow the
+// reference so that the tables are bit-identical to what it builds:
+//   AlptBasis   source/AlptBasis.cpp:16-130,256-340,473-560
+//   LagrBasis   source/LagrBasis.cpp:9-160,190-420,852-864
+//   Basis       source/Basis.cpp:8-81,121-173
+//   Quad::GL_1D source/Quad.cpp:31-80,224-244,140-146
+//   AllBasis    include/AllBasis.h:53-78
+//   OperatorMatrix1D include/OperatorMatrix1D.h:124-200
+//   Lag_pt_Alpt_1D   source/Interplation.cpp:16-99
+//   pw1d weights     source/Interplation.cpp:775-887
+//   Hash        source/Hash.cpp:55-167
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace adg {
+
+constexpr double ROUND_OFF = 1e-13;  // include/libs.h:28
+constexpr double PI = 3.1415926535897932384626433832795028841971693993751058209749445923078164062;
+
+enum class BasisKind : int { Alpert = 0, Lagrange = 1 };
+
+// 1D hierarchical index helpers.  "slot" is the level-ordered position of a
+// 1D cell: level 0 -> 0, level n>=1 support j (odd) -> 2^(n-1) + (j-1)/2
+// (Element::index_to_order_elem, source/Element.cpp:388-391).
+inline int slot_of(int level, int suppt) { return level == 0 ? 0 : (1 << (level - 1)) + (suppt - 1) / 2; }
+inline int level_of_slot(int s) { int n = 0; while ((1 << n) <= s) ++n; return n; }  // 0->0, 1->1, 2..3->2
+inline int suppt_of_slot(int s) { if (s == 0) return 1; int n = level_of_slot(s); return 2 * (s - (1 << (n - 1))) + 1; }
+
+// Support interval of the 1D cell (Basis::Basis, source/Basis.cpp:21-36).
+void cell_support(int level, int suppt, double& xl, double& xr);
+
+// A family of 1D basis functions up to level NMAX (AllBasis<T>).
+struct Basis1D {
+  BasisKind kind;
+  int pmax;       // polynomial degree
+  int msh_case;   // Lagrange point family (only msh_case 1 is provided)
+  int nmax;       // maximum mesh level
+  std::vector<double> msh0, msh1;  // Lagrange points on levels 0 and 1, mapped to [0,1]
+
+  Basis1D(BasisKind k, int pmax, int nmax, int msh_case = 1);
+  int size() const { return (1 << nmax) * (pmax + 1); }
+  void decode(int order, int& level, int& suppt, int& p) const;
+  // value (deriv 0/1) or one-sided limit (sgn -1/+1) of basis function `order` at x
+  double val(int order, double x, int deriv = 0, int sgn = 0) const;
+  // interpolation point of Lagrange basis `order` (LagrBasis::intep_pt)
+  double point(int order) const;
+};
+
+// GL 10-point rule on [tl,tr] as Quad::GL_1D evaluates it.
+template <class F> double gl10(F&& f, double tl, double tr);
+
+// Basis::product_volume(u, v, du, dv) with 10 GL points.
+double product_volume(const Basis1D& U, int ru, const Basis1D& V, int cv, int du, int dv);
+// Basis::product_edge_dis_v(u, v, sl_u, sl_v, 0, 0, "period").
+double product_edge_dis_v_period(const Basis1D& U, int ru, const Basis1D& V, int cv, int slu, int slv);
+
+// Dense level-ordered 1D operator tables, row = "from" (U) basis, column =
+// "to" (V) basis, row-major: m[r * cols + c].
+struct OperatorTables {
+  int rows = 0, cols = 0;
+  std::vector<double> u_v, u_vx, ulft_vjp, urgt_vjp, ujp_vjp;
+  OperatorTables(const Basis1D& U, const Basis1D& V, bool with_flux = true);
+};
+
+// T[alpt_order * n_lagr + lagr_order] = value of Alpert basis at the Lagrange
+// point (0 outside the closed support); the transpose that FastLagrIntp builds
+// from LagrInterpolation::Lag_pt_Alpt_1D (source/FastMultiplyLU.cpp:1339-1360).
+std::vector<double> alpt_at_lagr_points(const Basis1D& alpt, const Basis1D& lagr);
+
+// Diagonal of the Lagrange-Lagrange mass matrix (u_v(num,num)), used by the
+// interpolation refinement indicator (source/DGAdaptIntp.cpp:236-272).
+std::vector<double> lagr_mass_diag(const Basis1D& lagr);
+
+// Hierarchical-surplus data of one 1D Lagrange cell (LagrInterpolation::pwts):
+// for cell slot s (level >= 1) the P1 ancestor points (slot, point index) and
+// weights wt[p0][ic].  Tables are [slot][ic] and [slot][p0][ic].
+struct SurplusTables {
+  int p1 = 0, nslots = 0;
+  std::vector<int> anc_slot;   // [nslots * p1]
+  std::vector<int> anc_point;  // [nslots * p1]
+  std::vector<double> wt;      // [nslots * p1 * p1]
+  explicit SurplusTables(const Basis1D& lagr);
+};
+
+// Entries of the assembled const-coefficient advection operator in one
+// dimension: coef*u_vx + coef*(ulft_vjp if coef >= 0 else urgt_vjp), each
+// product dropped below 1e-13 as BilinearFormAlpt::assemble_matrix_alpt drops
+// triplets (source/BilinearForm.cpp:64-74,691-710).
+std::vector<double> linear_operator_table(const OperatorTables& aa, double coef);
+
+// Hash::hash_key arithmetic (int/double mix with truncation kept).
+class HashKey {
+ public:
+  HashKey();
+  int key(const int* level, const int* suppt, int dim) const;
+ private:
+  std::vector<std::vector<double>> bicoef_;
+};
+
+// Gauss-Legendre tables of Quad::read_GL_quad_1D (1..10 points).
+const std::vector<double>& gl_nodes(int n);
+const std::vector<double>& gl_weights(int n);
+
+}  // namespace adg

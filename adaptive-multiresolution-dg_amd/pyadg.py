@@ -1,0 +1,276 @@
+"""ctypes binding of libadg_hip.so (C-ABI in include/adg.h).
+
+Arrays are ordered by element hash key, per-element tensors row-major.  The
+library must be built in-tree (`make -C adaptive-multiresolution-dg_amd`);
+loading fails loudly when it is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libadg_hip.so")
+_LIB = None
+
+FIELD_UCOE, FIELD_RHS, FIELD_UP, FIELD_TN, FIELD_UINTP, FIELD_FUCOE = 0, 1, 2, 3, 4, 10
+TAB_ALPT_AT_LAGR, TAB_LA_UV, TAB_LA_UVX, TAB_LA_FLX = 0, 1, 2, 3
+TAB_AA_UV, TAB_AA_UVX, TAB_AA_ULFT, TAB_AA_URGT, TAB_AA_UJP = 4, 5, 6, 7, 8
+LU_FULL, LU_L, LU_U = 0, 1, 2
+FLUX_BURGERS, FLUX_SINCOS, FLUX_VARCOEF = 0, 1, 2
+RK_EULER, RK3 = 0, 3
+
+POINT_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p)
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("dof_stages", ctypes.c_int64), ("max_elem", ctypes.c_int64),
+                ("seconds", ctypes.c_double), ("final_time", ctypes.c_double)]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("flux_kind", ctypes.c_int), ("flux_param", ctypes.c_double), ("alpha", ctypes.c_double * 4),
+                ("wave_speed", ctypes.c_double * 4), ("cfl", ctypes.c_double), ("t_final", ctypes.c_double),
+                ("adaptive", ctypes.c_int), ("eps", ctypes.c_double), ("eta", ctypes.c_double),
+                ("max_steps", ctypes.c_int64)]
+
+
+class ICParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("shift", ctypes.c_double), ("base", ctypes.c_double),
+                ("ampl", ctypes.c_double), ("nbump", ctypes.c_int), ("bump_amp", ctypes.c_double * 8),
+                ("bump_width", ctypes.c_double * 8), ("bump_center", (ctypes.c_double * 4) * 8),
+                ("inside", ctypes.c_double), ("outside", ctypes.c_double), ("radius", ctypes.c_double),
+                ("center", ctypes.c_double * 4), ("scale", ctypes.c_double * 4), ("offset", ctypes.c_double * 4)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `make -C adaptive-multiresolution-dg_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        P = ctypes.POINTER
+        I32P, DP = P(ctypes.c_int32), P(dbl)
+        sig = {
+            "adg_create": (i32, [i32, i32, i32, i32, P(vp)]),
+            "adg_destroy": (i32, [vp]),
+            "adg_last_error": (ctypes.c_char_p, [vp]),
+            "adg_sync": (i32, [vp]),
+            "adg_grid_init": (i32, [vp, i32, i32]),
+            "adg_grid_set": (i32, [vp, I32P, I32P, i64]),
+            "adg_num_elem": (i64, [vp]),
+            "adg_grid_get": (i32, [vp, I32P, I32P, I32P]),
+            "adg_max_level": (i32, [vp, I32P]),
+            "adg_field_set": (i32, [vp, i32, DP]),
+            "adg_field_get": (i32, [vp, i32, DP]),
+            "adg_sweep": (i32, [vp, i32, i32, i32, i32, I32P, I32P, dbl, DP, DP]),
+            "adg_eval_up": (i32, [vp]),
+            "adg_flux": (i32, [vp, i32, DP]),
+            "adg_fp_to_coe": (i32, [vp]),
+            "adg_rhs_zero": (i32, [vp]),
+            "adg_rhs_lagr_vol": (i32, [vp]),
+            "adg_rhs_lagr_flx": (i32, [vp]),
+            "adg_rhs_penalty": (i32, [vp, DP]),
+            "adg_rhs_linear": (i32, [vp, DP]),
+            "adg_rk_init": (i32, [vp]),
+            "adg_rk_stage": (i32, [vp, i32, i32, dbl]),
+            "adg_copy_to_predict": (i32, [vp]),
+            "adg_copy_predict_to_ucoe": (i32, [vp]),
+            "adg_indicator_sumsq": (i32, [vp, DP]),
+            "adg_refine": (i32, [vp, dbl, P(i64)]),
+            "adg_coarsen": (i32, [vp, dbl, P(i64)]),
+            "adg_init_adaptive_intp": (i32, [vp, POINT_FN, vp, dbl]),
+            "adg_ic_eval": (dbl, [DP, i32, vp]),
+            "adg_run_linear": (i32, [vp, DP, dbl, i64, P(RunStats)]),
+            "adg_run_hyperbolic": (i32, [vp, P(Problem), P(RunStats)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def ic_eval_ptr():
+    """Address of the library's built-in point function adg_ic_eval."""
+    return ctypes.cast(lib().adg_ic_eval, ctypes.c_void_p).value
+
+
+class AdgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"adg error {code}: {msg}")
+        self.code = code
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+class Solution:
+    """One device-resident DG solution (a context of the C-ABI)."""
+
+    def __init__(self, dim, k, m, nmax, sparse=True, level_init=None):
+        self.dim, self.k, self.m, self.nmax = dim, k, m, nmax
+        self.K = (k + 1) ** dim
+        self.P = (m + 1) ** dim
+        h = ctypes.c_void_p()
+        rc = lib().adg_create(dim, k, m, nmax, ctypes.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = lib().adg_last_error(h).decode() if h.value else "create failed"
+            if h.value:
+                lib().adg_destroy(h)
+            self._h = None
+            raise AdgError(rc, msg)
+        self._keep = []
+        self._check(lib().adg_grid_init(self._h, int(sparse), nmax if level_init is None else level_init))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise AdgError(rc, lib().adg_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().adg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_elem(self):
+        return int(lib().adg_num_elem(self._h))
+
+    def grid(self):
+        E = self.num_elem
+        lev = np.zeros((E, self.dim), np.int32)
+        sup = np.zeros((E, self.dim), np.int32)
+        key = np.zeros(E, np.int32)
+        self._check(lib().adg_grid_get(self._h, _ip(lev), _ip(sup), _ip(key)))
+        return lev, sup, key
+
+    def set_grid(self, level, suppt):
+        lev = np.ascontiguousarray(level, np.int32)
+        sup = np.ascontiguousarray(suppt, np.int32)
+        self._check(lib().adg_grid_set(self._h, _ip(lev), _ip(sup), lev.shape[0]))
+
+    def max_level(self):
+        out = np.zeros(self.dim, np.int32)
+        self._check(lib().adg_max_level(self._h, _ip(out)))
+        return out
+
+    def _fsize(self, field):
+        return self.K if field in (FIELD_UCOE, FIELD_RHS, FIELD_TN) else self.P
+
+    def get(self, field=FIELD_UCOE):
+        out = np.zeros(self.num_elem * self._fsize(field))
+        self._check(lib().adg_field_get(self._h, field, _dp(out)))
+        return out
+
+    def set(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.size == self.num_elem * self._fsize(field)
+        self._check(lib().adg_field_set(self._h, field, _dp(v)))
+
+    def sweep(self, table, flx, lu, dim, size_in, size_out, coef, x):
+        si = np.asarray(size_in, np.int32)
+        so = np.asarray(size_out, np.int32)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.num_elem * int(np.prod(so)))
+        self._check(lib().adg_sweep(self._h, table, int(flx), lu, dim, _ip(si), _ip(so), coef, _dp(x), _dp(out)))
+        return out
+
+    def eval_up(self):
+        self._check(lib().adg_eval_up(self._h))
+
+    def flux(self, kind, params=(0.0,)):
+        p = np.asarray(params, np.float64)
+        self._check(lib().adg_flux(self._h, kind, _dp(p)))
+
+    def fp_to_coe(self):
+        self._check(lib().adg_fp_to_coe(self._h))
+
+    def rhs_zero(self):
+        self._check(lib().adg_rhs_zero(self._h))
+
+    def rhs_lagr_vol(self):
+        self._check(lib().adg_rhs_lagr_vol(self._h))
+
+    def rhs_lagr_flx(self):
+        self._check(lib().adg_rhs_lagr_flx(self._h))
+
+    def rhs_penalty(self, alpha):
+        a = np.asarray(alpha, np.float64)
+        self._check(lib().adg_rhs_penalty(self._h, _dp(a)))
+
+    def rhs_linear(self, coef):
+        c = np.asarray(coef, np.float64)
+        self._check(lib().adg_rhs_linear(self._h, _dp(c)))
+
+    def rk_init(self):
+        self._check(lib().adg_rk_init(self._h))
+
+    def rk_stage(self, scheme, stage, dt):
+        self._check(lib().adg_rk_stage(self._h, scheme, stage, dt))
+
+    def copy_to_predict(self):
+        self._check(lib().adg_copy_to_predict(self._h))
+
+    def copy_predict_to_ucoe(self):
+        self._check(lib().adg_copy_predict_to_ucoe(self._h))
+
+    def indicator_sumsq(self):
+        out = np.zeros(self.num_elem)
+        self._check(lib().adg_indicator_sumsq(self._h, _dp(out)))
+        return out
+
+    def refine(self, eps):
+        n = ctypes.c_int64()
+        self._check(lib().adg_refine(self._h, eps, ctypes.byref(n)))
+        return n.value
+
+    def coarsen(self, eta):
+        n = ctypes.c_int64()
+        self._check(lib().adg_coarsen(self._h, eta, ctypes.byref(n)))
+        return n.value
+
+    def init_adaptive_intp(self, func, eps):
+        """func(x: ndarray[dim]) -> float; evaluated on the host at Lagrange points."""
+        dim = self.dim
+
+        def cb(xp, d, user):
+            return float(func(np.ctypeslib.as_array(xp, shape=(dim,))))
+
+        cfun = POINT_FN(cb)
+        self._keep.append(cfun)
+        self._check(lib().adg_init_adaptive_intp(self._h, cfun, None, eps))
+
+    def init_adaptive_intp_builtin(self, params, eps):
+        """Initialise with the library's adg_ic_eval and an ICParams."""
+        self._keep.append(params)
+        fn = POINT_FN(ic_eval_ptr())
+        self._keep.append(fn)
+        self._check(lib().adg_init_adaptive_intp(self._h, fn, ctypes.cast(ctypes.pointer(params), ctypes.c_void_p), eps))
+
+    def run_linear(self, coef, dt, nsteps):
+        c = np.asarray(coef, np.float64)
+        st = RunStats()
+        self._check(lib().adg_run_linear(self._h, _dp(c), dt, nsteps, ctypes.byref(st)))
+        return st
+
+    def run_hyperbolic(self, problem):
+        st = RunStats()
+        self._check(lib().adg_run_hyperbolic(self._h, ctypes.byref(problem), ctypes.byref(st)))
+        return st
+
+    def sync(self):
+        self._check(lib().adg_sync(self._h))

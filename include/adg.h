@@ -1,0 +1,194 @@
+/* C-ABI of the MI355X adaptive sparse-grid DG library (libadg_hip.so).
+ *
+ * One context holds one DG solution (DGSolution / DGAdapt state) on one GPU:
+ * the sorted active-element table, the fiber maps, the 1D operator tables and
+ * the coefficient arrays, all resident in device memory.  Host arrays that
+ * cross this interface are borrowed for the duration of the call and are
+ * ordered by element hash key (ascending); per-element tensors are row-major
+ * (last dimension fastest), as VecMultiD stores them (include/VecMultiD.h:114-152
+ * in the reference).
+ *
+ * Every function returns 0 on success and a negative ADG_E_* code on error;
+ * adg_last_error() returns the message.  Calls on one context are ordered on
+ * that context's HIP stream; functions that hand results to the host
+ * synchronise that stream.  Contexts are independent; a context must not be
+ * used from two host threads at once.
+ *
+ * Each entry point names the reference interface it replaces
+ * (file:line in JuntaoHuang/adaptive-multiresolution-DG).
+ */
+#ifndef ADG_H
+#define ADG_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct adg_ctx adg_ctx;
+/* Initial-condition / point function: value at x[0..dim-1]. */
+typedef double (*adg_point_fn)(const double* x, int dim, void* user);
+
+enum {
+  ADG_OK = 0,
+  ADG_E_ARG = -1,      /* invalid argument or state */
+  ADG_E_HIP = -2,      /* HIP runtime error */
+  ADG_E_ANCESTOR = -3, /* surplus transform found a missing ancestor (grid has a hole) */
+  ADG_E_NONFINITE = -4,/* non-finite value detected */
+  ADG_E_INTERNAL = -5
+};
+
+/* Fields (coefficient arrays) of a context. */
+enum {
+  ADG_FIELD_UCOE = 0,   /* Element::ucoe_alpt, (k+1)^dim per element */
+  ADG_FIELD_RHS = 1,    /* Element::rhs */
+  ADG_FIELD_UP = 2,     /* Element::up_intp, values at Lagrange points, (m+1)^dim */
+  ADG_FIELD_TN = 3,     /* RK stage base u^n (ExplicitRK::ucoe_tn) */
+  ADG_FIELD_UINTP = 4,  /* Element::ucoe_intp (Lagrange surplus of the initial function) */
+  ADG_FIELD_FUCOE = 10  /* +d: Element::fucoe_intp[0][d], Lagrange surplus of flux d */
+};
+
+/* 1D operator tables (row = "from" basis, column = "to" basis). */
+enum {
+  ADG_TAB_ALPT_AT_LAGR = 0, /* Alpert basis at Lagrange points (FastLagrIntp::alpt_basis_Lagr_pt) */
+  ADG_TAB_LA_UV = 1,        /* OperatorMatrix1D<Lagr,Alpt>::u_v */
+  ADG_TAB_LA_UVX = 2,       /* OperatorMatrix1D<Lagr,Alpt>::u_vx */
+  ADG_TAB_LA_FLX = 3,       /* ulft_vjp + urgt_vjp (Lagr,Alpt) */
+  ADG_TAB_AA_UV = 4, ADG_TAB_AA_UVX = 5, ADG_TAB_AA_ULFT = 6, ADG_TAB_AA_URGT = 7,
+  ADG_TAB_AA_UJP = 8        /* OperatorMatrix1D<Alpt,Alpt>::ujp_vjp */
+};
+
+/* L/U split of a 1D sweep (FastMultiplyLU::transform_1D LU argument). */
+enum { ADG_LU_FULL = 0, ADG_LU_L = 1, ADG_LU_U = 2 };
+
+/* Point flux kinds (LagrInterpolation::eval_fp_Lag / eval_coe_u_Lag). */
+enum {
+  ADG_FLUX_BURGERS = 0,  /* f_d(u) = u*u*0.5 for every d (FluxFunction::burgers_flux_scalar) */
+  ADG_FLUX_SINCOS = 1,   /* f_0 = sin(u)*s, f_1 = cos(u)*s (FluxFunction::sin_flux/cos_flux) */
+  ADG_FLUX_VARCOEF = 2   /* f_d = a_d(x)*u, a_d(x) = 1 + s*sin(2*pi*x_{(d+1) mod dim}) */
+};
+
+/* RK schemes (ODESolver.h): forward Euler and SSP-RK3. */
+enum { ADG_RK_EULER = 0, ADG_RK3 = 3 };
+
+/* Context: statics of the reference (AlptBasis::PMAX = k, LagrBasis::PMAX = m,
+ * LagrBasis::msh_case = 1, Element::DIM = dim, DGSolution NMAX = nmax) plus the
+ * AllBasis / OperatorMatrix1D / LagrInterpolation tables built on the host and
+ * uploaded once.  Replaces the setup block of the example drivers
+ * (example/02_hyperbolic_05_burgers_adapt.cpp:148-166). */
+int adg_create(int dim, int k, int m, int nmax, adg_ctx** out);
+int adg_destroy(adg_ctx* c);
+const char* adg_last_error(adg_ctx* c);
+int adg_sync(adg_ctx* c);
+
+/* Grid: DGSolution/DGAdapt constructor (source/DGSolution.cpp:10-57,
+ * source/DGAdapt.cpp:5-37): sparse (level sum <= level_init) or full grid. */
+int adg_grid_init(adg_ctx* c, int sparse, int level_init);
+/* Replace the grid by an explicit element list (levels/supports, any order). */
+int adg_grid_set(adg_ctx* c, const int32_t* level, const int32_t* suppt, int64_t n);
+int64_t adg_num_elem(adg_ctx* c);
+/* Levels, supports (E x dim) and Hash::hash_key values, in hash order. */
+int adg_grid_get(adg_ctx* c, int32_t* level, int32_t* suppt, int32_t* hash);
+/* DGSolution::max_mesh_level_vec (source/DGSolution.cpp:948-959). */
+int adg_max_level(adg_ctx* c, int32_t* out);
+
+int adg_field_set(adg_ctx* c, int field, const double* host);
+int adg_field_get(adg_ctx* c, int field, double* host);
+
+/* One 1D sweep, FastMultiplyLU::transform_1D (source/FastMultiplyLU.cpp:436-512),
+ * on host arrays: out = coef * sum over related sources (vol or flx relation,
+ * L/U/full filter) of in * table. */
+int adg_sweep(adg_ctx* c, int table, int flx, int lu, int dim, const int32_t* size_in,
+              const int32_t* size_out, double coef, const double* in, double* out);
+
+/* FastLagrIntp::eval_up_Lagr (source/FastMultiplyLU.cpp:1362-1365). */
+int adg_eval_up(adg_ctx* c);
+/* LagrInterpolation::eval_fp_Lag / eval_coe_u_Lag (source/Interplation.cpp:256-295,648-696). */
+int adg_flux(adg_ctx* c, int kind, const double* params);
+/* LagrInterpolation::eval_fp_to_coe_D_Lag (source/Interplation.cpp:1222-1430). */
+int adg_fp_to_coe(adg_ctx* c);
+/* DGSolution::set_rhs_zero (source/DGSolution.cpp:1067-1076). */
+int adg_rhs_zero(adg_ctx* c);
+/* HyperbolicLagrRHS::rhs_vol_scalar / rhs_flx_intp_scalar (source/FastMultiplyLU.cpp:1125-1189). */
+int adg_rhs_lagr_vol(adg_ctx* c);
+int adg_rhs_lagr_flx(adg_ctx* c);
+/* HyperbolicAlptRHS::rhs_flx_penalty_scalar (source/FastMultiplyLU.cpp:1269-1277):
+ * Lax-Friedrichs penalty -alpha_d/2 [u][v]; equals the assembled
+ * HyperbolicAlpt::assemble_matrix_flx_scalar pair (source/BilinearForm.cpp:759-769). */
+int adg_rhs_penalty(adg_ctx* c, const double* alpha);
+/* HyperbolicAlpt::assemble_matrix_scalar + BilinearForm::multi (source/BilinearForm.cpp:691-710),
+ * applied matrix-free: rhs += A u for u_t + sum_d coef_d u_{x_d} = 0. */
+int adg_rhs_linear(adg_ctx* c, const double* coef);
+
+/* ExplicitRK::init / RK3SSP::step_stage / ForwardEuler::step_stage
+ * (source/ODESolver.cpp:209-213,285-301). */
+int adg_rk_init(adg_ctx* c);
+int adg_rk_stage(adg_ctx* c, int scheme, int stage, double dt);
+/* DGSolution::copy_ucoe_to_predict / copy_predict_to_ucoe (source/DGSolution.cpp:1151-1172). */
+int adg_copy_to_predict(adg_ctx* c);
+int adg_copy_predict_to_ucoe(adg_ctx* c);
+
+/* Per-element sum of squared Alpert coefficients; DGAdapt::indicator_norm is
+ * pow(sum, 0.5) (source/DGAdapt.cpp:1018-1030). */
+int adg_indicator_sumsq(adg_ctx* c, double* out);
+/* DGAdapt::refine / DGAdapt::coarsen (source/DGAdapt.cpp:371-404,675-688).
+ * New elements start with zero coefficients. */
+int adg_refine(adg_ctx* c, double eps, int64_t* n_added);
+int adg_coarsen(adg_ctx* c, double eta, int64_t* n_removed);
+
+/* DGAdaptIntp::init_adaptive_intp (source/DGAdaptIntp.cpp:35-45): adaptive
+ * Lagrange interpolation of f with refinement threshold eps, then the
+ * Lagrange-to-Alpert transform (FastLagrInit::eval_ucoe_Alpt_Lagr). */
+int adg_init_adaptive_intp(adg_ctx* c, adg_point_fn f, void* user, double eps);
+
+/* Built-in point functions of the benchmark configurations, usable as
+ * adg_point_fn with `user` pointing to an adg_ic_params.  Coordinates are the
+ * reference's unit cube [0,1]^dim.
+ *   kind 0: sin(2 pi (sum_d x_d - shift))
+ *   kind 1: prod_d sin(2 pi (x_d - shift))
+ *   kind 2: base + ampl sin(2 pi sum_d x_d) + sum_b A_b exp(-|x - c_b|^2 / w_b^2)
+ *           (periodic minimum-image distance)
+ *   kind 3: inside if |offset + scale*x - center| <= radius, else outside
+ *   kind 4: sum_b A_b exp(-|x - c_b|^2 / w_b^2) (periodic minimum-image distance) */
+typedef struct {
+  int kind;
+  double shift, base, ampl;
+  int nbump;
+  double bump_amp[8], bump_width[8], bump_center[8][4];
+  double inside, outside, radius, center[4], scale[4], offset[4];
+} adg_ic_params;
+double adg_ic_eval(const double* x, int dim, void* user);
+
+/* Whole-run drivers, the time loops of the reference example drivers. */
+typedef struct {
+  int64_t steps;          /* time steps taken */
+  int64_t dof_stages;     /* sum over RK stages of active DoF */
+  int64_t max_elem;       /* largest element count seen */
+  double seconds;         /* wall time of the loop */
+  double final_time;
+} adg_run_stats;
+
+/* RK3 on u_t + sum_d coef_d u_{x_d} = 0 with a fixed grid
+ * (example/02_hyperbolic_01_scalar_const_coefficient.cpp:148-179). */
+int adg_run_linear(adg_ctx* c, const double* coef, double dt, int64_t nsteps, adg_run_stats* st);
+
+/* Nonlinear / variable-coefficient hyperbolic run with Lagrange flux
+ * interpolation, LF penalty and (when adaptive) predict-refine-RK3-coarsen
+ * steps (example/02_hyperbolic_05_burgers_adapt.cpp:223-498). */
+typedef struct {
+  int flux_kind;           /* ADG_FLUX_* */
+  double flux_param;       /* s of ADG_FLUX_SINCOS / ADG_FLUX_VARCOEF */
+  double alpha[4];         /* Lax-Friedrichs constants per dimension */
+  double wave_speed[4];    /* dt = cfl / sum_d |c_d| 2^{L_d} */
+  double cfl;
+  double t_final;
+  int adaptive;            /* 0: fixed grid */
+  double eps, eta;         /* refine / coarsen thresholds */
+  int64_t max_steps;       /* 0: no limit */
+} adg_problem;
+int adg_run_hyperbolic(adg_ctx* c, const adg_problem* p, adg_run_stats* st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,176 @@
+"""GPU parity: the HIP library against the CPU oracle on the same inputs.
+
+FP64 results are compared bit for bit (both sum every output in ascending
+source-slot order with no FMA).  The one exception is the sin/cos flux, where
+the device libm and glibc may differ in the last bit: there the tolerance is
+1e-12 relative to the field's max norm.
+"""
+import numpy as np
+import pytest
+
+import pyadg
+import pyoracle
+from common import (first_diff, ic_params, init_both, oracle_run_hyperbolic, oracle_run_linear, problem,
+                    same_grid)
+
+pytestmark = pytest.mark.gpu
+
+rng = np.random.default_rng(1234)
+
+
+def pair(dim, k, m, nmax, sparse=True, level_init=None):
+    print(f"[case] dim={dim} k={k} m={m} nmax={nmax} sparse={sparse} level_init={level_init}", flush=True)
+    g = pyadg.Solution(dim, k, m, nmax, sparse, level_init)
+    o = pyoracle.Oracle(dim, k, m, nmax, sparse, level_init)
+    assert same_grid(g, o)
+    return g, o
+
+
+def adapted_pair(dim, k, m, nmax, level_init, params, eps):
+    g, o = pair(dim, k, m, nmax, True, level_init)
+    init_both(g, o, params, eps)
+    assert same_grid(g, o), "adaptive interpolation produced different element sets"
+    return g, o
+
+
+SWEEP_CASES = []
+for table in range(9):
+    for lu in (0, 1, 2):
+        for flx in (0, 1):
+            if table in (1, 2) and flx:
+                continue
+            SWEEP_CASES.append((table, lu, flx))
+
+
+@pytest.mark.parametrize("dim,k,m,nmax", [(2, 2, 3, 5), (3, 1, 2, 4)])
+def test_sweep_bitexact_sparse(dim, k, m, nmax):
+    g, o = pair(dim, k, m, nmax)
+    E = g.num_elem
+    for table, lu, flx in SWEEP_CASES:
+        pf = k if (table == 0 or table >= 4) else m
+        pt = m if table == 0 else k
+        for d in range(dim):
+            si = [pf + 1] * dim
+            so = list(si)
+            so[d] = pt + 1
+            x = rng.standard_normal(E * int(np.prod(si)))
+            a = g.sweep(table, flx, lu, d, si, so, 0.7, x)
+            b = o.sweep(table, flx, lu, d, si, so, 0.7, x)
+            assert np.array_equal(a, b), f"table={table} lu={lu} flx={flx} d={d}: {first_diff(a, b)}"
+
+
+def test_sweep_bitexact_adaptive_grid():
+    # an adaptively refined 2D grid: fibers with holes, uneven depth
+    p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.2, 0.06, (0.3, 0.7)), (0.1, 0.08, (0.8, 0.2))])
+    g, o = adapted_pair(2, 2, 3, 7, 3, p, 1e-4)
+    E = g.num_elem
+    assert E > 200
+    for table, lu, flx in SWEEP_CASES:
+        pf = 2 if (table == 0 or table >= 4) else 3
+        pt = 3 if table == 0 else 2
+        for d in range(2):
+            si = [pf + 1] * 2
+            so = list(si)
+            so[d] = pt + 1
+            x = rng.standard_normal(E * int(np.prod(si)))
+            a = g.sweep(table, flx, lu, d, si, so, -1.3, x)
+            b = o.sweep(table, flx, lu, d, si, so, -1.3, x)
+            assert np.array_equal(a, b), f"table={table} lu={lu} flx={flx} d={d}: {first_diff(a, b)}"
+
+
+def test_init_and_operators_bitexact_burgers():
+    p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.07, (0.4, 0.6))])
+    g, o = adapted_pair(2, 2, 3, 6, 4, p, 1e-4)
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.array_equal(a, b), "init ucoe: " + first_diff(a, b)
+    g.eval_up(); o.eval_up()
+    a, b = g.get(pyadg.FIELD_UP), o.get(pyoracle.FIELD_UP)
+    assert np.array_equal(a, b), "eval_up: " + first_diff(a, b)
+    g.flux(pyadg.FLUX_BURGERS); o.flux(0)
+    g.fp_to_coe(); o.fp_to_coe()
+    for d in range(2):
+        a, b = g.get(pyadg.FIELD_FUCOE + d), o.get(10 + d)
+        assert np.array_equal(a, b), f"fucoe[{d}]: " + first_diff(a, b)
+    g.rhs_zero(); o.rhs_zero()
+    g.rhs_lagr_vol(); o.rhs_lagr_vol()
+    a, b = g.get(pyadg.FIELD_RHS), o.get(pyoracle.FIELD_RHS)
+    assert np.array_equal(a, b), "rhs vol: " + first_diff(a, b)
+    g.rhs_lagr_flx(); o.rhs_lagr_flx()
+    a, b = g.get(pyadg.FIELD_RHS), o.get(pyoracle.FIELD_RHS)
+    assert np.array_equal(a, b), "rhs flx: " + first_diff(a, b)
+    g.rhs_penalty([1.2, 1.2]); o.rhs_penalty([1.2, 1.2])
+    a, b = g.get(pyadg.FIELD_RHS), o.get(pyoracle.FIELD_RHS)
+    assert np.array_equal(a, b), "rhs penalty: " + first_diff(a, b)
+    g.rk_init(); o.rk_init()
+    for st in range(3):
+        g.rk_stage(pyadg.RK3, st, 1e-3); o.rk_stage(3, st, 1e-3)
+        a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+        assert np.array_equal(a, b), f"rk stage {st}: " + first_diff(a, b)
+
+
+def test_linear_run_bitexact():
+    # config-1 shape at a reduced level: 2D, k=2, sparse N=6, RK3 steps of u_t + u_x + u_y = 0
+    p = ic_params(0, shift=0.0)
+    g, o = pair(2, 2, 3, 6)
+    init_both(g, o, p, 1e10)
+    dt = 2.0 ** -6 * 0.1 / 2
+    st = g.run_linear([1.0, 1.0], dt, 12)
+    oracle_run_linear(o, [1.0, 1.0], dt, 12)
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.array_equal(a, b), first_diff(a, b)
+    assert st.dof_stages == 12 * 3 * g.num_elem * 9
+
+
+def test_refine_coarsen_bitexact():
+    p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.2, 0.05, (0.25, 0.75))])
+    g, o = adapted_pair(2, 2, 3, 7, 3, p, 1e-3)
+    x = g.get(pyadg.FIELD_UCOE)
+    # perturb the coefficients so that both tests fire on many elements
+    y = x * (1.0 + 0.5 * rng.standard_normal(x.size))
+    g.set(pyadg.FIELD_UCOE, y); o.set(pyoracle.FIELD_UCOE, y)
+    na = g.refine(1e-4); nb = o.refine(1e-4)
+    assert na == nb and same_grid(g, o)
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.array_equal(a, b), first_diff(a, b)
+    ra = g.coarsen(3e-4); rb = o.coarsen(3e-4)
+    assert ra == rb and same_grid(g, o)
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.array_equal(a, b), first_diff(a, b)
+
+
+def test_adaptive_burgers_run_bitexact():
+    p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.08, (0.3, 0.6)), (0.1, 0.1, (0.7, 0.3))])
+    g, o = adapted_pair(2, 2, 3, 6, 4, p, 1e-4)
+    prob = problem(flux_kind=0, flux_param=0.0, alpha=[1.2, 1.2], wave_speed=[1.0, 1.0], cfl=0.2, t_final=0.02,
+                   adaptive=1, eps=1e-4, eta=1e-5, max_steps=4)
+    st = g.run_hyperbolic(prob)
+    steps, dof_stages, t = oracle_run_hyperbolic(o, prob)
+    assert st.steps == steps and st.dof_stages == dof_stages
+    assert same_grid(g, o), "element sets differ after adaptive steps"
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.array_equal(a, b), first_diff(a, b)
+
+
+def test_varcoef_4d_bitexact():
+    p = ic_params(1, shift=0.0)
+    g, o = pair(4, 1, 2, 4)
+    init_both(g, o, p, 1e10)
+    prob = problem(flux_kind=2, flux_param=0.5, alpha=[1.5] * 4, wave_speed=[1.5] * 4, cfl=0.2, t_final=1.0,
+                   adaptive=0, eps=0.0, eta=0.0, max_steps=2)
+    st = g.run_hyperbolic(prob)
+    oracle_run_hyperbolic(o, prob)
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.array_equal(a, b), first_diff(a, b)
+    assert st.steps == 2
+
+
+def test_sincos_flux_tolerance():
+    p = ic_params(3, inside=7 * np.pi / 4, outside=np.pi / 4, radius=1.0, center=[0.0, 0.0],
+                  scale=[4.0, 4.0], offset=[-2.0, -2.5])
+    g, o = adapted_pair(2, 2, 3, 6, 4, p, 1e-3)
+    prob = problem(flux_kind=1, flux_param=0.25, alpha=[0.25, 0.25], wave_speed=[0.25, 0.25], cfl=0.2, t_final=1.0,
+                   adaptive=0, eps=1e-3, eta=1e-4, max_steps=3)
+    g.run_hyperbolic(prob)
+    oracle_run_hyperbolic(o, prob)
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
