@@ -308,7 +308,7 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
   return guard(h, [&](adg::Context& c) {
     c.sync();
     const auto t0 = std::chrono::steady_clock::now();
-    double t = 0.;
+    double t = p->t_start;
     int64_t steps = 0, dof_stages = 0, max_elem = c.E();
     while (t < p->t_final && (p->max_steps <= 0 || steps < p->max_steps)) {
       double sum_c_dx = 0.;
@@ -347,6 +347,27 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
       st->max_elem = max_elem;
       st->seconds = std::chrono::duration<double>(t1 - t0).count();
       st->final_time = t;
+    }
+  });
+}
+
+int adg_state_save(adg_ctx* h, int slot) { return guard(h, [&](adg::Context& c) { c.save_state(slot); }); }
+int adg_state_restore(adg_ctx* h, int slot) { return guard(h, [&](adg::Context& c) { c.restore_state(slot); }); }
+
+int adg_profile(adg_ctx* h, int enable) {
+  return guard(h, [&](adg::Context& c) {
+    c.prof_reset();
+    c.prof_on = enable != 0;
+  });
+}
+
+int adg_profile_get(adg_ctx* h, double* ms, int64_t* count, double* bytes) {
+  return guard(h, [&](adg::Context& c) {
+    c.prof_collect();
+    for (int i = 0; i < adg::Context::PROF_NCAT; ++i) {
+      if (ms) ms[i] = c.prof_ms[i];
+      if (count) count[i] = c.prof_count[i];
+      if (bytes) bytes[i] = c.prof_bytes[i];
     }
   });
 }

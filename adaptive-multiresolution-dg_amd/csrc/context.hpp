@@ -112,6 +112,26 @@ class Context {
   SweepTable table(int id);
   void sync();
 
+  // In-memory snapshots of the solution state (grid + Alpert coefficients).
+  struct Snapshot { std::vector<int32_t> slots; DevBuf<double> u; int64_t E = 0; };
+  Snapshot snaps[4];
+  void save_state(int slot);
+  void restore_state(int slot);
+
+  // Kernel timing with HIP events on this context's stream.  Categories:
+  // 0 sweep, 1 surplus transform, 2 flux, 3 RK stage, 4 other.
+  enum { PROF_SWEEP = 0, PROF_SURPLUS, PROF_FLUX, PROF_RK, PROF_OTHER, PROF_NCAT };
+  bool prof_on = false;
+  struct ProfRec { hipEvent_t a, b; int cat; double bytes; };
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  double prof_ms[PROF_NCAT] = {0}, prof_bytes[PROF_NCAT] = {0};
+  int64_t prof_count[PROF_NCAT] = {0};
+  void prof_begin(int cat, double bytes);
+  void prof_end();
+  void prof_collect();
+  void prof_reset();
+
  private:
   void remap_field(DevBuf<double>& f, int per_elem, const std::vector<int32_t>& old_of_new, int64_t old_E);
   void surplus_transform(const double* in, double* out, double* scratch_a, double* scratch_b, int ncomp);

@@ -297,10 +297,52 @@ Context::Context(int dim, int k_, int m_, int nmax_)
 }
 
 Context::~Context() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (ProfRec& r : prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (hipEvent_t e : prof_pool) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
 void Context::sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+void Context::prof_begin(int cat, double bytes) {
+  if (!prof_on) return;
+  ProfRec r{};
+  for (hipEvent_t* ev : {&r.a, &r.b}) {
+    if (prof_pool.empty()) hip_check(hipEventCreate(ev), "hipEventCreate");
+    else { *ev = prof_pool.back(); prof_pool.pop_back(); }
+  }
+  r.cat = cat;
+  r.bytes = bytes;
+  hip_check(hipEventRecord(r.a, stream), "hipEventRecord");
+  prof_pending.push_back(r);
+}
+
+void Context::prof_end() {
+  if (!prof_on) return;
+  hip_check(hipEventRecord(prof_pending.back().b, stream), "hipEventRecord");
+  if (prof_pending.size() >= 4096) prof_collect();
+}
+
+void Context::prof_collect() {
+  if (prof_pending.empty()) return;
+  sync();
+  for (ProfRec& r : prof_pending) {
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+    prof_ms[r.cat] += ms;
+    prof_bytes[r.cat] += r.bytes;
+    prof_count[r.cat] += 1;
+    prof_pool.push_back(r.a);
+    prof_pool.push_back(r.b);
+  }
+  prof_pending.clear();
+}
+
+void Context::prof_reset() {
+  prof_collect();
+  for (int i = 0; i < PROF_NCAT; ++i) { prof_ms[i] = 0; prof_bytes[i] = 0; prof_count[i] = 0; }
+}
 
 SweepTable Context::table(int id) {
   switch (id) {
@@ -412,7 +454,9 @@ void Context::sweep(const double* in, const int* sin, double* out, double* accum
   a.lu = lu;
   a.coef = coef;
   if (sin[dd] != pf + 1 || sout[dd] != pt + 1) throw std::runtime_error("sweep: size mismatch in swept dimension");
+  prof_begin(PROF_SWEEP, 8.0 * static_cast<double>(a.E) * (a.Tin + a.Tout));
   launch_sweep(a, pf + 1, pt + 1, stream);
+  prof_end();
 }
 
 void Context::partial_sums(const SweepTable* tabs, const int* flx, int pf, int pt, double coef, const double* src,
@@ -479,7 +523,10 @@ void Context::flux(int kind, const double* params) {
   if (kind == 1 && D != 2) throw std::runtime_error("sin/cos flux is two-dimensional");
   const int64_t n = E() * P;
   if (n == 0) return;
+  const int nout = (kind == 0) ? 1 : D;
+  prof_begin(PROF_FLUX, 8.0 * static_cast<double>(n) * (1 + nout + (kind == 2 ? D : 0)));
   hipLaunchKernelGGL(k_flux, dim3(nblocks(n)), dim3(kBlock), 0, stream, up.p, fp.p, vcoef.p, n, kind, s, D);
+  prof_end();
 }
 
 void Context::surplus_transform(const double* in, double* out, double* sa, double* sb, int ncomp) {
@@ -494,8 +541,11 @@ void Context::surplus_transform(const double* in, double* out, double* sa, doubl
     a.slot = g_slot[d].p; a.fbase = g_fbase[d].p; a.fmap = g_fmap[d].p;
     a.anc_slot = t_anc_slot.p; a.anc_point = t_anc_point.p; a.wt = t_wt.p; a.err = err_flag.p;
     a.E = E(); a.comp_stride = E() * P; a.P = P; a.p1 = p1; a.stride_d = stride; a.ncomp = ncomp;
-    if (E() > 0)
+    if (E() > 0) {
+      prof_begin(PROF_SURPLUS, 8.0 * static_cast<double>(E()) * P * ncomp * 2);
       hipLaunchKernelGGL(k_surplus, dim3(nblocks(E() * P * ncomp)), dim3(kBlock), 0, stream, a);
+      prof_end();
+    }
     cur = dst;
   }
 }
@@ -546,9 +596,12 @@ void Context::rk_init() {
 }
 
 void Context::rk_stage(int scheme, int stage, double dt) {
-  if (E() > 0)
+  if (E() > 0) {
+    prof_begin(PROF_RK, 8.0 * static_cast<double>(E()) * K * 4);
     hipLaunchKernelGGL(k_rk, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, tn.p, rhs.p, E() * K, scheme,
                        stage, dt);
+    prof_end();
+  }
 }
 
 void Context::copy_to_predict() {
@@ -557,6 +610,23 @@ void Context::copy_to_predict() {
 
 void Context::copy_predict_to_ucoe() {
   if (E() > 0) hipLaunchKernelGGL(k_copy, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, pred.p, E() * K);
+}
+
+void Context::save_state(int slot) {
+  if (slot < 0 || slot >= 4) throw std::runtime_error("snapshot slot must be 0..3");
+  Snapshot& sn = snaps[slot];
+  sn.slots = grid.slot;
+  sn.E = E();
+  sn.u.ensure(std::max<int64_t>(E() * K, 1));
+  if (E() > 0) hipLaunchKernelGGL(k_copy, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, sn.u.p, u.p, E() * K);
+}
+
+void Context::restore_state(int slot) {
+  if (slot < 0 || slot >= 4) throw std::runtime_error("snapshot slot must be 0..3");
+  Snapshot& sn = snaps[slot];
+  if (sn.slots.empty()) throw std::runtime_error("snapshot slot is empty");
+  if (sn.slots != grid.slot) set_grid_slots(sn.slots, false);
+  if (E() > 0) hipLaunchKernelGGL(k_copy, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, sn.u.p, E() * K);
 }
 
 void Context::check_error_flag() {

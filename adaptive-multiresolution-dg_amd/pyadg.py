@@ -32,7 +32,7 @@ class Problem(ctypes.Structure):
     _fields_ = [("flux_kind", ctypes.c_int), ("flux_param", ctypes.c_double), ("alpha", ctypes.c_double * 4),
                 ("wave_speed", ctypes.c_double * 4), ("cfl", ctypes.c_double), ("t_final", ctypes.c_double),
                 ("adaptive", ctypes.c_int), ("eps", ctypes.c_double), ("eta", ctypes.c_double),
-                ("max_steps", ctypes.c_int64)]
+                ("max_steps", ctypes.c_int64), ("t_start", ctypes.c_double)]
 
 
 class ICParams(ctypes.Structure):
@@ -84,6 +84,10 @@ def lib():
             "adg_ic_eval": (dbl, [DP, i32, vp]),
             "adg_run_linear": (i32, [vp, DP, dbl, i64, P(RunStats)]),
             "adg_run_hyperbolic": (i32, [vp, P(Problem), P(RunStats)]),
+            "adg_state_save": (i32, [vp, i32]),
+            "adg_state_restore": (i32, [vp, i32]),
+            "adg_profile": (i32, [vp, i32]),
+            "adg_profile_get": (i32, [vp, DP, P(i64), DP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -271,6 +275,23 @@ class Solution:
         st = RunStats()
         self._check(lib().adg_run_hyperbolic(self._h, ctypes.byref(problem), ctypes.byref(st)))
         return st
+
+    def save_state(self, slot=0):
+        self._check(lib().adg_state_save(self._h, slot))
+
+    def restore_state(self, slot=0):
+        self._check(lib().adg_state_restore(self._h, slot))
+
+    def profile(self, enable=True):
+        self._check(lib().adg_profile(self._h, int(enable)))
+
+    def profile_get(self):
+        ms = np.zeros(5)
+        cnt = np.zeros(5, np.int64)
+        by = np.zeros(5)
+        self._check(lib().adg_profile_get(self._h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _dp(by)))
+        names = ["sweep", "surplus", "flux", "rk", "other"]
+        return {n: {"ms": float(ms[i]), "count": int(cnt[i]), "bytes": float(by[i])} for i, n in enumerate(names)}
 
     def sync(self):
         self._check(lib().adg_sync(self._h))

@@ -1,0 +1,150 @@
+"""Problem definitions of the benchmark configurations (synthetic, seeded).
+
+Coordinates are the reference's unit cube [0,1]^dim; config 5's physical
+domain [-2,2]x[-2.5,1.5] is mapped affinely onto it, which scales both flux
+components by 1/4.  Random parameters come from splitmix64 (the project's own
+counter-based generator), so a seed fixes a problem on any machine.
+"""
+import math
+from dataclasses import dataclass, field
+
+import pyadg
+
+MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.state = seed & MASK
+
+    def next(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, a, b):
+        return a + (b - a) * ((self.next() >> 11) * 2.0 ** -53)
+
+
+@dataclass
+class Workload:
+    name: str
+    description: str
+    dim: int
+    k: int
+    m: int
+    nmax: int
+    sparse: bool
+    level_init: int
+    ic: pyadg.ICParams
+    init_eps: float
+    linear: bool = False
+    coef: list = field(default_factory=list)   # linear advection speeds
+    dt: float = 0.0                             # linear runs: fixed step
+    nsteps: int = 0
+    problem: pyadg.Problem = None
+
+    def config(self):
+        return {"workload": self.name, "description": self.description, "dim": self.dim, "k": self.k,
+                "m": self.m, "nmax": self.nmax}
+
+
+def _ic(kind, **kw):
+    p = pyadg.ICParams()
+    p.kind = kind
+    for name, val in kw.items():
+        if name == "bumps":
+            p.nbump = len(val)
+            for b, (amp, width, center) in enumerate(val):
+                p.bump_amp[b], p.bump_width[b] = amp, width
+                for d, c in enumerate(center):
+                    p.bump_center[b][d] = c
+        elif name in ("center", "scale", "offset"):
+            arr = getattr(p, name)
+            for d, c in enumerate(val):
+                arr[d] = c
+        else:
+            setattr(p, name, val)
+    return p
+
+
+def _problem(**kw):
+    p = pyadg.Problem()
+    for name, val in kw.items():
+        if name in ("alpha", "wave_speed"):
+            arr = getattr(p, name)
+            for d, c in enumerate(val):
+                arr[d] = c
+        else:
+            setattr(p, name, val)
+    return p
+
+
+def c1_linear(nmax=7, t_final=1.0):
+    """2D linear advection u_t+u_x+u_y=0, u0=sin(2 pi (x+y)), k=2, sparse grid N,
+    RK3, dt = dx*0.1/2 (example/02_hyperbolic_01_scalar_const_coefficient.cpp:148-152)."""
+    dx = 1.0 / 2.0 ** nmax
+    dt = dx * 0.1 / 2
+    n = math.ceil(t_final / dt) + 1
+    dt = t_final / n
+    return Workload("c1_linear_advection_2d_k2_sparseN%d" % nmax,
+                    "2D linear advection, sparse grid N=%d, k=2, RK3, t=%g" % (nmax, t_final),
+                    2, 2, 3, nmax, True, nmax, _ic(0, shift=0.0), 1e10, linear=True, coef=[1.0, 1.0],
+                    dt=dt, nsteps=n)
+
+
+def c2_burgers(seed=1, nmax=8, level_init=6, t_final=0.1, eps=1e-4, eta=1e-5):
+    """2D Burgers, u0 = 0.5 + 0.25 sin(2 pi (x+y)) + 4 seeded Gaussian bumps,
+    k=2, Lagrange P3 flux interpolation, adaptive, RK3, LF alpha 1.2, cfl 0.2
+    (example/02_hyperbolic_05_burgers_adapt.cpp:214-244)."""
+    g = SplitMix64(seed)
+    bumps = []
+    for _ in range(4):
+        amp = g.uniform(0.05, 0.2)
+        width = g.uniform(0.05, 0.15)
+        center = (g.uniform(0.0, 1.0), g.uniform(0.0, 1.0))
+        bumps.append((amp, width, center))
+    ic = _ic(2, base=0.5, ampl=0.25, bumps=bumps)
+    prob = _problem(flux_kind=pyadg.FLUX_BURGERS, flux_param=0.0, alpha=[1.2, 1.2], wave_speed=[1.0, 1.0], cfl=0.2,
+                    t_final=t_final, adaptive=1, eps=eps, eta=eta, max_steps=0, t_start=0.0)
+    return Workload("c2_burgers_adaptive_2d_k2_m3_nmax%d" % nmax,
+                    "2D Burgers, 4 seeded bumps (seed %d), adaptive eps=%g eta=%g, NMAX=%d, sparse init N0=%d, t=%g"
+                    % (seed, eps, eta, nmax, level_init, t_final),
+                    2, 2, 3, nmax, True, level_init, ic, eps, problem=prob)
+
+
+def c3_varcoef_4d(nmax=8, t_final=0.5):
+    """4D u_t + sum_i a_i(x) u_{x_i} = 0, a_i = 1 + 0.5 sin(2 pi x_{i+1 mod 4}),
+    u0 = prod sin(2 pi x_i), k=1, Lagrange P2, sparse N=8, RK3, LF alpha 1.5
+    (example/02_hyperbolic_02_var_coeff_coarse_grid.cpp:242-250)."""
+    ic = _ic(1, shift=0.0)
+    prob = _problem(flux_kind=pyadg.FLUX_VARCOEF, flux_param=0.5, alpha=[1.5] * 4, wave_speed=[1.5] * 4, cfl=0.2,
+                    t_final=t_final, adaptive=0, eps=0.0, eta=0.0, max_steps=0, t_start=0.0)
+    return Workload("c3_varcoef_4d_k1_m2_sparseN%d" % nmax, "4D variable-coefficient advection, sparse N=%d, k=1" % nmax,
+                    4, 1, 2, nmax, True, nmax, ic, 1e10, problem=prob)
+
+
+def c5_rotating(seed=1, nmax=9, level_init=6, t_final=1.0, eps=1e-3):
+    """2D rotating wave u_t + (sin u)_x + (cos u)_y = 0 on [-2,2]x[-2.5,1.5]
+    mapped to [0,1]^2, u0 = 7 pi/4 inside a circle of seeded radius, pi/4
+    outside, k=2, Lagrange P3, adaptive eps, eta = eps/10."""
+    g = SplitMix64(seed)
+    radius = g.uniform(0.8, 1.2)
+    ic = _ic(3, inside=7 * math.pi / 4, outside=math.pi / 4, radius=radius, center=[0.0, 0.0], scale=[4.0, 4.0],
+             offset=[-2.0, -2.5])
+    prob = _problem(flux_kind=pyadg.FLUX_SINCOS, flux_param=0.25, alpha=[0.25, 0.25], wave_speed=[0.25, 0.25],
+                    cfl=0.2, t_final=t_final, adaptive=1, eps=eps, eta=eps / 10, max_steps=0, t_start=0.0)
+    return Workload("c5_rotating_wave_2d_k2_m3_nmax%d" % nmax,
+                    "2D rotating wave (KPP), circle radius %.4f, adaptive eps=%g, NMAX=%d" % (radius, eps, nmax),
+                    2, 2, 3, nmax, True, level_init, ic, eps, problem=prob)
+
+
+WORKLOADS = {"c1": c1_linear, "c2": c2_burgers, "c3": c3_varcoef_4d, "c5": c5_rotating}
+
+
+def make_solution(w):
+    s = pyadg.Solution(w.dim, w.k, w.m, w.nmax, w.sparse, w.level_init)
+    s.init_adaptive_intp_builtin(w.ic, w.init_eps)
+    return s

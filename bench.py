@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Throughput of the adaptive sparse-grid DG time step on one MI355X.
+
+A "step" is one full solve of the workload from its initial state (restored
+from a device-resident snapshot): for the adaptive configurations every time
+step is predict (forward Euler RHS), refine, three SSP-RK3 stages, coarsen.
+The metric counts, for every RK stage, the active degrees of freedom it
+updates (DoF-stage updates); the predict stage is done but not counted.
+`value` = DoF-stage updates / wall seconds of exactly --steps timed solves,
+after --warmup untimed solves, with all data resident on the GPU.
+
+Prints one JSON line.  The roofline object describes the dominant kernel (the
+1D fiber sweep), timed with HIP events on the library's stream during the timed
+region; the cpu_baseline object times the CPU oracle (a restatement of the
+reference's algorithm with its data structures and OpenMP loop) on the first
+steps of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "adaptive-multiresolution-dg_amd"))
+
+import pyadg  # noqa: E402
+import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "DoF-stage updates/s on 1 GPU: plain FP64 vs chosen form at equal L2 error"
+
+
+def cpu_baseline(w, max_seconds, threads):
+    """Times the CPU oracle on the same workload: whole solves from t=0 (a new
+    oracle per solve, its setup untimed) until max_seconds of stepping have
+    accumulated; for the linear workload, time steps until max_seconds."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    import runs
+
+    pyoracle.set_threads(threads)
+
+    def fresh():
+        o = pyoracle.Oracle(w.dim, w.k, w.m, w.nmax, w.sparse, w.level_init)
+        o._keep.append(w.ic)
+        o.init_adaptive_intp_c(pyadg.ic_eval_ptr(), ctypes.cast(ctypes.pointer(w.ic), ctypes.c_void_p), w.init_eps)
+        return o
+
+    sec, dof_stages, steps, solves = 0.0, 0, 0, 0
+    if w.linear:
+        o = fresh()
+        while sec < max_seconds and steps < w.nsteps:
+            n = min(20, w.nsteps - steps)
+            t0 = time.perf_counter()
+            dof_stages += runs.run_linear(o, w.coef, w.dt, n)
+            sec += time.perf_counter() - t0
+            steps += n
+        sample = "CPU oracle, first %d of %d time steps of the solve" % (steps, w.nsteps)
+    else:
+        while sec < max_seconds and solves < 5:
+            o = fresh()
+            prob = pyadg.Problem.from_buffer_copy(w.problem)
+            prob.t_start, prob.max_steps = 0.0, 0
+            t0 = time.perf_counter()
+            n, ds, _ = runs.run_hyperbolic(o, prob)
+            sec += time.perf_counter() - t0
+            dof_stages += ds
+            steps += n
+            solves += 1
+            o.close()
+        sample = "CPU oracle, %d full solve(s) of the workload (%d time steps)" % (solves, steps)
+    return {"value": dof_stages / sec, "unit": "DoF-stage updates/s", "cores": threads, "kind": "port",
+            "sample": sample + ": %d DoF-stage updates in %.2f s; reference data structures and OpenMP "
+                               "transform_1D loop, g++ -O3 -fopenmp" % (dof_stages, sec)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2", choices=sorted(workloads.WORKLOADS))
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="stepping time of the CPU oracle sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-events", action="store_true", help="time without per-kernel HIP events")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    make = workloads.WORKLOADS[args.workload]
+    w = make(seed=args.seed) if args.workload in ("c2", "c5") else make()
+    print(f"[bench] workload {w.name}: init", file=sys.stderr, flush=True)
+    sol = workloads.make_solution(w)
+    print(f"[bench] init done: {sol.num_elem} elements; warmup {args.warmup} steps", file=sys.stderr, flush=True)
+
+    sol.save_state(0)
+    prob = None if w.linear else pyadg.Problem.from_buffer_copy(w.problem)
+
+    def solve():
+        """One step of the benchmark: a full solve from the saved initial state."""
+        sol.restore_state(0)
+        if w.linear:
+            return sol.run_linear(w.coef, w.dt, w.nsteps)
+        prob.t_start, prob.max_steps = 0.0, 0
+        return sol.run_hyperbolic(prob)
+
+    for _ in range(args.warmup):
+        solve()
+    if not args.no_events:
+        sol.profile(True)
+    sol.sync()
+    dof_stages, time_steps, max_elem = 0, 0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = solve()
+        dof_stages += st.dof_stages
+        time_steps += st.steps
+        max_elem = max(max_elem, st.max_elem)
+    sol.sync()
+    sec = time.perf_counter() - t0
+    prof = sol.profile_get()
+    sol.profile(False)
+    print(f"[bench] timed {args.steps} solves ({time_steps} time steps) in {sec:.3f} s", file=sys.stderr, flush=True)
+
+    value = dof_stages / sec
+    sw = prof["sweep"]
+    roofline = None
+    if sw["count"] > 0 and sw["ms"] > 0:
+        avg_ms = sw["ms"] / sw["count"]
+        avg_bytes = sw["bytes"] / sw["count"]
+        achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sweep",
+                    "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes, "launches": sw["count"],
+                    "kernel_ms_by_category": {k: round(v["ms"], 3) for k, v in prof.items()}}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        print(f"[bench] cpu baseline on {threads} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(w, args.cpu_seconds, threads)
+        one = cpu_baseline(w, args.cpu_seconds / 4, 1)
+        cpu["one_thread_value"] = one["value"]
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "DoF-stage updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": sec / max(args.steps, 1) * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": dict(w.config(), form="plain FP64", step="one full solve from the saved initial state",
+                       time_steps_per_solve=time_steps // max(args.steps, 1), max_elements=int(max_elem),
+                       dof_stage_updates=int(dof_stages), sim_time=st.final_time),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

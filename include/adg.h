@@ -185,8 +185,22 @@ typedef struct {
   int adaptive;            /* 0: fixed grid */
   double eps, eta;         /* refine / coarsen thresholds */
   int64_t max_steps;       /* 0: no limit */
+  double t_start;          /* time of the current solution (runs can be continued) */
 } adg_problem;
 int adg_run_hyperbolic(adg_ctx* c, const adg_problem* p, adg_run_stats* st);
+
+/* In-memory snapshots (slots 0..3) of the element table and the Alpert
+ * coefficients; restore replaces the grid and coefficients of the context. */
+int adg_state_save(adg_ctx* c, int slot);
+int adg_state_restore(adg_ctx* c, int slot);
+
+/* Kernel timing: HIP events around every launch on the context's stream
+ * (enable = 1 starts a fresh record).  adg_profile_get fills, per category
+ * (0 sweep, 1 surplus transform, 2 flux, 3 RK stage, 4 other), the summed
+ * kernel milliseconds, launch count and algorithmic bytes (the bytes each
+ * kernel must move: inputs read once plus outputs written once). */
+int adg_profile(adg_ctx* c, int enable);
+int adg_profile_get(adg_ctx* c, double* ms, int64_t* count, double* bytes);
 
 #ifdef __cplusplus
 }

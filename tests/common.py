@@ -39,52 +39,9 @@ def init_both(gpu, orc, params, eps):
     orc.init_adaptive_intp_c(fn, ctypes.cast(ctypes.pointer(params), ctypes.c_void_p), eps)
 
 
-def oracle_hyperbolic_rhs(o, prob):
-    o.eval_up()
-    o.flux(prob.flux_kind, (prob.flux_param,))
-    o.fp_to_coe()
-    o.rhs_zero()
-    o.rhs_lagr_vol()
-    o.rhs_lagr_flx()
-    o.rhs_penalty([prob.alpha[d] for d in range(o.dim)])
-
-
-def oracle_run_hyperbolic(o, prob):
-    """The loop of adg_run_hyperbolic (reference example/02_hyperbolic_05_burgers_adapt.cpp:223-498)."""
-    t, steps, dof_stages = 0.0, 0, 0
-    while t < prob.t_final and (prob.max_steps <= 0 or steps < prob.max_steps):
-        lm = o.max_level()
-        s = 0.0
-        for d in range(o.dim):
-            s += abs(prob.wave_speed[d]) * 2.0 ** int(lm[d])
-        dt = prob.cfl / s
-        dt = min(dt, prob.t_final - t)
-        if prob.adaptive:
-            o.copy_to_predict()
-            o.rk_init()
-            oracle_hyperbolic_rhs(o, prob)
-            o.rk_stage(0, 0, dt)
-            o.refine(prob.eps)
-            o.copy_predict_to_ucoe()
-        o.rk_init()
-        for st in range(3):
-            oracle_hyperbolic_rhs(o, prob)
-            o.rk_stage(3, st, dt)
-        dof_stages += 3 * o.num_elem * o.K
-        if prob.adaptive:
-            o.coarsen(prob.eta)
-        t += dt
-        steps += 1
-    return steps, dof_stages, t
-
-
-def oracle_run_linear(o, coef, dt, nsteps):
-    for _ in range(nsteps):
-        o.rk_init()
-        for st in range(3):
-            o.rhs_zero()
-            o.rhs_linear(coef)
-            o.rk_stage(3, st, dt)
+from runs import hyperbolic_rhs as oracle_hyperbolic_rhs  # noqa: E402,F401
+from runs import run_hyperbolic as oracle_run_hyperbolic  # noqa: E402,F401
+from runs import run_linear as oracle_run_linear  # noqa: E402,F401
 
 
 def problem(**kw):
