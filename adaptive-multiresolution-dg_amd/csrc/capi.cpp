@@ -62,6 +62,14 @@ double periodic_dist2(const double* x, const double* c, int dim) {
   return r2;
 }
 
+adg::Context::StageProblem stage_problem(const adg_problem* p, int D) {
+  adg::Context::StageProblem sp;
+  sp.flux_kind = p->flux_kind;
+  sp.flux_param = p->flux_param;
+  for (int d = 0; d < D; ++d) sp.alpha[d] = p->alpha[d];
+  return sp;
+}
+
 // one RK3 stage of the Lagrange-interpolated hyperbolic operator
 void hyperbolic_rhs(adg::Context& c, const adg_problem* p) {
   const double fp = p->flux_param;
@@ -152,6 +160,11 @@ int adg_max_level(adg_ctx* h, int32_t* out) {
   });
 }
 
+int32_t adg_hash_key(const int32_t* level, const int32_t* suppt, int dim) {
+  static const adg::HashKey hk;
+  return hk.key(level, suppt, dim);
+}
+
 int adg_field_set(adg_ctx* h, int field, const double* host) {
   return guard(h, [&](adg::Context& c) {
     if (field >= ADG_FIELD_FUCOE) c.same_flux = false;
@@ -205,6 +218,15 @@ int adg_flux(adg_ctx* h, int kind, const double* params) {
   return guard(h, [&](adg::Context& c) {
     if (kind < 0 || kind > 2) throw std::runtime_error("unknown flux kind");
     c.flux(kind, params);
+  });
+}
+int adg_flux_set(adg_ctx* h, int ncomp, const double* fp) {
+  return guard(h, [&](adg::Context& c) {
+    if (ncomp != 1 && ncomp != c.D) throw std::runtime_error("ncomp must be 1 or dim");
+    const size_t n = static_cast<size_t>(c.E()) * c.P * ncomp;
+    adg::hip_check(hipMemcpyAsync(c.fp.p, fp, n * sizeof(double), hipMemcpyHostToDevice, c.stream), "flux upload");
+    c.same_flux = (ncomp == 1);
+    c.sync();
   });
 }
 int adg_fp_to_coe(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.fp_to_coe(); }); }
@@ -319,20 +341,25 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
       }
       double dt = p->cfl / sum_c_dx;
       dt = std::min(dt, p->t_final - t);
+      const adg::Context::StageProblem sp = stage_problem(p, c.D);
+      auto stage = [&](int scheme, int s) {
+        if (p->form == 1) {
+          c.stage_fused(sp, scheme, s, dt);
+        } else {
+          hyperbolic_rhs(c, p);
+          c.rk_stage(scheme, s, dt);
+        }
+      };
       if (p->adaptive) {
         c.copy_to_predict();
         c.rk_init();
-        hyperbolic_rhs(c, p);
-        c.rk_stage(ADG_RK_EULER, 0, dt);
+        stage(ADG_RK_EULER, 0);
         c.refine(p->eps);
         c.copy_predict_to_ucoe();
         max_elem = std::max(max_elem, c.E());
       }
       c.rk_init();
-      for (int s = 0; s < 3; ++s) {
-        hyperbolic_rhs(c, p);
-        c.rk_stage(ADG_RK3, s, dt);
-      }
+      for (int s = 0; s < 3; ++s) stage(ADG_RK3, s);
       dof_stages += 3 * c.E() * c.K;
       if (p->adaptive) c.coarsen(p->eta);
       t += dt;
@@ -348,6 +375,15 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
       st->seconds = std::chrono::duration<double>(t1 - t0).count();
       st->final_time = t;
     }
+  });
+}
+
+int adg_stage(adg_ctx* h, const adg_problem* p, int scheme, int stage, double dt) {
+  return guard(h, [&](adg::Context& c) {
+    if (!p) throw std::runtime_error("null problem");
+    if (!((scheme == ADG_RK_EULER && stage == 0) || (scheme == ADG_RK3 && stage >= 0 && stage <= 2)))
+      throw std::runtime_error("invalid RK scheme/stage");
+    c.stage_fused(stage_problem(p, c.D), scheme, stage, dt);
   });
 }
 

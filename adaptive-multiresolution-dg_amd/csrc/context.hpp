@@ -112,6 +112,21 @@ class Context {
   SweepTable table(int id);
   void sync();
 
+  // Fused RK stage: 3*dim launches per stage (first sweeps of every partial
+  // sum batched per phase; last sweeps, point flux, penalty and RK update
+  // fused into per-output kernels).  Bit-identical to the per-sweep path.
+  struct StageProblem {
+    int flux_kind = 0;
+    double flux_param = 0.;
+    double alpha[kMaxDim] = {0, 0, 0, 0};
+  };
+  void stage_fused(const StageProblem& p, int scheme, int stage, double dt);
+  struct FusedPlan;
+  FusedPlan& fused_plan(const StageProblem& p);
+  std::unique_ptr<FusedPlan> plans[2];
+  DevBuf<double> u_next, fused_tmp;
+  int64_t grid_version = 0;
+
   // In-memory snapshots of the solution state (grid + Alpert coefficients).
   struct Snapshot { std::vector<int32_t> slots; DevBuf<double> u; int64_t E = 0; };
   Snapshot snaps[4];

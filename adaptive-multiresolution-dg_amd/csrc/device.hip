@@ -217,6 +217,159 @@ __global__ __launch_bounds__(kBlock) void k_remap(double* dst, const double* src
   dst[i] = (o >= 0) ? src[static_cast<int64_t>(o) * T + t] : 0.;
 }
 
+// ---------------------------------------------------------------------------
+// Fused stage phases.  A "term" is one 1D sweep evaluated at a single output
+// value; a phase evaluates many terms in one launch (thread per output).
+struct TermDev {
+  const double* in;
+  const double* M;
+  const int32_t* slot;
+  const int32_t* fbase;
+  const int32_t* fmaxl;
+  const int32_t* fmap;
+  int ld, pt1, dd, Tin, flx, lu;
+  int sin[kMaxDim];  // strides of the input tensor
+  double coef;
+};
+
+struct TaskDev {  // a term written to an output buffer
+  TermDev t;
+  double* out;
+  int Tout;
+  int osz[kMaxDim];
+  int64_t begin;  // first work item of this task in the phase
+};
+
+// Value of one sweep at output (e, idx): coef * sum over related sources, in
+// ascending slot order, of sum_k in[src][idx with dd <- k] * M[s*PF1+k][st*pt1+idx[dd]].
+// Loads of up to four sources are issued before their terms are added in order.
+template <int PF1>
+__device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const int* idx, int D) {
+  int base = 0;
+  for (int d = 0; d < D; ++d)
+    if (d != t.dd) base += idx[d] * t.sin[d];
+  const int sdd = t.sin[t.dd];
+  const int st = t.slot[e];
+  const int fb = t.fbase[e];
+  const double* Mc = t.M + st * t.pt1 + idx[t.dd];
+  double acc = 0.;
+  for_each_related_ranges(st, t.fmaxl[e], t.flx != 0, t.lu, [&](int lo, int hi) {
+    for (int s = lo; s < hi; s += 4) {
+      int src[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) src[j] = (s + j < hi) ? t.fmap[fb + s + j] : -1;
+      double xv[4][PF1], mv[4][PF1];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double* x = t.in + static_cast<int64_t>(src[j] < 0 ? 0 : src[j]) * t.Tin + base;
+        const double* Mr = Mc + static_cast<int64_t>(s + j) * PF1 * t.ld;
+#pragma unroll
+        for (int kk = 0; kk < PF1; ++kk) {
+          xv[j][kk] = (src[j] >= 0) ? x[kk * sdd] : 0.;
+          mv[j][kk] = (src[j] >= 0) ? Mr[kk * t.ld] : 0.;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (src[j] >= 0) {
+#pragma unroll
+          for (int kk = 0; kk < PF1; ++kk) acc += xv[j][kk] * mv[j][kk];
+        }
+    }
+  });
+  return acc * t.coef;
+}
+
+__device__ __forceinline__ void unflatten(int o, const int* sz, int D, int* idx) {
+  for (int d = D - 1; d >= 0; --d) {
+    idx[d] = o % sz[d];
+    o /= sz[d];
+  }
+}
+
+template <int PF1>
+__global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntask, int64_t total, int D) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  int j = 0;
+  while (j + 1 < ntask && gid >= tasks[j + 1].begin) ++j;
+  const TaskDev& tk = tasks[j];
+  const int64_t local = gid - tk.begin;
+  const int64_t e = local / tk.Tout;
+  const int o = static_cast<int>(local - e * tk.Tout);
+  int idx[kMaxDim];
+  unflatten(o, tk.osz, D, idx);
+  tk.out[local] = term_value<PF1>(tk.t, e, idx, D);
+}
+
+// Last sweep of every eval_up partial sum, accumulated in ordering order into
+// the point value, followed by the point flux (LagrInterpolation::eval_fp_Lag).
+template <int PF1>
+__global__ __launch_bounds__(kBlock) void k_up_final(const TermDev* terms, int nterm, int64_t E, int P, int p1, int D,
+                                                      double* up, double* fp, const double* vcoef, int kind, double s,
+                                                      int ncomp) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= E * P) return;
+  const int64_t e = gid / P;
+  const int q = static_cast<int>(gid - e * P);
+  int sz[kMaxDim], idx[kMaxDim];
+  for (int d = 0; d < D; ++d) sz[d] = p1;
+  unflatten(q, sz, D, idx);
+  double v = 0.;
+  for (int t = 0; t < nterm; ++t) v += term_value<PF1>(terms[t], e, idx, D);
+  up[gid] = v;
+  const int64_t n = E * P;
+  if (kind == 0) {
+    fp[gid] = v * v * 0.5;
+  } else if (kind == 1) {
+    fp[gid] = sin(v) * s;
+    fp[n + gid] = cos(v) * s;
+  } else {
+    for (int c = 0; c < ncomp; ++c) fp[c * n + gid] = vcoef[c * n + gid] * v;
+  }
+}
+
+// Last sweep of every right-hand-side partial sum and the penalty sweeps,
+// accumulated in the reference order, then the RK stage update written to
+// u_next (u itself is still read by other threads' penalty sweeps).
+template <int PFM, int PFK>
+__global__ __launch_bounds__(kBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens, int npen,
+                                                       int64_t E, int K, int k1, int D, double* rhs, const double* u,
+                                                       const double* tn, double* u_next, int scheme, int stage,
+                                                       double dt) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= E * K) return;
+  const int64_t e = gid / K;
+  const int q = static_cast<int>(gid - e * K);
+  int sz[kMaxDim], idx[kMaxDim];
+  for (int d = 0; d < D; ++d) sz[d] = k1;
+  unflatten(q, sz, D, idx);
+  double r = 0.;
+  for (int t = 0; t < nterm; ++t) r += term_value<PFM>(terms[t], e, idx, D);
+  for (int t = 0; t < npen; ++t) r += term_value<PFK>(pens[t], e, idx, D);
+  rhs[gid] = r;
+  const double tt = tn[gid], uu = u[gid];
+  double out;
+  if (scheme == 0 || stage == 0) out = tt + dt * r;
+  else if (stage == 1) out = 3. / 4. * tt + 1. / 4. * (uu + dt * r);
+  else out = 1. / 3. * tt + 2. / 3. * (uu + dt * r);
+  u_next[gid] = out;
+}
+
+template <int PF1>
+void launch_phase_t(const TaskDev* tasks, int ntask, int64_t total, int D, hipStream_t s) {
+  hipLaunchKernelGGL((k_phase<PF1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D);
+}
+
+void launch_phase(int pf1, const TaskDev* tasks, int ntask, int64_t total, int D, hipStream_t s) {
+  switch (pf1) {
+    case 2: launch_phase_t<2>(tasks, ntask, total, D, s); return;
+    case 3: launch_phase_t<3>(tasks, ntask, total, D, s); return;
+    case 4: launch_phase_t<4>(tasks, ntask, total, D, s); return;
+  }
+  throw std::runtime_error("phase: unsupported basis size");
+}
+
 template <int PF1, int PT1>
 void launch_sweep_t(const SweepArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((k_sweep<PF1, PT1>), dim3(nblocks(a.E * a.R)), dim3(kBlock), 0, s, a);
@@ -295,6 +448,22 @@ Context::Context(int dim, int k_, int m_, int nmax_)
             "upload");
   sync();
 }
+
+struct Context::FusedPlan {
+  int64_t version = -1;
+  const double *u = nullptr, *tn = nullptr, *fucoe = nullptr, *tmp = nullptr;
+  double* unext = nullptr;
+  int flux_kind = -1;
+  double alpha[kMaxDim] = {0, 0, 0, 0};
+  std::vector<int> up_phase_ntask, rhs_phase_ntask;
+  std::vector<int64_t> up_phase_total, rhs_phase_total;
+  int n_up_terms = 0, n_rhs_terms = 0, n_pen = 0;
+  // algorithmic bytes per launch (8 B x elements x (inputs + outputs))
+  std::vector<double> up_phase_bytes, rhs_phase_bytes;
+  double up_final_bytes = 0, rhs_final_bytes = 0;
+  DevBuf<TaskDev> up_tasks, rhs_tasks;
+  DevBuf<TermDev> up_terms, rhs_terms, pens;
+};
 
 Context::~Context() {
   if (stream) (void)hipStreamSynchronize(stream);
@@ -415,6 +584,7 @@ void Context::set_grid_slots(const std::vector<int32_t>& slots, bool keep_fields
     new_add.assign(ng.E, 1);
   }
   grid = std::move(ng);
+  ++grid_version;
   upload_grid();
   ensure_fields();
   if (!keep_fields) {
@@ -610,6 +780,213 @@ void Context::copy_to_predict() {
 
 void Context::copy_predict_to_ucoe() {
   if (E() > 0) hipLaunchKernelGGL(k_copy, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, pred.p, E() * K);
+}
+
+namespace {
+
+void strides_of(const int* sz, int D, int* st) {
+  st[D - 1] = 1;
+  for (int d = D - 2; d >= 0; --d) st[d] = st[d + 1] * sz[d + 1];
+}
+
+int prod_of(const int* sz, int D) {
+  int p = 1;
+  for (int d = 0; d < D; ++d) p *= sz[d];
+  return p;
+}
+
+}  // namespace
+
+Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
+  // one cached plan per buffer of the u / u_next pair (the stage swaps them)
+  int slot = -1;
+  for (int i = 0; i < 2; ++i)
+    if (plans[i] && plans[i]->u == u.p) slot = i;
+  if (slot >= 0) {
+    FusedPlan& pl = *plans[slot];
+    const bool same_alpha = std::equal(pl.alpha, pl.alpha + D, p.alpha);
+    if (pl.version == grid_version && pl.tn == tn.p && pl.fucoe == fucoe.p && pl.unext == u_next.p &&
+        pl.tmp == fused_tmp.p && pl.flux_kind == p.flux_kind && same_alpha)
+      return pl;
+  } else {
+    slot = !plans[0] ? 0 : (!plans[1] ? 1 : 0);
+  }
+  // temporaries: (D-1) steps of 2^(D-1) eval_up chains and 2*D*2^(D-1) rhs chains
+  std::vector<std::vector<int>> lus;
+  const std::vector<std::vector<int>> dims = transform_orders(D, lus);
+  const int nord = static_cast<int>(dims.size());
+  const int mx = std::max(k, m) + 1;
+  int T = 1;
+  for (int d = 0; d < D; ++d) T *= mx;
+  const int nchains = nord + 2 * D * nord;
+  const size_t per = static_cast<size_t>(std::max<int64_t>(E(), 1)) * T;
+  fused_tmp.ensure(per * nchains * std::max(D - 1, 1));
+  u_next.ensure(static_cast<size_t>(std::max<int64_t>(E(), 1)) * K);
+  if (!plans[slot]) plans[slot].reset(new FusedPlan());
+  FusedPlan& pl = *plans[slot];
+  pl.version = grid_version;
+  pl.u = u.p; pl.tn = tn.p; pl.fucoe = fucoe.p; pl.unext = u_next.p; pl.tmp = fused_tmp.p;
+  pl.flux_kind = p.flux_kind;
+  std::copy(p.alpha, p.alpha + D, pl.alpha);
+  auto tmp = [&](int chain, int step) { return fused_tmp.p + (static_cast<size_t>(step) * nchains + chain) * per; };
+  auto make_term = [&](const double* in, const int* sin_sz, int dd, SweepTable tab, int pt1, bool flx, int lu,
+                       double coef) {
+    TermDev t{};
+    t.in = in; t.M = tab.dev; t.ld = tab.ld; t.pt1 = pt1; t.dd = dd;
+    t.slot = g_slot[dd].p; t.fbase = g_fbase[dd].p; t.fmaxl = g_fmaxl[dd].p; t.fmap = g_fmap[dd].p;
+    t.Tin = prod_of(sin_sz, D);
+    strides_of(sin_sz, D, t.sin);
+    t.flx = flx ? 1 : 0; t.lu = lu; t.coef = coef;
+    return t;
+  };
+  // one chain of partial sums: returns its last term; appends first-step tasks per phase
+  auto chain = [&](int cid, const double* src, int pf, int pt, const SweepTable* tabs, const int* flx, int o,
+                   double coef, std::vector<std::vector<TaskDev>>& phase_tasks) {
+    int sz[kMaxDim];
+    for (int d = 0; d < D; ++d) sz[d] = pf + 1;
+    const double* cur = src;
+    for (int st = 0; st < D; ++st) {
+      const int dd = dims[o][st];
+      int sin_sz[kMaxDim];
+      std::copy(sz, sz + D, sin_sz);
+      sz[dd] = pt + 1;
+      TermDev t = make_term(cur, sin_sz, dd, tabs[dd], pt + 1, flx[dd] != 0, lus[o][st], st == 0 ? coef : 1.0);
+      if (st == D - 1) return t;
+      TaskDev tk{};
+      tk.t = t;
+      tk.out = tmp(cid, st);
+      tk.Tout = prod_of(sz, D);
+      std::copy(sz, sz + D, tk.osz);
+      phase_tasks[st].push_back(tk);
+      cur = tk.out;
+    }
+    return TermDev{};
+  };
+  auto finish_phases = [&](std::vector<std::vector<TaskDev>>& ph, DevBuf<TaskDev>& dev, std::vector<int>& ntask,
+                           std::vector<int64_t>& total, std::vector<double>& bytes) {
+    std::vector<TaskDev> flat;
+    ntask.clear();
+    total.clear();
+    bytes.clear();
+    for (auto& tasks : ph) {
+      int64_t begin = 0;
+      double b = 0;
+      for (TaskDev& tk : tasks) {
+        tk.begin = begin;
+        begin += E() * tk.Tout;
+        b += 8.0 * E() * (tk.t.Tin + tk.Tout);
+      }
+      bytes.push_back(b);
+      ntask.push_back(static_cast<int>(tasks.size()));
+      total.push_back(begin);
+      flat.insert(flat.end(), tasks.begin(), tasks.end());
+    }
+    upload(dev, flat, stream);
+  };
+  // eval_up chains
+  {
+    std::vector<std::vector<TaskDev>> ph(std::max(D - 1, 0));
+    std::vector<TermDev> terms;
+    SweepTable tabs[kMaxDim];
+    int flx[kMaxDim];
+    for (int d = 0; d < D; ++d) { tabs[d] = table(0); flx[d] = 0; }
+    for (int o = 0; o < nord; ++o) terms.push_back(chain(o, u.p, k, m, tabs, flx, o, 1.0, ph));
+    finish_phases(ph, pl.up_tasks, pl.up_phase_ntask, pl.up_phase_total, pl.up_phase_bytes);
+    upload(pl.up_terms, terms, stream);
+    pl.n_up_terms = static_cast<int>(terms.size());
+    const int nc = (p.flux_kind == 0) ? 1 : D;
+    double b = 8.0 * E() * P * (1 + nc + (p.flux_kind == ADG_FLUX_VARCOEF_ID ? nc : 0));
+    for (const TermDev& t : terms) b += 8.0 * E() * t.Tin;
+    pl.up_final_bytes = b;
+  }
+  // rhs chains, in the reference order: volume terms for d = 0..D-1, then flux terms
+  {
+    std::vector<std::vector<TaskDev>> ph(std::max(D - 1, 0));
+    std::vector<TermDev> terms;
+    int cid = nord;
+    for (int ft = 0; ft < 2; ++ft) {
+      for (int d = 0; d < D; ++d) {
+        SweepTable tabs[kMaxDim];
+        int flx[kMaxDim];
+        for (int dd = 0; dd < D; ++dd) {
+          if (dd == d) { tabs[dd] = ft ? table(3) : table(2); flx[dd] = ft; }
+          else { tabs[dd] = table(1); flx[dd] = 0; }
+        }
+        const double* src = fucoe.p + static_cast<int64_t>(same_flux ? 0 : d) * E() * P;
+        for (int o = 0; o < nord; ++o) terms.push_back(chain(cid++, src, m, k, tabs, flx, o, ft ? 0.5 : 1.0, ph));
+      }
+    }
+    finish_phases(ph, pl.rhs_tasks, pl.rhs_phase_ntask, pl.rhs_phase_total, pl.rhs_phase_bytes);
+    upload(pl.rhs_terms, terms, stream);
+    pl.n_rhs_terms = static_cast<int>(terms.size());
+    double b = 8.0 * E() * K * (1 + 3 + D);  // rhs out, u, tn and u_next, penalty inputs
+    for (const TermDev& t : terms) b += 8.0 * E() * t.Tin;
+    pl.rhs_final_bytes = b;
+    std::vector<TermDev> pens;
+    int ksz[kMaxDim];
+    for (int d = 0; d < D; ++d) ksz[d] = k + 1;
+    for (int d = 0; d < D; ++d) pens.push_back(make_term(u.p, ksz, d, table(8), k + 1, true, LU_FULL, -p.alpha[d] / 2.));
+    upload(pl.pens, pens, stream);
+    pl.n_pen = D;
+  }
+  return pl;
+}
+
+void Context::stage_fused(const StageProblem& p, int scheme, int stage, double dt) {
+  if (E() == 0) return;
+  if (p.flux_kind == ADG_FLUX_VARCOEF_ID && (flux_kind != p.flux_kind || flux_param != p.flux_param)) {
+    flux_kind = p.flux_kind;
+    flux_param = p.flux_param;
+    compute_vcoef();
+  }
+  flux_kind = p.flux_kind;
+  flux_param = p.flux_param;
+  same_flux = (p.flux_kind == 0);
+  if (p.flux_kind == 1 && D != 2) throw std::runtime_error("sin/cos flux is two-dimensional");
+  FusedPlan& pl = fused_plan(p);
+  const int ncomp = same_flux ? 1 : D;
+  // point values and flux
+  int64_t off = 0;
+  for (size_t s = 0; s < pl.up_phase_ntask.size(); ++s) {
+    prof_begin(PROF_SWEEP, pl.up_phase_bytes[s]);
+    launch_phase(k + 1, pl.up_tasks.p + off, pl.up_phase_ntask[s], pl.up_phase_total[s], D, stream);
+    prof_end();
+    off += pl.up_phase_ntask[s];
+  }
+  {
+    const int64_t n = E() * P;
+    prof_begin(PROF_FLUX, pl.up_final_bytes);
+    switch (k + 1) {
+      case 2: hipLaunchKernelGGL((k_up_final<2>), dim3(nblocks(n)), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
+      case 3: hipLaunchKernelGGL((k_up_final<3>), dim3(nblocks(n)), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
+      case 4: hipLaunchKernelGGL((k_up_final<4>), dim3(nblocks(n)), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
+    }
+    prof_end();
+  }
+  surplus_transform(fp.p, fucoe.p, interA.p, interB.p, ncomp);
+  off = 0;
+  for (size_t s = 0; s < pl.rhs_phase_ntask.size(); ++s) {
+    prof_begin(PROF_SWEEP, pl.rhs_phase_bytes[s]);
+    launch_phase(m + 1, pl.rhs_tasks.p + off, pl.rhs_phase_ntask[s], pl.rhs_phase_total[s], D, stream);
+    prof_end();
+    off += pl.rhs_phase_ntask[s];
+  }
+  {
+    const int64_t n = E() * K;
+    prof_begin(PROF_RK, pl.rhs_final_bytes);
+#define ADG_RF(M1, K1)                                                                                           \
+  if (m + 1 == M1 && k + 1 == K1) {                                                                              \
+    hipLaunchKernelGGL((k_rhs_final<M1, K1>), dim3(nblocks(n)), dim3(kBlock), 0, stream, pl.rhs_terms.p,        \
+                       pl.n_rhs_terms, pl.pens.p, pl.n_pen, E(), K, k + 1, D, rhs.p, u.p, tn.p, u_next.p, scheme, \
+                       stage, dt);                                                                               \
+  }
+    ADG_RF(2, 2) ADG_RF(2, 3) ADG_RF(2, 4) ADG_RF(3, 2) ADG_RF(3, 3) ADG_RF(3, 4) ADG_RF(4, 2) ADG_RF(4, 3)
+    ADG_RF(4, 4)
+#undef ADG_RF
+    prof_end();
+  }
+  std::swap(u.p, u_next.p);
+  std::swap(u.cap, u_next.cap);
 }
 
 void Context::save_state(int slot) {

@@ -37,7 +37,7 @@ ADG_HD int slot_level(int s) {
 // Calls f(source_slot) for every related source slot at levels
 // [0, max_level], ascending.  flx selects the flux ("touching") relation.
 template <class F>
-ADG_HD void for_each_related(int st, int max_level, bool flx, int lu, F&& f) {
+ADG_HD void for_each_related_slots(int st, int max_level, bool flx, int lu, F&& f) {
   const int lt = slot_level(st);
   const int ls_begin = (lu == LU_L) ? lt + 1 : 0;
   const int ls_end = (lu == LU_U) ? (lt < max_level ? lt : max_level) : max_level;
@@ -86,6 +86,33 @@ ADG_HD void for_each_related(int st, int max_level, bool flx, int lu, F&& f) {
       }
     }
   }
+}
+
+}  // namespace adg
+
+namespace adg {
+
+// The same enumeration as maximal runs of consecutive slots: f(lo, hi) is
+// called for [lo, hi) ranges in ascending order.  Device kernels use the runs
+// to issue the loads of several sources before accumulating them in order.
+template <class F>
+ADG_HD void for_each_related_ranges(int st, int max_level, bool flx, int lu, F&& f) {
+  int plo = -1, phi = -1;
+  for_each_related_slots(st, max_level, flx, lu, [&](int s) {
+    if (s == phi) { ++phi; return; }
+    if (plo >= 0) f(plo, phi);
+    plo = s;
+    phi = s + 1;
+  });
+  if (plo >= 0) f(plo, phi);
+}
+
+// Slot-by-slot form built on the runs (used by the relation tests).
+template <class F>
+ADG_HD void for_each_related(int st, int max_level, bool flx, int lu, F&& f) {
+  for_each_related_ranges(st, max_level, flx, lu, [&](int lo, int hi) {
+    for (int s = lo; s < hi; ++s) f(s);
+  });
 }
 
 }  // namespace adg

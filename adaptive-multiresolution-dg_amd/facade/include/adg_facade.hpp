@@ -1,0 +1,300 @@
+// C++ facade with the reference's class names and signatures over the C-ABI
+// of libadg_hip.so (include/adg.h).  A driver written against the reference
+// headers (example/02_hyperbolic_*.cpp) compiles against these headers and runs
+// the hot path on the GPU: the DG solution lives in device memory owned by one
+// adg_ctx per DGSolution; the other classes borrow it as the reference classes
+// borrow their DGSolution pointer.
+//
+// Reference interfaces mirrored (include/*.h of the reference):
+//   Hash.h, AllBasis.h, AlptBasis.h, LagrBasis.h, HermBasis.h, OperatorMatrix1D.h,
+//   Element.h (static configuration), DGSolution.h, DGAdapt.h, DGAdaptIntp.h,
+//   FastMultiplyLU.h (FastLagrIntp, HyperbolicLagrRHS, HyperbolicDiffFluxLagrRHS,
+//   HyperbolicAlptRHS), Interpolation.h (LagrInterpolation, FluxFunction),
+//   BilinearForm.h (BilinearForm, HyperbolicAlpt), ODESolver.h (ODESolver,
+//   ForwardEuler, RK3SSP), Timer.h, Optparser.h.
+// Only the members the hyperbolic example drivers use are provided.
+#pragma once
+#include <chrono>
+#include <cmath>
+#include <functional>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "adg.h"
+
+namespace Const {
+const double ROUND_OFF = 1e-13;
+const double PI = 3.1415926535897932384626433832795028841971693993751058209749445923078164062;
+}  // namespace Const
+
+// Prints the C-ABI error and exits, as the reference reports errors (message + exit(1)).
+void adg_facade_check(adg_ctx* c, int rc, const char* what);
+
+class AlptBasis {
+ public:
+  static int PMAX;
+};
+
+class LagrBasis {
+ public:
+  static int PMAX;
+  static int msh_case;
+  static void set_interp_msh01();
+};
+
+class HermBasis {
+ public:
+  static int PMAX;
+  static int msh_case;
+  static void set_interp_msh01() {}
+};
+
+// Static configuration of the reference's Element class (include/Element.h:14-60).
+class Element {
+ public:
+  static int PMAX_alpt, PMAX_intp, DIM, VEC_NUM;
+  static std::vector<std::vector<bool>> is_intp;
+};
+
+class Interpolation {
+ public:
+  static int DIM, VEC_NUM;
+};
+
+class Hash {
+ public:
+  Hash() {}
+  int hash_key(const std::vector<int>& l, const std::vector<int>& j);
+};
+
+template <class T>
+class AllBasis {
+ public:
+  explicit AllBasis(const int max_level) : PMAX(T::PMAX), NMAX(max_level) {}
+  int size() const { return (1 << NMAX) * (PMAX + 1); }
+  const int PMAX;
+  const int NMAX;
+};
+
+// The 1D operator tables are built inside the library context; the facade
+// object only records the basis families and the boundary type.
+template <class U, class V>
+class OperatorMatrix1D {
+ public:
+  OperatorMatrix1D(const AllBasis<U>& u, const AllBasis<V>& v, const std::string& boundary_type)
+      : row(u.size()), col(v.size()), boundary(boundary_type) {
+    if (boundary_type != "period") {
+      std::cout << "OperatorMatrix1D: only periodic boundaries are provided by the GPU library" << std::endl;
+      std::exit(1);
+    }
+  }
+  const int row, col;
+  const std::string boundary;
+};
+
+class DGSolution {
+ public:
+  DGSolution(const bool sparse, const int level_init, const int NMAX, AllBasis<AlptBasis>& all_bas,
+             AllBasis<LagrBasis>& all_bas_Lag, AllBasis<HermBasis>& all_bas_Her, Hash& hash);
+  virtual ~DGSolution();
+  DGSolution(const DGSolution&) = delete;
+  DGSolution& operator=(const DGSolution&) = delete;
+
+  static int DIM;
+  static int VEC_NUM;
+  static std::vector<int> ind_var_vec;
+
+  int max_mesh_level() const;
+  std::vector<int> max_mesh_level_vec() const;
+  int size_elem() const;
+  int size_basis_alpt() const;
+  int get_dof() const;
+  void copy_ucoe_to_predict();
+  void copy_predict_to_ucoe();
+  void set_rhs_zero();
+  std::vector<double> get_L2_norm() const;
+
+  // Device handle of this solution (the library context).
+  adg_ctx* ctx() const { return ctx_; }
+  // Alpert coefficients keyed by hash key (host copy, for measurement).
+  std::unordered_map<int, std::vector<double>> coefficients_by_key() const;
+
+  const bool sparse;
+  const int level_init;
+  const int NMAX;
+
+ protected:
+  adg_ctx* ctx_ = nullptr;
+};
+
+class DGAdapt : public DGSolution {
+ public:
+  DGAdapt(const bool sparse, const int level_init, const int NMAX, AllBasis<AlptBasis>& all_bas,
+          AllBasis<LagrBasis>& all_bas_Lag, AllBasis<HermBasis>& all_bas_Her, Hash& hash, const double eps,
+          const double eta, const bool is_find_ptr_alpt, const bool is_find_ptr_intp,
+          const bool is_find_ptr_general = false);
+  static std::vector<int> indicator_var_adapt;
+  void refine();
+  void coarsen();
+  const double eps;
+  const double eta;
+};
+
+class DGAdaptIntp : public DGAdapt {
+ public:
+  DGAdaptIntp(const bool sparse, const int level_init, const int NMAX, AllBasis<AlptBasis>& all_bas,
+              AllBasis<LagrBasis>& all_bas_Lag, AllBasis<HermBasis>& all_bas_Her, Hash& hash, const double eps,
+              const double eta, const bool is_find_ptr_alpt, const bool is_find_ptr_intp,
+              const OperatorMatrix1D<LagrBasis, LagrBasis>& matrix_Lag,
+              const OperatorMatrix1D<HermBasis, HermBasis>& matrix_Her);
+  void init_adaptive_intp(std::function<double(std::vector<double>, int)> func);
+  double get_L2_error_split_adaptive_intp_scalar(DGSolution& exact_solu) const;
+};
+
+class FastLagrIntp {
+ public:
+  FastLagrIntp(DGSolution& dgsolution, const std::vector<std::vector<double>>& Lag_pt_Alpt_1D,
+               const std::vector<std::vector<double>>& Lag_pt_Alpt_1D_d1);
+  void eval_up_Lagr();
+  DGSolution* const dgsolution_ptr;
+};
+
+namespace FluxFunction {
+double burgers_flux_scalar(const double u);
+double sin_flux(const double u);
+double cos_flux(const double u);
+}  // namespace FluxFunction
+
+class LagrInterpolation {
+ public:
+  explicit LagrInterpolation(DGSolution& dgsolution);
+  // point tables are held by the library context; these stay empty
+  std::vector<std::vector<double>> Lag_pt_Alpt_1D, Lag_pt_Alpt_1D_d1;
+
+  // Host flux closure (the reference's signature): point values come back to
+  // the host, the closure runs per point, the flux values go back to the device.
+  void nonlinear_Lagr_fast(std::function<double(std::vector<double>, int, int)> func,
+                           std::vector<std::vector<bool>> is_intp, FastLagrIntp& fastLagr);
+  // Device flux by id (ADG_FLUX_BURGERS / ADG_FLUX_SINCOS): no host round trip.
+  void nonlinear_Lagr_fast(int flux_id, double flux_param, std::vector<std::vector<bool>> is_intp,
+                           FastLagrIntp& fastLagr);
+  // Variable coefficient a_d(x) u with a_d = 1 + s sin(2 pi x_{(d+1) mod dim}).
+  void var_coeff_u_Lagr_fast(double s, std::vector<std::vector<bool>> is_intp, FastLagrIntp& fastLagr);
+  DGSolution* const dgsolution_ptr;
+};
+
+class HyperbolicLagrRHS {
+ public:
+  HyperbolicLagrRHS(DGSolution& dgsolution, const OperatorMatrix1D<LagrBasis, AlptBasis>& oper_matx_lagr);
+  void rhs_vol_scalar();
+  void rhs_flx_intp_scalar();
+  DGSolution* const dgsolution_ptr;
+};
+
+class HyperbolicDiffFluxLagrRHS : public HyperbolicLagrRHS {
+ public:
+  using HyperbolicLagrRHS::HyperbolicLagrRHS;
+};
+
+class HyperbolicAlptRHS {
+ public:
+  HyperbolicAlptRHS(DGSolution& dgsolution, const OperatorMatrix1D<AlptBasis, AlptBasis>& oper_matx_alpt);
+  void rhs_flx_penalty_scalar(const std::vector<double>& lax_alpha);
+  DGSolution* const dgsolution_ptr;
+};
+
+// Operators the reference assembles into Eigen sparse matrices are recorded
+// here and applied matrix-free by the library.
+class BilinearForm {
+ public:
+  explicit BilinearForm(DGSolution& dgsolution);
+  virtual ~BilinearForm() {}
+  // rhs += A u
+  void apply_add() const;
+  DGSolution* const dgsolution_ptr;
+
+ protected:
+  std::vector<double> lin_coef_;            // const-coefficient advection speeds (assemble_matrix_scalar)
+  std::vector<double> lft_, rgt_;           // coefficients of ulft_vjp / urgt_vjp flux terms per dimension
+  bool has_lin_ = false, has_flx_ = false;
+};
+
+class HyperbolicAlpt : public BilinearForm {
+ public:
+  HyperbolicAlpt(DGSolution& dgsolution, const OperatorMatrix1D<AlptBasis, AlptBasis>& oper_matx_alpt);
+  void assemble_matrix_scalar(const std::vector<double>& eqnCoefficient);
+  void assemble_matrix_flx_scalar(const int dim, const int sign, const double coefficient);
+};
+
+class ODESolver {
+ public:
+  explicit ODESolver(BilinearForm& bilinearform);
+  explicit ODESolver(DGSolution& dg);
+  virtual ~ODESolver() {}
+  virtual void init();
+  virtual void step_rk() = 0;
+  virtual void step_stage(const int stage) = 0;
+  virtual void final() {}
+  void set_rhs_zero() {}
+  void add_rhs_to_eigenvec() {}
+  void add_rhs_matrix(const BilinearForm& bilinearform) { bilinearform.apply_add(); }
+  BilinearForm* const dgoperator_ptr;
+  DGSolution* const dgsolution_ptr;
+};
+
+class ForwardEuler : public ODESolver {
+ public:
+  ForwardEuler(BilinearForm& b, const double dt_) : ODESolver(b), dt(dt_) {}
+  ForwardEuler(DGSolution& d, const double dt_) : ODESolver(d), dt(dt_) {}
+  void step_rk() override;
+  void step_stage(const int stage) override;
+  const int num_stage = 1;
+  const double dt;
+};
+
+class RK3SSP : public ODESolver {
+ public:
+  RK3SSP(BilinearForm& b, const double dt_) : ODESolver(b), dt(dt_) {}
+  RK3SSP(DGSolution& d, const double dt_) : ODESolver(d), dt(dt_) {}
+  void step_rk() override;
+  void step_stage(const int stage) override;
+  const int num_stage = 3;
+  const double dt;
+};
+
+class Timer {
+ public:
+  Timer() { reset(); }
+  void reset() { t0_ = std::chrono::steady_clock::now(); }
+  double elapsed() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count(); }
+  void time(const std::string& what) const { std::cout << what << ": " << elapsed() << " seconds" << std::endl; }
+
+ private:
+  std::chrono::steady_clock::time_point t0_;
+};
+
+// Minimal command-line parser with the reference's OptionsParser calls.
+class OptionsParser {
+ public:
+  OptionsParser(int argc, char* argv[]) : argc_(argc), argv_(argv) {}
+  void AddOption(int* v, const char* s, const char* l, const char* help) { add(v, s, l, help, 'i'); }
+  void AddOption(double* v, const char* s, const char* l, const char* help) { add(v, s, l, help, 'd'); }
+  void AddOption(bool* v, const char* s, const char* l, const char* help) { add(v, s, l, help, 'b'); }
+  void AddOption(std::string* v, const char* s, const char* l, const char* help) { add(v, s, l, help, 's'); }
+  void Parse();
+  bool Good() const { return good_; }
+  void PrintUsage(std::ostream& os) const;
+  void PrintOptions(std::ostream& os) const;
+
+ private:
+  struct Opt { void* v; std::string s, l, help; char t; };
+  void add(void* v, const char* s, const char* l, const char* help, char t) { opts_.push_back({v, s, l, help, t}); }
+  int argc_;
+  char** argv_;
+  bool good_ = true;
+  std::vector<Opt> opts_;
+};

@@ -32,7 +32,7 @@ class Problem(ctypes.Structure):
     _fields_ = [("flux_kind", ctypes.c_int), ("flux_param", ctypes.c_double), ("alpha", ctypes.c_double * 4),
                 ("wave_speed", ctypes.c_double * 4), ("cfl", ctypes.c_double), ("t_final", ctypes.c_double),
                 ("adaptive", ctypes.c_int), ("eps", ctypes.c_double), ("eta", ctypes.c_double),
-                ("max_steps", ctypes.c_int64), ("t_start", ctypes.c_double)]
+                ("max_steps", ctypes.c_int64), ("t_start", ctypes.c_double), ("form", ctypes.c_int)]
 
 
 class ICParams(ctypes.Structure):
@@ -62,11 +62,13 @@ def lib():
             "adg_num_elem": (i64, [vp]),
             "adg_grid_get": (i32, [vp, I32P, I32P, I32P]),
             "adg_max_level": (i32, [vp, I32P]),
+            "adg_hash_key": (ctypes.c_int32, [I32P, I32P, i32]),
             "adg_field_set": (i32, [vp, i32, DP]),
             "adg_field_get": (i32, [vp, i32, DP]),
             "adg_sweep": (i32, [vp, i32, i32, i32, i32, I32P, I32P, dbl, DP, DP]),
             "adg_eval_up": (i32, [vp]),
             "adg_flux": (i32, [vp, i32, DP]),
+            "adg_flux_set": (i32, [vp, i32, DP]),
             "adg_fp_to_coe": (i32, [vp]),
             "adg_rhs_zero": (i32, [vp]),
             "adg_rhs_lagr_vol": (i32, [vp]),
@@ -84,6 +86,7 @@ def lib():
             "adg_ic_eval": (dbl, [DP, i32, vp]),
             "adg_run_linear": (i32, [vp, DP, dbl, i64, P(RunStats)]),
             "adg_run_hyperbolic": (i32, [vp, P(Problem), P(RunStats)]),
+            "adg_stage": (i32, [vp, P(Problem), i32, i32, dbl]),
             "adg_state_save": (i32, [vp, i32]),
             "adg_state_restore": (i32, [vp, i32]),
             "adg_profile": (i32, [vp, i32]),
@@ -200,6 +203,11 @@ class Solution:
         p = np.asarray(params, np.float64)
         self._check(lib().adg_flux(self._h, kind, _dp(p)))
 
+    def flux_set(self, values, ncomp):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.size == self.num_elem * self.P * ncomp
+        self._check(lib().adg_flux_set(self._h, ncomp, _dp(v)))
+
     def fp_to_coe(self):
         self._check(lib().adg_fp_to_coe(self._h))
 
@@ -275,6 +283,9 @@ class Solution:
         st = RunStats()
         self._check(lib().adg_run_hyperbolic(self._h, ctypes.byref(problem), ctypes.byref(st)))
         return st
+
+    def stage(self, problem, scheme, stage, dt):
+        self._check(lib().adg_stage(self._h, ctypes.byref(problem), scheme, stage, dt))
 
     def save_state(self, slot=0):
         self._check(lib().adg_state_save(self._h, slot))

@@ -91,6 +91,8 @@ int64_t adg_num_elem(adg_ctx* c);
 int adg_grid_get(adg_ctx* c, int32_t* level, int32_t* suppt, int32_t* hash);
 /* DGSolution::max_mesh_level_vec (source/DGSolution.cpp:948-959). */
 int adg_max_level(adg_ctx* c, int32_t* out);
+/* Hash::hash_key (source/Hash.cpp:55-114) of one element; host only. */
+int32_t adg_hash_key(const int32_t* level, const int32_t* suppt, int dim);
 
 int adg_field_set(adg_ctx* c, int field, const double* host);
 int adg_field_get(adg_ctx* c, int field, double* host);
@@ -105,6 +107,10 @@ int adg_sweep(adg_ctx* c, int table, int flx, int lu, int dim, const int32_t* si
 int adg_eval_up(adg_ctx* c);
 /* LagrInterpolation::eval_fp_Lag / eval_coe_u_Lag (source/Interplation.cpp:256-295,648-696). */
 int adg_flux(adg_ctx* c, int kind, const double* params);
+/* Point flux values computed by the caller (host closures of the reference's
+ * nonlinear_Lagr_fast): ncomp = 1 (one flux used for every dimension) or dim
+ * components of E x (m+1)^dim values each, in hash order. */
+int adg_flux_set(adg_ctx* c, int ncomp, const double* fp);
 /* LagrInterpolation::eval_fp_to_coe_D_Lag (source/Interplation.cpp:1222-1430). */
 int adg_fp_to_coe(adg_ctx* c);
 /* DGSolution::set_rhs_zero (source/DGSolution.cpp:1067-1076). */
@@ -186,8 +192,16 @@ typedef struct {
   double eps, eta;         /* refine / coarsen thresholds */
   int64_t max_steps;       /* 0: no limit */
   double t_start;          /* time of the current solution (runs can be continued) */
+  int form;                /* 0: one kernel per 1D sweep; 1: fused stages (3*dim launches per RK stage) */
 } adg_problem;
 int adg_run_hyperbolic(adg_ctx* c, const adg_problem* p, adg_run_stats* st);
+
+/* One fused RK stage of the same operator: point values, flux, surplus
+ * transform, right-hand side (volume, flux and LF penalty terms) and the stage
+ * update in 3*dim launches; bit-identical to adg_eval_up + adg_flux +
+ * adg_fp_to_coe + adg_rhs_zero + adg_rhs_lagr_vol + adg_rhs_lagr_flx +
+ * adg_rhs_penalty + adg_rk_stage.  Uses p->flux_kind, p->flux_param, p->alpha. */
+int adg_stage(adg_ctx* c, const adg_problem* p, int scheme, int stage, double dt);
 
 /* In-memory snapshots (slots 0..3) of the element table and the Alpert
  * coefficients; restore replaces the grid and coefficients of the context. */

@@ -174,3 +174,44 @@ def test_sincos_flux_tolerance():
     oracle_run_hyperbolic(o, prob)
     a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
     assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("dim,k,m,nmax,level_init,kind", [(2, 2, 3, 6, 4, 0), (2, 2, 3, 6, 4, 1), (3, 1, 2, 4, 3, 2),
+                                                          (4, 1, 2, 4, 4, 2)])
+def test_fused_stage_matches_per_sweep_path(dim, k, m, nmax, level_init, kind):
+    # the fused stage must reproduce the per-sweep kernels bit for bit (same sums, same order)
+    if kind == 0:
+        p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.08, (0.3, 0.6))])
+        prob = problem(flux_kind=0, alpha=[1.2] * dim, wave_speed=[1.0] * dim, cfl=0.2, t_final=1.0)
+    elif kind == 1:
+        p = ic_params(3, inside=7 * np.pi / 4, outside=np.pi / 4, radius=1.0, center=[0.0, 0.0], scale=[4.0, 4.0],
+                      offset=[-2.0, -2.5])
+        prob = problem(flux_kind=1, flux_param=0.25, alpha=[0.25] * dim, wave_speed=[0.25] * dim, cfl=0.2, t_final=1.0)
+    else:
+        p = ic_params(1, shift=0.1)
+        prob = problem(flux_kind=2, flux_param=0.5, alpha=[1.5] * dim, wave_speed=[1.5] * dim, cfl=0.2, t_final=1.0)
+    a, _ = adapted_pair(dim, k, m, nmax, level_init, p, 1e-4)
+    b = pyadg.Solution(dim, k, m, nmax, True, level_init)
+    b.init_adaptive_intp_builtin(p, 1e-4)
+    assert same_grid(a, b)
+    a.rk_init(); b.rk_init()
+    for st in range(3):
+        a.eval_up(); a.flux(prob.flux_kind, (prob.flux_param,)); a.fp_to_coe(); a.rhs_zero(); a.rhs_lagr_vol()
+        a.rhs_lagr_flx(); a.rhs_penalty([prob.alpha[d] for d in range(dim)]); a.rk_stage(pyadg.RK3, st, 1e-3)
+        b.stage(prob, pyadg.RK3, st, 1e-3)
+        for f in (pyadg.FIELD_UP, pyadg.FIELD_RHS, pyadg.FIELD_UCOE):
+            x, y = a.get(f), b.get(f)
+            assert np.array_equal(x, y), f"stage {st} field {f}: " + first_diff(x, y)
+
+
+def test_fused_adaptive_burgers_run_bitexact():
+    p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.08, (0.3, 0.6)), (0.1, 0.1, (0.7, 0.3))])
+    g, o = adapted_pair(2, 2, 3, 6, 4, p, 1e-4)
+    prob = problem(flux_kind=0, flux_param=0.0, alpha=[1.2, 1.2], wave_speed=[1.0, 1.0], cfl=0.2, t_final=0.03,
+                   adaptive=1, eps=1e-4, eta=1e-5, max_steps=6, form=1)
+    st = g.run_hyperbolic(prob)
+    steps, dof_stages, t = oracle_run_hyperbolic(o, prob)
+    assert st.steps == steps and st.dof_stages == dof_stages
+    assert same_grid(g, o), "element sets differ after adaptive steps"
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.array_equal(a, b), first_diff(a, b)
