@@ -120,6 +120,8 @@ int adg_grid_init(adg_ctx* h, int sparse, int level_init) {
   return guard(h, [&](adg::Context& c) {
     if (level_init < 0 || level_init > c.nmax) throw std::runtime_error("level_init must be in [0, nmax]");
     c.set_grid_slots(adg::HostGrid::initial_slots(c.D, level_init, sparse != 0), false);
+    c.inst_eps.clear();
+    c.inst_eta.clear();
     c.sync();
   });
 }
@@ -134,6 +136,8 @@ int adg_grid_set(adg_ctx* h, const int32_t* level, const int32_t* suppt, int64_t
       s[i] = adg::slot_of(level[i], suppt[i]);
     }
     c.set_grid_slots(s, false);
+    c.inst_eps.clear();
+    c.inst_eta.clear();
     c.sync();
   });
 }
@@ -144,10 +148,10 @@ int adg_grid_get(adg_ctx* h, int32_t* level, int32_t* suppt, int32_t* hash) {
   return guard(h, [&](adg::Context& c) {
     for (int64_t e = 0; e < c.E(); ++e) {
       for (int d = 0; d < c.D; ++d) {
-        if (level) level[e * c.D + d] = c.grid.level(e, d);
-        if (suppt) suppt[e * c.D + d] = c.grid.suppt(e, d);
+        if (level) level[e * c.D + d] = c.level(e, d);
+        if (suppt) suppt[e * c.D + d] = adg::suppt_of_slot(c.uslot[e * c.D + d]);
       }
-      if (hash) hash[e] = c.grid.hash[e];
+      if (hash) hash[e] = c.uhash[e];
     }
   });
 }
@@ -156,7 +160,7 @@ int adg_max_level(adg_ctx* h, int32_t* out) {
   return guard(h, [&](adg::Context& c) {
     for (int d = 0; d < c.D; ++d) out[d] = 0;
     for (int64_t e = 0; e < c.E(); ++e)
-      for (int d = 0; d < c.D; ++d) out[d] = std::max<int32_t>(out[d], c.grid.level(e, d));
+      for (int d = 0; d < c.D; ++d) out[d] = std::max<int32_t>(out[d], c.level(e, d));
   });
 }
 
@@ -330,41 +334,115 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
   return guard(h, [&](adg::Context& c) {
     c.sync();
     const auto t0 = std::chrono::steady_clock::now();
-    double t = p->t_start;
+    const int nb = c.B();
+    const std::vector<double> eps =
+        static_cast<int>(c.inst_eps.size()) == nb ? c.inst_eps : std::vector<double>(nb, p->eps);
+    const std::vector<double> eta =
+        static_cast<int>(c.inst_eta.size()) == nb ? c.inst_eta : std::vector<double>(nb, p->eta);
+    std::vector<double> t(nb, p->t_start);
+    std::vector<char> active(nb, 1), stashed(nb, 0);
+    std::vector<std::vector<int32_t>> fin_slots(nb);
+    std::vector<std::vector<double>> fin_u(nb);
     int64_t steps = 0, dof_stages = 0, max_elem = c.E();
-    while (t < p->t_final && (p->max_steps <= 0 || steps < p->max_steps)) {
-      double sum_c_dx = 0.;
-      for (int d = 0; d < c.D; ++d) {
-        int lmax = 0;
-        for (int64_t e = 0; e < c.E(); ++e) lmax = std::max(lmax, c.grid.level(e, d));
-        sum_c_dx += std::abs(p->wave_speed[d]) * std::pow(2., lmax);
+    const adg::Context::StageProblem sp = stage_problem(p, c.D);
+    auto any_active = [&]() {
+      for (int i = 0; i < nb; ++i)
+        if (active[i] && t[i] < p->t_final) return true;
+      return false;
+    };
+    for (int i = 0; i < nb; ++i) active[i] = t[i] < p->t_final;
+    while (any_active() && (p->max_steps <= 0 || steps < p->max_steps)) {
+      // dt = cfl / sum_d |c_d| 2^{L_d} per instance (example/02_hyperbolic_05_burgers_adapt.cpp:234-244)
+      c.inst_dt.assign(nb, 0.);
+      double dt1 = 0.;
+      for (int i = 0; i < nb; ++i) {
+        if (!active[i]) continue;
+        double sum_c_dx = 0.;
+        for (int d = 0; d < c.D; ++d) {
+          int lmax = 0;
+          for (int64_t e = c.off[i]; e < c.off[i + 1]; ++e) lmax = std::max(lmax, c.level(e, d));
+          sum_c_dx += std::abs(p->wave_speed[d]) * std::pow(2., lmax);
+        }
+        double dt = p->cfl / sum_c_dx;
+        dt = std::min(dt, p->t_final - t[i]);
+        c.inst_dt[i] = dt;
+        dt1 = dt;
       }
-      double dt = p->cfl / sum_c_dx;
-      dt = std::min(dt, p->t_final - t);
-      const adg::Context::StageProblem sp = stage_problem(p, c.D);
+      if (nb > 1) c.upload_inst_dt();
+      else c.use_inst_dt = false;
       auto stage = [&](int scheme, int s) {
         if (p->form == 1) {
-          c.stage_fused(sp, scheme, s, dt);
+          c.stage_fused(sp, scheme, s, dt1);
         } else {
           hyperbolic_rhs(c, p);
-          c.rk_stage(scheme, s, dt);
+          c.rk_stage(scheme, s, dt1);
         }
       };
       if (p->adaptive) {
         c.copy_to_predict();
         c.rk_init();
         stage(ADG_RK_EULER, 0);
-        c.refine(p->eps);
+        c.refine_instances(eps);
         c.copy_predict_to_ucoe();
         max_elem = std::max(max_elem, c.E());
       }
       c.rk_init();
       for (int s = 0; s < 3; ++s) stage(ADG_RK3, s);
       dof_stages += 3 * c.E() * c.K;
-      if (p->adaptive) c.coarsen(p->eta);
-      t += dt;
+      if (p->adaptive) c.coarsen_instances(eta);
       ++steps;
+      bool finished = false;
+      for (int i = 0; i < nb; ++i) {
+        if (!active[i]) continue;
+        t[i] += c.inst_dt[i];
+        if (t[i] >= p->t_final) { active[i] = 0; finished = true; }
+      }
+      // instances that reached t_final leave the batch while others continue
+      if (finished && any_active()) {
+        std::vector<double> uall(static_cast<size_t>(c.E()) * c.K);
+        if (c.E() > 0)
+          adg::hip_check(hipMemcpyAsync(uall.data(), c.u.p, uall.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                        c.stream), "download");
+        c.sync();
+        std::vector<std::vector<int32_t>> ns(nb);
+        for (int i = 0; i < nb; ++i) {
+          if (!active[i] && !stashed[i]) {
+            fin_slots[i] = c.grids[i].slot;
+            fin_u[i].assign(uall.begin() + c.off[i] * c.K, uall.begin() + c.off[i + 1] * c.K);
+            stashed[i] = 1;
+          } else if (active[i]) {
+            ns[i] = c.grids[i].slot;
+          }
+        }
+        c.set_grids(ns, true);
+      }
     }
+    // put every instance back, at its final state
+    bool any_stashed = false;
+    for (int i = 0; i < nb; ++i) any_stashed = any_stashed || stashed[i];
+    if (any_stashed) {
+      std::vector<double> uall(static_cast<size_t>(c.E()) * c.K);
+      if (c.E() > 0)
+        adg::hip_check(hipMemcpyAsync(uall.data(), c.u.p, uall.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                      c.stream), "download");
+      c.sync();
+      std::vector<std::vector<int32_t>> ns(nb);
+      std::vector<double> unew;
+      for (int i = 0; i < nb; ++i) {
+        if (stashed[i]) {
+          ns[i] = fin_slots[i];
+          unew.insert(unew.end(), fin_u[i].begin(), fin_u[i].end());
+        } else {
+          ns[i] = c.grids[i].slot;
+          unew.insert(unew.end(), uall.begin() + c.off[i] * c.K, uall.begin() + c.off[i + 1] * c.K);
+        }
+      }
+      c.set_grids(ns, false);
+      if (!unew.empty())
+        adg::hip_check(hipMemcpyAsync(c.u.p, unew.data(), unew.size() * sizeof(double), hipMemcpyHostToDevice,
+                                      c.stream), "upload");
+    }
+    c.use_inst_dt = false;
     c.sync();
     c.check_error_flag();
     const auto t1 = std::chrono::steady_clock::now();
@@ -373,8 +451,47 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
       st->dof_stages = dof_stages;
       st->max_elem = max_elem;
       st->seconds = std::chrono::duration<double>(t1 - t0).count();
-      st->final_time = t;
+      double tmin = p->t_final;
+      for (double v : t) tmin = std::min(tmin, v);
+      st->final_time = tmin;
     }
+  });
+}
+
+int adg_grid_init_batch(adg_ctx* h, int ninst, int sparse, int level_init) {
+  return guard(h, [&](adg::Context& c) {
+    if (ninst < 1) throw std::runtime_error("ninst must be >= 1");
+    if (level_init < 0 || level_init > c.nmax) throw std::runtime_error("level_init must be in [0, nmax]");
+    const std::vector<int32_t> s = adg::HostGrid::initial_slots(c.D, level_init, sparse != 0);
+    c.set_grids(std::vector<std::vector<int32_t>>(ninst, s), false);
+    c.inst_eps.clear();
+    c.inst_eta.clear();
+    c.sync();
+  });
+}
+
+int adg_init_adaptive_intp_batch(adg_ctx* h, adg_point_fn f, void* const* users, double eps) {
+  return guard(h, [&](adg::Context& c) {
+    if (!f || !users) throw std::runtime_error("null point function or user array");
+    c.init_adaptive_intp_batch(f, users, eps);
+  });
+}
+
+int adg_num_instances(adg_ctx* h) { return (h && h->c) ? h->c->B() : -1; }
+
+int adg_instance_range(adg_ctx* h, int inst, int64_t* first, int64_t* count) {
+  return guard(h, [&](adg::Context& c) {
+    if (inst < 0 || inst >= c.B()) throw std::runtime_error("instance out of range");
+    if (first) *first = c.off[inst];
+    if (count) *count = c.off[inst + 1] - c.off[inst];
+  });
+}
+
+int adg_set_instance_thresholds(adg_ctx* h, const double* eps, const double* eta) {
+  return guard(h, [&](adg::Context& c) {
+    if (!eps || !eta) throw std::runtime_error("null threshold array");
+    c.inst_eps.assign(eps, eps + c.B());
+    c.inst_eta.assign(eta, eta + c.B());
   });
 }
 

@@ -42,9 +42,13 @@ struct DevBuf {
   }
 };
 
+// A 1D operator table on the device, stored target-major in cell blocks:
+// blk[((st * ns + s) * pf1 + k) * pt1 + i] = M(row s*pf1 + k, column st*pt1 + i)
+// of the dense level-ordered table (row = source basis, column = target basis),
+// so the sources of one target cell are read from consecutive blocks.
 struct SweepTable {
   const double* dev = nullptr;
-  int ld = 0;  // columns
+  int ns = 0;  // 1D cells (2^NMAX)
 };
 
 class Context {
@@ -58,7 +62,21 @@ class Context {
   // host state
   Basis1D alpt, lagr;
   HashKey hk;
-  HostGrid grid;
+  // One element table per independent solve ("instance"); the device holds
+  // their union, instance after instance.  Fibers never cross instances, so
+  // every kernel serves all instances in one launch.
+  std::vector<HostGrid> grids;
+  std::vector<int64_t> off{0};          // union offset of each instance, size B+1
+  std::vector<int32_t> uslot, uinst;    // union: slots [E*D], instance [E]
+  std::vector<int32_t> uhash;           // union: hash keys [E]
+  int B() const { return static_cast<int>(grids.size()); }
+  int level(int64_t e, int d) const { return level_of_slot(uslot[e * D + d]); }
+  // per-instance adaptivity thresholds set by the caller (empty: the run's
+  // scalar eps / eta apply to every instance) and per-instance step sizes
+  std::vector<double> inst_eps, inst_eta, inst_dt;
+  DevBuf<double> d_inst_dt;
+  DevBuf<int32_t> g_inst;
+  bool use_inst_dt = false;
   std::vector<double> up_host;    // point values of the adaptive interpolation
   std::vector<char> new_add;      // Element::new_add of the adaptive interpolation
   std::vector<double> lin_coef;   // coefficients of the uploaded linear tables
@@ -81,9 +99,11 @@ class Context {
   DevBuf<int32_t> err_flag;
   bool same_flux = false;  // all flux components equal (Burgers): one surplus transform
 
-  int64_t E() const { return grid.E; }
+  int64_t E() const { return off.back(); }
 
   void set_grid_slots(const std::vector<int32_t>& slots, bool keep_fields);
+  // Replace all instance grids; keep_fields remaps u and the predict copy.
+  void set_grids(const std::vector<std::vector<int32_t>>& slots, bool keep_fields);
   void upload_grid();
   void ensure_fields();
 
@@ -106,7 +126,12 @@ class Context {
   void indicator_sumsq(double* host_out);
   int64_t refine(double eps);
   int64_t coarsen(double eta);
+  // one threshold per instance
+  int64_t refine_instances(const std::vector<double>& eps);
+  int64_t coarsen_instances(const std::vector<double>& eta);
   void init_adaptive_intp(double (*f)(const double*, int, void*), void* user, double eps);
+  void init_adaptive_intp_batch(double (*f)(const double*, int, void*), void* const* users, double eps);
+  void upload_inst_dt();
   void check_error_flag();
 
   SweepTable table(int id);
@@ -128,7 +153,7 @@ class Context {
   int64_t grid_version = 0;
 
   // In-memory snapshots of the solution state (grid + Alpert coefficients).
-  struct Snapshot { std::vector<int32_t> slots; DevBuf<double> u; int64_t E = 0; };
+  struct Snapshot { std::vector<std::vector<int32_t>> slots; DevBuf<double> u; int64_t E = 0; };
   Snapshot snaps[4];
   void save_state(int slot);
   void restore_state(int slot);

@@ -10,7 +10,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <atomic>
+#include <functional>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 #include "context.hpp"
 #include "fiber.hpp"
@@ -19,6 +23,30 @@ namespace adg {
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Runs fn(i) for i in [0, n) on up to 16 host threads (instances are independent).
+void parallel_for(int n, const std::function<void(int)>& fn) {
+  const int nt = std::min(n, std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))));
+  if (n <= 8 || nt <= 1) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  std::vector<std::thread> th;
+  std::exception_ptr err;
+  std::mutex mu;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&]() {
+      try {
+        for (int i = next++; i < n; i = next++) fn(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> g(mu);
+        err = std::current_exception();
+      }
+    });
+  for (auto& x : th) x.join();
+  if (err) std::rethrow_exception(err);
 }
 
 namespace {
@@ -40,7 +68,7 @@ struct SweepArgs {
   const int32_t* fmaxl;
   const int32_t* fmap;
   int64_t E;
-  int R, Tin, Tout, sdd_in, sdd_out, ld;
+  int R, Tin, Tout, sdd_in, sdd_out, ns;
   int nrest;
   int rsz[3], rsin[3], rsout[3];
   int flx, lu;
@@ -62,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
   }
   const int st = a.slot[e];
   const int fb = a.fbase[e];
-  const double* Mcol = a.M + st * PT1;
+  const double* Mst = a.M + static_cast<int64_t>(st) * a.ns * PF1 * PT1;
   double acc[PT1];
 #pragma unroll
   for (int i = 0; i < PT1; ++i) acc[i] = 0.;
@@ -70,12 +98,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
     const int src = a.fmap[fb + s];
     if (src < 0) return;
     const double* x = a.in + static_cast<int64_t>(src) * a.Tin + bin;
-    const double* Mr = Mcol + static_cast<int64_t>(s) * PF1 * a.ld;
+    const double* Mr = Mst + s * PF1 * PT1;
 #pragma unroll
     for (int kk = 0; kk < PF1; ++kk) {
       const double xv = x[kk * a.sdd_in];
 #pragma unroll
-      for (int i = 0; i < PT1; ++i) acc[i] += xv * Mr[kk * a.ld + i];
+      for (int i = 0; i < PT1; ++i) acc[i] += xv * Mr[kk * PT1 + i];
     }
   });
   const int64_t ob = e * a.Tout + bout;
@@ -144,9 +172,10 @@ __global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
 
 // SSP-RK3 / forward Euler stage (source/ODESolver.cpp:209-213,285-301).
 __global__ __launch_bounds__(kBlock) void k_rk(double* u, const double* tn, const double* r, int64_t n, int scheme,
-                                                int stage, double dt) {
+                                                int stage, double dt, const double* dtv, const int32_t* inst, int K) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (dtv) dt = dtv[inst[i / K]];
   const double t = tn[i], uu = u[i], rr = r[i];
   double out;
   if (scheme == 0 || stage == 0) out = t + dt * rr;
@@ -227,7 +256,7 @@ struct TermDev {
   const int32_t* fbase;
   const int32_t* fmaxl;
   const int32_t* fmap;
-  int ld, pt1, dd, Tin, flx, lu;
+  int ns, pt1, dd, Tin, flx, lu;
   int sin[kMaxDim];  // strides of the input tensor
   double coef;
 };
@@ -251,7 +280,7 @@ __device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const 
   const int sdd = t.sin[t.dd];
   const int st = t.slot[e];
   const int fb = t.fbase[e];
-  const double* Mc = t.M + st * t.pt1 + idx[t.dd];
+  const double* Mc = t.M + static_cast<int64_t>(st) * t.ns * PF1 * t.pt1 + idx[t.dd];
   double acc = 0.;
   for_each_related_ranges(st, t.fmaxl[e], t.flx != 0, t.lu, [&](int lo, int hi) {
     for (int s = lo; s < hi; s += 4) {
@@ -262,11 +291,11 @@ __device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const 
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const double* x = t.in + static_cast<int64_t>(src[j] < 0 ? 0 : src[j]) * t.Tin + base;
-        const double* Mr = Mc + static_cast<int64_t>(s + j) * PF1 * t.ld;
+        const double* Mr = Mc + (s + j) * PF1 * t.pt1;
 #pragma unroll
         for (int kk = 0; kk < PF1; ++kk) {
           xv[j][kk] = (src[j] >= 0) ? x[kk * sdd] : 0.;
-          mv[j][kk] = (src[j] >= 0) ? Mr[kk * t.ld] : 0.;
+          mv[j][kk] = (src[j] >= 0) ? Mr[kk * t.pt1] : 0.;
         }
       }
 #pragma unroll
@@ -304,56 +333,94 @@ __global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntas
 
 // Last sweep of every eval_up partial sum, accumulated in ordering order into
 // the point value, followed by the point flux (LagrInterpolation::eval_fp_Lag).
+// Threads are (point, term) pairs: every term is computed by its own thread
+// and the point's thread 0 adds the terms in order, so the dependent-load
+// chains of the terms overlap while the sum keeps the reference order.
 template <int PF1>
-__global__ __launch_bounds__(kBlock) void k_up_final(const TermDev* terms, int nterm, int64_t E, int P, int p1, int D,
-                                                      double* up, double* fp, const double* vcoef, int kind, double s,
-                                                      int ncomp) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= E * P) return;
-  const int64_t e = gid / P;
-  const int q = static_cast<int>(gid - e * P);
-  int sz[kMaxDim], idx[kMaxDim];
-  for (int d = 0; d < D; ++d) sz[d] = p1;
-  unflatten(q, sz, D, idx);
+__global__ __launch_bounds__(kBlock) void k_up_final(const TermDev* terms, int nterm, int ntp, int64_t E, int P, int p1,
+                                                      int D, double* up, double* fp, const double* vcoef, int kind,
+                                                      double s, int ncomp) {
+  __shared__ double vals[kBlock];
+  const int per_block = kBlock / ntp;
+  const int lo = threadIdx.x / ntp, t = threadIdx.x - lo * ntp;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * per_block + lo;
+  const bool live = gid < E * P && lo < per_block;
+  int64_t e = 0;
+  int q = 0;
+  if (live) {
+    e = gid / P;
+    q = static_cast<int>(gid - e * P);
+  }
   double v = 0.;
-  for (int t = 0; t < nterm; ++t) v += term_value<PF1>(terms[t], e, idx, D);
-  up[gid] = v;
+  if (live && t < nterm) {
+    int sz[kMaxDim], idx[kMaxDim];
+    for (int d = 0; d < D; ++d) sz[d] = p1;
+    unflatten(q, sz, D, idx);
+    v = term_value<PF1>(terms[t], e, idx, D);
+  }
+  vals[threadIdx.x] = v;
+  __syncthreads();
+  if (!live || t != 0) return;
+  double u = 0.;
+  for (int j = 0; j < nterm; ++j) u += vals[lo * ntp + j];
+  up[gid] = u;
   const int64_t n = E * P;
   if (kind == 0) {
-    fp[gid] = v * v * 0.5;
+    fp[gid] = u * u * 0.5;
   } else if (kind == 1) {
-    fp[gid] = sin(v) * s;
-    fp[n + gid] = cos(v) * s;
+    fp[gid] = sin(u) * s;
+    fp[n + gid] = cos(u) * s;
   } else {
-    for (int c = 0; c < ncomp; ++c) fp[c * n + gid] = vcoef[c * n + gid] * v;
+    for (int c = 0; c < ncomp; ++c) fp[c * n + gid] = vcoef[c * n + gid] * u;
   }
 }
 
 // Last sweep of every right-hand-side partial sum and the penalty sweeps,
 // accumulated in the reference order, then the RK stage update written to
-// u_next (u itself is still read by other threads' penalty sweeps).
+// u_next (u itself is still read by other threads' penalty sweeps).  Threads
+// are (DoF, term) pairs as in k_up_final.
 template <int PFM, int PFK>
 __global__ __launch_bounds__(kBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens, int npen,
-                                                       int64_t E, int K, int k1, int D, double* rhs, const double* u,
-                                                       const double* tn, double* u_next, int scheme, int stage,
-                                                       double dt) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= E * K) return;
-  const int64_t e = gid / K;
-  const int q = static_cast<int>(gid - e * K);
-  int sz[kMaxDim], idx[kMaxDim];
-  for (int d = 0; d < D; ++d) sz[d] = k1;
-  unflatten(q, sz, D, idx);
+                                                       int ntp, int64_t E, int K, int k1, int D, double* rhs,
+                                                       const double* u, const double* tn, double* u_next, int scheme,
+                                                       int stage, double dt, const double* dtv, const int32_t* inst) {
+  __shared__ double vals[kBlock];
+  const int per_block = kBlock / ntp;
+  const int lo = threadIdx.x / ntp, t = threadIdx.x - lo * ntp;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * per_block + lo;
+  const bool live = gid < E * K && lo < per_block;
+  int64_t e = 0;
+  int q = 0;
+  if (live) {
+    e = gid / K;
+    q = static_cast<int>(gid - e * K);
+  }
+  double v = 0.;
+  if (live && t < nterm + npen) {
+    int sz[kMaxDim], idx[kMaxDim];
+    for (int d = 0; d < D; ++d) sz[d] = k1;
+    unflatten(q, sz, D, idx);
+    v = (t < nterm) ? term_value<PFM>(terms[t], e, idx, D) : term_value<PFK>(pens[t - nterm], e, idx, D);
+  }
+  vals[threadIdx.x] = v;
+  __syncthreads();
+  if (!live || t != 0) return;
   double r = 0.;
-  for (int t = 0; t < nterm; ++t) r += term_value<PFM>(terms[t], e, idx, D);
-  for (int t = 0; t < npen; ++t) r += term_value<PFK>(pens[t], e, idx, D);
+  for (int j = 0; j < nterm + npen; ++j) r += vals[lo * ntp + j];
   rhs[gid] = r;
+  if (dtv) dt = dtv[inst[e]];
   const double tt = tn[gid], uu = u[gid];
   double out;
   if (scheme == 0 || stage == 0) out = tt + dt * r;
   else if (stage == 1) out = 3. / 4. * tt + 1. / 4. * (uu + dt * r);
   else out = 1. / 3. * tt + 2. / 3. * (uu + dt * r);
   u_next[gid] = out;
+}
+
+inline int pow2_at_least(int n) {
+  int p = 1;
+  while (p < n) p *= 2;
+  return p;
 }
 
 template <int PF1>
@@ -381,6 +448,19 @@ void launch_sweep(const SweepArgs& a, int pf1, int pt1, hipStream_t s) {
   ADG_SW(4, 2) ADG_SW(4, 3) ADG_SW(4, 4)
 #undef ADG_SW
   throw std::runtime_error("sweep: unsupported basis sizes (degrees 1..3 are provided)");
+}
+
+// Dense level-ordered table (rows ns*pf1, cols ns*pt1) -> target-major cell blocks.
+std::vector<double> to_blocks(const std::vector<double>& M, int ns, int pf1, int pt1) {
+  std::vector<double> b(M.size());
+  const size_t cols = static_cast<size_t>(ns) * pt1;
+  for (int st = 0; st < ns; ++st)
+    for (int sc = 0; sc < ns; ++sc)
+      for (int kk = 0; kk < pf1; ++kk)
+        for (int i = 0; i < pt1; ++i)
+          b[((static_cast<size_t>(st) * ns + sc) * pf1 + kk) * pt1 + i] =
+              M[(static_cast<size_t>(sc) * pf1 + kk) * cols + static_cast<size_t>(st) * pt1 + i];
+  return b;
 }
 
 template <class T>
@@ -427,17 +507,18 @@ Context::Context(int dim, int k_, int m_, int nmax_)
   na = alpt.size();
   nl = lagr.size();
   OperatorTables aa(alpt, alpt), la(lagr, alpt);
-  upload(t_pt, alpt_at_lagr_points(alpt, lagr), stream);
-  upload(t_la_uv, la.u_v, stream);
-  upload(t_la_uvx, la.u_vx, stream);
+  const int ns = 1 << nmax;
+  upload(t_pt, to_blocks(alpt_at_lagr_points(alpt, lagr), ns, k + 1, m + 1), stream);
+  upload(t_la_uv, to_blocks(la.u_v, ns, m + 1, k + 1), stream);
+  upload(t_la_uvx, to_blocks(la.u_vx, ns, m + 1, k + 1), stream);
   std::vector<double> flx(la.ulft_vjp.size());
   for (size_t i = 0; i < flx.size(); ++i) flx[i] = la.ulft_vjp[i] + la.urgt_vjp[i];
-  upload(t_la_flx, flx, stream);
-  upload(t_aa[0], aa.u_v, stream);
-  upload(t_aa[1], aa.u_vx, stream);
-  upload(t_aa[2], aa.ulft_vjp, stream);
-  upload(t_aa[3], aa.urgt_vjp, stream);
-  upload(t_aa[4], aa.ujp_vjp, stream);
+  upload(t_la_flx, to_blocks(flx, ns, m + 1, k + 1), stream);
+  upload(t_aa[0], to_blocks(aa.u_v, ns, k + 1, k + 1), stream);
+  upload(t_aa[1], to_blocks(aa.u_vx, ns, k + 1, k + 1), stream);
+  upload(t_aa[2], to_blocks(aa.ulft_vjp, ns, k + 1, k + 1), stream);
+  upload(t_aa[3], to_blocks(aa.urgt_vjp, ns, k + 1, k + 1), stream);
+  upload(t_aa[4], to_blocks(aa.ujp_vjp, ns, k + 1, k + 1), stream);
   upload(t_lmass, lagr_mass_diag(lagr), stream);
   SurplusTables sur(lagr);
   upload(t_anc_slot, sur.anc_slot, stream);
@@ -514,23 +595,52 @@ void Context::prof_reset() {
 }
 
 SweepTable Context::table(int id) {
+  const int ns = 1 << nmax;
   switch (id) {
-    case 0: return {t_pt.p, nl};
-    case 1: return {t_la_uv.p, na};
-    case 2: return {t_la_uvx.p, na};
-    case 3: return {t_la_flx.p, na};
-    case 4: case 5: case 6: case 7: case 8: return {t_aa[id - 4].p, na};
+    case 0: return {t_pt.p, ns};
+    case 1: return {t_la_uv.p, ns};
+    case 2: return {t_la_uvx.p, ns};
+    case 3: return {t_la_flx.p, ns};
+    case 4: case 5: case 6: case 7: case 8: return {t_aa[id - 4].p, ns};
   }
   throw std::runtime_error("unknown table id");
 }
 
 void Context::upload_grid() {
-  for (int d = 0; d < D; ++d) {
-    upload(g_slot[d], grid.dslot[d], stream);
-    upload(g_fbase[d], grid.fbase[d], stream);
-    upload(g_fmaxl[d], grid.fmaxl[d], stream);
-    upload(g_fmap[d], grid.fmap[d], stream);
+  // union of the instance tables: element indices and fiber ids are shifted by
+  // the instance offsets, so fibers stay within their instance
+  const int64_t n = E();
+  uslot.resize(n * D);
+  uhash.resize(n);
+  uinst.resize(n);
+  std::vector<int32_t> dsl(n), fb(n), fml(n), fm;
+  for (int i = 0; i < B(); ++i) {
+    const HostGrid& g = grids[i];
+    std::copy(g.slot.begin(), g.slot.end(), uslot.begin() + off[i] * D);
+    std::copy(g.hash.begin(), g.hash.end(), uhash.begin() + off[i]);
+    std::fill(uinst.begin() + off[i], uinst.begin() + off[i + 1], i);
   }
+  for (int d = 0; d < D; ++d) {
+    fm.clear();
+    int64_t fib_off = 0;
+    for (int i = 0; i < B(); ++i) {
+      const HostGrid& g = grids[i];
+      if (g.E == 0) continue;
+      for (int64_t e = 0; e < g.E; ++e) {
+        dsl[off[i] + e] = g.dslot[d][e];
+        fb[off[i] + e] = g.fbase[d][e] + static_cast<int32_t>(fib_off << nmax);
+        fml[off[i] + e] = g.fmaxl[d][e];
+      }
+      for (int32_t v : g.fmap[d]) fm.push_back(v < 0 ? -1 : v + static_cast<int32_t>(off[i]));
+      fib_off += g.nfib[d];
+    }
+    if (static_cast<int64_t>(fm.size()) > (int64_t{1} << 31) - 1) throw std::runtime_error("fiber map exceeds int32 range");
+    upload(g_slot[d], dsl, stream);
+    upload(g_fbase[d], fb, stream);
+    upload(g_fmaxl[d], fml, stream);
+    upload(g_fmap[d], fm, stream);
+  }
+  upload(g_inst, uinst, stream);
 }
 
 void Context::ensure_fields() {
@@ -559,19 +669,31 @@ void Context::remap_field(DevBuf<double>& f, int per_elem, const std::vector<int
 }
 
 void Context::set_grid_slots(const std::vector<int32_t>& slots, bool keep_fields) {
-  HostGrid ng;
-  ng.build(D, nmax, slots, hk);
-  if (keep_fields && grid.E > 0) {
-    const std::vector<int32_t> m = map_new_to_old(grid, ng);
+  set_grids(std::vector<std::vector<int32_t>>{slots}, keep_fields);
+}
+
+void Context::set_grids(const std::vector<std::vector<int32_t>>& slots, bool keep_fields) {
+  const int nb = static_cast<int>(slots.size());
+  std::vector<HostGrid> ng(nb);
+  parallel_for(nb, [&](int i) { ng[i].build(D, nmax, slots[i], hk); });
+  std::vector<int64_t> noff(nb + 1, 0);
+  for (int i = 0; i < nb; ++i) noff[i + 1] = noff[i] + ng[i].E;
+  const int64_t newE = noff[nb];
+  if (keep_fields && E() > 0 && nb == B()) {
+    std::vector<int32_t> m(newE, -1);
+    parallel_for(nb, [&](int i) {
+      const std::vector<int32_t> mi = map_new_to_old(grids[i], ng[i]);
+      for (int64_t j = 0; j < ng[i].E; ++j) m[noff[i] + j] = mi[j] < 0 ? -1 : static_cast<int32_t>(off[i] + mi[j]);
+    });
     upload(g_remap, m, stream);
-    const int64_t oldE = grid.E;
+    const int64_t oldE = E();
     remap_field(u, K, m, oldE);
     remap_field(pred, K, m, oldE);
     // host-side interpolation state follows the grid
     if (!up_host.empty()) {
-      std::vector<double> nu(static_cast<size_t>(ng.E) * P, 0.);
-      std::vector<char> na_(ng.E, 1);
-      for (int64_t i = 0; i < ng.E; ++i)
+      std::vector<double> nu(static_cast<size_t>(newE) * P, 0.);
+      std::vector<char> na_(newE, 1);
+      for (int64_t i = 0; i < newE; ++i)
         if (m[i] >= 0) {
           std::copy(&up_host[static_cast<size_t>(m[i]) * P], &up_host[static_cast<size_t>(m[i]) * P] + P, &nu[i * P]);
           na_[i] = 0;
@@ -581,9 +703,10 @@ void Context::set_grid_slots(const std::vector<int32_t>& slots, bool keep_fields
     }
   } else {
     up_host.clear();
-    new_add.assign(ng.E, 1);
+    new_add.assign(newE, 1);
   }
-  grid = std::move(ng);
+  grids = std::move(ng);
+  off = noff;
   ++grid_version;
   upload_grid();
   ensure_fields();
@@ -596,11 +719,16 @@ void Context::set_grid_slots(const std::vector<int32_t>& slots, bool keep_fields
   if (flux_kind == ADG_FLUX_VARCOEF_ID) compute_vcoef();
 }
 
+void Context::upload_inst_dt() {
+  upload(d_inst_dt, inst_dt, stream);
+  use_inst_dt = true;
+}
+
 void Context::sweep(const double* in, const int* sin, double* out, double* accum, const int* sout, int dd,
                     SweepTable tab, bool flx, int lu, double coef, int pf, int pt) {
   if (E() == 0) return;
   SweepArgs a{};
-  a.in = in; a.out = out; a.accum = accum; a.M = tab.dev; a.ld = tab.ld;
+  a.in = in; a.out = out; a.accum = accum; a.M = tab.dev; a.ns = tab.ns;
   a.slot = g_slot[dd].p; a.fbase = g_fbase[dd].p; a.fmaxl = g_fmaxl[dd].p; a.fmap = g_fmap[dd].p;
   a.E = E();
   int stin[kMaxDim], stout[kMaxDim];
@@ -671,7 +799,7 @@ void Context::compute_vcoef() {
       for (int d = D - 1; d >= 0; --d) { idx[d] = rem % (m + 1); rem /= (m + 1); }
       for (int d = 0; d < D; ++d) {
         const int dn = (d + 1) % D;
-        const double x = lagr.point(grid.slot[e * D + dn] * (m + 1) + idx[dn]);
+        const double x = lagr.point(uslot[e * D + dn] * (m + 1) + idx[dn]);
         h[(static_cast<size_t>(d) * E() + e) * P + q] = 1. + flux_param * std::sin(2. * PI * x);
       }
     }
@@ -755,10 +883,10 @@ void Context::rhs_penalty(const double* alpha) {
 void Context::rhs_linear(const double* coef) {
   if (lin_coef.size() != static_cast<size_t>(D) || !std::equal(lin_coef.begin(), lin_coef.end(), coef)) {
     OperatorTables aa(alpt, alpt);
-    for (int d = 0; d < D; ++d) upload(t_lin[d], linear_operator_table(aa, coef[d]), stream);
+    for (int d = 0; d < D; ++d) upload(t_lin[d], to_blocks(linear_operator_table(aa, coef[d]), 1 << nmax, k + 1, k + 1), stream);
     lin_coef.assign(coef, coef + D);
   }
-  for (int d = 0; d < D; ++d) rhs_single_dim({t_lin[d].p, na}, d, 1.0);
+  for (int d = 0; d < D; ++d) rhs_single_dim({t_lin[d].p, 1 << nmax}, d, 1.0);
 }
 
 void Context::rk_init() {
@@ -769,7 +897,7 @@ void Context::rk_stage(int scheme, int stage, double dt) {
   if (E() > 0) {
     prof_begin(PROF_RK, 8.0 * static_cast<double>(E()) * K * 4);
     hipLaunchKernelGGL(k_rk, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, tn.p, rhs.p, E() * K, scheme,
-                       stage, dt);
+                       stage, dt, use_inst_dt ? d_inst_dt.p : nullptr, g_inst.p, K);
     prof_end();
   }
 }
@@ -832,7 +960,7 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
   auto make_term = [&](const double* in, const int* sin_sz, int dd, SweepTable tab, int pt1, bool flx, int lu,
                        double coef) {
     TermDev t{};
-    t.in = in; t.M = tab.dev; t.ld = tab.ld; t.pt1 = pt1; t.dd = dd;
+    t.in = in; t.M = tab.dev; t.ns = tab.ns; t.pt1 = pt1; t.dd = dd;
     t.slot = g_slot[dd].p; t.fbase = g_fbase[dd].p; t.fmaxl = g_fmaxl[dd].p; t.fmap = g_fmap[dd].p;
     t.Tin = prod_of(sin_sz, D);
     strides_of(sin_sz, D, t.sin);
@@ -955,11 +1083,14 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   }
   {
     const int64_t n = E() * P;
+    const int ntp = pow2_at_least(pl.n_up_terms);
+    if (ntp > kBlock) throw std::runtime_error("too many eval_up terms");
+    const unsigned nb = static_cast<unsigned>((n + kBlock / ntp - 1) / (kBlock / ntp));
     prof_begin(PROF_FLUX, pl.up_final_bytes);
     switch (k + 1) {
-      case 2: hipLaunchKernelGGL((k_up_final<2>), dim3(nblocks(n)), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
-      case 3: hipLaunchKernelGGL((k_up_final<3>), dim3(nblocks(n)), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
-      case 4: hipLaunchKernelGGL((k_up_final<4>), dim3(nblocks(n)), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
+      case 2: hipLaunchKernelGGL((k_up_final<2>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
+      case 3: hipLaunchKernelGGL((k_up_final<3>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
+      case 4: hipLaunchKernelGGL((k_up_final<4>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
     }
     prof_end();
   }
@@ -973,12 +1104,15 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   }
   {
     const int64_t n = E() * K;
+    const int ntp = pow2_at_least(pl.n_rhs_terms + pl.n_pen);
+    if (ntp > kBlock) throw std::runtime_error("too many right-hand-side terms");
+    const unsigned nb = static_cast<unsigned>((n + kBlock / ntp - 1) / (kBlock / ntp));
     prof_begin(PROF_RK, pl.rhs_final_bytes);
-#define ADG_RF(M1, K1)                                                                                           \
-  if (m + 1 == M1 && k + 1 == K1) {                                                                              \
-    hipLaunchKernelGGL((k_rhs_final<M1, K1>), dim3(nblocks(n)), dim3(kBlock), 0, stream, pl.rhs_terms.p,        \
-                       pl.n_rhs_terms, pl.pens.p, pl.n_pen, E(), K, k + 1, D, rhs.p, u.p, tn.p, u_next.p, scheme, \
-                       stage, dt);                                                                               \
+#define ADG_RF(M1, K1)                                                                                             \
+  if (m + 1 == M1 && k + 1 == K1) {                                                                                \
+    hipLaunchKernelGGL((k_rhs_final<M1, K1>), dim3(nb), dim3(kBlock), 0, stream, pl.rhs_terms.p, pl.n_rhs_terms,  \
+                       pl.pens.p, pl.n_pen, ntp, E(), K, k + 1, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt,   \
+                       use_inst_dt ? d_inst_dt.p : nullptr, g_inst.p);                                              \
   }
     ADG_RF(2, 2) ADG_RF(2, 3) ADG_RF(2, 4) ADG_RF(3, 2) ADG_RF(3, 3) ADG_RF(3, 4) ADG_RF(4, 2) ADG_RF(4, 3)
     ADG_RF(4, 4)
@@ -992,7 +1126,8 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
 void Context::save_state(int slot) {
   if (slot < 0 || slot >= 4) throw std::runtime_error("snapshot slot must be 0..3");
   Snapshot& sn = snaps[slot];
-  sn.slots = grid.slot;
+  sn.slots.clear();
+  for (const HostGrid& g : grids) sn.slots.push_back(g.slot);
   sn.E = E();
   sn.u.ensure(std::max<int64_t>(E() * K, 1));
   if (E() > 0) hipLaunchKernelGGL(k_copy, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, sn.u.p, u.p, E() * K);
@@ -1002,7 +1137,9 @@ void Context::restore_state(int slot) {
   if (slot < 0 || slot >= 4) throw std::runtime_error("snapshot slot must be 0..3");
   Snapshot& sn = snaps[slot];
   if (sn.slots.empty()) throw std::runtime_error("snapshot slot is empty");
-  if (sn.slots != grid.slot) set_grid_slots(sn.slots, false);
+  bool same = static_cast<int>(sn.slots.size()) == B();
+  for (int i = 0; same && i < B(); ++i) same = (sn.slots[i] == grids[i].slot);
+  if (!same) set_grids(sn.slots, false);
   if (E() > 0) hipLaunchKernelGGL(k_copy, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, sn.u.p, E() * K);
 }
 
@@ -1025,40 +1162,64 @@ void Context::indicator_sumsq(double* host_out) {
   sync();
 }
 
-int64_t Context::refine(double eps) {
+int64_t Context::refine(double eps) { return refine_instances(std::vector<double>(B(), eps)); }
+
+int64_t Context::coarsen(double eta) { return coarsen_instances(std::vector<double>(B(), eta)); }
+
+int64_t Context::refine_instances(const std::vector<double>& eps) {
+  if (static_cast<int>(eps.size()) != B()) throw std::runtime_error("refine: one threshold per instance expected");
   std::vector<double> s(E());
   indicator_sumsq(s.data());
   for (double& v : s) v = std::pow(v, 1. / 2.);
-  int64_t added = 0;
-  std::vector<int32_t> ns = refine_slots(grid, s, eps, added);
-  if (added > 0) set_grid_slots(ns, true);
-  return added;
+  std::vector<std::vector<int32_t>> ns(B());
+  std::vector<int64_t> added(B(), 0);
+  parallel_for(B(), [&](int i) {
+    const std::vector<double> ni(s.begin() + off[i], s.begin() + off[i + 1]);
+    ns[i] = refine_slots(grids[i], ni, eps[i], added[i]);
+  });
+  int64_t tot = 0;
+  for (int64_t a : added) tot += a;
+  if (tot > 0) set_grids(ns, true);
+  return tot;
 }
 
-int64_t Context::coarsen(double eta) {
+int64_t Context::coarsen_instances(const std::vector<double>& eta) {
+  if (static_cast<int>(eta.size()) != B()) throw std::runtime_error("coarsen: one threshold per instance expected");
   std::vector<double> s(E());
   indicator_sumsq(s.data());
   for (double& v : s) v = std::pow(v, 1. / 2.);
-  int64_t removed = 0;
-  std::vector<int32_t> ns = coarsen_slots(grid, s, eta, removed);
-  if (removed > 0) set_grid_slots(ns, true);
-  return removed;
+  std::vector<std::vector<int32_t>> ns(B());
+  std::vector<int64_t> removed(B(), 0);
+  parallel_for(B(), [&](int i) {
+    const std::vector<double> ni(s.begin() + off[i], s.begin() + off[i + 1]);
+    ns[i] = coarsen_slots(grids[i], ni, eta[i], removed[i]);
+  });
+  int64_t tot = 0;
+  for (int64_t r : removed) tot += r;
+  if (tot > 0) set_grids(ns, true);
+  return tot;
 }
 
 void Context::init_adaptive_intp(double (*f)(const double*, int, void*), void* user, double eps) {
+  std::vector<void*> users(B(), user);
+  init_adaptive_intp_batch(f, users.data(), eps);
+}
+
+void Context::init_adaptive_intp_batch(double (*f)(const double*, int, void*), void* const* users, double eps) {
   if (up_host.size() != static_cast<size_t>(E()) * P) {
     up_host.assign(static_cast<size_t>(E()) * P, 0.);
     new_add.assign(E(), 1);
   }
-  std::vector<double> x(D);
   auto eval_new = [&]() {
+    std::vector<double> x(D);
     for (int64_t e = 0; e < E(); ++e) {
       if (!new_add[e]) continue;
+      void* user = users[uinst[e]];
       for (int q = 0; q < P; ++q) {
         int rem = q;
         int idx[kMaxDim];
         for (int d = D - 1; d >= 0; --d) { idx[d] = rem % (m + 1); rem /= (m + 1); }
-        for (int d = 0; d < D; ++d) x[d] = lagr.point(grid.slot[e * D + d] * (m + 1) + idx[d]);
+        for (int d = 0; d < D; ++d) x[d] = lagr.point(uslot[e * D + d] * (m + 1) + idx[d]);
         up_host[e * P + q] = f(x.data(), D, user);
       }
     }
@@ -1072,20 +1233,27 @@ void Context::init_adaptive_intp(double (*f)(const double*, int, void*), void* u
     LagrNormArgs a{};
     a.c = uintp.p; a.lmass = t_lmass.p; a.out = sumsq.p; a.E = E(); a.D = D; a.P = P; a.p1 = m + 1;
     for (int d = 0; d < D; ++d) a.slot[d] = g_slot[d].p;
-    hipLaunchKernelGGL(k_lagr_norm, dim3(nblocks(E())), dim3(kBlock), 0, stream, a);
+    if (E() > 0) hipLaunchKernelGGL(k_lagr_norm, dim3(nblocks(E())), dim3(kBlock), 0, stream, a);
     std::vector<double> s(E());
-    hip_check(hipMemcpyAsync(s.data(), sumsq.p, E() * sizeof(double), hipMemcpyDeviceToHost, stream), "download");
+    if (E() > 0)
+      hip_check(hipMemcpyAsync(s.data(), sumsq.p, E() * sizeof(double), hipMemcpyDeviceToHost, stream), "download");
     sync();
     check_error_flag();
     for (double& v : s) v = std::pow(v, 0.5);
+    std::vector<std::vector<int32_t>> ns(B());
     int64_t added = 0;
-    std::vector<int32_t> ns = refine_slots(grid, s, eps, added);
+    for (int i = 0; i < B(); ++i) {
+      const std::vector<double> ni(s.begin() + off[i], s.begin() + off[i + 1]);
+      int64_t ai = 0;
+      ns[i] = refine_slots(grids[i], ni, eps, ai);
+      added += ai;
+    }
     if (added == 0) break;
-    set_grid_slots(ns, true);
+    set_grids(ns, true);
     eval_new();
   }
   // FastLagrInit::eval_ucoe_Alpt_Lagr: Lagrange surplus -> Alpert coefficients
-  hipLaunchKernelGGL(k_fill, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, E() * K, 0.);
+  if (E() > 0) hipLaunchKernelGGL(k_fill, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, E() * K, 0.);
   SweepTable tabs[kMaxDim];
   int flx[kMaxDim];
   for (int d = 0; d < D; ++d) { tabs[d] = table(1); flx[d] = 0; }

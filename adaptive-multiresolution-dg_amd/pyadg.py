@@ -91,6 +91,11 @@ def lib():
             "adg_state_restore": (i32, [vp, i32]),
             "adg_profile": (i32, [vp, i32]),
             "adg_profile_get": (i32, [vp, DP, P(i64), DP]),
+            "adg_grid_init_batch": (i32, [vp, i32, i32, i32]),
+            "adg_init_adaptive_intp_batch": (i32, [vp, POINT_FN, P(vp), dbl]),
+            "adg_num_instances": (i32, [vp]),
+            "adg_instance_range": (i32, [vp, i32, P(i64), P(i64)]),
+            "adg_set_instance_thresholds": (i32, [vp, DP, DP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -272,6 +277,49 @@ class Solution:
         fn = POINT_FN(ic_eval_ptr())
         self._keep.append(fn)
         self._check(lib().adg_init_adaptive_intp(self._h, fn, ctypes.cast(ctypes.pointer(params), ctypes.c_void_p), eps))
+
+    # ---- batches of independent solves of one discretisation ----
+    def init_batch(self, ninst, sparse=True, level_init=None):
+        """Replace the grid by ninst copies of the initial grid."""
+        self._check(lib().adg_grid_init_batch(self._h, ninst, int(sparse), self.nmax if level_init is None else level_init))
+
+    @property
+    def num_instances(self):
+        return int(lib().adg_num_instances(self._h))
+
+    def instance_range(self, i):
+        a, n = ctypes.c_int64(), ctypes.c_int64()
+        self._check(lib().adg_instance_range(self._h, i, ctypes.byref(a), ctypes.byref(n)))
+        return a.value, n.value
+
+    def init_adaptive_intp_builtin_batch(self, params_list, eps):
+        """Initialise instance i with adg_ic_eval and params_list[i]."""
+        if len(params_list) != self.num_instances:
+            raise ValueError("one ICParams per instance expected")
+        self._keep.extend(params_list)
+        fn = POINT_FN(ic_eval_ptr())
+        self._keep.append(fn)
+        users = (ctypes.c_void_p * len(params_list))(
+            *[ctypes.cast(ctypes.pointer(p), ctypes.c_void_p).value for p in params_list])
+        self._keep.append(users)
+        self._check(lib().adg_init_adaptive_intp_batch(self._h, fn, users, eps))
+
+    def set_instance_thresholds(self, eps, eta):
+        e = np.ascontiguousarray(eps, np.float64)
+        t = np.ascontiguousarray(eta, np.float64)
+        if e.size != self.num_instances or t.size != self.num_instances:
+            raise ValueError("one threshold per instance expected")
+        self._check(lib().adg_set_instance_thresholds(self._h, _dp(e), _dp(t)))
+
+    def instance_grid(self, i):
+        """(level, suppt, hash) of instance i."""
+        a, n = self.instance_range(i)
+        lv, sp, hk = self.grid()
+        return lv[a:a + n], sp[a:a + n], hk[a:a + n]
+
+    def instance_get(self, i, field=FIELD_UCOE):
+        a, n = self.instance_range(i)
+        return self.get(field)[a * self.K:(a + n) * self.K]
 
     def run_linear(self, coef, dt, nsteps):
         c = np.asarray(coef, np.float64)

@@ -148,3 +148,17 @@ def make_solution(w):
     s = pyadg.Solution(w.dim, w.k, w.m, w.nmax, w.sparse, w.level_init)
     s.init_adaptive_intp_builtin(w.ic, w.init_eps)
     return s
+
+
+def make_batch(workloads):
+    """One context holding every workload of the list as an instance.  All must
+    share (dim, k, m, nmax, sparse, level_init, init_eps)."""
+    w0 = workloads[0]
+    for w in workloads[1:]:
+        if (w.dim, w.k, w.m, w.nmax, w.sparse, w.level_init, w.init_eps) != (
+                w0.dim, w0.k, w0.m, w0.nmax, w0.sparse, w0.level_init, w0.init_eps):
+            raise ValueError("instances of one batch must share the discretisation")
+    s = pyadg.Solution(w0.dim, w0.k, w0.m, w0.nmax, w0.sparse, w0.level_init)
+    s.init_batch(len(workloads), w0.sparse, w0.level_init)
+    s.init_adaptive_intp_builtin_batch([w.ic for w in workloads], w0.init_eps)
+    return s

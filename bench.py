@@ -88,35 +88,49 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=8.0, help="stepping time of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="time without per-kernel HIP events")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="independent seeded instances solved together in the chosen form (c2, c5)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     make = workloads.WORKLOADS[args.workload]
-    w = make(seed=args.seed) if args.workload in ("c2", "c5") else make()
+    batched = args.workload in ("c2", "c5") and args.batch > 1
+
+    def workload(seed):
+        return make(seed=seed) if args.workload in ("c2", "c5") else make()
+
+    w = workload(args.seed)
     print(f"[bench] workload {w.name}: init", file=sys.stderr, flush=True)
     sol = workloads.make_solution(w)
     print(f"[bench] init done: {sol.num_elem} elements; warmup {args.warmup} steps", file=sys.stderr, flush=True)
-
     sol.save_state(0)
+    bsol = None
+    if batched:
+        ws = [workload(args.seed + i) for i in range(args.batch)]
+        print(f"[bench] batch of {args.batch} instances (seeds {args.seed}..{args.seed + args.batch - 1}): init",
+              file=sys.stderr, flush=True)
+        bsol = workloads.make_batch(ws)
+        bsol.save_state(0)
+        print(f"[bench] batch init done: {bsol.num_elem} elements", file=sys.stderr, flush=True)
 
-    def measure(form):
+    def measure(s, form):
         """Time --steps full solves (after --warmup) of one form; returns figures and HIP-event profile."""
         prob = None if w.linear else pyadg.Problem.from_buffer_copy(w.problem)
 
         def solve():
-            sol.restore_state(0)
+            s.restore_state(0)
             if w.linear:
-                return sol.run_linear(w.coef, w.dt, w.nsteps)
+                return s.run_linear(w.coef, w.dt, w.nsteps)
             prob.t_start, prob.max_steps, prob.form = 0.0, 0, form
-            return sol.run_hyperbolic(prob)
+            return s.run_hyperbolic(prob)
 
         for _ in range(args.warmup):
             solve()
         if not args.no_events:
-            sol.profile(True)
-        sol.sync()
+            s.profile(True)
+        s.sync()
         dof_stages, time_steps, max_elem = 0, 0, 0
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -124,19 +138,21 @@ def main():
             dof_stages += st.dof_stages
             time_steps += st.steps
             max_elem = max(max_elem, st.max_elem)
-        sol.sync()
+        s.sync()
         sec = time.perf_counter() - t0
-        prof = sol.profile_get()
-        sol.profile(False)
-        u = sol.get(pyadg.FIELD_UCOE)
-        print(f"[bench] form {form}: {args.steps} solves ({time_steps} time steps) in {sec:.3f} s",
-              file=sys.stderr, flush=True)
+        prof = s.profile_get()
+        s.profile(False)
+        u = s.instance_get(0) if s.num_instances > 1 else s.get(pyadg.FIELD_UCOE)
+        print(f"[bench] form {form}, {s.num_instances} instance(s): {args.steps} solves ({time_steps} time steps) "
+              f"in {sec:.3f} s, {dof_stages / sec:.4g} DoF-stage/s", file=sys.stderr, flush=True)
         return {"value": dof_stages / sec, "sec": sec, "dof_stages": dof_stages, "time_steps": time_steps,
                 "max_elem": max_elem, "prof": prof, "u": u, "sim_time": st.final_time}
 
-    plain = measure(0)
-    chosen = plain if w.linear else measure(1)
+    plain = measure(sol, 0)
+    fused = plain if w.linear else measure(sol, 1)
+    chosen = measure(bsol, 1) if batched else fused
     value = chosen["value"]
+    # instance 0 of the batch is the plain problem itself: equal L2 error means equal coefficients
     identical = bool(np.array_equal(plain["u"], chosen["u"]))
 
     roofline = None
@@ -166,13 +182,16 @@ def main():
         "warmup": args.warmup, "ms_per_step": chosen["sec"] / max(args.steps, 1) * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": dict(w.config(), step="one full solve from the saved initial state",
-                       form="chosen: fused FP64 stages (3*dim launches per RK stage)",
-                       plain_form="plain FP64, one kernel per 1D sweep",
-                       plain_value=plain["value"], chosen_value=chosen["value"],
+                       form=("chosen: fused FP64 stages (3*dim launches per RK stage), %d seeded instances "
+                             "(seeds %d..%d) per launch" % (args.batch, args.seed, args.seed + args.batch - 1))
+                       if batched else "chosen: fused FP64 stages (3*dim launches per RK stage)",
+                       plain_form="plain FP64, one kernel per 1D sweep, one instance",
+                       plain_value=plain["value"], fused_single_instance_value=fused["value"],
+                       chosen_value=chosen["value"], batch=args.batch if batched else 1,
                        speedup_vs_plain=chosen["value"] / plain["value"],
                        forms_bit_identical=identical, l2_difference_between_forms=float(
                            np.sqrt(np.sum((plain["u"] - chosen["u"]) ** 2))),
-                       time_steps_per_solve=chosen["time_steps"] // max(args.steps, 1),
+                       time_steps_per_solve=plain["time_steps"] // max(args.steps, 1),
                        max_elements=int(chosen["max_elem"]), dof_stage_updates=int(chosen["dof_stages"]),
                        sim_time=chosen["sim_time"]),
         "roofline": roofline, "cpu_baseline": cpu,

@@ -196,6 +196,22 @@ typedef struct {
 } adg_problem;
 int adg_run_hyperbolic(adg_ctx* c, const adg_problem* p, adg_run_stats* st);
 
+/* Batches of independent solves.  A context may hold several instances of the
+ * same discretisation (dim, k, m, nmax): their element tables are stored one
+ * after another, fibers never cross instances, and every kernel serves all
+ * instances in one launch.  Arrays crossing the C-ABI are ordered by instance,
+ * then by hash key.  adg_run_hyperbolic advances every instance with its own
+ * time step and thresholds until each reaches t_final (instances that finish
+ * early leave the batch and are put back at the end). */
+int adg_grid_init_batch(adg_ctx* c, int ninst, int sparse, int level_init);
+/* users[i] is passed to f for the points of instance i. */
+int adg_init_adaptive_intp_batch(adg_ctx* c, adg_point_fn f, void* const* users, double eps);
+int adg_num_instances(adg_ctx* c);
+/* Elements [first, first + count) of the union belong to instance inst. */
+int adg_instance_range(adg_ctx* c, int inst, int64_t* first, int64_t* count);
+/* Per-instance refine / coarsen thresholds (arrays of adg_num_instances values). */
+int adg_set_instance_thresholds(adg_ctx* c, const double* eps, const double* eta);
+
 /* One fused RK stage of the same operator: point values, flux, surplus
  * transform, right-hand side (volume, flux and LF penalty terms) and the stage
  * update in 3*dim launches; bit-identical to adg_eval_up + adg_flux +
