@@ -217,6 +217,9 @@ int adg_sweep(adg_ctx* h, int table, int flx, int lu, int dim, const int32_t* si
   });
 }
 
+int adg_set_level_sum_cap(adg_ctx* h, int cap) {
+  return guard(h, [&](adg::Context& c) { c.lsum_cap = cap < 0 ? -1 : cap; });
+}
 int adg_eval_up(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.eval_up(); }); }
 int adg_flux(adg_ctx* h, int kind, const double* params) {
   return guard(h, [&](adg::Context& c) {
@@ -242,7 +245,8 @@ int adg_rhs_linear(adg_ctx* h, const double* coef) { return guard(h, [&](adg::Co
 int adg_rk_init(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rk_init(); }); }
 int adg_rk_stage(adg_ctx* h, int scheme, int stage, double dt) {
   return guard(h, [&](adg::Context& c) {
-    if (!((scheme == ADG_RK_EULER && stage == 0) || (scheme == ADG_RK3 && stage >= 0 && stage <= 2)))
+    if (!((scheme == ADG_RK_EULER && stage == 0) || (scheme == ADG_RK2 && stage >= 0 && stage <= 1) ||
+          (scheme == ADG_RK3 && stage >= 0 && stage <= 2)))
       throw std::runtime_error("invalid RK scheme/stage");
     c.rk_stage(scheme, stage, dt);
   });
@@ -498,7 +502,8 @@ int adg_set_instance_thresholds(adg_ctx* h, const double* eps, const double* eta
 int adg_stage(adg_ctx* h, const adg_problem* p, int scheme, int stage, double dt) {
   return guard(h, [&](adg::Context& c) {
     if (!p) throw std::runtime_error("null problem");
-    if (!((scheme == ADG_RK_EULER && stage == 0) || (scheme == ADG_RK3 && stage >= 0 && stage <= 2)))
+    if (!((scheme == ADG_RK_EULER && stage == 0) || (scheme == ADG_RK2 && stage >= 0 && stage <= 1) ||
+          (scheme == ADG_RK3 && stage >= 0 && stage <= 2)))
       throw std::runtime_error("invalid RK scheme/stage");
     c.stage_fused(stage_problem(p, c.D), scheme, stage, dt);
   });

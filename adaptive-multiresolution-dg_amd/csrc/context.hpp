@@ -93,6 +93,10 @@ class Context {
   // grid arrays
   DevBuf<int32_t> g_slot[kMaxDim], g_fbase[kMaxDim], g_fmaxl[kMaxDim], g_fmap[kMaxDim];
   DevBuf<int32_t> g_remap;
+  DevBuf<int32_t> g_lsum;  // level sum of each element
+  // Level-sum cap of the reference's _coarse_grid variants: sweeps and
+  // operators ignore elements above it (their output is zero).  -1: none.
+  int lsum_cap = -1;
 
   // fields
   DevBuf<double> u, tn, pred, rhs, up, uintp, fp, fucoe, interA, interB, tmpA, tmpB, vcoef, sumsq, remap_tmp;

@@ -67,8 +67,10 @@ struct SweepArgs {
   const int32_t* fbase;
   const int32_t* fmaxl;
   const int32_t* fmap;
+  const int32_t* lsum;  // level sum per element
   int64_t E;
   int R, Tin, Tout, sdd_in, sdd_out, ns;
+  int cap;  // level-sum cap (coarse grid); negative: none
   int nrest;
   int rsz[3], rsin[3], rsout[3];
   int flx, lu;
@@ -94,7 +96,24 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
   double acc[PT1];
 #pragma unroll
   for (int i = 0; i < PT1; ++i) acc[i] = 0.;
-  for_each_related(st, a.fmaxl[e], a.flx != 0, a.lu, [&](int s) {
+  const int64_t ob = e * a.Tout + bout;
+  int maxl = a.fmaxl[e];
+  if (a.cap >= 0) {
+    // coarse grid (FastMultiplyLU::transform_1D_coarse_grid): a target above
+    // the cap outputs zero; sources in its fiber share its other levels, so
+    // the cap bounds their level in dd
+    const int ls = a.lsum[e];
+    if (ls > a.cap) {
+#pragma unroll
+      for (int i = 0; i < PT1; ++i) {
+        if (a.out) a.out[ob + i * a.sdd_out] = 0.;
+        if (a.accum) a.accum[ob + i * a.sdd_out] += 0.;
+      }
+      return;
+    }
+    maxl = min(maxl, a.cap - ls + slot_level(st));
+  }
+  for_each_related(st, maxl, a.flx != 0, a.lu, [&](int s) {
     const int src = a.fmap[fb + s];
     if (src < 0) return;
     const double* x = a.in + static_cast<int64_t>(src) * a.Tin + bin;
@@ -106,7 +125,6 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
       for (int i = 0; i < PT1; ++i) acc[i] += xv * Mr[kk * PT1 + i];
     }
   });
-  const int64_t ob = e * a.Tout + bout;
 #pragma unroll
   for (int i = 0; i < PT1; ++i) {
     const double v = acc[i] * a.coef;
@@ -170,18 +188,22 @@ __global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
   a.out[c * a.comp_stride + e * a.P + q] = coe;
 }
 
-// SSP-RK3 / forward Euler stage (source/ODESolver.cpp:209-213,285-301).
+// One explicit RK stage update from u_n (t), the current stage value (uu) and
+// the right-hand side (rr): forward Euler (source/ODESolver.cpp:209-213),
+// RK2 midpoint (:247-259) and SSP-RK3 (:285-301).
+__device__ __forceinline__ double rk_combine(int scheme, int stage, double dt, double t, double uu, double rr) {
+  if (scheme == 2) return stage == 0 ? t + 0.5 * dt * rr : t + dt * rr;
+  if (scheme == 0 || stage == 0) return t + dt * rr;
+  if (stage == 1) return 3. / 4. * t + 1. / 4. * (uu + dt * rr);
+  return 1. / 3. * t + 2. / 3. * (uu + dt * rr);
+}
+
 __global__ __launch_bounds__(kBlock) void k_rk(double* u, const double* tn, const double* r, int64_t n, int scheme,
                                                 int stage, double dt, const double* dtv, const int32_t* inst, int K) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (dtv) dt = dtv[inst[i / K]];
-  const double t = tn[i], uu = u[i], rr = r[i];
-  double out;
-  if (scheme == 0 || stage == 0) out = t + dt * rr;
-  else if (stage == 1) out = 3. / 4. * t + 1. / 4. * (uu + dt * rr);
-  else out = 1. / 3. * t + 2. / 3. * (uu + dt * rr);
-  u[i] = out;
+  u[i] = rk_combine(scheme, stage, dt, tn[i], u[i], r[i]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(double* p, int64_t n, double v) {
@@ -256,6 +278,7 @@ struct TermDev {
   const int32_t* fbase;
   const int32_t* fmaxl;
   const int32_t* fmap;
+  const int32_t* lsum;
   int ns, pt1, dd, Tin, flx, lu;
   int sin[kMaxDim];  // strides of the input tensor
   double coef;
@@ -273,7 +296,7 @@ struct TaskDev {  // a term written to an output buffer
 // ascending slot order, of sum_k in[src][idx with dd <- k] * M[s*PF1+k][st*pt1+idx[dd]].
 // Loads of up to four sources are issued before their terms are added in order.
 template <int PF1>
-__device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const int* idx, int D) {
+__device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const int* idx, int D, int cap) {
   int base = 0;
   for (int d = 0; d < D; ++d)
     if (d != t.dd) base += idx[d] * t.sin[d];
@@ -281,8 +304,14 @@ __device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const 
   const int st = t.slot[e];
   const int fb = t.fbase[e];
   const double* Mc = t.M + static_cast<int64_t>(st) * t.ns * PF1 * t.pt1 + idx[t.dd];
+  int maxl = t.fmaxl[e];
+  if (cap >= 0) {  // coarse grid, as in k_sweep
+    const int ls = t.lsum[e];
+    if (ls > cap) return 0.;
+    maxl = min(maxl, cap - ls + slot_level(st));
+  }
   double acc = 0.;
-  for_each_related_ranges(st, t.fmaxl[e], t.flx != 0, t.lu, [&](int lo, int hi) {
+  for_each_related_ranges(st, maxl, t.flx != 0, t.lu, [&](int lo, int hi) {
     for (int s = lo; s < hi; s += 4) {
       int src[4];
 #pragma unroll
@@ -317,7 +346,7 @@ __device__ __forceinline__ void unflatten(int o, const int* sz, int D, int* idx)
 }
 
 template <int PF1>
-__global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntask, int64_t total, int D) {
+__global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntask, int64_t total, int D, int cap) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= total) return;
   int j = 0;
@@ -328,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntas
   const int o = static_cast<int>(local - e * tk.Tout);
   int idx[kMaxDim];
   unflatten(o, tk.osz, D, idx);
-  tk.out[local] = term_value<PF1>(tk.t, e, idx, D);
+  tk.out[local] = term_value<PF1>(tk.t, e, idx, D, cap);
 }
 
 // Last sweep of every eval_up partial sum, accumulated in ordering order into
@@ -339,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntas
 template <int PF1>
 __global__ __launch_bounds__(kBlock) void k_up_final(const TermDev* terms, int nterm, int ntp, int64_t E, int P, int p1,
                                                       int D, double* up, double* fp, const double* vcoef, int kind,
-                                                      double s, int ncomp) {
+                                                      double s, int ncomp, int cap) {
   __shared__ double vals[kBlock];
   const int per_block = kBlock / ntp;
   const int lo = threadIdx.x / ntp, t = threadIdx.x - lo * ntp;
@@ -356,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_up_final(const TermDev* terms, int n
     int sz[kMaxDim], idx[kMaxDim];
     for (int d = 0; d < D; ++d) sz[d] = p1;
     unflatten(q, sz, D, idx);
-    v = term_value<PF1>(terms[t], e, idx, D);
+    v = term_value<PF1>(terms[t], e, idx, D, cap);
   }
   vals[threadIdx.x] = v;
   __syncthreads();
@@ -383,7 +412,8 @@ template <int PFM, int PFK>
 __global__ __launch_bounds__(kBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens, int npen,
                                                        int ntp, int64_t E, int K, int k1, int D, double* rhs,
                                                        const double* u, const double* tn, double* u_next, int scheme,
-                                                       int stage, double dt, const double* dtv, const int32_t* inst) {
+                                                       int stage, double dt, const double* dtv, const int32_t* inst,
+                                                       int cap) {
   __shared__ double vals[kBlock];
   const int per_block = kBlock / ntp;
   const int lo = threadIdx.x / ntp, t = threadIdx.x - lo * ntp;
@@ -400,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_final(const TermDev* terms, int 
     int sz[kMaxDim], idx[kMaxDim];
     for (int d = 0; d < D; ++d) sz[d] = k1;
     unflatten(q, sz, D, idx);
-    v = (t < nterm) ? term_value<PFM>(terms[t], e, idx, D) : term_value<PFK>(pens[t - nterm], e, idx, D);
+    v = (t < nterm) ? term_value<PFM>(terms[t], e, idx, D, cap) : term_value<PFK>(pens[t - nterm], e, idx, D, cap);
   }
   vals[threadIdx.x] = v;
   __syncthreads();
@@ -409,12 +439,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_final(const TermDev* terms, int 
   for (int j = 0; j < nterm + npen; ++j) r += vals[lo * ntp + j];
   rhs[gid] = r;
   if (dtv) dt = dtv[inst[e]];
-  const double tt = tn[gid], uu = u[gid];
-  double out;
-  if (scheme == 0 || stage == 0) out = tt + dt * r;
-  else if (stage == 1) out = 3. / 4. * tt + 1. / 4. * (uu + dt * r);
-  else out = 1. / 3. * tt + 2. / 3. * (uu + dt * r);
-  u_next[gid] = out;
+  u_next[gid] = rk_combine(scheme, stage, dt, tn[gid], u[gid], r);
 }
 
 inline int pow2_at_least(int n) {
@@ -424,15 +449,15 @@ inline int pow2_at_least(int n) {
 }
 
 template <int PF1>
-void launch_phase_t(const TaskDev* tasks, int ntask, int64_t total, int D, hipStream_t s) {
-  hipLaunchKernelGGL((k_phase<PF1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D);
+void launch_phase_t(const TaskDev* tasks, int ntask, int64_t total, int D, int cap, hipStream_t s) {
+  hipLaunchKernelGGL((k_phase<PF1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap);
 }
 
-void launch_phase(int pf1, const TaskDev* tasks, int ntask, int64_t total, int D, hipStream_t s) {
+void launch_phase(int pf1, const TaskDev* tasks, int ntask, int64_t total, int D, int cap, hipStream_t s) {
   switch (pf1) {
-    case 2: launch_phase_t<2>(tasks, ntask, total, D, s); return;
-    case 3: launch_phase_t<3>(tasks, ntask, total, D, s); return;
-    case 4: launch_phase_t<4>(tasks, ntask, total, D, s); return;
+    case 2: launch_phase_t<2>(tasks, ntask, total, D, cap, s); return;
+    case 3: launch_phase_t<3>(tasks, ntask, total, D, cap, s); return;
+    case 4: launch_phase_t<4>(tasks, ntask, total, D, cap, s); return;
   }
   throw std::runtime_error("phase: unsupported basis size");
 }
@@ -641,6 +666,10 @@ void Context::upload_grid() {
     upload(g_fmap[d], fm, stream);
   }
   upload(g_inst, uinst, stream);
+  std::vector<int32_t> ls(n, 0);
+  for (int64_t e = 0; e < n; ++e)
+    for (int d = 0; d < D; ++d) ls[e] += level_of_slot(uslot[e * D + d]);
+  upload(g_lsum, ls, stream);
 }
 
 void Context::ensure_fields() {
@@ -730,6 +759,7 @@ void Context::sweep(const double* in, const int* sin, double* out, double* accum
   SweepArgs a{};
   a.in = in; a.out = out; a.accum = accum; a.M = tab.dev; a.ns = tab.ns;
   a.slot = g_slot[dd].p; a.fbase = g_fbase[dd].p; a.fmaxl = g_fmaxl[dd].p; a.fmap = g_fmap[dd].p;
+  a.lsum = g_lsum.p; a.cap = lsum_cap;
   a.E = E();
   int stin[kMaxDim], stout[kMaxDim];
   stin[D - 1] = 1; stout[D - 1] = 1;
@@ -962,6 +992,7 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     TermDev t{};
     t.in = in; t.M = tab.dev; t.ns = tab.ns; t.pt1 = pt1; t.dd = dd;
     t.slot = g_slot[dd].p; t.fbase = g_fbase[dd].p; t.fmaxl = g_fmaxl[dd].p; t.fmap = g_fmap[dd].p;
+    t.lsum = g_lsum.p;
     t.Tin = prod_of(sin_sz, D);
     strides_of(sin_sz, D, t.sin);
     t.flx = flx ? 1 : 0; t.lu = lu; t.coef = coef;
@@ -1077,7 +1108,7 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   int64_t off = 0;
   for (size_t s = 0; s < pl.up_phase_ntask.size(); ++s) {
     prof_begin(PROF_SWEEP, pl.up_phase_bytes[s]);
-    launch_phase(k + 1, pl.up_tasks.p + off, pl.up_phase_ntask[s], pl.up_phase_total[s], D, stream);
+    launch_phase(k + 1, pl.up_tasks.p + off, pl.up_phase_ntask[s], pl.up_phase_total[s], D, lsum_cap, stream);
     prof_end();
     off += pl.up_phase_ntask[s];
   }
@@ -1088,9 +1119,9 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
     const unsigned nb = static_cast<unsigned>((n + kBlock / ntp - 1) / (kBlock / ntp));
     prof_begin(PROF_FLUX, pl.up_final_bytes);
     switch (k + 1) {
-      case 2: hipLaunchKernelGGL((k_up_final<2>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
-      case 3: hipLaunchKernelGGL((k_up_final<3>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
-      case 4: hipLaunchKernelGGL((k_up_final<4>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp); break;
+      case 2: hipLaunchKernelGGL((k_up_final<2>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
+      case 3: hipLaunchKernelGGL((k_up_final<3>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
+      case 4: hipLaunchKernelGGL((k_up_final<4>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
     }
     prof_end();
   }
@@ -1098,7 +1129,7 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   off = 0;
   for (size_t s = 0; s < pl.rhs_phase_ntask.size(); ++s) {
     prof_begin(PROF_SWEEP, pl.rhs_phase_bytes[s]);
-    launch_phase(m + 1, pl.rhs_tasks.p + off, pl.rhs_phase_ntask[s], pl.rhs_phase_total[s], D, stream);
+    launch_phase(m + 1, pl.rhs_tasks.p + off, pl.rhs_phase_ntask[s], pl.rhs_phase_total[s], D, lsum_cap, stream);
     prof_end();
     off += pl.rhs_phase_ntask[s];
   }
@@ -1112,7 +1143,7 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   if (m + 1 == M1 && k + 1 == K1) {                                                                                \
     hipLaunchKernelGGL((k_rhs_final<M1, K1>), dim3(nb), dim3(kBlock), 0, stream, pl.rhs_terms.p, pl.n_rhs_terms,  \
                        pl.pens.p, pl.n_pen, ntp, E(), K, k + 1, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt,   \
-                       use_inst_dt ? d_inst_dt.p : nullptr, g_inst.p);                                              \
+                       use_inst_dt ? d_inst_dt.p : nullptr, g_inst.p, lsum_cap);                                    \
   }
     ADG_RF(2, 2) ADG_RF(2, 3) ADG_RF(2, 4) ADG_RF(3, 2) ADG_RF(3, 3) ADG_RF(3, 4) ADG_RF(4, 2) ADG_RF(4, 3)
     ADG_RF(4, 4)

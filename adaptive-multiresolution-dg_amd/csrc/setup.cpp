@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <map>
 #include <stdexcept>
 
@@ -428,6 +429,33 @@ SurplusTables::SurplusTables(const Basis1D& lagr) {
       }
     }
   }
+}
+
+std::vector<double> project_1d(const Basis1D& B, const std::function<double(double)>& func) {
+  // linspace(0., 1., 17) accumulates val += 1/16 (include/subs.h:62-73)
+  std::vector<double> xint(17);
+  {
+    const double h = (1. - 0.) / static_cast<double>(17 - 1);
+    double v = 0.;
+    for (double& x : xint) { x = v; v += h; }
+  }
+  std::vector<double> out(B.size());
+  for (int b = 0; b < B.size(); ++b) {
+    int n, j, p;
+    B.decode(b, n, j, p);
+    auto fx = [&](const double& x) { return B.val(b, x, 0, 0) * func(x); };
+    if (n <= 4) {
+      double integral = 0.;
+      for (int i = 0; i < 16; ++i) integral += gl10(fx, xint[i], xint[i + 1]);
+      out[b] = integral;
+    } else {
+      double xl, xr;
+      cell_support(n, j, xl, xr);
+      const double xmid = (xl + xr) / 2.;
+      out[b] = gl10(fx, xl, xmid) + gl10(fx, xmid, xr);
+    }
+  }
+  return out;
 }
 
 std::vector<double> linear_operator_table(const OperatorTables& aa, double coef) {

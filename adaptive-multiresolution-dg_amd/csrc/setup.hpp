@@ -8,7 +8,7 @@
 // reference so that the tables are bit-identical to what it builds:
 //   AlptBasis   source/AlptBasis.cpp:16-130,256-340,473-560
 //   LagrBasis   source/LagrBasis.cpp:9-160,190-420,852-864
-//   Basis       source/Basis.cpp:8-81,121-173
+//   Basis       source/Basis.cpp:8-81,121-173 (product_function :38-59)
 //   Quad::GL_1D source/Quad.cpp:31-80,224-244,140-146
 //   AllBasis    include/AllBasis.h:53-78
 //   OperatorMatrix1D include/OperatorMatrix1D.h:124-200
@@ -17,6 +17,7 @@
 //   Hash        source/Hash.cpp:55-167
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -89,6 +90,12 @@ struct SurplusTables {
   std::vector<double> wt;      // [nslots * p1 * p1]
   explicit SurplusTables(const Basis1D& lagr);
 };
+
+// L2 projection of func onto every basis function of B (AllBasis::projection
+// with Basis::product_function, include/AllBasis.h:100-108,
+// source/Basis.cpp:38-59): 10-point GL over the 16 cells of level 4 for bases
+// of level <= 4, over the two halves of the support otherwise.
+std::vector<double> project_1d(const Basis1D& B, const std::function<double(double)>& func);
 
 // Entries of the assembled const-coefficient advection operator in one
 // dimension: coef*u_vx + coef*(ulft_vjp if coef >= 0 else urgt_vjp), each

@@ -273,6 +273,21 @@ void RK3SSP::step_rk() {
   }
 }
 
+void RK2Midpoint::step_stage(const int stage) {
+  adg_facade_check(dgsolution_ptr->ctx(), adg_rk_stage(dgsolution_ptr->ctx(), ADG_RK2, stage, dt), "RK2Midpoint");
+}
+
+void RK2Midpoint::step_rk() {
+  // RK2Midpoint::step_rk (source/ODESolver.cpp:238-245): u1 = u_n + dt/2 A u_n, u = u_n + dt A u1
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_rk_init(c), "RK2Midpoint::step_rk");
+  for (int s = 0; s < 2; ++s) {
+    adg_facade_check(c, adg_rhs_zero(c), "RK2Midpoint::step_rk");
+    dgoperator_ptr->apply_add();
+    step_stage(s);
+  }
+}
+
 // ---------------------------------------------------------------------------
 
 void OptionsParser::Parse() {

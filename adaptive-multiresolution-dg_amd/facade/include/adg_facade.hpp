@@ -11,7 +11,7 @@
 //   FastMultiplyLU.h (FastLagrIntp, HyperbolicLagrRHS, HyperbolicDiffFluxLagrRHS,
 //   HyperbolicAlptRHS), Interpolation.h (LagrInterpolation, FluxFunction),
 //   BilinearForm.h (BilinearForm, HyperbolicAlpt), ODESolver.h (ODESolver,
-//   ForwardEuler, RK3SSP), Timer.h, Optparser.h.
+//   ForwardEuler, RK2Midpoint, RK3SSP), Timer.h, Optparser.h.
 // Only the members the hyperbolic example drivers use are provided.
 #pragma once
 #include <chrono>
@@ -263,6 +263,16 @@ class RK3SSP : public ODESolver {
   void step_rk() override;
   void step_stage(const int stage) override;
   const int num_stage = 3;
+  const double dt;
+};
+
+class RK2Midpoint : public ODESolver {
+ public:
+  RK2Midpoint(BilinearForm& b, const double dt_) : ODESolver(b), dt(dt_) {}
+  RK2Midpoint(DGSolution& d, const double dt_) : ODESolver(d), dt(dt_) {}
+  void step_rk() override;
+  void step_stage(const int stage) override;
+  const int num_stage = 2;
   const double dt;
 };
 

@@ -18,7 +18,7 @@ TAB_ALPT_AT_LAGR, TAB_LA_UV, TAB_LA_UVX, TAB_LA_FLX = 0, 1, 2, 3
 TAB_AA_UV, TAB_AA_UVX, TAB_AA_ULFT, TAB_AA_URGT, TAB_AA_UJP = 4, 5, 6, 7, 8
 LU_FULL, LU_L, LU_U = 0, 1, 2
 FLUX_BURGERS, FLUX_SINCOS, FLUX_VARCOEF = 0, 1, 2
-RK_EULER, RK3 = 0, 3
+RK_EULER, RK2, RK3 = 0, 2, 3
 
 POINT_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p)
 
@@ -67,6 +67,7 @@ def lib():
             "adg_field_get": (i32, [vp, i32, DP]),
             "adg_sweep": (i32, [vp, i32, i32, i32, i32, I32P, I32P, dbl, DP, DP]),
             "adg_eval_up": (i32, [vp]),
+            "adg_set_level_sum_cap": (i32, [vp, i32]),
             "adg_flux": (i32, [vp, i32, DP]),
             "adg_flux_set": (i32, [vp, i32, DP]),
             "adg_fp_to_coe": (i32, [vp]),
@@ -200,6 +201,10 @@ class Solution:
         out = np.zeros(self.num_elem * int(np.prod(so)))
         self._check(lib().adg_sweep(self._h, table, int(flx), lu, dim, _ip(si), _ip(so), coef, _dp(x), _dp(out)))
         return out
+
+    def set_level_sum_cap(self, cap):
+        """Coarse grid: sweeps ignore elements with level sum above cap (cap < 0: none)."""
+        self._check(lib().adg_set_level_sum_cap(self._h, cap))
 
     def eval_up(self):
         self._check(lib().adg_eval_up(self._h))

@@ -68,8 +68,8 @@ enum {
   ADG_FLUX_VARCOEF = 2   /* f_d = a_d(x)*u, a_d(x) = 1 + s*sin(2*pi*x_{(d+1) mod dim}) */
 };
 
-/* RK schemes (ODESolver.h): forward Euler and SSP-RK3. */
-enum { ADG_RK_EULER = 0, ADG_RK3 = 3 };
+/* RK schemes (ODESolver.h): forward Euler, RK2 midpoint and SSP-RK3. */
+enum { ADG_RK_EULER = 0, ADG_RK2 = 2, ADG_RK3 = 3 };
 
 /* Context: statics of the reference (AlptBasis::PMAX = k, LagrBasis::PMAX = m,
  * LagrBasis::msh_case = 1, Element::DIM = dim, DGSolution NMAX = nmax) plus the
@@ -103,6 +103,18 @@ int adg_field_get(adg_ctx* c, int field, double* host);
 int adg_sweep(adg_ctx* c, int table, int flx, int lu, int dim, const int32_t* size_in,
               const int32_t* size_out, double coef, const double* in, double* out);
 
+/* Coarse grid: the _coarse_grid variants of the reference
+ * (FastMultiplyLU::transform_1D_coarse_grid, source/FastMultiplyLU.cpp:514-594;
+ * FastLagrIntp::eval_up_Lagr_coarse_grid, :1367-1370;
+ * HyperbolicLagrRHS::rhs_vol_scalar_coarse_grid / rhs_flx_intp_scalar_coarse_grid,
+ * :1143-1217; BilinearFormAlpt::assemble_matrix_alpt_coarse_grid,
+ * source/BilinearForm.cpp:89-159).  While cap >= 0, every sweep and operator
+ * (adg_sweep, adg_eval_up, adg_rhs_lagr_*, adg_rhs_penalty, adg_rhs_linear,
+ * adg_stage) ignores elements whose level sum exceeds cap: they are not
+ * sources and their output is zero.  The point flux and the surplus transform
+ * are not affected (as in the reference).  cap < 0 removes the cap. */
+int adg_set_level_sum_cap(adg_ctx* c, int cap);
+
 /* FastLagrIntp::eval_up_Lagr (source/FastMultiplyLU.cpp:1362-1365). */
 int adg_eval_up(adg_ctx* c);
 /* LagrInterpolation::eval_fp_Lag / eval_coe_u_Lag (source/Interplation.cpp:256-295,648-696). */
@@ -126,8 +138,8 @@ int adg_rhs_penalty(adg_ctx* c, const double* alpha);
  * applied matrix-free: rhs += A u for u_t + sum_d coef_d u_{x_d} = 0. */
 int adg_rhs_linear(adg_ctx* c, const double* coef);
 
-/* ExplicitRK::init / RK3SSP::step_stage / ForwardEuler::step_stage
- * (source/ODESolver.cpp:209-213,285-301). */
+/* ExplicitRK::init / RK3SSP::step_stage / RK2Midpoint::step_stage /
+ * ForwardEuler::step_stage (source/ODESolver.cpp:93,209-213,247-259,285-301). */
 int adg_rk_init(adg_ctx* c);
 int adg_rk_stage(adg_ctx* c, int scheme, int stage, double dt);
 /* DGSolution::copy_ucoe_to_predict / copy_predict_to_ucoe (source/DGSolution.cpp:1151-1172). */

@@ -28,7 +28,7 @@
 //   surplus transform         source/Interplation.cpp:1053-1220,1222-1430
 //   adaptive interpolation    source/DGAdaptIntp.cpp:35-45,57-115,184-272; source/Interplation.cpp:736-770,4137-4146
 //   assembled operator        source/BilinearForm.cpp:25-87,691-710,759-769
-//   RK stages                 source/ODESolver.cpp:209-213,285-301
+//   RK stages                 source/ODESolver.cpp:209-213,247-259,285-301
 #include "oracle.h"
 
 #include <omp.h>
@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstdio>
 #include <map>
+#include <numeric>
 #include <set>
 #include <stdexcept>
 #include <unordered_map>
@@ -358,6 +359,14 @@ struct or_ctx {
     for (int d = static_cast<int>(s.size()) - 2; d >= 0; --d) st[d] = st[d + 1] * s[d + 1];
     return st;
   }
+  // Level-sum cap of the _coarse_grid variants (source/FastMultiplyLU.cpp:514-594,
+  // source/BilinearForm.cpp:89-159): elements whose level sum exceeds it are
+  // neither targets (their output stays zero) nor sources.  Negative: no cap.
+  int cap = -1;
+  bool capped(const OElem& e) const {
+    return cap >= 0 && std::accumulate(e.level.begin(), e.level.end(), 0) > cap;
+  }
+
   void transform_1D(const double* M, int cols, int lu, bool flx, int dd, int pf, int ptd, double coef,
                     const std::vector<int>& sf, const std::vector<int>& st) {
     const std::vector<int> strf = strides(sf);
@@ -373,11 +382,12 @@ struct or_ctx {
           const std::vector<OElem*>& rel = flx ? e.flx[dd] : e.vol[dd];
           e.to.assign(nto, 0.);
           std::vector<int> idx(D, 0);
-          for (int t = 0; t < nto; ++t) {
+          for (int t = 0; t < nto && !capped(e); ++t) {
             const int ord_to = e.ord[dd] * (ptd + 1) + idx[dd];
             int base = 0;
             for (int d = 0; d < D; ++d) if (d != dd) base += idx[d] * strf[d];
             for (OElem* s : rel) {
+              if (capped(*s)) continue;
               if (lu == adg_lu_L && e.level[dd] >= s->level[dd]) continue;
               if (lu == adg_lu_U && e.level[dd] < s->level[dd]) continue;
               int off = base;
@@ -389,7 +399,8 @@ struct or_ctx {
             }
             for (int d = D - 1; d >= 0; --d) { if (++idx[d] < st[d]) break; idx[d] = 0; }
           }
-          for (double& v : e.to) v *= coef;
+          if (!capped(e))
+            for (double& v : e.to) v *= coef;
         }
       }
     }
@@ -728,6 +739,7 @@ void or_sweep(or_ctx* c, int table, int flx, int lu, int dim, const int32_t* siz
 }
 
 void or_eval_up(or_ctx* c) { c->eval_up(); }
+void or_set_level_sum_cap(or_ctx* c, int cap) { c->cap = cap; }
 void or_flux(or_ctx* c, int kind, const double* params) { c->flux(kind, params); }
 void or_fp_to_coe(or_ctx* c) { c->fp_to_coe(); }
 void or_rhs_zero(or_ctx* c) { for (auto& it : c->dg) std::fill(it.second.rhs.begin(), it.second.rhs.end(), 0.); }
@@ -756,8 +768,10 @@ void or_rhs_linear_assembled(or_ctx* c, const double* coef) {
   auto assemble = [&](double cf, int dim, const std::vector<double>& M, bool flx) {
     std::map<std::pair<int64_t, int64_t>, double> local;
     for (OElem* t : v) {
+      if (c->capped(*t)) continue;
       const auto& rel = flx ? t->flx[dim] : t->vol[dim];
       for (OElem* s : rel) {
+        if (c->capped(*s)) continue;
         std::vector<int> it(D, 0);
         for (int ti = 0; ti < K; ++ti) {
           std::vector<int> is(D, 0);
@@ -801,7 +815,8 @@ void or_rk_stage(or_ctx* c, int scheme, int stage, double dt) {
     for (int i = 0; i < c->K; ++i) {
       const double tn = e.tn[i], u = e.ucoe[i], r = e.rhs[i];
       double out;
-      if (scheme == 0 || stage == 0) out = tn + dt * r;
+      if (scheme == 2) out = (stage == 0) ? tn + 0.5 * dt * r : tn + dt * r;  // RK2Midpoint, ODESolver.cpp:247-259
+      else if (scheme == 0 || stage == 0) out = tn + dt * r;
       else if (stage == 1) out = 3. / 4. * tn + 1. / 4. * (u + dt * r);
       else out = 1. / 3. * tn + 2. / 3. * (u + dt * r);
       e.ucoe[i] = out;

@@ -32,6 +32,9 @@ void or_sweep(or_ctx* c, int table, int flx, int lu, int dim, const int32_t* siz
               const int32_t* size_out, double coef, const double* in, double* out);
 
 void or_eval_up(or_ctx* c);                                   /* FastLagrIntp::eval_up_Lagr */
+/* _coarse_grid variants: sweeps and assembled operators skip elements whose
+ * level sum exceeds cap (negative: no cap). */
+void or_set_level_sum_cap(or_ctx* c, int cap);
 /* flux at points: kind 0 burgers u^2/2 (all dims), 1 sin/cos rotating wave scaled by p[0],
  * 2 var-coeff a_d(x)*u with a_d = 1 + p[0]*sin(2*pi*x_{(d+1)%dim}) */
 void or_flux(or_ctx* c, int kind, const double* params);

@@ -33,6 +33,7 @@ def lib():
             "or_set_field": (None, [vp, i32, P(dbl)]),
             "or_sweep": (None, [vp, i32, i32, i32, i32, P(ctypes.c_int32), P(ctypes.c_int32), dbl, P(dbl), P(dbl)]),
             "or_eval_up": (None, [vp]),
+            "or_set_level_sum_cap": (None, [vp, ctypes.c_int]),
             "or_flux": (None, [vp, i32, P(dbl)]),
             "or_fp_to_coe": (None, [vp]),
             "or_rhs_zero": (None, [vp]),
@@ -137,6 +138,9 @@ class Oracle:
 
     def eval_up(self):
         lib().or_eval_up(self._h)
+
+    def set_level_sum_cap(self, cap):
+        lib().or_set_level_sum_cap(self._h, cap)
 
     def flux(self, kind, params=(0.0,)):
         p = np.asarray(params, np.float64)

@@ -10,8 +10,8 @@ import pytest
 
 import pyadg
 import pyoracle
-from common import (first_diff, ic_params, init_both, oracle_run_hyperbolic, oracle_run_linear, problem,
-                    same_grid)
+from common import (first_diff, ic_params, init_both, oracle_hyperbolic_rhs, oracle_run_hyperbolic, oracle_run_linear,
+                    problem, same_grid)
 
 pytestmark = pytest.mark.gpu
 
@@ -215,3 +215,51 @@ def test_fused_adaptive_burgers_run_bitexact():
     assert same_grid(g, o), "element sets differ after adaptive steps"
     a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
     assert np.array_equal(a, b), first_diff(a, b)
+
+
+def test_rk2_midpoint_stages_bitexact():
+    # RK2Midpoint::step_stage (source/ODESolver.cpp:247-259) on the per-sweep and the fused path
+    p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.08, (0.3, 0.6))])
+    g, o = adapted_pair(2, 2, 3, 6, 4, p, 1e-4)
+    f = pyadg.Solution(2, 2, 3, 6, True, 4)
+    f.init_adaptive_intp_builtin(p, 1e-4)
+    prob = problem(flux_kind=0, alpha=[1.2, 1.2], wave_speed=[1.0, 1.0], cfl=0.2, t_final=1.0)
+    g.rk_init(); o.rk_init(); f.rk_init()
+    for st in range(2):
+        g.eval_up(); g.flux(0); g.fp_to_coe(); g.rhs_zero(); g.rhs_lagr_vol(); g.rhs_lagr_flx()
+        g.rhs_penalty([1.2, 1.2]); g.rk_stage(pyadg.RK2, st, 2e-3)
+        oracle_hyperbolic_rhs(o, prob); o.rk_stage(2, st, 2e-3)
+        f.stage(prob, pyadg.RK2, st, 2e-3)
+        a, b, c = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE), f.get(pyadg.FIELD_UCOE)
+        assert np.array_equal(a, b), f"rk2 stage {st}: " + first_diff(a, b)
+        assert np.array_equal(c, b), f"fused rk2 stage {st}: " + first_diff(c, b)
+
+
+@pytest.mark.parametrize("dim,k,m,nmax,cap", [(2, 2, 3, 6, 4), (3, 1, 2, 4, 3)])
+def test_coarse_grid_cap_bitexact(dim, k, m, nmax, cap):
+    # the _coarse_grid variants: per-sweep path and fused stage against the oracle
+    p = ic_params(1, shift=0.1)
+    g, o = pair(dim, k, m, nmax)
+    init_both(g, o, p, 1e10)
+    f = pyadg.Solution(dim, k, m, nmax, True, nmax)
+    f.init_adaptive_intp_builtin(p, 1e10)
+    for s in (g, o, f):
+        s.set_level_sum_cap(cap)
+    g.eval_up(); o.eval_up()
+    a, b = g.get(pyadg.FIELD_UP), o.get(pyoracle.FIELD_UP)
+    assert np.array_equal(a, b), "eval_up: " + first_diff(a, b)
+    prob = problem(flux_kind=2, flux_param=0.5, alpha=[1.5] * dim, wave_speed=[1.5] * dim, cfl=0.2, t_final=1.0)
+    g.rk_init(); o.rk_init(); f.rk_init()
+    for st in range(2):
+        g.eval_up(); g.flux(2, (0.5,)); g.fp_to_coe(); g.rhs_zero(); g.rhs_lagr_vol(); g.rhs_lagr_flx()
+        g.rhs_penalty([1.5] * dim); g.rk_stage(pyadg.RK2, st, 1e-3)
+        oracle_hyperbolic_rhs(o, prob); o.rk_stage(2, st, 1e-3)
+        f.stage(prob, pyadg.RK2, st, 1e-3)
+        a, b, c = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE), f.get(pyadg.FIELD_UCOE)
+        assert np.array_equal(a, b), f"stage {st}: " + first_diff(a, b)
+        assert np.array_equal(c, b), f"fused stage {st}: " + first_diff(c, b)
+    coef = [1.0, -0.5, 0.75][:dim]
+    g.rhs_zero(); o.rhs_zero()
+    g.rhs_linear(coef); o.rhs_linear(coef)
+    a, b = g.get(pyadg.FIELD_RHS), o.get(pyoracle.FIELD_RHS)
+    assert np.array_equal(a, b), "linear: " + first_diff(a, b)

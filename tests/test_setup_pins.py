@@ -351,3 +351,25 @@ def test_linear_advection_convergence():
         errs.append(o.l2_error_split(ex))
     orders = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
     assert all(o_ > 1.8 for o_ in orders), (errs, orders)
+
+
+@pytest.mark.parametrize("scheme,nstage,order_expected", [(0, 1, 1), (2, 2, 2), (3, 3, 3)])
+def test_rk_temporal_order(scheme, nstage, order_expected):
+    # Fixed grid, dt, dt/2, dt/4: successive differences shrink by 2^order
+    # (forward Euler 1, RK2 midpoint 2, SSP-RK3 3), which a wrong stage
+    # coefficient breaks.  The linear operator is the one pinned above.
+    n, T = 3, 0.05
+    sols = []
+    for nsteps in (20, 40, 80):
+        o = pyoracle.Oracle(2, 2, 3, n, True, n)
+        o.init_adaptive_intp(lambda x: math.sin(2 * PI * (x[0] + x[1])), 1e10)
+        dt = T / nsteps
+        for _ in range(nsteps):
+            o.rk_init()
+            for st in range(nstage):
+                o.rhs_zero()
+                o.rhs_linear([1.0, 1.0])
+                o.rk_stage(scheme, st, dt)
+        sols.append(o.get(pyoracle.FIELD_UCOE))
+    r = np.linalg.norm(sols[0] - sols[1]) / np.linalg.norm(sols[1] - sols[2])
+    assert abs(math.log2(r) - order_expected) < 0.15, (r, math.log2(r))
