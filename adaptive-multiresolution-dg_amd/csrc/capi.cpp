@@ -217,6 +217,22 @@ int adg_sweep(adg_ctx* h, int table, int flx, int lu, int dim, const int32_t* si
   });
 }
 
+int adg_lagr_points(adg_ctx* h, double* x) {
+  return guard(h, [&](adg::Context& c) {
+    if (!x) throw std::runtime_error("null output");
+    const int p1 = c.m + 1;
+    for (int64_t e = 0; e < c.E(); ++e)
+      for (int q = 0; q < c.P; ++q) {
+        int rem = q;
+        for (int d = c.D - 1; d >= 0; --d) {
+          const int pd = rem % p1;
+          rem /= p1;
+          x[(e * c.P + q) * c.D + d] = c.lagr.point(c.uslot[e * c.D + d] * p1 + pd);
+        }
+      }
+  });
+}
+
 int adg_set_level_sum_cap(adg_ctx* h, int cap) {
   return guard(h, [&](adg::Context& c) { c.lsum_cap = cap < 0 ? -1 : cap; });
 }
@@ -308,6 +324,130 @@ double adg_ic_eval(const double* x, int dim, void* user) {
     }
   }
   return 0.;
+}
+
+int adg_init_separable(adg_ctx* h, adg_sep_fn f, void* user, int nfunc, double eps) {
+  return guard(h, [&](adg::Context& c) {
+    if (!f) throw std::runtime_error("null separable function");
+    c.init_separable(f, user, nfunc, eps);
+  });
+}
+
+double adg_sep_eval(double x, int d, int ifunc, void* user) {
+  const adg_ic_params* p = static_cast<const adg_ic_params*>(user);
+  const double pi = adg::PI;
+  if (p->kind == 0) {
+    if (d == 0) return ifunc == 0 ? std::sin(2. * pi * (x - p->shift)) : std::cos(2. * pi * (x - p->shift));
+    return ifunc == 0 ? std::cos(2. * pi * x) : std::sin(2. * pi * x);
+  }
+  return std::sin(2. * pi * (x - p->shift));
+}
+
+int adg_filter(adg_ctx* h, double damp_coef, const double* wave_speed, double dt, int start_level_sum) {
+  return guard(h, [&](adg::Context& c) {
+    if (!wave_speed) throw std::runtime_error("null wave speed");
+    std::vector<double> sp(static_cast<size_t>(c.E()) * c.D);
+    for (int64_t e = 0; e < c.E(); ++e)
+      for (int d = 0; d < c.D; ++d) sp[e * c.D + d] = wave_speed[d];
+    c.filter(damp_coef, sp.data(), dt, start_level_sum);
+  });
+}
+
+int adg_filter_local(adg_ctx* h, double damp_coef, adg_coef_fn speed, void* user, double dt, int start_level_sum) {
+  return guard(h, [&](adg::Context& c) {
+    if (!speed) throw std::runtime_error("null speed function");
+    if (c.D != 2 && c.D != 3) throw std::runtime_error("filter_local() supports dim 2 and 3");
+    std::vector<double> sp(static_cast<size_t>(c.E()) * c.D, 0.);
+    const int nc = 1 << c.D;
+    for (int64_t e = 0; e < c.E(); ++e) {
+      double xl[3], xr[3];
+      for (int d = 0; d < c.D; ++d) {
+        const int s = c.uslot[e * c.D + d];
+        adg::cell_support(adg::level_of_slot(s), adg::suppt_of_slot(s), xl[d], xr[d]);
+      }
+      for (int corner = 0; corner < nc; ++corner) {  // corners in the reference's nested-loop order
+        double x[3];
+        for (int d = 0; d < c.D; ++d) x[d] = ((corner >> (c.D - 1 - d)) & 1) ? xr[d] : xl[d];
+        for (int d = 0; d < c.D; ++d)
+          sp[e * c.D + d] = std::max(std::abs(speed(x, d, user)), sp[e * c.D + d]);
+      }
+    }
+    c.filter(damp_coef, sp.data(), dt, start_level_sum);
+  });
+}
+
+int adg_eval_tensor(adg_ctx* h, int nx, const double* x1d, double* out) {
+  return guard(h, [&](adg::Context& c) {
+    if (!x1d || !out) throw std::runtime_error("null coordinates or output");
+    c.eval_tensor(nx, x1d, out);
+  });
+}
+
+int adg_error_no_separable(adg_ctx* h, adg_point_fn f, void* user, int gauss_points, double* err) {
+  return guard(h, [&](adg::Context& c) {
+    if (!f || !err) throw std::runtime_error("null function or output");
+    if (gauss_points < 1 || gauss_points > 10) throw std::runtime_error("gauss_points must be in 1..10");
+    int lmax = 0;
+    for (int64_t e = 0; e < c.E(); ++e)
+      for (int d = 0; d < c.D; ++d) lmax = std::max(lmax, c.level(e, d));
+    const int N = 1 << lmax;
+    const double h = (1. - 0.) / N;
+    const std::vector<double>& qx = adg::gl_nodes(gauss_points);
+    const std::vector<double>& qw = adg::gl_weights(gauss_points);
+    const int g = gauss_points;
+    // Quad::linear_map(x, -1, 1, tl, tr) of every 1D node of every cell
+    std::vector<double> x1d(static_cast<size_t>(N) * g);
+    for (int i = 0; i < N; ++i) {
+      const double tl = i * h, tr = (i + 1) * h;
+      for (int q = 0; q < g; ++q) x1d[i * g + q] = (qx[q] - (-1.)) / (1. - (-1.)) * (tr - tl) + tl;
+    }
+    const int nx = N * g;
+    int64_t npts = 1;
+    for (int d = 0; d < c.D; ++d) npts *= nx;
+    std::vector<double> uh(static_cast<size_t>(npts));
+    c.eval_tensor(nx, x1d.data(), uh.data());
+    // cells, then the points of a cell, in IterativeNestedLoop order (last index fastest)
+    const int D = c.D;
+    int64_t ncell = 1, nq = 1;
+    for (int d = 0; d < D; ++d) { ncell *= N; nq *= g; }
+    std::vector<double> qwm(nq);
+    for (int64_t q = 0; q < nq; ++q) {
+      double w = 1;
+      int64_t r = q;
+      std::vector<int> qi(D);
+      for (int d = D - 1; d >= 0; --d) { qi[d] = static_cast<int>(r % g); r /= g; }
+      for (int d = 0; d < D; ++d) w *= qw[qi[d]];
+      qwm[q] = w;
+    }
+    double L1 = 0, L2 = 0, Linf = 0;
+    std::vector<double> x(D), av(nq);
+    std::vector<int> ci(D), qi(D);
+    for (int64_t cell = 0; cell < ncell; ++cell) {
+      int64_t r = cell;
+      for (int d = D - 1; d >= 0; --d) { ci[d] = static_cast<int>(r % N); r /= N; }
+      for (int64_t q = 0; q < nq; ++q) {
+        int64_t rq = q;
+        for (int d = D - 1; d >= 0; --d) { qi[d] = static_cast<int>(rq % g); rq /= g; }
+        int64_t idx = 0;
+        for (int d = 0; d < D; ++d) {
+          const int ix = ci[d] * g + qi[d];
+          x[d] = x1d[ix];
+          idx = idx * nx + ix;
+        }
+        av[q] = std::abs(std::abs(uh[idx] - f(x.data(), D, user)));
+      }
+      for (int64_t q = 0; q < nq; ++q) {
+        L1 += qwm[q] * av[q];
+        L2 += qwm[q] * av[q] * av[q];
+        Linf = std::max(Linf, av[q]);
+      }
+    }
+    L1 *= std::pow(h / 2., D);
+    L2 = std::pow(L2 * std::pow(h / 2., D), 0.5);
+    err[0] = L1;
+    err[1] = L2;
+    err[2] = Linf;
+  });
 }
 
 int adg_run_linear(adg_ctx* h, const double* coef, double dt, int64_t nsteps, adg_run_stats* st) {

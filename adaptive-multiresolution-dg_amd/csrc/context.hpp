@@ -135,6 +135,15 @@ class Context {
   int64_t coarsen_instances(const std::vector<double>& eta);
   void init_adaptive_intp(double (*f)(const double*, int, void*), void* user, double eps);
   void init_adaptive_intp_batch(double (*f)(const double*, int, void*), void* const* users, double eps);
+  void init_separable(double (*f)(double, int, int, void*), void* user, int nfunc, double eps);
+  // u at every point of the tensor grid x1d^D (first dimension slowest)
+  void eval_tensor(int nx, const double* x1d, double* out_host);
+  DevBuf<uint64_t> eval_hkey;
+  DevBuf<int32_t> eval_hval, eval_ncand, eval_cslot;
+  DevBuf<double> eval_cval, eval_out;
+  uint64_t eval_hmask = 0;
+  int64_t eval_version = -1;
+  void filter(double damp, const double* speed, double dt, int start);
   void upload_inst_dt();
   void check_error_flag();
 

@@ -211,6 +211,12 @@ __global__ __launch_bounds__(kBlock) void k_fill(double* p, int64_t n, double v)
   if (i < n) p[i] = v;
 }
 
+// u *= f elementwise (the filter's per-element damping factors, expanded per DoF).
+__global__ __launch_bounds__(kBlock) void k_scale(double* u, const double* f, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) u[i] *= f[i];
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy(double* dst, const double* src, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
@@ -256,6 +262,80 @@ __global__ __launch_bounds__(kBlock) void k_lagr_norm(LagrNormArgs a) {
     norm += val;
   }
   a.out[e] = norm;
+}
+
+// ---------------------------------------------------------------------------
+// Point values of the DG solution on a tensor grid of coordinates
+// (DGSolution::val, source/DGSolution.cpp:613-625 with Element::val,
+// source/Element.cpp:123-147).  For every 1D coordinate the host lists the
+// cells whose closed support contains it, level by level, with the k+1 basis
+// values there; a point visits every combination of its per-dimension cells,
+// finds the element through an open-addressing table keyed by the packed
+// slots, and adds c * prod_d v_d over the element's coefficients.
+struct EvalArgs {
+  const double* u;
+  const uint64_t* hkey;   // table keys (kEmptyKey: free)
+  const int32_t* hval;    // element index of each key
+  uint64_t hmask;
+  const int32_t* ncand;   // [nx] candidates of each 1D coordinate
+  const int32_t* cslot;   // [nx * cm]
+  const double* cval;     // [nx * cm * k1]
+  double* out;            // [nx^D]
+  int64_t npts;
+  int nx, cm, k1, D, K;
+};
+
+constexpr uint64_t kEmptyKey = ~uint64_t{0};
+
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void k_eval_tensor(EvalArgs a) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= a.npts) return;
+  int ix[kMaxDim], nc[kMaxDim], c[kMaxDim];
+  int64_t rem = gid;
+  for (int d = a.D - 1; d >= 0; --d) {
+    ix[d] = static_cast<int>(rem % a.nx);
+    rem /= a.nx;
+    nc[d] = a.ncand[ix[d]];
+    c[d] = 0;
+  }
+  double total = 0.;
+  for (int d = 0; d < a.D; ++d)
+    if (nc[d] == 0) { a.out[gid] = 0.; return; }
+  while (true) {
+    uint64_t key = 0;
+    for (int d = 0; d < a.D; ++d) key |= static_cast<uint64_t>(a.cslot[ix[d] * a.cm + c[d]]) << (16 * d);
+    uint64_t h = mix64(key) & a.hmask;
+    int32_t e = -1;
+    while (a.hkey[h] != kEmptyKey) {
+      if (a.hkey[h] == key) { e = a.hval[h]; break; }
+      h = (h + 1) & a.hmask;
+    }
+    if (e >= 0) {
+      double value = 0.;
+      for (int q = 0; q < a.K; ++q) {
+        double val_bas = a.u[static_cast<int64_t>(e) * a.K + q];
+        int r = q, stride = a.K;
+        for (int d = 0; d < a.D; ++d) {
+          stride /= a.k1;
+          const int p = r / stride;
+          r -= p * stride;
+          val_bas *= a.cval[(ix[d] * a.cm + c[d]) * a.k1 + p];
+        }
+        value += val_bas;
+      }
+      total += value;
+    }
+    int d = a.D - 1;
+    while (d >= 0 && ++c[d] == nc[d]) { c[d] = 0; --d; }
+    if (d < 0) break;
+  }
+  a.out[gid] = total;
 }
 
 __global__ __launch_bounds__(kBlock) void k_remap(double* dst, const double* src, const int32_t* old_of_new, int64_t E,
@@ -1237,6 +1317,12 @@ void Context::init_adaptive_intp(double (*f)(const double*, int, void*), void* u
 }
 
 void Context::init_adaptive_intp_batch(double (*f)(const double*, int, void*), void* const* users, double eps) {
+  // initialisation is never restricted to a coarse grid
+  struct CapGuard {
+    int& c; int saved;
+    explicit CapGuard(int& cap) : c(cap), saved(cap) { c = -1; }
+    ~CapGuard() { c = saved; }
+  } cap_guard(lsum_cap);
   if (up_host.size() != static_cast<size_t>(E()) * P) {
     up_host.assign(static_cast<size_t>(E()) * P, 0.);
     new_add.assign(E(), 1);
@@ -1289,6 +1375,149 @@ void Context::init_adaptive_intp_batch(double (*f)(const double*, int, void*), v
   int flx[kMaxDim];
   for (int d = 0; d < D; ++d) { tabs[d] = table(1); flx[d] = 0; }
   partial_sums(tabs, flx, m, k, 1.0, uintp.p, u.p);
+  sync();
+}
+
+// DGAdapt::init_separable_scalar_sum (source/DGAdapt.cpp:120-147): project
+// sum_i prod_d f_i(x_d) onto the grid, then refine every leaf whose
+// coefficient norm exceeds eps with separably initialised children
+// (refine_init_separable_scalar_sum, :850-894) until no leaf does.
+void Context::init_separable(double (*f)(double, int, int, void*), void* user, int nfunc, double eps) {
+  if (B() != 1) throw std::runtime_error("separable initialisation: one instance expected");
+  if (nfunc < 1) throw std::runtime_error("separable initialisation: at least one function expected");
+  // DGSolution::seperable_project (source/DGSolution.cpp:150-167)
+  std::vector<std::vector<double>> c1(static_cast<size_t>(nfunc) * D);
+  for (int i = 0; i < nfunc; ++i)
+    for (int d = 0; d < D; ++d) c1[i * D + d] = project_1d(alpt, [&](double x) { return f(x, d, i, user); });
+  // DGSolution::init_elem_separable (source/DGSolution.cpp:169-184), functions
+  // in order, added to the coefficients the elements hold (new ones hold zero)
+  const HostGrid g0 = grids[0];
+  std::vector<double> u0(static_cast<size_t>(g0.E) * K);
+  if (g0.E > 0)
+    hip_check(hipMemcpyAsync(u0.data(), u.p, u0.size() * sizeof(double), hipMemcpyDeviceToHost, stream), "download");
+  sync();
+  std::vector<double> uh;
+  auto coefficients = [&](const HostGrid& g) {
+    const std::vector<int32_t> old = map_new_to_old(g0, g);
+    uh.assign(static_cast<size_t>(g.E) * K, 0.);
+    for (int64_t e = 0; e < g.E; ++e)
+      if (old[e] >= 0) std::copy(&u0[static_cast<size_t>(old[e]) * K], &u0[static_cast<size_t>(old[e]) * K] + K, &uh[e * K]);
+    for (int64_t e = 0; e < g.E; ++e)
+      for (int q = 0; q < K; ++q) {
+        int idx[kMaxDim];
+        int rem = q;
+        for (int d = D - 1; d >= 0; --d) { idx[d] = rem % (k + 1); rem /= (k + 1); }
+        for (int i = 0; i < nfunc; ++i) {
+          double coe = 1.;
+          for (int d = 0; d < D; ++d) coe *= c1[i * D + d][g.slot[e * D + d] * (k + 1) + idx[d]];
+          uh[e * K + q] += coe;
+        }
+      }
+  };
+  while (true) {
+    coefficients(grids[0]);
+    std::vector<double> norm(grids[0].E);
+    for (int64_t e = 0; e < grids[0].E; ++e) {  // VecMultiD::norm(2.)
+      double sq = 0.;
+      for (int q = 0; q < K; ++q) sq += std::pow(uh[e * K + q], 2.);
+      norm[e] = std::pow(sq, 1. / 2.);
+    }
+    int64_t added = 0;
+    std::vector<int32_t> ns = refine_slots(grids[0], norm, eps, added);
+    if (added == 0) break;
+    set_grid_slots(ns, false);
+  }
+  set_grid_slots(grids[0].slot, false);
+  upload(u, uh, stream);
+  sync();
+}
+
+// DGAdapt::filter / filter_local (source/DGAdapt.cpp:461-555): every element
+// with level sum >= start is scaled by exp(-c (sum_d a_d 2^l_d)^2),
+// c = damp * dt^2/2 * pi^2; speed[e*D+d] is a_d for element e.
+void Context::filter(double damp, const double* speed, double dt, int start) {
+  const double coefficient = damp * (dt * dt / 2.0) * (M_PI * M_PI);
+  std::vector<double> fac(E(), 1.);
+  std::vector<char> on(E(), 0);
+  for (int64_t e = 0; e < E(); ++e) {
+    int ls = 0;
+    for (int d = 0; d < D; ++d) ls += level(e, d);
+    if (ls < start) continue;
+    double index_sum = 0.0;
+    for (int d = 0; d < D; ++d) index_sum += speed[e * D + d] * std::pow(2.0, level(e, d));
+    fac[e] = std::exp(-coefficient * index_sum * index_sum);
+    on[e] = 1;
+  }
+  std::vector<double> f(static_cast<size_t>(E()) * K);
+  for (int64_t e = 0; e < E(); ++e)
+    for (int q = 0; q < K; ++q) f[e * K + q] = on[e] ? fac[e] : 1.;
+  ensure_fields();  // tmpA holds at least E*K values, so the upload below keeps it
+  upload(tmpA, f, stream);
+  if (E() > 0) hipLaunchKernelGGL(k_scale, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, tmpA.p, E() * K);
+}
+
+void Context::eval_tensor(int nx, const double* x1d, double* out_host) {
+  if (B() != 1) throw std::runtime_error("point evaluation: one instance expected");
+  if (nx < 1) throw std::runtime_error("point evaluation: empty coordinate set");
+  int64_t npts = 1;
+  for (int d = 0; d < D; ++d) {
+    npts *= nx;
+    if (npts > (int64_t{1} << 34)) throw std::runtime_error("point evaluation: grid too large");
+  }
+  // slot -> element table of the current grid
+  if (eval_version != grid_version) {
+    uint64_t cap = 2;
+    while (cap < 2 * static_cast<uint64_t>(std::max<int64_t>(E(), 1))) cap <<= 1;
+    std::vector<uint64_t> hk_(cap, kEmptyKey);
+    std::vector<int32_t> hv(cap, -1);
+    for (int64_t e = 0; e < E(); ++e) {
+      uint64_t key = 0;
+      for (int d = 0; d < D; ++d) key |= static_cast<uint64_t>(uslot[e * D + d]) << (16 * d);
+      uint64_t h = mix64(key) & (cap - 1);
+      while (hk_[h] != kEmptyKey) h = (h + 1) & (cap - 1);
+      hk_[h] = key;
+      hv[h] = static_cast<int32_t>(e);
+    }
+    upload(eval_hkey, hk_, stream);
+    upload(eval_hval, hv, stream);
+    eval_hmask = cap - 1;
+    eval_version = grid_version;
+  }
+  // cells whose closed support holds each coordinate, with the basis values there
+  const int k1 = k + 1, cm = 2 * (nmax + 1);
+  std::vector<int32_t> ncand(nx, 0), cslot(static_cast<size_t>(nx) * cm, 0);
+  std::vector<double> cval(static_cast<size_t>(nx) * cm * k1, 0.);
+  for (int i = 0; i < nx; ++i) {
+    const double x = x1d[i];
+    int n = 0;
+    auto add = [&](int level, int suppt) {
+      double xl, xr;
+      cell_support(level, suppt, xl, xr);
+      if (x < xl || x > xr) return;
+      const int sl = slot_of(level, suppt);
+      cslot[i * cm + n] = sl;
+      for (int p = 0; p < k1; ++p) cval[(static_cast<size_t>(i) * cm + n) * k1 + p] = alpt.val(sl * k1 + p, x, 0, 0);
+      ++n;
+    };
+    for (int l = 0; l <= nmax; ++l) {
+      if (l <= 1) { add(l, 1); continue; }
+      const int ncell = 1 << (l - 1);
+      const int c0 = static_cast<int>(std::floor(x * ncell));
+      for (int cc = c0 - 1; cc <= c0 + 1; ++cc)
+        if (cc >= 0 && cc < ncell) add(l, 2 * cc + 1);
+    }
+    ncand[i] = n;
+  }
+  upload(eval_ncand, ncand, stream);
+  upload(eval_cslot, cslot, stream);
+  upload(eval_cval, cval, stream);
+  eval_out.ensure(static_cast<size_t>(npts));
+  EvalArgs a{};
+  a.u = u.p; a.hkey = eval_hkey.p; a.hval = eval_hval.p; a.hmask = eval_hmask;
+  a.ncand = eval_ncand.p; a.cslot = eval_cslot.p; a.cval = eval_cval.p; a.out = eval_out.p;
+  a.npts = npts; a.nx = nx; a.cm = cm; a.k1 = k1; a.D = D; a.K = K;
+  hipLaunchKernelGGL(k_eval_tensor, dim3(nblocks(npts)), dim3(kBlock), 0, stream, a);
+  hip_check(hipMemcpyAsync(out_host, eval_out.p, npts * sizeof(double), hipMemcpyDeviceToHost, stream), "download");
   sync();
 }
 

@@ -6,6 +6,8 @@
 
 #include "adg_facade.hpp"
 
+#include <fstream>
+
 int AlptBasis::PMAX = 0;
 int LagrBasis::PMAX = 0;
 int LagrBasis::msh_case = 1;
@@ -102,6 +104,122 @@ std::unordered_map<int, std::vector<double>> DGSolution::coefficients_by_key() c
   return out;
 }
 
+namespace {
+double call_exact(const double* x, int dim, void* user) {
+  auto* f = static_cast<std::function<double(std::vector<double>)>*>(user);
+  return (*f)(std::vector<double>(x, x + dim));
+}
+}  // namespace
+
+std::vector<double> DGSolution::get_error_no_separable_scalar(std::function<double(std::vector<double>)> func,
+                                                              const int gauss_points) const {
+  std::vector<double> err(3);
+  adg_facade_check(ctx_, adg_error_no_separable(ctx_, call_exact, &func, gauss_points, err.data()),
+                   "get_error_no_separable_scalar");
+  return err;
+}
+
+std::vector<double> DGSolution::val_tensor(const std::vector<double>& x1d) const {
+  size_t n = 1;
+  for (int d = 0; d < DIM; ++d) n *= x1d.size();
+  std::vector<double> out(n);
+  adg_facade_check(ctx_, adg_eval_tensor(ctx_, static_cast<int>(x1d.size()), x1d.data(), out.data()), "DGSolution::val");
+  return out;
+}
+
+std::vector<double> DGSolution::val(const std::vector<double>& x, const std::vector<int>& derivative) const {
+  for (int d : derivative)
+    if (d != 0) {
+      std::cout << "DGSolution::val: only point values (zero derivatives) are provided" << std::endl;
+      std::exit(1);
+    }
+  // the point is the tensor grid of one coordinate per dimension only when all are equal;
+  // evaluate on the grid {x_0, ..., x_{D-1}}^D and pick the entry (x_0, ..., x_{D-1})
+  const std::vector<double> grid = val_tensor(x);
+  size_t idx = 0;
+  for (int d = 0; d < DIM; ++d) idx = idx * x.size() + d;
+  return {grid[idx]};
+}
+
+void DGSolution::elements(std::vector<int>& level, std::vector<int>& suppt) const {
+  const int E = size_elem();
+  level.resize(static_cast<size_t>(E) * DIM);
+  suppt.resize(static_cast<size_t>(E) * DIM);
+  std::vector<int32_t> key(E), lv(level.size()), sp(suppt.size());
+  adg_facade_check(ctx_, adg_grid_get(ctx_, lv.data(), sp.data(), key.data()), "DGSolution::elements");
+  std::copy(lv.begin(), lv.end(), level.begin());
+  std::copy(sp.begin(), sp.end(), suppt.begin());
+}
+
+IO::IO(DGSolution& dgsolution)
+    : dgsolution_ptr(&dgsolution), max_mesh(dgsolution.nmax_level()), num_grid_point(1 << max_mesh),
+      dx(1. / num_grid_point), x_1D(num_grid_point) {
+  for (int i = 0; i < num_grid_point; i++) x_1D[i] = dx * (i + 0.5);
+}
+
+void IO::output_num(const std::string& file_name) const {
+  // IO::output_num_1D/2D/3D (source/IO.cpp:168-186,279-297,390-414)
+  const int dim = DGSolution::DIM;
+  if (dim < 1 || dim > 3) {
+    std::cout << "dimension out of range in IO::output_num()" << std::endl;
+    std::exit(1);
+  }
+  const std::vector<double> u = dgsolution_ptr->val_tensor(x_1D);
+  std::ofstream output(file_name);
+  const size_t n = x_1D.size();
+  for (size_t i = 0; i < u.size(); ++i) {
+    size_t r = i;
+    std::vector<double> x(dim);
+    for (int d = dim - 1; d >= 0; --d) { x[d] = x_1D[r % n]; r /= n; }
+    for (int d = 0; d < dim; ++d) output << x[d] << "   ";
+    output << u[i];
+    if (dim == 1) output << "   ";
+    output << std::endl;
+  }
+}
+
+namespace {
+void element_lines(const DGSolution& dg, const std::string& file_name, int what) {
+  std::vector<int> level, suppt;
+  dg.elements(level, suppt);
+  const int D = DGSolution::DIM;
+  std::ofstream output(file_name);
+  for (size_t e = 0; e < level.size() / D; ++e) {
+    for (int d = 0; d < D; ++d) {
+      const int l = level[e * D + d], j = suppt[e * D + d];
+      double xl = 0., xr = 1.;
+      if (l > 1) {  // Basis::Basis support (source/Basis.cpp:21-31)
+        xl = std::pow(2., -l + 1.) * (j - 1.) / 2.;
+        xr = std::pow(2., -l + 1.) * (j + 1.) / 2.;
+      }
+      if (what == 0) output << (xl + xr) / 2. << " ";
+      else if (what == 1) output << xl << " " << xr << " ";
+      else output << l << " " << xl << " " << xr << " ";
+    }
+    output << std::endl;
+  }
+}
+}  // namespace
+
+void IO::output_element_center(const std::string& file_name) const { element_lines(*dgsolution_ptr, file_name, 0); }
+void IO::output_element_support(const std::string& file_name) const { element_lines(*dgsolution_ptr, file_name, 1); }
+void IO::output_element_level_support(const std::string& file_name) const {
+  element_lines(*dgsolution_ptr, file_name, 2);
+}
+
+void IO::write_error_time(const double tn, const std::vector<double>& err, const int init,
+                          const std::string file_name) const {
+  std::ofstream output(file_name, init == 0 ? std::ofstream::out : (std::ofstream::out | std::ofstream::app));
+  output << tn << " " << err[1] << std::endl;
+}
+
+void IO::write_error(const int NMAX, const std::vector<double>& err, const std::string file_name) const {
+  std::ofstream output(file_name);
+  output << NMAX << " ";
+  for (auto const& x : err) output << x << " ";
+  output << std::endl;
+}
+
 DGAdapt::DGAdapt(const bool sparse_, const int level_init_, const int NMAX_, AllBasis<AlptBasis>& a,
                  AllBasis<LagrBasis>& l, AllBasis<HermBasis>& h, Hash& hash, const double eps_, const double eta_,
                  const bool, const bool, const bool)
@@ -109,6 +227,39 @@ DGAdapt::DGAdapt(const bool sparse_, const int level_init_, const int NMAX_, All
 
 void DGAdapt::refine() { adg_facade_check(ctx_, adg_refine(ctx_, eps, nullptr), "DGAdapt::refine"); }
 void DGAdapt::coarsen() { adg_facade_check(ctx_, adg_coarsen(ctx_, eta, nullptr), "DGAdapt::coarsen"); }
+
+namespace {
+double call_separable(double x, int d, int ifunc, void* user) {
+  auto* f = static_cast<std::vector<std::function<double(double, int)>>*>(user);
+  return (*f)[ifunc](x, d);
+}
+double call_coefficient(const double* x, int d, void* user) {
+  auto* f = static_cast<std::function<double(std::vector<double>, int)>*>(user);
+  return (*f)(std::vector<double>(x, x + DGSolution::DIM), d);
+}
+// Sets the level-sum cap of the _coarse_grid variants for one call.
+struct CoarseGrid {
+  adg_ctx* c;
+  CoarseGrid(adg_ctx* ctx, int mesh_nmax) : c(ctx) { adg_facade_check(c, adg_set_level_sum_cap(c, mesh_nmax), "coarse grid"); }
+  ~CoarseGrid() { adg_set_level_sum_cap(c, -1); }
+};
+}  // namespace
+
+void DGAdapt::init_separable_scalar_sum(std::vector<std::function<double(double, int)>> sum_func) {
+  adg_facade_check(ctx_, adg_init_separable(ctx_, call_separable, &sum_func, static_cast<int>(sum_func.size()), eps),
+                   "init_separable_scalar_sum");
+}
+
+void DGAdapt::filter(const double damp_coef, const std::vector<double>& wave_speed, const double dt,
+                     const int filter_start_level_sum) {
+  adg_facade_check(ctx_, adg_filter(ctx_, damp_coef, wave_speed.data(), dt, filter_start_level_sum), "DGAdapt::filter");
+}
+
+void DGAdapt::filter_local(const double damp_coef, std::function<double(std::vector<double>, int)> wave_speed_func,
+                           const double dt, const int filter_start_level_sum) {
+  adg_facade_check(ctx_, adg_filter_local(ctx_, damp_coef, call_coefficient, &wave_speed_func, dt, filter_start_level_sum),
+                   "DGAdapt::filter_local");
+}
 
 DGAdaptIntp::DGAdaptIntp(const bool sparse_, const int level_init_, const int NMAX_, AllBasis<AlptBasis>& a,
                          AllBasis<LagrBasis>& l, AllBasis<HermBasis>& h, Hash& hash, const double eps_,
@@ -154,6 +305,11 @@ FastLagrIntp::FastLagrIntp(DGSolution& dg, const std::vector<std::vector<double>
 
 void FastLagrIntp::eval_up_Lagr() { adg_facade_check(dgsolution_ptr->ctx(), adg_eval_up(dgsolution_ptr->ctx()), "eval_up_Lagr"); }
 
+void FastLagrIntp::eval_up_Lagr_coarse_grid(const int mesh_nmax) {
+  CoarseGrid cg(dgsolution_ptr->ctx(), mesh_nmax);
+  eval_up_Lagr();
+}
+
 double FluxFunction::burgers_flux_scalar(const double u) { return u * u * 0.5; }
 double FluxFunction::sin_flux(const double u) { return sin(u); }
 double FluxFunction::cos_flux(const double u) { return cos(u); }
@@ -188,6 +344,43 @@ void LagrInterpolation::nonlinear_Lagr_fast(int flux_id, double flux_param, std:
   adg_facade_check(c, adg_fp_to_coe(c), "nonlinear_Lagr_fast");
 }
 
+namespace {
+// LagrInterpolation::eval_coe_u_Lag (source/Interplation.cpp:648-696) with a host
+// closure, then the surplus transform
+void coe_times_u(adg_ctx* c, std::function<double(std::vector<double>, int)>& coe_func,
+                 const std::vector<std::vector<bool>>& is_intp) {
+  const int D = DGSolution::DIM;
+  int P = 1;
+  for (int d = 0; d < D; ++d) P *= LagrBasis::PMAX + 1;
+  const int64_t n = adg_num_elem(c) * P;
+  std::vector<double> up(n), x(n * D), fp(n * D, 0.);
+  adg_facade_check(c, adg_field_get(c, ADG_FIELD_UP, up.data()), "var_coeff_u_Lagr_fast");
+  adg_facade_check(c, adg_lagr_points(c, x.data()), "var_coeff_u_Lagr_fast");
+  std::vector<double> pos(D);
+  for (int64_t i = 0; i < n; ++i) {
+    std::copy(&x[i * D], &x[i * D] + D, pos.begin());
+    for (int d = 0; d < D; ++d)
+      if (is_intp.empty() || is_intp[0][d]) fp[d * n + i] = coe_func(pos, d) * up[i];
+  }
+  adg_facade_check(c, adg_flux_set(c, D, fp.data()), "var_coeff_u_Lagr_fast");
+  adg_facade_check(c, adg_fp_to_coe(c), "var_coeff_u_Lagr_fast");
+}
+}  // namespace
+
+void LagrInterpolation::var_coeff_u_Lagr_fast(std::function<double(std::vector<double>, int)> coe_func,
+                                              std::vector<std::vector<bool>> is_intp, FastLagrIntp& fastLagr) {
+  fastLagr.eval_up_Lagr();
+  coe_times_u(dgsolution_ptr->ctx(), coe_func, is_intp);
+}
+
+void LagrInterpolation::var_coeff_u_Lagr_fast_coarse_grid(std::function<double(std::vector<double>, int)> coe_func,
+                                                          std::vector<std::vector<bool>> is_intp,
+                                                          FastLagrIntp& fastLagr, const int mesh_nmax) {
+  // source/Interplation.cpp:4215-4225: only the point evaluation is restricted
+  fastLagr.eval_up_Lagr_coarse_grid(mesh_nmax);
+  coe_times_u(dgsolution_ptr->ctx(), coe_func, is_intp);
+}
+
 void LagrInterpolation::var_coeff_u_Lagr_fast(double s, std::vector<std::vector<bool>>, FastLagrIntp& fastLagr) {
   adg_ctx* c = dgsolution_ptr->ctx();
   fastLagr.eval_up_Lagr();
@@ -199,6 +392,14 @@ HyperbolicLagrRHS::HyperbolicLagrRHS(DGSolution& dg, const OperatorMatrix1D<Lagr
     : dgsolution_ptr(&dg) {}
 void HyperbolicLagrRHS::rhs_vol_scalar() { adg_facade_check(dgsolution_ptr->ctx(), adg_rhs_lagr_vol(dgsolution_ptr->ctx()), "rhs_vol_scalar"); }
 void HyperbolicLagrRHS::rhs_flx_intp_scalar() { adg_facade_check(dgsolution_ptr->ctx(), adg_rhs_lagr_flx(dgsolution_ptr->ctx()), "rhs_flx_intp_scalar"); }
+void HyperbolicLagrRHS::rhs_vol_scalar_coarse_grid(const int mesh_nmax) {
+  CoarseGrid cg(dgsolution_ptr->ctx(), mesh_nmax);
+  rhs_vol_scalar();
+}
+void HyperbolicLagrRHS::rhs_flx_intp_scalar_coarse_grid(const int mesh_nmax) {
+  CoarseGrid cg(dgsolution_ptr->ctx(), mesh_nmax);
+  rhs_flx_intp_scalar();
+}
 
 HyperbolicAlptRHS::HyperbolicAlptRHS(DGSolution& dg, const OperatorMatrix1D<AlptBasis, AlptBasis>&)
     : dgsolution_ptr(&dg) {}
@@ -211,8 +412,17 @@ void HyperbolicAlptRHS::rhs_flx_penalty_scalar(const std::vector<double>& lax_al
 BilinearForm::BilinearForm(DGSolution& dg)
     : dgsolution_ptr(&dg), lin_coef_(DGSolution::DIM, 0.), lft_(DGSolution::DIM, 0.), rgt_(DGSolution::DIM, 0.) {}
 
+void BilinearForm::set_cap(int mesh_nmax) {
+  if ((has_lin_ || has_flx_) && mesh_nmax_ != mesh_nmax) {
+    std::cout << "BilinearForm: terms assembled with different coarse-grid levels are not provided" << std::endl;
+    std::exit(1);
+  }
+  mesh_nmax_ = mesh_nmax;
+}
+
 void BilinearForm::apply_add() const {
   adg_ctx* c = dgsolution_ptr->ctx();
+  CoarseGrid cg(c, mesh_nmax_);
   if (has_lin_) adg_facade_check(c, adg_rhs_linear(c, lin_coef_.data()), "BilinearForm (advection)");
   if (has_flx_) {
     // coef_l ulft_vjp + coef_r urgt_vjp with coef_l = -coef_r is coef_r ujp_vjp, the LF penalty
@@ -231,12 +441,28 @@ void BilinearForm::apply_add() const {
 
 HyperbolicAlpt::HyperbolicAlpt(DGSolution& dg, const OperatorMatrix1D<AlptBasis, AlptBasis>&) : BilinearForm(dg) {}
 
+void HyperbolicAlpt::assemble_matrix_scalar_coarse_grid(const std::vector<double>& c, const int mesh_nmax) {
+  set_cap(mesh_nmax);
+  for (int d = 0; d < DGSolution::DIM; ++d) lin_coef_[d] += c[d];
+  has_lin_ = true;
+}
+
+void HyperbolicAlpt::assemble_matrix_flx_scalar_coarse_grid(const int dim, const int sign, const int mesh_nmax,
+                                                            const double coefficient) {
+  set_cap(mesh_nmax);
+  if (sign == -1) lft_[dim] += coefficient;
+  else rgt_[dim] += coefficient;
+  has_flx_ = true;
+}
+
 void HyperbolicAlpt::assemble_matrix_scalar(const std::vector<double>& c) {
+  set_cap(-1);
   for (int d = 0; d < DGSolution::DIM; ++d) lin_coef_[d] += c[d];
   has_lin_ = true;
 }
 
 void HyperbolicAlpt::assemble_matrix_flx_scalar(const int dim, const int sign, const double coefficient) {
+  set_cap(-1);
   if (sign == -1) lft_[dim] += coefficient;
   else rgt_[dim] += coefficient;
   has_flx_ = true;

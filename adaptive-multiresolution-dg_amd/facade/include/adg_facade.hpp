@@ -14,14 +14,28 @@
 //   ForwardEuler, RK2Midpoint, RK3SSP), Timer.h, Optparser.h.
 // Only the members the hyperbolic example drivers use are provided.
 #pragma once
+// the standard headers a driver gets through the reference's include/libs.h
+#include <algorithm>
+#include <array>
+#include <cassert>
 #include <chrono>
 #include <cmath>
+#include <cstring>
+#include <ctime>
+#include <fstream>
 #include <functional>
+#include <iomanip>
 #include <iostream>
+#include <iterator>
 #include <map>
 #include <memory>
+#include <numeric>
+#include <random>
+#include <set>
+#include <sstream>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "adg.h"
@@ -110,6 +124,16 @@ class DGSolution {
 
   int max_mesh_level() const;
   std::vector<int> max_mesh_level_vec() const;
+  int nmax_level() const { return NMAX; }
+  // DGSolution::get_error_no_separable_scalar (source/DGSolution.cpp:326-335): {L1, L2, Linf}
+  std::vector<double> get_error_no_separable_scalar(std::function<double(std::vector<double>)> func,
+                                                    const int gauss_points) const;
+  // DGSolution::val (source/DGSolution.cpp:613-625), zero derivatives only
+  std::vector<double> val(const std::vector<double>& x, const std::vector<int>& derivative) const;
+  // u on the tensor grid x1d^DIM (first dimension slowest)
+  std::vector<double> val_tensor(const std::vector<double>& x1d) const;
+  // level / support of every element, hash-key order
+  void elements(std::vector<int>& level, std::vector<int>& suppt) const;
   int size_elem() const;
   int size_basis_alpt() const;
   int get_dof() const;
@@ -140,6 +164,13 @@ class DGAdapt : public DGSolution {
   static std::vector<int> indicator_var_adapt;
   void refine();
   void coarsen();
+  // DGAdapt::init_separable_scalar_sum (source/DGAdapt.cpp:120-147)
+  void init_separable_scalar_sum(std::vector<std::function<double(double, int)>> sum_func);
+  // DGAdapt::filter / filter_local (source/DGAdapt.cpp:461-555)
+  void filter(const double damp_coef, const std::vector<double>& wave_speed, const double dt,
+              const int filter_start_level_sum);
+  void filter_local(const double damp_coef, std::function<double(std::vector<double>, int)> wave_speed_func,
+                    const double dt, const int filter_start_level_sum);
   const double eps;
   const double eta;
 };
@@ -160,6 +191,7 @@ class FastLagrIntp {
   FastLagrIntp(DGSolution& dgsolution, const std::vector<std::vector<double>>& Lag_pt_Alpt_1D,
                const std::vector<std::vector<double>>& Lag_pt_Alpt_1D_d1);
   void eval_up_Lagr();
+  void eval_up_Lagr_coarse_grid(const int mesh_nmax);
   DGSolution* const dgsolution_ptr;
 };
 
@@ -182,8 +214,14 @@ class LagrInterpolation {
   // Device flux by id (ADG_FLUX_BURGERS / ADG_FLUX_SINCOS): no host round trip.
   void nonlinear_Lagr_fast(int flux_id, double flux_param, std::vector<std::vector<bool>> is_intp,
                            FastLagrIntp& fastLagr);
-  // Variable coefficient a_d(x) u with a_d = 1 + s sin(2 pi x_{(d+1) mod dim}).
+  // Variable coefficient a_d(x) u with a_d = 1 + s sin(2 pi x_{(d+1) mod dim}) (device).
   void var_coeff_u_Lagr_fast(double s, std::vector<std::vector<bool>> is_intp, FastLagrIntp& fastLagr);
+  // Host coefficient closure coe_func(x, d) (the reference's signature).
+  void var_coeff_u_Lagr_fast(std::function<double(std::vector<double>, int)> coe_func,
+                             std::vector<std::vector<bool>> is_intp, FastLagrIntp& fastLagr);
+  void var_coeff_u_Lagr_fast_coarse_grid(std::function<double(std::vector<double>, int)> coe_func,
+                                         std::vector<std::vector<bool>> is_intp, FastLagrIntp& fastLagr,
+                                         const int mesh_nmax);
   DGSolution* const dgsolution_ptr;
 };
 
@@ -192,6 +230,8 @@ class HyperbolicLagrRHS {
   HyperbolicLagrRHS(DGSolution& dgsolution, const OperatorMatrix1D<LagrBasis, AlptBasis>& oper_matx_lagr);
   void rhs_vol_scalar();
   void rhs_flx_intp_scalar();
+  void rhs_vol_scalar_coarse_grid(const int mesh_nmax);
+  void rhs_flx_intp_scalar_coarse_grid(const int mesh_nmax);
   DGSolution* const dgsolution_ptr;
 };
 
@@ -221,6 +261,8 @@ class BilinearForm {
   std::vector<double> lin_coef_;            // const-coefficient advection speeds (assemble_matrix_scalar)
   std::vector<double> lft_, rgt_;           // coefficients of ulft_vjp / urgt_vjp flux terms per dimension
   bool has_lin_ = false, has_flx_ = false;
+  int mesh_nmax_ = -1;                      // level-sum cap of the _coarse_grid assembly (-1: none)
+  void set_cap(int mesh_nmax);
 };
 
 class HyperbolicAlpt : public BilinearForm {
@@ -228,6 +270,9 @@ class HyperbolicAlpt : public BilinearForm {
   HyperbolicAlpt(DGSolution& dgsolution, const OperatorMatrix1D<AlptBasis, AlptBasis>& oper_matx_alpt);
   void assemble_matrix_scalar(const std::vector<double>& eqnCoefficient);
   void assemble_matrix_flx_scalar(const int dim, const int sign, const double coefficient);
+  void assemble_matrix_scalar_coarse_grid(const std::vector<double>& eqnCoefficient, const int mesh_nmax);
+  void assemble_matrix_flx_scalar_coarse_grid(const int dim, const int sign, const int mesh_nmax,
+                                              const double coefficient = 1.);
 };
 
 class ODESolver {
@@ -274,6 +319,27 @@ class RK2Midpoint : public ODESolver {
   void step_stage(const int stage) override;
   const int num_stage = 2;
   const double dt;
+};
+
+// Text outputs of the reference's IO class (source/IO.cpp): point values on
+// the cell-centre grid of level NMAX, element centres and supports, errors.
+class IO {
+ public:
+  explicit IO(DGSolution& dgsolution);
+  void output_num(const std::string& file_name) const;
+  void output_element_center(const std::string& file_name) const;
+  void output_element_support(const std::string& file_name) const;
+  void output_element_level_support(const std::string& file_name) const;
+  void write_error_time(const double tn, const std::vector<double>& err, const int init,
+                        const std::string file_name) const;
+  void write_error(const int NMAX, const std::vector<double>& err, const std::string file_name) const;
+
+ private:
+  DGSolution* const dgsolution_ptr;
+  const int max_mesh;
+  const int num_grid_point;
+  const double dx;
+  std::vector<double> x_1D;
 };
 
 class Timer {

@@ -43,6 +43,10 @@ class ICParams(ctypes.Structure):
                 ("center", ctypes.c_double * 4), ("scale", ctypes.c_double * 4), ("offset", ctypes.c_double * 4)]
 
 
+SEP_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
+COEF_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p)
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -68,6 +72,7 @@ def lib():
             "adg_sweep": (i32, [vp, i32, i32, i32, i32, I32P, I32P, dbl, DP, DP]),
             "adg_eval_up": (i32, [vp]),
             "adg_set_level_sum_cap": (i32, [vp, i32]),
+            "adg_lagr_points": (i32, [vp, DP]),
             "adg_flux": (i32, [vp, i32, DP]),
             "adg_flux_set": (i32, [vp, i32, DP]),
             "adg_fp_to_coe": (i32, [vp]),
@@ -92,6 +97,12 @@ def lib():
             "adg_state_restore": (i32, [vp, i32]),
             "adg_profile": (i32, [vp, i32]),
             "adg_profile_get": (i32, [vp, DP, P(i64), DP]),
+            "adg_init_separable": (i32, [vp, SEP_FN, vp, i32, dbl]),
+            "adg_sep_eval": (dbl, [dbl, i32, i32, vp]),
+            "adg_filter": (i32, [vp, dbl, DP, dbl, i32]),
+            "adg_filter_local": (i32, [vp, dbl, COEF_FN, vp, dbl, i32]),
+            "adg_eval_tensor": (i32, [vp, i32, DP, DP]),
+            "adg_error_no_separable": (i32, [vp, POINT_FN, vp, i32, DP]),
             "adg_grid_init_batch": (i32, [vp, i32, i32, i32]),
             "adg_init_adaptive_intp_batch": (i32, [vp, POINT_FN, P(vp), dbl]),
             "adg_num_instances": (i32, [vp]),
@@ -109,6 +120,11 @@ def lib():
 def ic_eval_ptr():
     """Address of the library's built-in point function adg_ic_eval."""
     return ctypes.cast(lib().adg_ic_eval, ctypes.c_void_p).value
+
+
+def sep_eval_ptr():
+    """Address of the library's built-in separable function adg_sep_eval."""
+    return ctypes.cast(lib().adg_sep_eval, ctypes.c_void_p).value
 
 
 class AdgError(RuntimeError):
@@ -202,6 +218,12 @@ class Solution:
         self._check(lib().adg_sweep(self._h, table, int(flx), lu, dim, _ip(si), _ip(so), coef, _dp(x), _dp(out)))
         return out
 
+    def lagr_points(self):
+        """Coordinates of every element's Lagrange points, shape (E, P, dim)."""
+        x = np.zeros(self.num_elem * self.P * self.dim)
+        self._check(lib().adg_lagr_points(self._h, _dp(x)))
+        return x.reshape(self.num_elem, self.P, self.dim)
+
     def set_level_sum_cap(self, cap):
         """Coarse grid: sweeps ignore elements with level sum above cap (cap < 0: none)."""
         self._check(lib().adg_set_level_sum_cap(self._h, cap))
@@ -282,6 +304,55 @@ class Solution:
         fn = POINT_FN(ic_eval_ptr())
         self._keep.append(fn)
         self._check(lib().adg_init_adaptive_intp(self._h, fn, ctypes.cast(ctypes.pointer(params), ctypes.c_void_p), eps))
+
+    def init_separable(self, funcs, eps):
+        """funcs[i](x, d) -> float: u0 = sum_i prod_d funcs[i](x_d, d) (host callbacks)."""
+        def cb(x, d, i, user):
+            return float(funcs[i](x, d))
+
+        cfun = SEP_FN(cb)
+        self._keep.append(cfun)
+        self._check(lib().adg_init_separable(self._h, cfun, None, len(funcs), eps))
+
+    def init_separable_builtin(self, params, nfunc, eps):
+        """Initialise with the library's adg_sep_eval and an ICParams."""
+        self._keep.append(params)
+        fn = SEP_FN(sep_eval_ptr())
+        self._keep.append(fn)
+        self._check(lib().adg_init_separable(self._h, fn, ctypes.cast(ctypes.pointer(params), ctypes.c_void_p), nfunc,
+                                             eps))
+
+    def filter(self, damp, wave_speed, dt, start_level_sum):
+        w = np.ascontiguousarray(wave_speed, np.float64)
+        self._check(lib().adg_filter(self._h, damp, _dp(w), dt, start_level_sum))
+
+    def filter_local(self, damp, speed, dt, start_level_sum):
+        """speed(x: ndarray[dim], d) -> float, evaluated on the host at element corners."""
+        dim = self.dim
+
+        def cb(xp, d, user):
+            return float(speed(np.ctypeslib.as_array(xp, shape=(dim,)), d))
+
+        cfun = COEF_FN(cb)
+        self._keep.append(cfun)
+        self._check(lib().adg_filter_local(self._h, damp, cfun, None, dt, start_level_sum))
+
+    def eval_tensor(self, x1d):
+        """u on the tensor grid x1d^dim, shape (len(x1d),) * dim."""
+        x = np.ascontiguousarray(x1d, np.float64)
+        out = np.zeros(x.size ** self.dim)
+        self._check(lib().adg_eval_tensor(self._h, x.size, _dp(x), _dp(out)))
+        return out.reshape((x.size,) * self.dim)
+
+    def error_no_separable_builtin(self, params, gauss_points):
+        """[L1, L2, Linf] of u - f, f = adg_ic_eval with params."""
+        self._keep.append(params)
+        fn = POINT_FN(ic_eval_ptr())
+        self._keep.append(fn)
+        err = np.zeros(3)
+        self._check(lib().adg_error_no_separable(self._h, fn, ctypes.cast(ctypes.pointer(params), ctypes.c_void_p),
+                                                 gauss_points, _dp(err)))
+        return err
 
     # ---- batches of independent solves of one discretisation ----
     def init_batch(self, ninst, sparse=True, level_init=None):

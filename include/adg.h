@@ -103,6 +103,11 @@ int adg_field_get(adg_ctx* c, int field, double* host);
 int adg_sweep(adg_ctx* c, int table, int flx, int lu, int dim, const int32_t* size_in,
               const int32_t* size_out, double coef, const double* in, double* out);
 
+/* Coordinates of every element's Lagrange points (LagrBasis::intep_pt,
+ * source/LagrBasis.cpp:19-28): x[(e*P + q)*dim + d], points in row-major
+ * order.  For host flux / coefficient closures. */
+int adg_lagr_points(adg_ctx* c, double* x);
+
 /* Coarse grid: the _coarse_grid variants of the reference
  * (FastMultiplyLU::transform_1D_coarse_grid, source/FastMultiplyLU.cpp:514-594;
  * FastLagrIntp::eval_up_Lagr_coarse_grid, :1367-1370;
@@ -176,6 +181,46 @@ typedef struct {
   double inside, outside, radius, center[4], scale[4], offset[4];
 } adg_ic_params;
 double adg_ic_eval(const double* x, int dim, void* user);
+
+/* Separable initial function u0(x) = sum_i prod_d f_i(x_d, d): f returns the
+ * factor of term ifunc in dimension d at coordinate x. */
+typedef double (*adg_sep_fn)(double x, int d, int ifunc, void* user);
+/* DGAdapt::init_separable_scalar_sum (source/DGAdapt.cpp:120-147): the 1D L2
+ * projections of every factor (DGSolution::seperable_project,
+ * source/DGSolution.cpp:150-167, with Basis::product_function,
+ * source/Basis.cpp:38-59) give every element's coefficients as sums of
+ * products (init_elem_separable, :169-184), added to the current ones; leaves
+ * whose coefficient norm exceeds eps get all children, initialised the same
+ * way, until no leaf does (refine_init_separable_scalar_sum,
+ * source/DGAdapt.cpp:850-894).  One instance only. */
+int adg_init_separable(adg_ctx* c, adg_sep_fn f, void* user, int nfunc, double eps);
+/* Built-in separable functions, `user` pointing to an adg_ic_params:
+ *   kind 0, dim 2, nfunc 2: sin(2 pi (x_0 - shift + x_1)) as
+ *     sin(2 pi (x_0 - shift)) cos(2 pi x_1) + cos(2 pi (x_0 - shift)) sin(2 pi x_1)
+ *   kind 1, nfunc 1: prod_d sin(2 pi (x_d - shift)) */
+double adg_sep_eval(double x, int d, int ifunc, void* user);
+
+/* Coefficient field: component d at point x. */
+typedef double (*adg_coef_fn)(const double* x, int d, void* user);
+/* DGAdapt::filter (source/DGAdapt.cpp:461-483): the coefficients of every
+ * element with level sum >= start_level_sum are multiplied by
+ * exp(-c (sum_d wave_speed[d] 2^{l_d})^2), c = damp_coef dt^2/2 pi^2. */
+int adg_filter(adg_ctx* c, double damp_coef, const double* wave_speed, double dt, int start_level_sum);
+/* DGAdapt::filter_local (source/DGAdapt.cpp:485-555): the same with the
+ * element's local speed max over its corners of |speed(x, d)| (dim 2 or 3). */
+int adg_filter_local(adg_ctx* c, double damp_coef, adg_coef_fn speed, void* user, double dt, int start_level_sum);
+
+/* DGSolution::val (source/DGSolution.cpp:613-625, Element::val,
+ * source/Element.cpp:123-147) at every point of the tensor grid x1d^dim:
+ * out[i_0 * nx^(dim-1) + ... + i_{dim-1}] = u(x1d[i_0], ..., x1d[i_{dim-1}]).
+ * Every element whose closed support holds the point contributes.  One
+ * instance only. */
+int adg_eval_tensor(adg_ctx* c, int nx, const double* x1d, double* out);
+/* DGSolution::get_error_no_separable_scalar (source/DGSolution.cpp:326-335)
+ * with Quad::norm_multiD (source/Quad.cpp:261-317): L1, L2 and Linf norms of
+ * |u - f| by gauss_points-point Gauss-Legendre rules on the 2^L cells per
+ * dimension, L the maximum active mesh level.  err[0..2] = L1, L2, Linf. */
+int adg_error_no_separable(adg_ctx* c, adg_point_fn f, void* user, int gauss_points, double* err);
 
 /* Whole-run drivers, the time loops of the reference example drivers. */
 typedef struct {

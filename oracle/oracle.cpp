@@ -37,6 +37,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <numeric>
 #include <set>
@@ -253,8 +254,13 @@ struct or_ctx {
     for (auto& it : dg) if (it.second.chd.empty()) leaf0[it.first] = &it.second;
   }
 
+  // called for every element add_elem inserts (separable initialisation sets
+  // the coefficients of new elements through it)
+  std::function<void(OElem&)> on_add;
+
   void add_elem(const std::vector<int>& l, const std::vector<int>& j) {
     OElem e0 = make_elem(l, j);
+    if (on_add) on_add(e0);
     auto ins = dg.emplace(e0.hash, std::move(e0));
     OElem& e = ins.first->second;
     e.num_total_par = num_all_par(e.level);
@@ -329,6 +335,62 @@ struct or_ctx {
     update_leaf();
     update_leaf_zero_child();
     return static_cast<int64_t>(dg.size() - before);
+  }
+
+  // DGAdapt::init_separable_scalar_sum (source/DGAdapt.cpp:120-147):
+  // seperable_project (source/DGSolution.cpp:150-167), init_elem_separable
+  // (:169-184) on every element, then refine_init_separable_scalar_sum
+  // (source/DGAdapt.cpp:850-894) with check_hole_init_separable_scalar_sum.
+  void init_separable(double (*f)(double, int, int, void*), void* user, int nfunc, double eps) {
+    std::vector<std::vector<double>> c1(static_cast<size_t>(nfunc) * D);
+    for (int i = 0; i < nfunc; ++i)
+      for (int d = 0; d < D; ++d) c1[i * D + d] = adg::project_1d(alpt, [&](double x) { return f(x, d, i, user); });
+    auto init_elem = [&](OElem& e, int i) {
+      std::vector<int> idx(D, 0);
+      for (int q = 0; q < K; ++q) {
+        double coe = 1.;
+        for (int d = 0; d < D; ++d) coe *= c1[i * D + d][e.ord[d] * (k + 1) + idx[d]];
+        e.ucoe[q] += coe;
+        for (int d = D - 1; d >= 0; --d) { if (++idx[d] < k + 1) break; idx[d] = 0; }
+      }
+    };
+    for (auto& it : dg)
+      for (int i = 0; i < nfunc; ++i) init_elem(it.second, i);
+    on_add = [&](OElem& e) { for (int i = 0; i < nfunc; ++i) init_elem(e, i); };
+    while (true) {
+      bool flag = false;
+      std::vector<OElem*> lf;
+      for (auto& it : leaf) lf.push_back(it.second);
+      for (OElem* e : lf) {
+        if (norm2(e->ucoe) > eps) {
+          flag = true;
+          for (const auto& idx : index_all_chd(e->level, e->suppt)) {
+            const int h = key(idx[0], idx[1]);
+            if (e->chd.find(h) == e->chd.end()) add_elem(idx[0], idx[1]);
+          }
+        }
+      }
+      if (!flag) break;
+      check_hole();
+      update_leaf();
+    }
+    on_add = nullptr;
+    update_leaf_zero_child();
+  }
+
+  // DGAdapt::filter / filter_local (source/DGAdapt.cpp:461-555)
+  void filter(double damp, const std::function<double(const OElem&, int)>& speed, double dt, int start) {
+    const double coefficient = damp * (dt * dt / 2.0) * (M_PI * M_PI);
+    for (auto& it : dg) {
+      OElem& e = it.second;
+      const int level_sum = std::accumulate(e.level.begin(), e.level.end(), 0);
+      if (level_sum >= start) {
+        double index_sum = 0.0;
+        for (int d = 0; d < D; ++d) index_sum += speed(e, d) * std::pow(2.0, e.level[d]);
+        const double fac = std::exp(-coefficient * index_sum * index_sum);
+        for (double& v : e.ucoe) v *= fac;
+      }
+    }
   }
 
   int64_t coarsen(double eta) {
@@ -846,6 +908,111 @@ void or_max_level(or_ctx* c, int32_t* out) {
 }
 
 void or_init_adaptive_intp(or_ctx* c, or_func_t f, void* user, double eps) { c->init_adaptive_intp(f, user, eps); }
+
+void or_init_separable(or_ctx* c, or_sep_t f, void* user, int nfunc, double eps) { c->init_separable(f, user, nfunc, eps); }
+
+// DGSolution::val (source/DGSolution.cpp:613-625) with Element::val
+// (source/Element.cpp:123-147): every element whose closed support holds x,
+// in map order.
+static double dg_val(or_ctx* c, const double* x) {
+  double value = 0.;
+  for (auto& it : c->dg) {
+    const OElem& e = it.second;
+    bool inside = true;
+    for (int d = 0; d < c->D; ++d)
+      if (x[d] < e.xl[d] || x[d] > e.xr[d]) { inside = false; break; }
+    double v = 0.;
+    if (inside) {
+      std::vector<int> idx(c->D, 0);
+      for (int q = 0; q < c->K; ++q) {
+        double val_bas = e.ucoe[q];
+        for (int d = 0; d < c->D; ++d) val_bas *= c->alpt.val(e.ord[d] * (c->k + 1) + idx[d], x[d], 0, 0);
+        v += val_bas;
+        for (int d = c->D - 1; d >= 0; --d) { if (++idx[d] < c->k + 1) break; idx[d] = 0; }
+      }
+    }
+    value += v;
+  }
+  return value;
+}
+
+void or_eval_tensor(or_ctx* c, int nx, const double* x1d, double* out) {
+  int64_t n = 1;
+  for (int d = 0; d < c->D; ++d) n *= nx;
+  std::vector<double> x(c->D);
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t r = i;
+    for (int d = c->D - 1; d >= 0; --d) { x[d] = x1d[r % nx]; r /= nx; }
+    out[i] = dg_val(c, x.data());
+  }
+}
+
+// DGSolution::get_error_no_separable_scalar + Quad::norm_multiD
+// (source/DGSolution.cpp:326-335, source/Quad.cpp:261-317)
+void or_error_no_separable(or_ctx* c, or_func_t f, void* user, int points, double* err) {
+  int lmax = 0;
+  for (auto& it : c->dg) for (int l : it.second.level) lmax = std::max(lmax, l);
+  const int D = c->D;
+  const std::vector<double>& qx1 = adg::gl_nodes(points);
+  const std::vector<double>& qw1 = adg::gl_weights(points);
+  std::vector<std::vector<int>> qidx;
+  {
+    std::vector<int> s(D, 0);
+    while (true) {
+      qidx.push_back(s);
+      int d = D - 1;
+      while (d >= 0 && ++s[d] == points) { s[d] = 0; --d; }
+      if (d < 0) break;
+    }
+  }
+  std::vector<double> qw;
+  for (auto& ind : qidx) { double w = 1; for (int i : ind) w *= qw1[i]; qw.push_back(w); }
+  const int N = 1 << lmax;
+  const double h = (1. - 0.) / N;
+  double L1 = 0, L2 = 0, Linf = 0;
+  std::vector<int> cell(D, 0);
+  std::vector<double> x(D);
+  while (true) {
+    std::vector<double> av(qidx.size());
+    for (size_t i = 0; i < qidx.size(); ++i) {
+      for (int d = 0; d < D; ++d) {
+        const double tl = cell[d] * h, tr = (cell[d] + 1) * h;
+        x[d] = (qx1[qidx[i][d]] - (-1.)) / (1. - (-1.)) * (tr - tl) + tl;
+      }
+      av[i] = std::abs(std::abs(dg_val(c, x.data()) - f(x.data(), D, user)));
+    }
+    for (size_t i = 0; i < qidx.size(); ++i) {
+      L1 += qw[i] * av[i];
+      L2 += qw[i] * av[i] * av[i];
+      Linf = std::max(Linf, av[i]);
+    }
+    int d = D - 1;
+    while (d >= 0 && ++cell[d] == N) { cell[d] = 0; --d; }
+    if (d < 0) break;
+  }
+  L1 *= std::pow(h / 2., D);
+  L2 = std::pow(L2 * std::pow(h / 2., D), 0.5);
+  err[0] = L1; err[1] = L2; err[2] = Linf;
+}
+
+void or_filter(or_ctx* c, double damp, const double* speed, double dt, int start) {
+  c->filter(damp, [&](const OElem&, int d) { return speed[d]; }, dt, start);
+}
+
+void or_filter_local(or_ctx* c, double damp, or_coef_t speed, void* user, double dt, int start) {
+  // local speed: max over the element's corners of |a_d| (dim 2 and 3)
+  if (c->D != 2 && c->D != 3) throw std::runtime_error("filter_local: dim 2 or 3");
+  c->filter(damp, [&](const OElem& e, int d) {
+    double loc = 0.0;
+    const int nc = 1 << c->D;
+    for (int corner = 0; corner < nc; ++corner) {
+      double x[3];
+      for (int dd = 0; dd < c->D; ++dd) x[dd] = ((corner >> (c->D - 1 - dd)) & 1) ? e.xr[dd] : e.xl[dd];
+      loc = std::max(std::abs(speed(x, d, user)), loc);
+    }
+    return loc;
+  }, dt, start);
+}
 
 double or_l2_error_split(or_ctx* num, or_ctx* exact) {
   double err2 = 0.0;

@@ -54,6 +54,17 @@ int64_t or_refine(or_ctx* c, double eps);
 int64_t or_coarsen(or_ctx* c, double eta);
 void or_max_level(or_ctx* c, int32_t* out);
 void or_init_adaptive_intp(or_ctx* c, or_func_t f, void* user, double eps);
+typedef double (*or_sep_t)(double x, int d, int ifunc, void* user);
+typedef double (*or_coef_t)(const double* x, int d, void* user);
+/* DGAdapt::init_separable_scalar_sum */
+void or_init_separable(or_ctx* c, or_sep_t f, void* user, int nfunc, double eps);
+/* DGSolution::val on the tensor grid x1d^dim */
+void or_eval_tensor(or_ctx* c, int nx, const double* x1d, double* out);
+/* DGSolution::get_error_no_separable_scalar: err = {L1, L2, Linf} */
+void or_error_no_separable(or_ctx* c, or_func_t f, void* user, int points, double* err);
+/* DGAdapt::filter, DGAdapt::filter_local */
+void or_filter(or_ctx* c, double damp, const double* speed, double dt, int start);
+void or_filter_local(or_ctx* c, double damp, or_coef_t speed, void* user, double dt, int start);
 double or_l2_error_split(or_ctx* num, or_ctx* exact);
 void or_set_threads(int n);
 

@@ -13,6 +13,8 @@ _LIB = None
 
 FIELD_UCOE, FIELD_RHS, FIELD_UP, FIELD_TN, FIELD_UINTP = 0, 1, 2, 3, 4
 FUNC_T = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p)
+SEP_T = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
+COEF_T = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p)
 
 
 def lib():
@@ -51,6 +53,11 @@ def lib():
             "or_coarsen": (i64, [vp, dbl]),
             "or_max_level": (None, [vp, P(ctypes.c_int32)]),
             "or_init_adaptive_intp": (None, [vp, FUNC_T, vp, dbl]),
+            "or_init_separable": (None, [vp, SEP_T, vp, i32, dbl]),
+            "or_eval_tensor": (None, [vp, i32, P(dbl), P(dbl)]),
+            "or_error_no_separable": (None, [vp, FUNC_T, vp, i32, P(dbl)]),
+            "or_filter": (None, [vp, dbl, P(dbl), dbl, i32]),
+            "or_filter_local": (None, [vp, dbl, COEF_T, vp, dbl, i32]),
             "or_l2_error_split": (dbl, [vp, vp]),
             "or_set_threads": (None, [i32]),
             "or_table_size": (i64, [vp, i32, P(ctypes.c_int32), P(ctypes.c_int32)]),
@@ -207,6 +214,54 @@ class Oracle:
 
     def init_adaptive_intp_c(self, cfunc_ptr, user_ptr, eps):
         lib().or_init_adaptive_intp(self._h, ctypes.cast(cfunc_ptr, FUNC_T), user_ptr, eps)
+
+    def init_separable(self, funcs, eps):
+        """funcs[i](x, d) -> float: u0 = sum_i prod_d funcs[i](x_d, d)."""
+        def cb(x, d, i, user):
+            return float(funcs[i](x, d))
+
+        cfun = SEP_T(cb)
+        self._keep.append(cfun)
+        lib().or_init_separable(self._h, cfun, None, len(funcs), eps)
+
+    def init_separable_c(self, cfunc_ptr, user_ptr, nfunc, eps):
+        lib().or_init_separable(self._h, ctypes.cast(cfunc_ptr, SEP_T), user_ptr, nfunc, eps)
+
+    def eval_tensor(self, x1d):
+        x = np.ascontiguousarray(x1d, np.float64)
+        out = np.zeros(x.size ** self.dim)
+        lib().or_eval_tensor(self._h, x.size, _dp(x), _dp(out))
+        return out.reshape((x.size,) * self.dim)
+
+    def error_no_separable(self, func, points):
+        dim = self.dim
+
+        def cb(xp, d, user):
+            return float(func(np.ctypeslib.as_array(xp, shape=(dim,))))
+
+        cfun = FUNC_T(cb)
+        err = np.zeros(3)
+        lib().or_error_no_separable(self._h, cfun, None, points, _dp(err))
+        return err
+
+    def error_no_separable_c(self, cfunc_ptr, user_ptr, points):
+        err = np.zeros(3)
+        lib().or_error_no_separable(self._h, ctypes.cast(cfunc_ptr, FUNC_T), user_ptr, points, _dp(err))
+        return err
+
+    def filter(self, damp, wave_speed, dt, start_level_sum):
+        w = np.ascontiguousarray(wave_speed, np.float64)
+        lib().or_filter(self._h, damp, _dp(w), dt, start_level_sum)
+
+    def filter_local(self, damp, speed, dt, start_level_sum):
+        dim = self.dim
+
+        def cb(xp, d, user):
+            return float(speed(np.ctypeslib.as_array(xp, shape=(dim,)), d))
+
+        cfun = COEF_T(cb)
+        self._keep.append(cfun)
+        lib().or_filter_local(self._h, damp, cfun, None, dt, start_level_sum)
 
     def table(self, table_id):
         r = ctypes.c_int32()

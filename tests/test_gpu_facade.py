@@ -33,3 +33,19 @@ def test_linear_driver_error():
     out = run("linear_advection", "-N", "5", "-tf", "0.1")
     err = float(re.search(r"L2 error at final time: ([0-9.eE+-]+)", out).group(1))
     assert 0 < err < 2e-3
+
+
+def test_var_coeff_coarse_grid_driver(tmp_path):
+    # coarse-grid stages, RK2 midpoint, filter_local, IO outputs and the quadrature error through the facade
+    errs = []
+    for sep in ("0", "1"):
+        prefix = str(tmp_path / f"sep{sep}_")
+        out = run("var_coeff_coarse_grid", "-NM", "5", "-N1", "4", "-N2", "5", "-tf", "0.2", "-sep", sep, "-o", prefix)
+        e = [float(v) for v in re.search(r"error at final time: ([^\n]+)", out).group(1).split(",")]
+        assert 0 < e[0] <= e[1] <= e[2] and e[1] < 0.05, e
+        errs.append(e[1])
+        rows = open(prefix + "profile2D.txt").read().split("\n")
+        assert len([r for r in rows if r.strip()]) == 32 * 32
+        assert open(prefix + "error.txt").read().startswith("5 ")
+    # both initialisations approximate the same bump
+    assert abs(errs[0] - errs[1]) < 0.5 * max(errs)

@@ -5,6 +5,8 @@ source-slot order with no FMA).  The one exception is the sin/cos flux, where
 the device libm and glibc may differ in the last bit: there the tolerance is
 1e-12 relative to the field's max norm.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -263,3 +265,51 @@ def test_coarse_grid_cap_bitexact(dim, k, m, nmax, cap):
     g.rhs_linear(coef); o.rhs_linear(coef)
     a, b = g.get(pyadg.FIELD_RHS), o.get(pyoracle.FIELD_RHS)
     assert np.array_equal(a, b), "linear: " + first_diff(a, b)
+
+
+@pytest.mark.parametrize("dim,k,nmax,level_init,kind,nfunc,eps", [(2, 2, 7, 2, 0, 2, 1e-4), (4, 1, 6, 6, 1, 1, 1e10),
+                                                                   (3, 2, 6, 1, 1, 1, 1e-3)])
+def test_separable_init_bitexact(dim, k, nmax, level_init, kind, nfunc, eps):
+    # DGAdapt::init_separable_scalar_sum with refinement, against the oracle
+    p = ic_params(kind, shift=0.05)
+    g, o = pair(dim, k, k + 1, nmax, True, level_init)
+    g.init_separable_builtin(p, nfunc, eps)
+    o._keep.append(p)
+    o.init_separable_c(pyadg.sep_eval_ptr(), ctypes.cast(ctypes.pointer(p), ctypes.c_void_p), nfunc, eps)
+    assert same_grid(g, o), "element sets differ"
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.array_equal(a, b), first_diff(a, b)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_filters_bitexact(dim):
+    p = ic_params(1, shift=0.1)
+    g, o = pair(dim, 1, 2, 5)
+    init_both(g, o, p, 1e10)
+    g.filter(0.9, [1.0, 0.5, 2.0][:dim], 2e-3, 3)
+    o.filter(0.9, [1.0, 0.5, 2.0][:dim], 2e-3, 3)
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.array_equal(a, b), "filter: " + first_diff(a, b)
+
+    def speed(x, d):
+        return -x[(d + 1) % dim] + 0.5
+
+    g.filter_local(0.9, speed, 2e-3, 2)
+    o.filter_local(0.9, speed, 2e-3, 2)
+    a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
+    assert np.array_equal(a, b), "filter_local: " + first_diff(a, b)
+
+
+def test_point_values_and_quadrature_error():
+    # DGSolution::val on a tensor grid (boundaries included) and get_error_no_separable_scalar;
+    # element contributions are summed in another order than the oracle's map order: 1e-13 relative
+    p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.08, (0.3, 0.6))])
+    g, o = adapted_pair(2, 2, 3, 6, 3, p, 1e-4)
+    x1d = np.concatenate([np.linspace(0.0, 1.0, 65), np.array([0.3141, 0.77])])
+    a, b = g.eval_tensor(x1d), o.eval_tensor(x1d)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b)), np.max(np.abs(a - b))
+    q = ic_params(0, shift=0.0)
+    for pts in (1, 3):
+        ea = g.error_no_separable_builtin(q, pts)
+        eb = o.error_no_separable_c(pyadg.ic_eval_ptr(), ctypes.cast(ctypes.pointer(q), ctypes.c_void_p), pts)
+        assert np.all(np.abs(ea - eb) <= 1e-12 * np.abs(eb)), (ea, eb)
