@@ -497,6 +497,7 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
     for (int i = 0; i < nb; ++i) active[i] = t[i] < p->t_final;
     while (any_active() && (p->max_steps <= 0 || steps < p->max_steps)) {
       // dt = cfl / sum_d |c_d| 2^{L_d} per instance (example/02_hyperbolic_05_burgers_adapt.cpp:234-244)
+      adg::Context::HostTimer step_timer(c, adg::Context::HOST_STEP);
       c.inst_dt.assign(nb, 0.);
       double dt1 = 0.;
       for (int i = 0; i < nb; ++i) {
@@ -656,6 +657,13 @@ int adg_profile(adg_ctx* h, int enable) {
   return guard(h, [&](adg::Context& c) {
     c.prof_reset();
     c.prof_on = enable != 0;
+  });
+}
+
+int adg_profile_host(adg_ctx* h, double* ms) {
+  return guard(h, [&](adg::Context& c) {
+    for (int i = 0; i < adg::Context::HOST_NCAT; ++i)
+      if (ms) ms[i] = c.host_ms[i];
   });
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -92,6 +93,7 @@ class Context {
 
   // grid arrays
   DevBuf<int32_t> g_slot[kMaxDim], g_fbase[kMaxDim], g_fmaxl[kMaxDim], g_fmap[kMaxDim];
+  DevBuf<uint32_t> g_fbits[kMaxDim];  // occupancy bit of every fiber-map entry
   DevBuf<int32_t> g_remap;
   DevBuf<int32_t> g_lsum;  // level sum of each element
   // Level-sum cap of the reference's _coarse_grid variants: sweeps and
@@ -107,7 +109,9 @@ class Context {
 
   void set_grid_slots(const std::vector<int32_t>& slots, bool keep_fields);
   // Replace all instance grids; keep_fields remaps u and the predict copy.
-  void set_grids(const std::vector<std::vector<int32_t>>& slots, bool keep_fields);
+  // changed (optional): instances flagged 0 keep their current table.
+  void set_grids(const std::vector<std::vector<int32_t>>& slots, bool keep_fields,
+                 const std::vector<char>* changed = nullptr);
   void upload_grid();
   void ensure_fields();
 
@@ -178,6 +182,19 @@ class Context {
   struct ProfRec { hipEvent_t a, b; int cat; double bytes; };
   std::vector<ProfRec> prof_pending;
   std::vector<hipEvent_t> prof_pool;
+  // host wall time (inclusive) of the adaptive bookkeeping, recorded while profiling
+  enum { HOST_ADAPT = 0, HOST_GRID, HOST_PLAN, HOST_STEP, HOST_NCAT };
+  double host_ms[HOST_NCAT] = {0};
+  struct HostTimer {
+    Context& c;
+    int cat;
+    std::chrono::steady_clock::time_point t0;
+    HostTimer(Context& ctx, int category) : c(ctx), cat(category), t0(std::chrono::steady_clock::now()) {}
+    ~HostTimer() {
+      if (c.prof_on)
+        c.host_ms[cat] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+  };
   double prof_ms[PROF_NCAT] = {0}, prof_bytes[PROF_NCAT] = {0};
   int64_t prof_count[PROF_NCAT] = {0};
   void prof_begin(int cat, double bytes);

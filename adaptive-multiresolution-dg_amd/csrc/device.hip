@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <atomic>
+#include <condition_variable>
 #include <functional>
 #include <mutex>
 #include <numeric>
@@ -25,28 +26,91 @@ void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// Runs fn(i) for i in [0, n) on up to 16 host threads (instances are independent).
+// Runs fn(i) for i in [0, n) on a persistent pool of up to 15 host threads
+// plus the caller (instances are independent).  The pool starts on first use;
+// one parallel_for runs at a time.
+namespace {
+class HostPool {
+ public:
+  static HostPool& get() {
+    static HostPool pool;
+    return pool;
+  }
+  void run(int n, const std::function<void(int)>& fn) {
+    std::lock_guard<std::mutex> one(run_mu_);
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      fn_ = &fn;
+      n_ = n;
+      next_.store(0);
+      active_ = static_cast<int>(workers_.size());
+      err_ = nullptr;
+      ++gen_;
+    }
+    cv_.notify_all();
+    work();
+    std::unique_lock<std::mutex> g(mu_);
+    done_cv_.wait(g, [&] { return active_ == 0; });
+    fn_ = nullptr;
+    if (err_) std::rethrow_exception(err_);
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (auto& t : workers_) t.join();
+  }
+
+ private:
+  HostPool() {
+    const int nt = std::max(0, std::min(15, static_cast<int>(std::thread::hardware_concurrency()) - 1));
+    for (int t = 0; t < nt; ++t) workers_.emplace_back([this] { loop(); });
+  }
+  void work() {
+    for (int i = next_++; i < n_; i = next_++) {
+      try {
+        (*fn_)(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> g(mu_);
+        if (!err_) err_ = std::current_exception();
+      }
+    }
+  }
+  void loop() {
+    uint64_t seen = 0;
+    while (true) {
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return gen_ != seen; });
+        seen = gen_;
+        if (stop_) return;
+      }
+      work();
+      std::lock_guard<std::mutex> g(mu_);
+      if (--active_ == 0) done_cv_.notify_one();
+    }
+  }
+  std::vector<std::thread> workers_;
+  std::mutex mu_, run_mu_;
+  std::condition_variable cv_, done_cv_;
+  const std::function<void(int)>* fn_ = nullptr;
+  int n_ = 0, active_ = 0;
+  std::atomic<int> next_{0};
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+  std::exception_ptr err_;
+};
+}  // namespace
+
 void parallel_for(int n, const std::function<void(int)>& fn) {
-  const int nt = std::min(n, std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))));
-  if (n <= 8 || nt <= 1) {
+  if (n <= 8) {
     for (int i = 0; i < n; ++i) fn(i);
     return;
   }
-  std::atomic<int> next{0};
-  std::vector<std::thread> th;
-  std::exception_ptr err;
-  std::mutex mu;
-  for (int t = 0; t < nt; ++t)
-    th.emplace_back([&]() {
-      try {
-        for (int i = next++; i < n; i = next++) fn(i);
-      } catch (...) {
-        std::lock_guard<std::mutex> g(mu);
-        err = std::current_exception();
-      }
-    });
-  for (auto& x : th) x.join();
-  if (err) std::rethrow_exception(err);
+  HostPool::get().run(n, fn);
 }
 
 namespace {
@@ -54,6 +118,7 @@ namespace {
 constexpr int kBlock = 256;
 
 inline unsigned nblocks(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
 
 // ---------------------------------------------------------------------------
 // One 1D sweep (FastMultiplyLU::transform_1D, source/FastMultiplyLU.cpp:436-512).
@@ -67,7 +132,8 @@ struct SweepArgs {
   const int32_t* fbase;
   const int32_t* fmaxl;
   const int32_t* fmap;
-  const int32_t* lsum;  // level sum per element
+  const uint32_t* fbits;  // occupancy bits of the fiber map
+  const int32_t* lsum;    // level sum per element
   int64_t E;
   int R, Tin, Tout, sdd_in, sdd_out, ns;
   int cap;  // level-sum cap (coarse grid); negative: none
@@ -113,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
     }
     maxl = min(maxl, a.cap - ls + slot_level(st));
   }
-  for_each_related(st, maxl, a.flx != 0, a.lu, [&](int s) {
+  for_each_present(st, maxl, a.flx != 0, a.lu, a.fbits, fb, [&](int s) {
     const int src = a.fmap[fb + s];
     if (src < 0) return;
     const double* x = a.in + static_cast<int64_t>(src) * a.Tin + bin;
@@ -215,6 +281,27 @@ __global__ __launch_bounds__(kBlock) void k_fill(double* p, int64_t n, double v)
 __global__ __launch_bounds__(kBlock) void k_scale(double* u, const double* f, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) u[i] *= f[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// fmap[fbase[e] + slot[e]] = e and the matching occupancy bit: every element
+// owns one entry of its fiber.
+__global__ __launch_bounds__(kBlock) void k_fmap_scatter(int32_t* fmap, uint32_t* fbits, const int32_t* fbase,
+                                                         const int32_t* slot, int64_t E) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int b = fbase[e] + slot[e];
+  fmap[b] = static_cast<int32_t>(e);
+  atomicOr(&fbits[b >> 5], 1u << (b & 31));
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy(double* dst, const double* src, int64_t n) {
@@ -358,6 +445,7 @@ struct TermDev {
   const int32_t* fbase;
   const int32_t* fmaxl;
   const int32_t* fmap;
+  const uint32_t* fbits;
   const int32_t* lsum;
   int ns, pt1, dd, Tin, flx, lu;
   int sin[kMaxDim];  // strides of the input tensor
@@ -391,30 +479,37 @@ __device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const 
     maxl = min(maxl, cap - ls + slot_level(st));
   }
   double acc = 0.;
-  for_each_related_ranges(st, maxl, t.flx != 0, t.lu, [&](int lo, int hi) {
-    for (int s = lo; s < hi; s += 4) {
-      int src[4];
+  // sources are gathered four at a time so that their loads are in flight together
+  int pend[4];
+  int np = 0;
+  auto flush = [&]() {
+    int src[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) src[j] = (s + j < hi) ? t.fmap[fb + s + j] : -1;
-      double xv[4][PF1], mv[4][PF1];
+    for (int j = 0; j < 4; ++j) src[j] = (j < np) ? t.fmap[fb + pend[j]] : -1;
+    double xv[4][PF1], mv[4][PF1];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const double* x = t.in + static_cast<int64_t>(src[j] < 0 ? 0 : src[j]) * t.Tin + base;
-        const double* Mr = Mc + (s + j) * PF1 * t.pt1;
+    for (int j = 0; j < 4; ++j) {
+      const double* x = t.in + static_cast<int64_t>(src[j] < 0 ? 0 : src[j]) * t.Tin + base;
+      const double* Mr = Mc + (j < np ? pend[j] : 0) * PF1 * t.pt1;
 #pragma unroll
-        for (int kk = 0; kk < PF1; ++kk) {
-          xv[j][kk] = (src[j] >= 0) ? x[kk * sdd] : 0.;
-          mv[j][kk] = (src[j] >= 0) ? Mr[kk * t.pt1] : 0.;
-        }
+      for (int kk = 0; kk < PF1; ++kk) {
+        xv[j][kk] = (src[j] >= 0) ? x[kk * sdd] : 0.;
+        mv[j][kk] = (src[j] >= 0) ? Mr[kk * t.pt1] : 0.;
       }
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (src[j] >= 0) {
-#pragma unroll
-          for (int kk = 0; kk < PF1; ++kk) acc += xv[j][kk] * mv[j][kk];
-        }
     }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (src[j] >= 0) {
+#pragma unroll
+        for (int kk = 0; kk < PF1; ++kk) acc += xv[j][kk] * mv[j][kk];
+      }
+    np = 0;
+  };
+  for_each_present(st, maxl, t.flx != 0, t.lu, t.fbits, fb, [&](int s) {
+    pend[np++] = s;
+    if (np == 4) flush();
   });
+  if (np > 0) flush();
   return acc * t.coef;
 }
 
@@ -697,6 +792,7 @@ void Context::prof_collect() {
 void Context::prof_reset() {
   prof_collect();
   for (int i = 0; i < PROF_NCAT; ++i) { prof_ms[i] = 0; prof_bytes[i] = 0; prof_count[i] = 0; }
+  for (double& v : host_ms) v = 0;
 }
 
 SweepTable Context::table(int id) {
@@ -718,7 +814,7 @@ void Context::upload_grid() {
   uslot.resize(n * D);
   uhash.resize(n);
   uinst.resize(n);
-  std::vector<int32_t> dsl(n), fb(n), fml(n), fm;
+  std::vector<int32_t> dsl(n), fb(n), fml(n);
   for (int i = 0; i < B(); ++i) {
     const HostGrid& g = grids[i];
     std::copy(g.slot.begin(), g.slot.end(), uslot.begin() + off[i] * D);
@@ -726,24 +822,32 @@ void Context::upload_grid() {
     std::fill(uinst.begin() + off[i], uinst.begin() + off[i + 1], i);
   }
   for (int d = 0; d < D; ++d) {
-    fm.clear();
     int64_t fib_off = 0;
+    for (int i = 0; i < B(); ++i) fib_off += grids[i].nfib[d];
+    if ((fib_off << nmax) > (int64_t{1} << 31) - 1) throw std::runtime_error("fiber map exceeds int32 range");
+    fib_off = 0;
     for (int i = 0; i < B(); ++i) {
       const HostGrid& g = grids[i];
-      if (g.E == 0) continue;
       for (int64_t e = 0; e < g.E; ++e) {
         dsl[off[i] + e] = g.dslot[d][e];
         fb[off[i] + e] = g.fbase[d][e] + static_cast<int32_t>(fib_off << nmax);
         fml[off[i] + e] = g.fmaxl[d][e];
       }
-      for (int32_t v : g.fmap[d]) fm.push_back(v < 0 ? -1 : v + static_cast<int32_t>(off[i]));
       fib_off += g.nfib[d];
     }
-    if (static_cast<int64_t>(fm.size()) > (int64_t{1} << 31) - 1) throw std::runtime_error("fiber map exceeds int32 range");
     upload(g_slot[d], dsl, stream);
     upload(g_fbase[d], fb, stream);
     upload(g_fmaxl[d], fml, stream);
-    upload(g_fmap[d], fm, stream);
+    // dense fiber map, scattered on the device: fmap[fbase[e] + slot[e]] = e
+    const int64_t nmap = std::max<int64_t>(fib_off << nmax, 1);
+    g_fmap[d].ensure(static_cast<size_t>(nmap));
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(nmap)), dim3(kBlock), 0, stream, g_fmap[d].p, nmap, -1);
+    const int64_t nwords = (nmap + 31) / 32;
+    g_fbits[d].ensure(static_cast<size_t>(nwords));
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(nwords)), dim3(kBlock), 0, stream, g_fbits[d].p, nwords, 0u);
+    if (n > 0)
+      hipLaunchKernelGGL(k_fmap_scatter, dim3(nblocks(n)), dim3(kBlock), 0, stream, g_fmap[d].p, g_fbits[d].p,
+                         g_fbase[d].p, g_slot[d].p, n);
   }
   upload(g_inst, uinst, stream);
   std::vector<int32_t> ls(n, 0);
@@ -781,16 +885,26 @@ void Context::set_grid_slots(const std::vector<int32_t>& slots, bool keep_fields
   set_grids(std::vector<std::vector<int32_t>>{slots}, keep_fields);
 }
 
-void Context::set_grids(const std::vector<std::vector<int32_t>>& slots, bool keep_fields) {
+void Context::set_grids(const std::vector<std::vector<int32_t>>& slots, bool keep_fields,
+                        const std::vector<char>* changed) {
+  HostTimer timer(*this, HOST_GRID);
   const int nb = static_cast<int>(slots.size());
+  // instances whose element set did not change keep their table
+  auto reuse = [&](int i) { return changed && nb == B() && !(*changed)[i]; };
   std::vector<HostGrid> ng(nb);
-  parallel_for(nb, [&](int i) { ng[i].build(D, nmax, slots[i], hk); });
+  parallel_for(nb, [&](int i) {
+    if (!reuse(i)) ng[i].build(D, nmax, slots[i], hk);
+  });
   std::vector<int64_t> noff(nb + 1, 0);
-  for (int i = 0; i < nb; ++i) noff[i + 1] = noff[i] + ng[i].E;
+  for (int i = 0; i < nb; ++i) noff[i + 1] = noff[i] + (reuse(i) ? grids[i].E : ng[i].E);
   const int64_t newE = noff[nb];
   if (keep_fields && E() > 0 && nb == B()) {
     std::vector<int32_t> m(newE, -1);
     parallel_for(nb, [&](int i) {
+      if (reuse(i)) {
+        for (int64_t j = 0; j < grids[i].E; ++j) m[noff[i] + j] = static_cast<int32_t>(off[i] + j);
+        return;
+      }
       const std::vector<int32_t> mi = map_new_to_old(grids[i], ng[i]);
       for (int64_t j = 0; j < ng[i].E; ++j) m[noff[i] + j] = mi[j] < 0 ? -1 : static_cast<int32_t>(off[i] + mi[j]);
     });
@@ -814,6 +928,8 @@ void Context::set_grids(const std::vector<std::vector<int32_t>>& slots, bool kee
     up_host.clear();
     new_add.assign(newE, 1);
   }
+  for (int i = 0; i < nb; ++i)
+    if (reuse(i)) ng[i] = std::move(grids[i]);
   grids = std::move(ng);
   off = noff;
   ++grid_version;
@@ -839,6 +955,7 @@ void Context::sweep(const double* in, const int* sin, double* out, double* accum
   SweepArgs a{};
   a.in = in; a.out = out; a.accum = accum; a.M = tab.dev; a.ns = tab.ns;
   a.slot = g_slot[dd].p; a.fbase = g_fbase[dd].p; a.fmaxl = g_fmaxl[dd].p; a.fmap = g_fmap[dd].p;
+  a.fbits = g_fbits[dd].p;
   a.lsum = g_lsum.p; a.cap = lsum_cap;
   a.E = E();
   int stin[kMaxDim], stout[kMaxDim];
@@ -1049,6 +1166,7 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
   } else {
     slot = !plans[0] ? 0 : (!plans[1] ? 1 : 0);
   }
+  HostTimer timer(*this, HOST_PLAN);
   // temporaries: (D-1) steps of 2^(D-1) eval_up chains and 2*D*2^(D-1) rhs chains
   std::vector<std::vector<int>> lus;
   const std::vector<std::vector<int>> dims = transform_orders(D, lus);
@@ -1072,6 +1190,7 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     TermDev t{};
     t.in = in; t.M = tab.dev; t.ns = tab.ns; t.pt1 = pt1; t.dd = dd;
     t.slot = g_slot[dd].p; t.fbase = g_fbase[dd].p; t.fmaxl = g_fmaxl[dd].p; t.fmap = g_fmap[dd].p;
+    t.fbits = g_fbits[dd].p;
     t.lsum = g_lsum.p;
     t.Tin = prod_of(sin_sz, D);
     strides_of(sin_sz, D, t.sin);
@@ -1278,6 +1397,7 @@ int64_t Context::refine(double eps) { return refine_instances(std::vector<double
 int64_t Context::coarsen(double eta) { return coarsen_instances(std::vector<double>(B(), eta)); }
 
 int64_t Context::refine_instances(const std::vector<double>& eps) {
+  HostTimer timer(*this, HOST_ADAPT);
   if (static_cast<int>(eps.size()) != B()) throw std::runtime_error("refine: one threshold per instance expected");
   std::vector<double> s(E());
   indicator_sumsq(s.data());
@@ -1289,12 +1409,14 @@ int64_t Context::refine_instances(const std::vector<double>& eps) {
     ns[i] = refine_slots(grids[i], ni, eps[i], added[i]);
   });
   int64_t tot = 0;
-  for (int64_t a : added) tot += a;
-  if (tot > 0) set_grids(ns, true);
+  std::vector<char> changed(B(), 0);
+  for (int i = 0; i < B(); ++i) { tot += added[i]; changed[i] = added[i] > 0; }
+  if (tot > 0) set_grids(ns, true, &changed);
   return tot;
 }
 
 int64_t Context::coarsen_instances(const std::vector<double>& eta) {
+  HostTimer timer(*this, HOST_ADAPT);
   if (static_cast<int>(eta.size()) != B()) throw std::runtime_error("coarsen: one threshold per instance expected");
   std::vector<double> s(E());
   indicator_sumsq(s.data());
@@ -1306,8 +1428,9 @@ int64_t Context::coarsen_instances(const std::vector<double>& eta) {
     ns[i] = coarsen_slots(grids[i], ni, eta[i], removed[i]);
   });
   int64_t tot = 0;
-  for (int64_t r : removed) tot += r;
-  if (tot > 0) set_grids(ns, true);
+  std::vector<char> changed(B(), 0);
+  for (int i = 0; i < B(); ++i) { tot += removed[i]; changed[i] = removed[i] > 0; }
+  if (tot > 0) set_grids(ns, true, &changed);
   return tot;
 }
 

@@ -19,6 +19,7 @@
 // in ascending slot order (level-major, then position), which is the
 // summation order the device kernels and the oracle both use.
 #pragma once
+#include <cstdint>
 
 #if defined(__HIPCC__)
 #define ADG_HD __host__ __device__ __forceinline__
@@ -112,6 +113,32 @@ template <class F>
 ADG_HD void for_each_related(int st, int max_level, bool flx, int lu, F&& f) {
   for_each_related_ranges(st, max_level, flx, lu, [&](int lo, int hi) {
     for (int s = lo; s < hi; ++s) f(s);
+  });
+}
+
+// Present source slots of a target, ascending.  Short runs (ancestors and flux
+// neighbours) are passed on as they are and resolved through the fiber map;
+// longer runs (descendants) are read from the fiber's occupancy bits, 32
+// slots per load, so that absent cells cost nothing.
+template <class F>
+ADG_HD void for_each_present(int st, int maxl, bool flx, int lu, const uint32_t* bits, int fb, F&& f) {
+  for_each_related_ranges(st, maxl, flx, lu, [&](int lo, int hi) {
+    if (hi - lo <= 2) {
+      for (int s = lo; s < hi; ++s) f(s);
+      return;
+    }
+    const int b0 = fb + lo, b1 = fb + hi;  // bit range [b0, b1)
+    for (int w = b0 >> 5; w <= (b1 - 1) >> 5; ++w) {
+      uint32_t m = bits[w];
+      const int wb = w << 5;
+      if (wb < b0) m &= ~0u << (b0 - wb);
+      if (wb + 32 > b1) m &= ~0u >> (wb + 32 - b1);
+      while (m) {
+        const int b = __builtin_ctz(m);
+        m &= m - 1;
+        f(wb + b - fb);
+      }
+    }
   });
 }
 

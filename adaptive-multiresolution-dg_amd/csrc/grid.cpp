@@ -63,12 +63,10 @@ void HostGrid::build(int D_, int nmax_, const std::vector<int32_t>& slots, const
       dslot[dd][e] = slot[e * D + dd];
     }
     nfib[dd] = static_cast<int32_t>(fid.size());
-    fmap[dd].assign(static_cast<size_t>(nfib[dd]) << nmax, -1);
     std::vector<int32_t> maxl(nfib[dd], 0);
     for (int64_t e = 0; e < n; ++e) {
       const int f = fiber_of[e];
       fbase[dd][e] = f << nmax;
-      fmap[dd][(static_cast<size_t>(f) << nmax) + dslot[dd][e]] = static_cast<int32_t>(e);
       maxl[f] = std::max(maxl[f], level_of_slot(dslot[dd][e]));
     }
     for (int64_t e = 0; e < n; ++e) fmaxl[dd][e] = maxl[fiber_of[e]];

@@ -6,9 +6,10 @@
 // dimension d the table also carries the fiber structure the device kernels
 // use instead of the reference's per-element neighbour pointer sets
 // (Element::ptr_vol_alpt / ptr_flx_alpt, include/Element.h:152-155):
-//   fbase[d][e]  offset of e's d-fiber in fmap[d] (fiber id << nmax)
-//   fmap[d][fbase + s] = element index holding slot s of that fiber, or -1
+//   fbase[d][e]  offset of e's d-fiber in the device fiber map (fiber id << nmax)
 //   fmaxl[d][e]  deepest level present in e's d-fiber
+// The dense fiber map itself (fmap[d][fbase + s] = element holding slot s of
+// the fiber, or -1) is scattered on the device from fbase and the slots.
 // The refine / coarsen rules are the reference's (DGAdapt::refine,
 // DGAdapt::coarsen, source/DGAdapt.cpp:371-404,675-688,982-1016,1250-1385)
 // expressed as set operations on the table.
@@ -32,7 +33,6 @@ struct HostGrid {
   std::vector<int32_t> dslot[kMaxDim];      // [E]  slot of element e in dim d
   std::vector<int32_t> fbase[kMaxDim];      // [E]
   std::vector<int32_t> fmaxl[kMaxDim];      // [E]
-  std::vector<int32_t> fmap[kMaxDim];       // [nfib << nmax]
   int32_t nfib[kMaxDim] = {0, 0, 0, 0};
 
   static uint64_t pack(const int32_t* s, int D) {

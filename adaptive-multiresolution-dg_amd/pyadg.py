@@ -97,6 +97,7 @@ def lib():
             "adg_state_restore": (i32, [vp, i32]),
             "adg_profile": (i32, [vp, i32]),
             "adg_profile_get": (i32, [vp, DP, P(i64), DP]),
+            "adg_profile_host": (i32, [vp, DP]),
             "adg_init_separable": (i32, [vp, SEP_FN, vp, i32, dbl]),
             "adg_sep_eval": (dbl, [dbl, i32, i32, vp]),
             "adg_filter": (i32, [vp, dbl, DP, dbl, i32]),
@@ -427,6 +428,11 @@ class Solution:
         self._check(lib().adg_profile_get(self._h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _dp(by)))
         names = ["sweep", "surplus", "flux", "rk", "other"]
         return {n: {"ms": float(ms[i]), "count": int(cnt[i]), "bytes": float(by[i])} for i, n in enumerate(names)}
+
+    def profile_host(self):
+        ms = np.zeros(4)
+        self._check(lib().adg_profile_host(self._h, _dp(ms)))
+        return dict(zip(["adapt", "grid", "plan", "step"], (float(v) for v in ms)))
 
     def sync(self):
         self._check(lib().adg_sync(self._h))

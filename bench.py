@@ -141,12 +141,13 @@ def main():
         s.sync()
         sec = time.perf_counter() - t0
         prof = s.profile_get()
+        host = s.profile_host()
         s.profile(False)
         u = s.instance_get(0) if s.num_instances > 1 else s.get(pyadg.FIELD_UCOE)
         print(f"[bench] form {form}, {s.num_instances} instance(s): {args.steps} solves ({time_steps} time steps) "
               f"in {sec:.3f} s, {dof_stages / sec:.4g} DoF-stage/s", file=sys.stderr, flush=True)
         return {"value": dof_stages / sec, "sec": sec, "dof_stages": dof_stages, "time_steps": time_steps,
-                "max_elem": max_elem, "prof": prof, "u": u, "sim_time": st.final_time}
+                "max_elem": max_elem, "prof": prof, "host": host, "u": u, "sim_time": st.final_time}
 
     plain = measure(sol, 0)
     fused = plain if w.linear else measure(sol, 1)
@@ -167,7 +168,8 @@ def main():
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": names[cat],
                     "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes, "launches": pc["count"],
-                    "kernel_ms_by_category": {k: round(v["ms"], 3) for k, v in prof.items()}}
+                    "kernel_ms_by_category": {k: round(v["ms"], 3) for k, v in prof.items()},
+                    "host_ms_inclusive": {k: round(v, 3) for k, v in chosen["host"].items()}}
 
     cpu = None
     if not args.no_cpu:

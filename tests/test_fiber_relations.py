@@ -26,6 +26,9 @@ def enum_lib():
     L = ctypes.CDLL(so)
     L.fiber_related.restype = ctypes.c_int
     L.fiber_related.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+    L.fiber_present.restype = ctypes.c_int
+    L.fiber_present.argtypes = [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_uint32), ctypes.c_int,
+                                                     ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
     return L
 
 
@@ -86,3 +89,32 @@ def pyadg_ic(p, x):
         t = np.minimum(t, 1 - t)
         v += p.bump_amp[b] * np.exp(-(t @ t) / p.bump_width[b] ** 2)
     return v
+
+
+@pytest.mark.parametrize("nmax", [3, 5, 8])
+def test_present_enumeration_matches_filtered_relations(enum_lib, nmax):
+    # the occupancy-bit walk yields exactly the present related slots, in ascending order,
+    # for fibers that start anywhere inside a word of bits
+    rng = np.random.default_rng(nmax)
+    ns = 1 << nmax
+    f_rel = enum_lib.fiber_related
+    f_pre = enum_lib.fiber_present
+    for trial in range(40):
+        fb = int(rng.integers(0, 96))
+        present = rng.random(ns) < rng.choice([0.1, 0.5, 0.9])
+        bits = np.zeros((fb + ns + 31) // 32 + 1, np.uint32)
+        for s in np.nonzero(present)[0]:
+            b = fb + int(s)
+            bits[b >> 5] |= np.uint32(1 << (b & 31))
+        bits_p = bits.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        out = np.zeros(4 * ns, np.int32)
+        out_p = out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        for st in rng.integers(0, ns, 12):
+            for flx in (0, 1):
+                for lu in (0, 1, 2):
+                    maxl = int(rng.integers(0, nmax + 1))
+                    n = f_rel(int(st), maxl, flx, lu, out_p, out.size)
+                    want = [int(s) for s in out[:n] if present[s]]
+                    n = f_pre(int(st), maxl, flx, lu, bits_p, fb, out_p, out.size)
+                    got = [int(s) for s in out[:n] if present[s]]
+                    assert got == want, (st, flx, lu, maxl)

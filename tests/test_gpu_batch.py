@@ -77,3 +77,23 @@ def test_batch_per_instance_thresholds():
         assert np.array_equal(bhk, hk) and np.array_equal(blv, lv), f"instance {i}: element sets differ"
         bu = b.instance_get(i)
         assert np.array_equal(bu, u), f"instance {i}: " + first_diff(bu, u)
+
+
+def test_large_batch_matches_single_gpu_solves():
+    # more instances than the serial threshold of the host pool: table builds run on several threads
+    seeds = list(range(11, 23))
+    ws = [workloads.c2_burgers(seed=s, nmax=5, level_init=3, t_final=0.02) for s in seeds]
+    print(f"[case] batch of {len(seeds)} instances, nmax=5", flush=True)
+    b = workloads.make_batch(ws)
+    prob = ws[0].problem
+    prob.form = 1
+    b.run_hyperbolic(prob)
+    for i, w in enumerate(ws):
+        s = workloads.make_solution(w)
+        w.problem.form = 1
+        s.run_hyperbolic(w.problem)
+        lv, sp, hk = s.grid()
+        blv, bsp, bhk = b.instance_grid(i)
+        assert np.array_equal(bhk, hk) and np.array_equal(blv, lv), f"instance {i}: element sets differ"
+        a, c = b.instance_get(i), s.get(pyadg.FIELD_UCOE)
+        assert np.array_equal(a, c), f"instance {i}: " + first_diff(a, c)
