@@ -183,7 +183,7 @@ class Context {
   std::vector<ProfRec> prof_pending;
   std::vector<hipEvent_t> prof_pool;
   // host wall time (inclusive) of the adaptive bookkeeping, recorded while profiling
-  enum { HOST_ADAPT = 0, HOST_GRID, HOST_PLAN, HOST_STEP, HOST_NCAT };
+  enum { HOST_ADAPT = 0, HOST_GRID, HOST_PLAN, HOST_STEP, HOST_BUILD, HOST_UPLOAD, HOST_NCAT };
   double host_ms[HOST_NCAT] = {0};
   struct HostTimer {
     Context& c;

@@ -116,6 +116,18 @@ void parallel_for(int n, const std::function<void(int)>& fn) {
 namespace {
 
 constexpr int kBlock = 256;
+// Final fused kernels: a block is (terms x outputs); each wavefront evaluates
+// one term for consecutive outputs (same element, same fiber walk), the
+// terms of an output are then added in order from LDS.
+constexpr int kFinalBlock = 1024;
+
+// outputs per block of a final kernel with `nterm` terms: a power of two, at
+// most one wavefront, with nterm * outputs <= kFinalBlock
+inline int final_outputs_per_block(int nterm) {
+  int o = 64;
+  while (o > 1 && o * nterm > kFinalBlock) o /= 2;
+  return o;
+}
 
 inline unsigned nblocks(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
@@ -479,16 +491,17 @@ __device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const 
     maxl = min(maxl, cap - ls + slot_level(st));
   }
   double acc = 0.;
-  // sources are gathered four at a time so that their loads are in flight together
-  int pend[4];
+  // sources are gathered NB at a time so that their loads are in flight together
+  constexpr int NB = (PF1 <= 4) ? 4 : 2;
+  int pend[NB];
   int np = 0;
   auto flush = [&]() {
-    int src[4];
+    int src[NB];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) src[j] = (j < np) ? t.fmap[fb + pend[j]] : -1;
-    double xv[4][PF1], mv[4][PF1];
+    for (int j = 0; j < NB; ++j) src[j] = (j < np) ? t.fmap[fb + pend[j]] : -1;
+    double xv[NB][PF1], mv[NB][PF1];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NB; ++j) {
       const double* x = t.in + static_cast<int64_t>(src[j] < 0 ? 0 : src[j]) * t.Tin + base;
       const double* Mr = Mc + (j < np ? pend[j] : 0) * PF1 * t.pt1;
 #pragma unroll
@@ -498,7 +511,7 @@ __device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const 
       }
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < NB; ++j)
       if (src[j] >= 0) {
 #pragma unroll
         for (int kk = 0; kk < PF1; ++kk) acc += xv[j][kk] * mv[j][kk];
@@ -507,7 +520,7 @@ __device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const 
   };
   for_each_present(st, maxl, t.flx != 0, t.lu, t.fbits, fb, [&](int s) {
     pend[np++] = s;
-    if (np == 4) flush();
+    if (np == NB) flush();
   });
   if (np > 0) flush();
   return acc * t.coef;
@@ -541,14 +554,13 @@ __global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntas
 // and the point's thread 0 adds the terms in order, so the dependent-load
 // chains of the terms overlap while the sum keeps the reference order.
 template <int PF1>
-__global__ __launch_bounds__(kBlock) void k_up_final(const TermDev* terms, int nterm, int ntp, int64_t E, int P, int p1,
-                                                      int D, double* up, double* fp, const double* vcoef, int kind,
-                                                      double s, int ncomp, int cap) {
-  __shared__ double vals[kBlock];
-  const int per_block = kBlock / ntp;
-  const int lo = threadIdx.x / ntp, t = threadIdx.x - lo * ntp;
+__global__ __launch_bounds__(kFinalBlock) void k_up_final(const TermDev* terms, int nterm, int per_block, int64_t E,
+                                                           int P, int p1, int D, double* up, double* fp,
+                                                           const double* vcoef, int kind, double s, int ncomp, int cap) {
+  __shared__ double vals[kFinalBlock];
+  const int t = threadIdx.x / per_block, lo = threadIdx.x - t * per_block;
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * per_block + lo;
-  const bool live = gid < E * P && lo < per_block;
+  const bool live = gid < E * P;
   int64_t e = 0;
   int q = 0;
   if (live) {
@@ -566,7 +578,7 @@ __global__ __launch_bounds__(kBlock) void k_up_final(const TermDev* terms, int n
   __syncthreads();
   if (!live || t != 0) return;
   double u = 0.;
-  for (int j = 0; j < nterm; ++j) u += vals[lo * ntp + j];
+  for (int j = 0; j < nterm; ++j) u += vals[j * per_block + lo];
   up[gid] = u;
   const int64_t n = E * P;
   if (kind == 0) {
@@ -584,16 +596,15 @@ __global__ __launch_bounds__(kBlock) void k_up_final(const TermDev* terms, int n
 // u_next (u itself is still read by other threads' penalty sweeps).  Threads
 // are (DoF, term) pairs as in k_up_final.
 template <int PFM, int PFK>
-__global__ __launch_bounds__(kBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens, int npen,
-                                                       int ntp, int64_t E, int K, int k1, int D, double* rhs,
-                                                       const double* u, const double* tn, double* u_next, int scheme,
-                                                       int stage, double dt, const double* dtv, const int32_t* inst,
-                                                       int cap) {
-  __shared__ double vals[kBlock];
-  const int per_block = kBlock / ntp;
-  const int lo = threadIdx.x / ntp, t = threadIdx.x - lo * ntp;
+__global__ __launch_bounds__(kFinalBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens,
+                                                            int npen, int per_block, int64_t E, int K, int k1, int D,
+                                                            double* rhs, const double* u, const double* tn,
+                                                            double* u_next, int scheme, int stage, double dt,
+                                                            const double* dtv, const int32_t* inst, int cap) {
+  __shared__ double vals[kFinalBlock];
+  const int t = threadIdx.x / per_block, lo = threadIdx.x - t * per_block;
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * per_block + lo;
-  const bool live = gid < E * K && lo < per_block;
+  const bool live = gid < E * K;
   int64_t e = 0;
   int q = 0;
   if (live) {
@@ -611,17 +622,12 @@ __global__ __launch_bounds__(kBlock) void k_rhs_final(const TermDev* terms, int 
   __syncthreads();
   if (!live || t != 0) return;
   double r = 0.;
-  for (int j = 0; j < nterm + npen; ++j) r += vals[lo * ntp + j];
+  for (int j = 0; j < nterm + npen; ++j) r += vals[j * per_block + lo];
   rhs[gid] = r;
   if (dtv) dt = dtv[inst[e]];
   u_next[gid] = rk_combine(scheme, stage, dt, tn[gid], u[gid], r);
 }
 
-inline int pow2_at_least(int n) {
-  int p = 1;
-  while (p < n) p *= 2;
-  return p;
-}
 
 template <int PF1>
 void launch_phase_t(const TaskDev* tasks, int ntask, int64_t total, int D, int cap, hipStream_t s) {
@@ -633,6 +639,7 @@ void launch_phase(int pf1, const TaskDev* tasks, int ntask, int64_t total, int D
     case 2: launch_phase_t<2>(tasks, ntask, total, D, cap, s); return;
     case 3: launch_phase_t<3>(tasks, ntask, total, D, cap, s); return;
     case 4: launch_phase_t<4>(tasks, ntask, total, D, cap, s); return;
+    case 5: launch_phase_t<5>(tasks, ntask, total, D, cap, s); return;
   }
   throw std::runtime_error("phase: unsupported basis size");
 }
@@ -646,8 +653,9 @@ void launch_sweep(const SweepArgs& a, int pf1, int pt1, hipStream_t s) {
 #define ADG_SW(F, T) if (pf1 == F && pt1 == T) { launch_sweep_t<F, T>(a, s); return; }
   ADG_SW(2, 2) ADG_SW(2, 3) ADG_SW(2, 4) ADG_SW(3, 2) ADG_SW(3, 3) ADG_SW(3, 4)
   ADG_SW(4, 2) ADG_SW(4, 3) ADG_SW(4, 4)
+  ADG_SW(2, 5) ADG_SW(3, 5) ADG_SW(4, 5) ADG_SW(5, 2) ADG_SW(5, 3) ADG_SW(5, 4)
 #undef ADG_SW
-  throw std::runtime_error("sweep: unsupported basis sizes (degrees 1..3 are provided)");
+  throw std::runtime_error("sweep: unsupported basis sizes (Alpert degrees 1..3, Lagrange 1..4 are provided)");
 }
 
 // Dense level-ordered table (rows ns*pf1, cols ns*pt1) -> target-major cell blocks.
@@ -699,7 +707,7 @@ Context::Context(int dim, int k_, int m_, int nmax_)
     : D(dim), k(k_), m(m_), nmax(nmax_),
       alpt(BasisKind::Alpert, k_, nmax_), lagr(BasisKind::Lagrange, m_, nmax_) {
   if (D < 1 || D > kMaxDim) throw std::runtime_error("dimension must be 1..4");
-  if (k < 1 || k > 3 || m < 1 || m > 3) throw std::runtime_error("degrees k and m must be 1..3");
+  if (k < 1 || k > 3 || m < 1 || m > 4) throw std::runtime_error("degrees must be k in 1..3 and m in 1..4");
   if (nmax < 1 || nmax > 13) throw std::runtime_error("maximum level must be 1..13");
   K = 1; P = 1;
   for (int d = 0; d < D; ++d) { K *= k + 1; P *= m + 1; }
@@ -808,6 +816,7 @@ SweepTable Context::table(int id) {
 }
 
 void Context::upload_grid() {
+  HostTimer timer(*this, HOST_UPLOAD);
   // union of the instance tables: element indices and fiber ids are shifted by
   // the instance offsets, so fibers stay within their instance
   const int64_t n = E();
@@ -892,9 +901,12 @@ void Context::set_grids(const std::vector<std::vector<int32_t>>& slots, bool kee
   // instances whose element set did not change keep their table
   auto reuse = [&](int i) { return changed && nb == B() && !(*changed)[i]; };
   std::vector<HostGrid> ng(nb);
-  parallel_for(nb, [&](int i) {
-    if (!reuse(i)) ng[i].build(D, nmax, slots[i], hk);
-  });
+  {
+    HostTimer tb(*this, HOST_BUILD);
+    parallel_for(nb, [&](int i) {
+      if (!reuse(i)) ng[i].build(D, nmax, slots[i], hk);
+    });
+  }
   std::vector<int64_t> noff(nb + 1, 0);
   for (int i = 0; i < nb; ++i) noff[i + 1] = noff[i] + (reuse(i) ? grids[i].E : ng[i].E);
   const int64_t newE = noff[nb];
@@ -1313,14 +1325,15 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   }
   {
     const int64_t n = E() * P;
-    const int ntp = pow2_at_least(pl.n_up_terms);
-    if (ntp > kBlock) throw std::runtime_error("too many eval_up terms");
-    const unsigned nb = static_cast<unsigned>((n + kBlock / ntp - 1) / (kBlock / ntp));
+    const int per_block = final_outputs_per_block(pl.n_up_terms);
+    if (per_block * pl.n_up_terms > kFinalBlock) throw std::runtime_error("too many eval_up terms");
+    const unsigned nb = static_cast<unsigned>((n + per_block - 1) / per_block);
+    const unsigned bs = static_cast<unsigned>(per_block * pl.n_up_terms);
     prof_begin(PROF_FLUX, pl.up_final_bytes);
     switch (k + 1) {
-      case 2: hipLaunchKernelGGL((k_up_final<2>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
-      case 3: hipLaunchKernelGGL((k_up_final<3>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
-      case 4: hipLaunchKernelGGL((k_up_final<4>), dim3(nb), dim3(kBlock), 0, stream, pl.up_terms.p, pl.n_up_terms, ntp, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
+      case 2: hipLaunchKernelGGL((k_up_final<2>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
+      case 3: hipLaunchKernelGGL((k_up_final<3>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
+      case 4: hipLaunchKernelGGL((k_up_final<4>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
     }
     prof_end();
   }
@@ -1334,18 +1347,20 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   }
   {
     const int64_t n = E() * K;
-    const int ntp = pow2_at_least(pl.n_rhs_terms + pl.n_pen);
-    if (ntp > kBlock) throw std::runtime_error("too many right-hand-side terms");
-    const unsigned nb = static_cast<unsigned>((n + kBlock / ntp - 1) / (kBlock / ntp));
+    const int nt = pl.n_rhs_terms + pl.n_pen;
+    const int per_block = final_outputs_per_block(nt);
+    if (per_block * nt > kFinalBlock) throw std::runtime_error("too many right-hand-side terms");
+    const unsigned nb = static_cast<unsigned>((n + per_block - 1) / per_block);
+    const unsigned bs = static_cast<unsigned>(per_block * nt);
     prof_begin(PROF_RK, pl.rhs_final_bytes);
 #define ADG_RF(M1, K1)                                                                                             \
   if (m + 1 == M1 && k + 1 == K1) {                                                                                \
-    hipLaunchKernelGGL((k_rhs_final<M1, K1>), dim3(nb), dim3(kBlock), 0, stream, pl.rhs_terms.p, pl.n_rhs_terms,  \
-                       pl.pens.p, pl.n_pen, ntp, E(), K, k + 1, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt,   \
+    hipLaunchKernelGGL((k_rhs_final<M1, K1>), dim3(nb), dim3(bs), 0, stream, pl.rhs_terms.p, pl.n_rhs_terms,      \
+                       pl.pens.p, pl.n_pen, per_block, E(), K, k + 1, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt, \
                        use_inst_dt ? d_inst_dt.p : nullptr, g_inst.p, lsum_cap);                                    \
   }
     ADG_RF(2, 2) ADG_RF(2, 3) ADG_RF(2, 4) ADG_RF(3, 2) ADG_RF(3, 3) ADG_RF(3, 4) ADG_RF(4, 2) ADG_RF(4, 3)
-    ADG_RF(4, 4)
+    ADG_RF(4, 4) ADG_RF(5, 2) ADG_RF(5, 3) ADG_RF(5, 4)
 #undef ADG_RF
     prof_end();
   }

@@ -3,7 +3,6 @@
 #include <algorithm>
 #include <numeric>
 #include <stdexcept>
-#include <unordered_set>
 
 namespace adg {
 
@@ -38,38 +37,43 @@ void HostGrid::build(int D_, int nmax_, const std::vector<int32_t>& slots, const
   E = n;
   slot.resize(n * D);
   hash.resize(n);
-  index.clear();
-  index.reserve(n * 2);
+  index.reset(static_cast<size_t>(n));
   for (int64_t i = 0; i < n; ++i) {
     const int64_t e = order[i];
     for (int d = 0; d < D; ++d) slot[i * D + d] = slots[e * D + d];
     hash[i] = keys[e];
     if (i > 0 && hash[i] == hash[i - 1]) throw std::runtime_error("grid: duplicate hash key");
-    index[pack(&slot[i * D], D)] = static_cast<int32_t>(i);
+    index.insert(pack(&slot[i * D], D), static_cast<int32_t>(i));
   }
-  // fibers per dimension
+  // fibers per dimension: elements sorted by their slots outside dd, runs of
+  // equal keys are fibers
+  std::vector<uint64_t> fk(n);
+  std::vector<int64_t> by(n);
   for (int dd = 0; dd < D; ++dd) {
     dslot[dd].resize(n);
     fbase[dd].resize(n);
     fmaxl[dd].resize(n);
-    std::unordered_map<uint64_t, int32_t> fid;
-    fid.reserve(n);
-    std::vector<int32_t> fiber_of(n);
     for (int64_t e = 0; e < n; ++e) {
       int32_t s[kMaxDim];
       for (int d = 0; d < D; ++d) s[d] = (d == dd) ? 0 : slot[e * D + d];
-      auto it = fid.emplace(pack(s, D), static_cast<int32_t>(fid.size())).first;
-      fiber_of[e] = it->second;
+      fk[e] = pack(s, D);
       dslot[dd][e] = slot[e * D + dd];
     }
-    nfib[dd] = static_cast<int32_t>(fid.size());
-    std::vector<int32_t> maxl(nfib[dd], 0);
-    for (int64_t e = 0; e < n; ++e) {
-      const int f = fiber_of[e];
-      fbase[dd][e] = f << nmax;
-      maxl[f] = std::max(maxl[f], level_of_slot(dslot[dd][e]));
+    std::iota(by.begin(), by.end(), 0);
+    std::sort(by.begin(), by.end(), [&](int64_t a, int64_t b) { return fk[a] < fk[b]; });
+    int32_t nf = 0;
+    for (int64_t i = 0; i < n;) {
+      int64_t j = i;
+      int maxl = 0;
+      while (j < n && fk[by[j]] == fk[by[i]]) { maxl = std::max(maxl, level_of_slot(dslot[dd][by[j]])); ++j; }
+      for (int64_t q = i; q < j; ++q) {
+        fbase[dd][by[q]] = nf << nmax;
+        fmaxl[dd][by[q]] = maxl;
+      }
+      ++nf;
+      i = j;
     }
-    for (int64_t e = 0; e < n; ++e) fmaxl[dd][e] = maxl[fiber_of[e]];
+    nfib[dd] = nf;
   }
 }
 
@@ -99,12 +103,12 @@ std::vector<int32_t> HostGrid::initial_slots(int D, int level_init, bool sparse)
 std::vector<int32_t> refine_slots(const HostGrid& g, const std::vector<double>& norm, double eps, int64_t& added) {
   const int D = g.D;
   std::vector<int32_t> out = g.slot;
-  std::unordered_set<uint64_t> present;
-  present.reserve(g.E * 2);
-  for (int64_t e = 0; e < g.E; ++e) present.insert(HostGrid::pack(&g.slot[e * D], D));
+  FlatIndex present;
+  present.reset(static_cast<size_t>(g.E) * 2);
+  for (int64_t e = 0; e < g.E; ++e) present.insert(HostGrid::pack(&g.slot[e * D], D), 0);
   std::vector<int32_t> work;  // newly added slot vectors, for the hole closure
   auto add = [&](const int32_t* s) {
-    if (present.insert(HostGrid::pack(s, D)).second) {
+    if (present.insert(HostGrid::pack(s, D), 0)) {
       out.insert(out.end(), s, s + D);
       work.insert(work.end(), s, s + D);
     }

@@ -15,7 +15,6 @@
 // expressed as set operations on the table.
 #pragma once
 #include <cstdint>
-#include <unordered_map>
 #include <vector>
 
 #include "setup.hpp"
@@ -24,12 +23,61 @@ namespace adg {
 
 constexpr int kMaxDim = 4;
 
+// Open-addressing map from packed slot vectors to element indices (linear
+// probing, load factor <= 1/2): one allocation instead of one per element.
+class FlatIndex {
+ public:
+  static constexpr uint64_t kEmpty = ~uint64_t{0};
+  void reset(size_t n) {
+    size_t cap = 16;
+    while (cap < 2 * n + 2) cap <<= 1;
+    key_.assign(cap, kEmpty);
+    val_.assign(cap, -1);
+    mask_ = cap - 1;
+    size_ = 0;
+  }
+  // inserts (k, v) unless k is present; returns true when inserted
+  bool insert(uint64_t k, int32_t v) {
+    if (2 * (size_ + 1) > key_.size()) grow();
+    for (uint64_t i = hash(k) & mask_;; i = (i + 1) & mask_) {
+      if (key_[i] == kEmpty) { key_[i] = k; val_[i] = v; ++size_; return true; }
+      if (key_[i] == k) return false;
+    }
+  }
+  int32_t find(uint64_t k) const {
+    if (key_.empty()) return -1;
+    for (uint64_t i = hash(k) & mask_;; i = (i + 1) & mask_) {
+      if (key_[i] == k) return val_[i];
+      if (key_[i] == kEmpty) return -1;
+    }
+  }
+  size_t size() const { return size_; }
+
+ private:
+  static uint64_t hash(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  void grow() {
+    std::vector<uint64_t> k = std::move(key_);
+    std::vector<int32_t> v = std::move(val_);
+    reset(k.empty() ? 8 : k.size());
+    for (size_t i = 0; i < k.size(); ++i)
+      if (k[i] != kEmpty) insert(k[i], v[i]);
+  }
+  std::vector<uint64_t> key_;
+  std::vector<int32_t> val_;
+  uint64_t mask_ = 0;
+  size_t size_ = 0;
+};
+
 struct HostGrid {
   int D = 0, nmax = 0;
   int64_t E = 0;
   std::vector<int32_t> slot;                // [E * D]
   std::vector<int32_t> hash;                // [E]
-  std::unordered_map<uint64_t, int32_t> index;
+  FlatIndex index;
   std::vector<int32_t> dslot[kMaxDim];      // [E]  slot of element e in dim d
   std::vector<int32_t> fbase[kMaxDim];      // [E]
   std::vector<int32_t> fmaxl[kMaxDim];      // [E]
@@ -42,10 +90,7 @@ struct HostGrid {
   }
   int level(int64_t e, int d) const { return level_of_slot(slot[e * D + d]); }
   int suppt(int64_t e, int d) const { return suppt_of_slot(slot[e * D + d]); }
-  int32_t find(const int32_t* s) const {
-    auto it = index.find(pack(s, D));
-    return it == index.end() ? -1 : it->second;
-  }
+  int32_t find(const int32_t* s) const { return index.find(pack(s, D)); }
   int num_total_children(int64_t e) const;
 
   // Build the table from a list of element slot vectors (any order, no duplicates).

@@ -203,8 +203,22 @@ static double lagr_phi(double xt, int msh, int P, int p) {
       if (p == 2) return (x < 0) ? 0. : 125. / 2. * (x - 1. / 5.) * (x - 3. / 5.) * (x - 4. / 5.);
       if (p == 3) return (x < 0) ? 0. : 125. / 6. * (x - 1. / 5.) * (x - 2. / 5.) * (x - 3. / 5.);
     }
+  } else if (P == 4) {
+    if (msh == 0) {
+      if (p == 0) return 72. / 5. * (x - 1. / 3.) * (x - 1. / 6.) * (x + 1. / 3.) * (x + 5. / 12.);
+      if (p == 1) return -768. / 7. * (x - 1. / 3.) * (x - 1. / 6.) * (x + 1. / 3.) * (x + 2. / 3.);
+      if (p == 2) return 108 * (x - 1. / 3.) * (x - 1. / 6.) * (x + 5. / 12.) * (x + 2. / 3.);
+      if (p == 3) return -864. / 35. * (x - 1. / 3.) * (x + 1. / 3.) * (x + 5. / 12.) * (x + 2. / 3.);
+      if (p == 4) return 12 * (x - 1. / 6.) * (x + 1. / 3.) * (x + 5. / 12.) * (x + 2. / 3.);
+    } else {
+      if (p == 0) return (x <= 0) ? 1152. / 5. * (x + 1. / 3.) * (x + 5. / 12.) * (x + 2. / 3.) * (x + 17. / 24.) : 0.;
+      if (p == 1) return (x <= 0) ? -12288. / 7. * (x + 1. / 3.) * (x + 5. / 12.) * (x + 2. / 3.) * (x + 5. / 6.) : 0.;
+      if (p == 2) return (x < 0) ? 0. : -12288. / 7. * (x - 2. / 3.) * (x - 7. / 12.) * (x - 1. / 3.) * (x - 1. / 6.);
+      if (p == 3) return (x < 0) ? 0. : -13824. / 35. * (x - 2. / 3.) * (x - 1. / 3.) * (x - 7. / 24.) * (x - 1. / 6.);
+      if (p == 4) return (x < 0) ? 0. : 192 * (x - 7. / 12.) * (x - 1. / 3.) * (x - 7. / 24.) * (x - 1. / 6.);
+    }
   }
-  throw std::runtime_error("Lagrange basis: only degree 1..3, msh_case 1 are provided");
+  throw std::runtime_error("Lagrange basis: only degree 1..4, msh_case 1 are provided");
 }
 
 static double lagr_val0(double x, int n, int j, int p, int P) {
@@ -224,6 +238,10 @@ Basis1D::Basis1D(BasisKind k, int pmax_, int nmax_, int msh_case_)
     if (pmax == 1) { msh0 = {-1. / 3., 1. / 3.}; msh1 = {-2. / 3., 2. / 3.}; }
     else if (pmax == 2) { msh0 = {-2. / 3., -1. / 3., 1. / 3.}; msh1 = {-5. / 6., 1. / 6., 2. / 3.}; }
     else if (pmax == 3) { msh0 = {-3. / 5., -1. / 5., 1. / 5., 3. / 5.}; msh1 = {-4. / 5., -2. / 5., 2. / 5., 4. / 5.}; }
+    else if (pmax == 4) {
+      msh0 = {-2. / 3., -5. / 12., -1. / 3., 1. / 6., 1. / 3.};
+      msh1 = {-5. / 6., -17. / 24., 7. / 24., 7. / 12., 2. / 3.};
+    }
     else throw std::runtime_error("Lagrange basis: degree out of range");
     for (int i = 0; i < pmax + 1; ++i) {
       msh0[i] = (msh0[i] + 1) / 2.;
@@ -513,6 +531,29 @@ static int hk_pow(int x, int p) {
 }
 
 int HashKey::key(const int* l, const int* j, int d) const {
+  if (d < 1 || d > 4) return key_direct(l, j, d);
+  int code = 0;
+  for (int t = 0; t < d; ++t) {
+    if (l[t] < 0 || l[t] > 15) return key_direct(l, j, d);
+    code = code * 16 + l[t];
+  }
+  std::call_once(level_once_[d], [&]() {
+    std::vector<int>& tab = level_part_[d];
+    tab.assign(static_cast<size_t>(1) << (4 * d), 0);
+    std::vector<int> lv(d), ones(d, 1);
+    for (size_t c = 0; c < tab.size(); ++c) {
+      size_t r = c;
+      for (int t = d - 1; t >= 0; --t) { lv[t] = static_cast<int>(r % 16); r /= 16; }
+      // j = 1 in every dimension makes ind1 = 0: the key is the level part
+      tab[c] = key_direct(lv.data(), ones.data(), d);
+    }
+  });
+  int ind1 = 0;
+  for (int t = 0; t < d; ++t) ind1 = ind1 * hk_pow(2, l[t]) + (j[t] - 1) / 2;
+  return ind1 + level_part_[d][code];
+}
+
+int HashKey::key_direct(const int* l, const int* j, int d) const {
   int ind1 = 0, ind2 = 0, ind3 = 0;
   for (int t = 0; t < d; ++t) ind1 = ind1 * hk_pow(2, l[t]) + (j[t] - 1) / 2;
   int sum = l[0];

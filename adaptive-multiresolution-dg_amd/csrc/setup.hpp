@@ -18,6 +18,7 @@
 #pragma once
 #include <cstdint>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -109,7 +110,13 @@ class HashKey {
   HashKey();
   int key(const int* level, const int* suppt, int dim) const;
  private:
+  // the reference's arithmetic, term by term
+  int key_direct(const int* level, const int* suppt, int dim) const;
   std::vector<std::vector<double>> bicoef_;
+  // per dimension: the level-vector part (ind2 + ind3) of the key for every
+  // level vector with levels 0..15, computed once with key_direct's arithmetic
+  mutable std::vector<int> level_part_[5];
+  mutable std::once_flag level_once_[5];
 };
 
 // Gauss-Legendre tables of Quad::read_GL_quad_1D (1..10 points).

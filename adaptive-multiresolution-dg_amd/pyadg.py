@@ -430,9 +430,9 @@ class Solution:
         return {n: {"ms": float(ms[i]), "count": int(cnt[i]), "bytes": float(by[i])} for i, n in enumerate(names)}
 
     def profile_host(self):
-        ms = np.zeros(4)
+        ms = np.zeros(6)
         self._check(lib().adg_profile_host(self._h, _dp(ms)))
-        return dict(zip(["adapt", "grid", "plan", "step"], (float(v) for v in ms)))
+        return dict(zip(["adapt", "grid", "plan", "step", "build", "upload"], (float(v) for v in ms)))
 
     def sync(self):
         self._check(lib().adg_sync(self._h))

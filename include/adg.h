@@ -290,7 +290,8 @@ int adg_profile(adg_ctx* c, int enable);
 int adg_profile_get(adg_ctx* c, double* ms, int64_t* count, double* bytes);
 /* Host wall time (ms, inclusive) while profiling: [0] refine/coarsen
  * decisions with their grid rebuilds, [1] grid rebuilds and uploads, [2] fused
- * plan builds, [3] whole time steps of adg_run_hyperbolic. */
+ * plan builds, [3] whole time steps of adg_run_hyperbolic, [4] element-table
+ * builds inside [1], [5] device table uploads inside [1].  ms has 6 entries. */
 int adg_profile_host(adg_ctx* c, double* ms);
 
 #ifdef __cplusplus

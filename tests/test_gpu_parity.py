@@ -44,7 +44,7 @@ for table in range(9):
             SWEEP_CASES.append((table, lu, flx))
 
 
-@pytest.mark.parametrize("dim,k,m,nmax", [(2, 2, 3, 5), (3, 1, 2, 4)])
+@pytest.mark.parametrize("dim,k,m,nmax", [(2, 2, 3, 5), (3, 1, 2, 4), (2, 3, 4, 4)])
 def test_sweep_bitexact_sparse(dim, k, m, nmax):
     g, o = pair(dim, k, m, nmax)
     E = g.num_elem
@@ -179,7 +179,7 @@ def test_sincos_flux_tolerance():
 
 
 @pytest.mark.parametrize("dim,k,m,nmax,level_init,kind", [(2, 2, 3, 6, 4, 0), (2, 2, 3, 6, 4, 1), (3, 1, 2, 4, 3, 2),
-                                                          (4, 1, 2, 4, 4, 2)])
+                                                          (4, 1, 2, 4, 4, 2), (2, 3, 4, 5, 4, 2)])
 def test_fused_stage_matches_per_sweep_path(dim, k, m, nmax, level_init, kind):
     # the fused stage must reproduce the per-sweep kernels bit for bit (same sums, same order)
     if kind == 0:

@@ -189,8 +189,9 @@ def test_alpert_flux_tables_independent():
         assert abs(lft - ul[r, c]) < 1e-5 and abs(rgt - ur[r, c]) < 1e-5
 
 
-def test_lagrange_basis_interpolation_property():
-    m, nmax = 3, 4
+@pytest.mark.parametrize("m", [1, 2, 3, 4])
+def test_lagrange_basis_interpolation_property(m):
+    nmax = 4
     o = pyoracle.Oracle(1, 1, m, nmax, False, 0)
     n = (1 << nmax) * (m + 1)
     pts = [o.lagr_point(i) for i in range(n)]
@@ -205,7 +206,7 @@ def test_lagrange_basis_interpolation_property():
                 assert abs(v) < 1e-12, (i, j, v)
 
 
-@pytest.mark.parametrize("dim,m", [(2, 3), (3, 2)])
+@pytest.mark.parametrize("dim,m", [(2, 3), (3, 2), (2, 4), (2, 1)])
 def test_surplus_vanishes_for_polynomials(dim, m):
     # interpolation of a polynomial of degree <= m in each variable is exact, so
     # every surplus on levels >= 1 vanishes (pins points, weights and the transform)
