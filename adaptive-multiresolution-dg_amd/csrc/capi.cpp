@@ -640,6 +640,13 @@ int adg_set_instance_thresholds(adg_ctx* h, const double* eps, const double* eta
   });
 }
 
+int adg_set_kernel_variant(adg_ctx* h, int variant) {
+  return guard(h, [&](adg::Context& c) {
+    if (variant < -1 || variant > 2) throw std::runtime_error("kernel variant must be -1, 0, 1 or 2");
+    c.kernel_variant = variant;
+  });
+}
+
 int adg_stage(adg_ctx* h, const adg_problem* p, int scheme, int stage, double dt) {
   return guard(h, [&](adg::Context& c) {
     if (!p) throw std::runtime_error("null problem");

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <functional>
+#include <map>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -165,6 +167,13 @@ class Context {
   void stage_fused(const StageProblem& p, int scheme, int stage, double dt);
   struct FusedPlan;
   FusedPlan& fused_plan(const StageProblem& p);
+  // thread mapping of the fused kernels: 0 per output, 1 per (element, rest
+  // index), 2 per (element, 4 rest indices); -1 lets the autotuner choose
+  // per kernel class and grid-size class
+  enum { TUNE_UP_PHASE = 0, TUNE_UP_FINAL, TUNE_RHS_PHASE, TUNE_RHS_FINAL };
+  int kernel_variant = -1;
+  std::map<int, int> tuned;
+  void tuned_launch(int cls, const std::function<void(int)>& launch);
   std::unique_ptr<FusedPlan> plans[2];
   DevBuf<double> u_next, fused_tmp;
   int64_t grid_version = 0;

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <atomic>
@@ -121,13 +122,6 @@ constexpr int kBlock = 256;
 // terms of an output are then added in order from LDS.
 constexpr int kFinalBlock = 1024;
 
-// outputs per block of a final kernel with `nterm` terms: a power of two, at
-// most one wavefront, with nterm * outputs <= kFinalBlock
-inline int final_outputs_per_block(int nterm) {
-  int o = 64;
-  while (o > 1 && o * nterm > kFinalBlock) o /= 2;
-  return o;
-}
 
 inline unsigned nblocks(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
@@ -460,6 +454,7 @@ struct TermDev {
   const uint32_t* fbits;
   const int32_t* lsum;
   int ns, pt1, dd, Tin, flx, lu;
+  int isz[kMaxDim];  // sizes of the input tensor
   int sin[kMaxDim];  // strides of the input tensor
   double coef;
 };
@@ -469,7 +464,8 @@ struct TaskDev {  // a term written to an output buffer
   double* out;
   int Tout;
   int osz[kMaxDim];
-  int64_t begin;  // first work item of this task in the phase
+  int R;              // rest indices per element (Tin / input size in the swept dimension)
+  int64_t begin_v[3];  // first work item of this task in the phase, per thread mapping
 };
 
 // Value of one sweep at output (e, idx): coef * sum over related sources, in
@@ -533,14 +529,15 @@ __device__ __forceinline__ void unflatten(int o, const int* sz, int D, int* idx)
   }
 }
 
+// Per-output form of k_phase: thread per (task, element, output).
 template <int PF1>
-__global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntask, int64_t total, int D, int cap) {
+__global__ __launch_bounds__(kBlock) void k_phase_pt(const TaskDev* tasks, int ntask, int64_t total, int D, int cap) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= total) return;
   int j = 0;
-  while (j + 1 < ntask && gid >= tasks[j + 1].begin) ++j;
+  while (j + 1 < ntask && gid >= tasks[j + 1].begin_v[0]) ++j;
   const TaskDev& tk = tasks[j];
-  const int64_t local = gid - tk.begin;
+  const int64_t local = gid - tk.begin_v[0];
   const int64_t e = local / tk.Tout;
   const int o = static_cast<int>(local - e * tk.Tout);
   int idx[kMaxDim];
@@ -548,13 +545,10 @@ __global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntas
   tk.out[local] = term_value<PF1>(tk.t, e, idx, D, cap);
 }
 
-// Last sweep of every eval_up partial sum, accumulated in ordering order into
-// the point value, followed by the point flux (LagrInterpolation::eval_fp_Lag).
-// Threads are (point, term) pairs: every term is computed by its own thread
-// and the point's thread 0 adds the terms in order, so the dependent-load
-// chains of the terms overlap while the sum keeps the reference order.
+// Per-output form of k_up_final: threads are (point, term) pairs, one term per
+// wavefront, and the point's first thread adds the terms in order.
 template <int PF1>
-__global__ __launch_bounds__(kFinalBlock) void k_up_final(const TermDev* terms, int nterm, int per_block, int64_t E,
+__global__ __launch_bounds__(kFinalBlock) void k_up_final_pt(const TermDev* terms, int nterm, int per_block, int64_t E,
                                                            int P, int p1, int D, double* up, double* fp,
                                                            const double* vcoef, int kind, double s, int ncomp, int cap) {
   __shared__ double vals[kFinalBlock];
@@ -591,12 +585,9 @@ __global__ __launch_bounds__(kFinalBlock) void k_up_final(const TermDev* terms, 
   }
 }
 
-// Last sweep of every right-hand-side partial sum and the penalty sweeps,
-// accumulated in the reference order, then the RK stage update written to
-// u_next (u itself is still read by other threads' penalty sweeps).  Threads
-// are (DoF, term) pairs as in k_up_final.
+// Per-output form of k_rhs_final: threads are (DoF, term) pairs.
 template <int PFM, int PFK>
-__global__ __launch_bounds__(kFinalBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens,
+__global__ __launch_bounds__(kFinalBlock) void k_rhs_final_pt(const TermDev* terms, int nterm, const TermDev* pens,
                                                             int npen, int per_block, int64_t E, int K, int k1, int D,
                                                             double* rhs, const double* u, const double* tn,
                                                             double* u_next, int scheme, int stage, double dt,
@@ -629,19 +620,234 @@ __global__ __launch_bounds__(kFinalBlock) void k_rhs_final(const TermDev* terms,
 }
 
 
-template <int PF1>
-void launch_phase_t(const TaskDev* tasks, int ntask, int64_t total, int D, int cap, hipStream_t s) {
-  hipLaunchKernelGGL((k_phase<PF1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap);
+// A thread evaluates one term for up to kRC "rest" indices of one element
+// (the indices of the dimensions other than the swept one) and every output
+// index of the swept dimension: one walk over the fiber serves all of them.
+constexpr int kRC = 4;
+
+// Input and output offsets of rest index r (row-major over the dimensions
+// other than dd) in tensors with input sizes isz / strides si and output
+// strides so.
+__device__ __forceinline__ void rest_offsets(int r, int dd, int D, const int* isz, const int* si, const int* so,
+                                             int& bin, int& bout) {
+  bin = 0;
+  bout = 0;
+  for (int d = D - 1; d >= 0; --d) {
+    if (d == dd) continue;
+    const int q = r % isz[d];
+    r /= isz[d];
+    bin += q * si[d];
+    bout += q * so[d];
+  }
 }
 
-void launch_phase(int pf1, const TaskDev* tasks, int ntask, int64_t total, int D, int cap, hipStream_t s) {
-  switch (pf1) {
-    case 2: launch_phase_t<2>(tasks, ntask, total, D, cap, s); return;
-    case 3: launch_phase_t<3>(tasks, ntask, total, D, cap, s); return;
-    case 4: launch_phase_t<4>(tasks, ntask, total, D, cap, s); return;
-    case 5: launch_phase_t<5>(tasks, ntask, total, D, cap, s); return;
+__device__ __forceinline__ void strides_dev(const int* sz, int D, int* st) {
+  st[D - 1] = 1;
+  for (int d = D - 2; d >= 0; --d) st[d] = st[d + 1] * sz[d + 1];
+}
+
+// acc[r][i] = sum over related sources of element e, in ascending slot order,
+// of sum_k in[src][bin[r] + k*sdd] * M(s*PF1+k, st*PF1+i), for r < nr.
+// Returns false when e lies above the coarse-grid cap (its outputs are zero).
+template <int PF1, int PT1>
+__device__ __forceinline__ bool term_block(const TermDev& t, int64_t e, const int* bin, int nr, int cap,
+                                           double (&acc)[kRC][PT1]) {
+#pragma unroll
+  for (int r = 0; r < kRC; ++r)
+#pragma unroll
+    for (int i = 0; i < PT1; ++i) acc[r][i] = 0.;
+  const int st = t.slot[e];
+  const int fb = t.fbase[e];
+  int maxl = t.fmaxl[e];
+  if (cap >= 0) {  // coarse grid, as in k_sweep
+    const int ls = t.lsum[e];
+    if (ls > cap) return false;
+    maxl = min(maxl, cap - ls + slot_level(st));
   }
-  throw std::runtime_error("phase: unsupported basis size");
+  const int sdd = t.sin[t.dd];
+  const double* Mst = t.M + static_cast<int64_t>(st) * t.ns * PF1 * PT1;
+  for_each_present(st, maxl, t.flx != 0, t.lu, t.fbits, fb, [&](int s) {
+    const int src = t.fmap[fb + s];
+    if (src < 0) return;
+    const double* Mr = Mst + s * PF1 * PT1;
+    double m[PF1][PT1];
+#pragma unroll
+    for (int k = 0; k < PF1; ++k)
+#pragma unroll
+      for (int i = 0; i < PT1; ++i) m[k][i] = Mr[k * PT1 + i];
+    const double* xs = t.in + static_cast<int64_t>(src) * t.Tin;
+#pragma unroll
+    for (int r = 0; r < kRC; ++r) {
+      if (r >= nr) break;
+      double xv[PF1];
+#pragma unroll
+      for (int k = 0; k < PF1; ++k) xv[k] = xs[bin[r] + k * sdd];
+#pragma unroll
+      for (int i = 0; i < PT1; ++i)
+#pragma unroll
+        for (int k = 0; k < PF1; ++k) acc[r][i] += xv[k] * m[k][i];
+    }
+  });
+  return true;
+}
+
+// Evaluates term t for (element e, chunk c) and stores coef * value at
+// out[bout + i*so_dd] (zero above the coarse-grid cap).
+template <int PF1, int PT1>
+__device__ __forceinline__ void term_chunk_store(const TermDev& t, int64_t e, int c, int rc, int D, const int* so,
+                                                 int cap, double* out) {
+  const int R = t.Tin / t.isz[t.dd];
+  const int r0 = c * rc;
+  const int nr = min(rc, R - r0);
+  int bin[kRC], bout[kRC];
+#pragma unroll
+  for (int r = 0; r < kRC; ++r) {
+    bin[r] = 0;
+    bout[r] = 0;
+    if (r < nr) rest_offsets(r0 + r, t.dd, D, t.isz, t.sin, so, bin[r], bout[r]);
+  }
+  double acc[kRC][PT1];
+  const bool live = term_block<PF1, PT1>(t, e, bin, nr, cap, acc);
+  const int sod = so[t.dd];
+#pragma unroll
+  for (int r = 0; r < kRC; ++r) {
+    if (r >= nr) break;
+#pragma unroll
+    for (int i = 0; i < PT1; ++i) out[bout[r] + i * sod] = live ? acc[r][i] * t.coef : 0.;
+  }
+}
+
+// First sweeps of the partial sums of one phase: thread per (task, element,
+// chunk of rc rest indices); variant v (1: rc = 1, 2: rc = kRC) selects the
+// work-item offsets.
+template <int PF1, int PT1>
+__global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntask, int64_t total, int D, int cap,
+                                                  int v, int rc) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  int j = 0;
+  while (j + 1 < ntask && gid >= tasks[j + 1].begin_v[v]) ++j;
+  const TaskDev& tk = tasks[j];
+  const int nch = (tk.R + rc - 1) / rc;
+  const int64_t local = gid - tk.begin_v[v];
+  const int64_t e = local / nch;
+  const int c = static_cast<int>(local - e * nch);
+  int so[kMaxDim];
+  strides_dev(tk.osz, D, so);
+  term_chunk_store<PF1, PT1>(tk.t, e, c, rc, D, so, cap, tk.out + e * tk.Tout);
+}
+
+// Last sweep of every eval_up partial sum, accumulated in ordering order into
+// the point value, followed by the point flux (LagrInterpolation::eval_fp_Lag).
+// A block holds EB elements; threads are (term, element, chunk), term-major,
+// and write their values to LDS; the point values then add the terms in order.
+template <int PF1, int PT1>
+__global__ __launch_bounds__(kFinalBlock) void k_up_final(const TermDev* terms, int nterm, int EB, int nch, int rc,
+                                                           int64_t E, int P, int D, double* up, double* fp,
+                                                           const double* vcoef, int kind, double s, int ncomp, int cap) {
+  extern __shared__ double vals[];  // [nterm][EB][P]
+  const int tid = threadIdx.x;
+  const int per_term = EB * nch;
+  const int t = tid / per_term, rem = tid - t * per_term;
+  const int el = rem / nch, c = rem - (rem / nch) * nch;
+  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * EB;
+  int so[kMaxDim], osz[kMaxDim];
+  for (int d = 0; d < D; ++d) osz[d] = PT1;
+  strides_dev(osz, D, so);
+  if (t < nterm && e0 + el < E)
+    term_chunk_store<PF1, PT1>(terms[t], e0 + el, c, rc, D, so, cap, vals + (static_cast<int64_t>(t) * EB + el) * P);
+  __syncthreads();
+  const int64_t n = E * P;
+  for (int o = tid; o < EB * P; o += blockDim.x) {
+    const int ol = o / P, q = o - ol * P;
+    if (e0 + ol >= E) break;
+    double u = 0.;
+    for (int j = 0; j < nterm; ++j) u += vals[(static_cast<int64_t>(j) * EB + ol) * P + q];
+    const int64_t gid = (e0 + ol) * P + q;
+    up[gid] = u;
+    if (kind == 0) {
+      fp[gid] = u * u * 0.5;
+    } else if (kind == 1) {
+      fp[gid] = sin(u) * s;
+      fp[n + gid] = cos(u) * s;
+    } else {
+      for (int cc = 0; cc < ncomp; ++cc) fp[cc * n + gid] = vcoef[cc * n + gid] * u;
+    }
+  }
+}
+
+// Last sweep of every right-hand-side partial sum and the penalty sweeps,
+// accumulated in the reference order, then the RK stage update written to
+// u_next (u itself is still read by other threads' penalty sweeps).  Same
+// block structure as k_up_final.
+template <int PFM, int PFK>
+__global__ __launch_bounds__(kFinalBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens,
+                                                            int npen, int EB, int nch, int rc, int64_t E, int K, int D,
+                                                            double* rhs, const double* u, const double* tn,
+                                                            double* u_next, int scheme, int stage, double dt,
+                                                            const double* dtv, const int32_t* inst, int cap) {
+  extern __shared__ double vals[];  // [nterm + npen][EB][K]
+  const int tid = threadIdx.x;
+  const int per_term = EB * nch;
+  const int t = tid / per_term, rem = tid - t * per_term;
+  const int el = rem / nch, c = rem - (rem / nch) * nch;
+  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * EB;
+  int so[kMaxDim], osz[kMaxDim];
+  for (int d = 0; d < D; ++d) osz[d] = PFK;
+  strides_dev(osz, D, so);
+  if (e0 + el < E) {
+    double* out = vals + (static_cast<int64_t>(t) * EB + el) * K;
+    if (t < nterm) term_chunk_store<PFM, PFK>(terms[t], e0 + el, c, rc, D, so, cap, out);
+    else if (t < nterm + npen) term_chunk_store<PFK, PFK>(pens[t - nterm], e0 + el, c, rc, D, so, cap, out);
+  }
+  __syncthreads();
+  for (int o = tid; o < EB * K; o += blockDim.x) {
+    const int ol = o / K, q = o - ol * K;
+    if (e0 + ol >= E) break;
+    double r = 0.;
+    for (int j = 0; j < nterm + npen; ++j) r += vals[(static_cast<int64_t>(j) * EB + ol) * K + q];
+    const int64_t gid = (e0 + ol) * K + q;
+    rhs[gid] = r;
+    const double dte = dtv ? dtv[inst[e0 + ol]] : dt;
+    u_next[gid] = rk_combine(scheme, stage, dte, tn[gid], u[gid], r);
+  }
+}
+
+template <int PF1, int PT1>
+void launch_phase_t(const TaskDev* tasks, int ntask, int64_t total, int D, int cap, int v, hipStream_t s) {
+  if (v == 0)
+    hipLaunchKernelGGL((k_phase_pt<PF1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap);
+  else
+    hipLaunchKernelGGL((k_phase<PF1, PT1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap, v,
+                       v == 1 ? 1 : kRC);
+}
+
+// v: 0 thread per output, 1 thread per (element, rest index), 2 thread per
+// (element, kRC rest indices); total counts the work items of that mapping.
+void launch_phase(int pf1, int pt1, const TaskDev* tasks, int ntask, int64_t total, int D, int cap, int v,
+                  hipStream_t s) {
+#define ADG_PH(F, T) if (pf1 == F && pt1 == T) { launch_phase_t<F, T>(tasks, ntask, total, D, cap, v, s); return; }
+  ADG_PH(2, 2) ADG_PH(2, 3) ADG_PH(2, 4) ADG_PH(2, 5) ADG_PH(3, 2) ADG_PH(3, 3) ADG_PH(3, 4) ADG_PH(3, 5)
+  ADG_PH(4, 2) ADG_PH(4, 3) ADG_PH(4, 4) ADG_PH(4, 5) ADG_PH(5, 2) ADG_PH(5, 3) ADG_PH(5, 4)
+#undef ADG_PH
+  throw std::runtime_error("phase: unsupported basis sizes");
+}
+
+// Outputs per block of a per-output final kernel with nterm terms: a power
+// of two, at most one wavefront, with nterm * outputs <= kFinalBlock.
+inline int final_outputs_per_block(int nterm) {
+  int o = 64;
+  while (o > 1 && o * nterm > kFinalBlock) o /= 2;
+  return o;
+}
+
+// Elements per block of a final kernel with nt terms, nch chunks per element
+// and `outs` outputs per element: at most kFinalBlock threads and 64 KB of LDS.
+inline int final_elements_per_block(int nt, int nch, int outs) {
+  int eb = std::min(kFinalBlock / (nt * nch), 65536 / (nt * outs * 8));
+  if (eb < 1) throw std::runtime_error("final kernel: too many terms for one block");
+  if (nch == 1 && eb >= 64) eb -= eb % 64;  // whole wavefronts per term
+  return eb;
 }
 
 template <int PF1, int PT1>
@@ -745,7 +951,7 @@ struct Context::FusedPlan {
   int flux_kind = -1;
   double alpha[kMaxDim] = {0, 0, 0, 0};
   std::vector<int> up_phase_ntask, rhs_phase_ntask;
-  std::vector<int64_t> up_phase_total, rhs_phase_total;
+  std::vector<std::array<int64_t, 3>> up_phase_total, rhs_phase_total;  // work items per thread mapping
   int n_up_terms = 0, n_rhs_terms = 0, n_pen = 0;
   // algorithmic bytes per launch (8 B x elements x (inputs + outputs))
   std::vector<double> up_phase_bytes, rhs_phase_bytes;
@@ -1205,6 +1411,7 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     t.fbits = g_fbits[dd].p;
     t.lsum = g_lsum.p;
     t.Tin = prod_of(sin_sz, D);
+    std::copy(sin_sz, sin_sz + D, t.isz);
     strides_of(sin_sz, D, t.sin);
     t.flx = flx ? 1 : 0; t.lu = lu; t.coef = coef;
     return t;
@@ -1227,23 +1434,26 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
       tk.out = tmp(cid, st);
       tk.Tout = prod_of(sz, D);
       std::copy(sz, sz + D, tk.osz);
+      tk.R = t.Tin / sin_sz[dd];
       phase_tasks[st].push_back(tk);
       cur = tk.out;
     }
     return TermDev{};
   };
   auto finish_phases = [&](std::vector<std::vector<TaskDev>>& ph, DevBuf<TaskDev>& dev, std::vector<int>& ntask,
-                           std::vector<int64_t>& total, std::vector<double>& bytes) {
+                           std::vector<std::array<int64_t, 3>>& total, std::vector<double>& bytes) {
     std::vector<TaskDev> flat;
     ntask.clear();
     total.clear();
     bytes.clear();
     for (auto& tasks : ph) {
-      int64_t begin = 0;
+      std::array<int64_t, 3> begin{0, 0, 0};
       double b = 0;
       for (TaskDev& tk : tasks) {
-        tk.begin = begin;
-        begin += E() * tk.Tout;
+        for (int v = 0; v < 3; ++v) tk.begin_v[v] = begin[v];
+        begin[0] += E() * tk.Tout;
+        begin[1] += E() * tk.R;
+        begin[2] += E() * ((tk.R + kRC - 1) / kRC);
         b += 8.0 * E() * (tk.t.Tin + tk.Tout);
       }
       bytes.push_back(b);
@@ -1319,49 +1529,93 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   int64_t off = 0;
   for (size_t s = 0; s < pl.up_phase_ntask.size(); ++s) {
     prof_begin(PROF_SWEEP, pl.up_phase_bytes[s]);
-    launch_phase(k + 1, pl.up_tasks.p + off, pl.up_phase_ntask[s], pl.up_phase_total[s], D, lsum_cap, stream);
+    const TaskDev* tasks = pl.up_tasks.p + off;
+    const int nt = pl.up_phase_ntask[s];
+    const std::array<int64_t, 3> tot = pl.up_phase_total[s];
+    tuned_launch(TUNE_UP_PHASE, [&](int v) { launch_phase(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
     prof_end();
     off += pl.up_phase_ntask[s];
   }
   {
-    const int64_t n = E() * P;
-    const int per_block = final_outputs_per_block(pl.n_up_terms);
-    if (per_block * pl.n_up_terms > kFinalBlock) throw std::runtime_error("too many eval_up terms");
-    const unsigned nb = static_cast<unsigned>((n + per_block - 1) / per_block);
-    const unsigned bs = static_cast<unsigned>(per_block * pl.n_up_terms);
     prof_begin(PROF_FLUX, pl.up_final_bytes);
-    switch (k + 1) {
-      case 2: hipLaunchKernelGGL((k_up_final<2>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
-      case 3: hipLaunchKernelGGL((k_up_final<3>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
-      case 4: hipLaunchKernelGGL((k_up_final<4>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap); break;
-    }
+    tuned_launch(TUNE_UP_FINAL, [&](int v) {
+      if (v == 0) {
+        const int64_t n = E() * P;
+        const int per_block = final_outputs_per_block(pl.n_up_terms);
+        const unsigned nb = static_cast<unsigned>((n + per_block - 1) / per_block);
+        const unsigned bs = static_cast<unsigned>(per_block * pl.n_up_terms);
+#define ADG_UP(K1)                                                                                                 \
+  if (k + 1 == K1)                                                                                                 \
+    hipLaunchKernelGGL((k_up_final_pt<K1>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, \
+                       E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap);
+        ADG_UP(2) ADG_UP(3) ADG_UP(4)
+#undef ADG_UP
+        return;
+      }
+      const int rc = v == 1 ? 1 : kRC;
+      const int R = P / (m + 1);
+      const int nch = (R + rc - 1) / rc;
+      const int EB = final_elements_per_block(pl.n_up_terms, nch, P);
+      const unsigned nb = static_cast<unsigned>((E() + EB - 1) / EB);
+      const unsigned bs = static_cast<unsigned>(pl.n_up_terms * EB * nch);
+      const size_t lds = sizeof(double) * pl.n_up_terms * EB * P;
+#define ADG_UF(K1, M1)                                                                                             \
+  if (k + 1 == K1 && m + 1 == M1)                                                                                  \
+    hipLaunchKernelGGL((k_up_final<K1, M1>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p, pl.n_up_terms, EB, nch, \
+                       rc, E(), P, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap);
+      ADG_UF(2, 2) ADG_UF(2, 3) ADG_UF(2, 4) ADG_UF(2, 5) ADG_UF(3, 2) ADG_UF(3, 3) ADG_UF(3, 4) ADG_UF(3, 5)
+      ADG_UF(4, 2) ADG_UF(4, 3) ADG_UF(4, 4) ADG_UF(4, 5)
+#undef ADG_UF
+    });
     prof_end();
   }
   surplus_transform(fp.p, fucoe.p, interA.p, interB.p, ncomp);
   off = 0;
   for (size_t s = 0; s < pl.rhs_phase_ntask.size(); ++s) {
     prof_begin(PROF_SWEEP, pl.rhs_phase_bytes[s]);
-    launch_phase(m + 1, pl.rhs_tasks.p + off, pl.rhs_phase_ntask[s], pl.rhs_phase_total[s], D, lsum_cap, stream);
+    const TaskDev* tasks = pl.rhs_tasks.p + off;
+    const int nt = pl.rhs_phase_ntask[s];
+    const std::array<int64_t, 3> tot = pl.rhs_phase_total[s];
+    tuned_launch(TUNE_RHS_PHASE, [&](int v) { launch_phase(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
     prof_end();
     off += pl.rhs_phase_ntask[s];
   }
   {
-    const int64_t n = E() * K;
-    const int nt = pl.n_rhs_terms + pl.n_pen;
-    const int per_block = final_outputs_per_block(nt);
-    if (per_block * nt > kFinalBlock) throw std::runtime_error("too many right-hand-side terms");
-    const unsigned nb = static_cast<unsigned>((n + per_block - 1) / per_block);
-    const unsigned bs = static_cast<unsigned>(per_block * nt);
     prof_begin(PROF_RK, pl.rhs_final_bytes);
+    const int nt = pl.n_rhs_terms + pl.n_pen;
+    const double* dtv = use_inst_dt ? d_inst_dt.p : nullptr;
+    tuned_launch(TUNE_RHS_FINAL, [&](int v) {
+      if (v == 0) {
+        const int64_t n = E() * K;
+        const int per_block = final_outputs_per_block(nt);
+        const unsigned nb = static_cast<unsigned>((n + per_block - 1) / per_block);
+        const unsigned bs = static_cast<unsigned>(per_block * nt);
+#define ADG_RP(M1, K1)                                                                                             \
+  if (m + 1 == M1 && k + 1 == K1)                                                                                  \
+    hipLaunchKernelGGL((k_rhs_final_pt<M1, K1>), dim3(nb), dim3(bs), 0, stream, pl.rhs_terms.p, pl.n_rhs_terms,    \
+                       pl.pens.p, pl.n_pen, per_block, E(), K, k + 1, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, \
+                       dt, dtv, g_inst.p, lsum_cap);
+        ADG_RP(2, 2) ADG_RP(2, 3) ADG_RP(2, 4) ADG_RP(3, 2) ADG_RP(3, 3) ADG_RP(3, 4) ADG_RP(4, 2) ADG_RP(4, 3)
+        ADG_RP(4, 4) ADG_RP(5, 2) ADG_RP(5, 3) ADG_RP(5, 4)
+#undef ADG_RP
+        return;
+      }
+      const int rc = v == 1 ? 1 : kRC;
+      const int R = K / (k + 1);
+      const int nch = (R + rc - 1) / rc;
+      const int EB = final_elements_per_block(nt, nch, K);
+      const unsigned nb = static_cast<unsigned>((E() + EB - 1) / EB);
+      const unsigned bs = static_cast<unsigned>(nt * EB * nch);
+      const size_t lds = sizeof(double) * nt * EB * K;
 #define ADG_RF(M1, K1)                                                                                             \
-  if (m + 1 == M1 && k + 1 == K1) {                                                                                \
-    hipLaunchKernelGGL((k_rhs_final<M1, K1>), dim3(nb), dim3(bs), 0, stream, pl.rhs_terms.p, pl.n_rhs_terms,      \
-                       pl.pens.p, pl.n_pen, per_block, E(), K, k + 1, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt, \
-                       use_inst_dt ? d_inst_dt.p : nullptr, g_inst.p, lsum_cap);                                    \
-  }
-    ADG_RF(2, 2) ADG_RF(2, 3) ADG_RF(2, 4) ADG_RF(3, 2) ADG_RF(3, 3) ADG_RF(3, 4) ADG_RF(4, 2) ADG_RF(4, 3)
-    ADG_RF(4, 4) ADG_RF(5, 2) ADG_RF(5, 3) ADG_RF(5, 4)
+  if (m + 1 == M1 && k + 1 == K1)                                                                                  \
+    hipLaunchKernelGGL((k_rhs_final<M1, K1>), dim3(nb), dim3(bs), lds, stream, pl.rhs_terms.p, pl.n_rhs_terms,     \
+                       pl.pens.p, pl.n_pen, EB, nch, rc, E(), K, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt,  \
+                       dtv, g_inst.p, lsum_cap);
+      ADG_RF(2, 2) ADG_RF(2, 3) ADG_RF(2, 4) ADG_RF(3, 2) ADG_RF(3, 3) ADG_RF(3, 4) ADG_RF(4, 2) ADG_RF(4, 3)
+      ADG_RF(4, 4) ADG_RF(5, 2) ADG_RF(5, 3) ADG_RF(5, 4)
 #undef ADG_RF
+    });
     prof_end();
   }
   std::swap(u.p, u_next.p);
@@ -1657,6 +1911,43 @@ void Context::eval_tensor(int nx, const double* x1d, double* out_host) {
   hipLaunchKernelGGL(k_eval_tensor, dim3(nblocks(npts)), dim3(kBlock), 0, stream, a);
   hip_check(hipMemcpyAsync(out_host, eval_out.p, npts * sizeof(double), hipMemcpyDeviceToHost, stream), "download");
   sync();
+}
+
+// Thread-mapping autotuner of the fused stage kernels.  The three mappings
+// compute identical values (same sums in the same order), so the first time
+// a kernel class meets a grid-size class, every mapping is launched once on
+// the live data with events around it and the fastest is kept; the outputs of
+// the last launch stand.  A forced mapping (kernel_variant >= 0) skips this.
+void Context::tuned_launch(int cls, const std::function<void(int)>& launch) {
+  if (kernel_variant >= 0) {
+    launch(kernel_variant);
+    return;
+  }
+  int bucket = 0;
+  while ((int64_t{1} << (bucket + 1)) <= E()) ++bucket;
+  const int key = cls * 64 + bucket;
+  auto it = tuned.find(key);
+  if (it != tuned.end()) {
+    launch(it->second);
+    return;
+  }
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "hipEventCreate");
+  hip_check(hipEventCreate(&b), "hipEventCreate");
+  float best = 0.f;
+  int bv = 0;
+  for (int v = 0; v < 3; ++v) {
+    hip_check(hipEventRecord(a, stream), "hipEventRecord");
+    launch(v);
+    hip_check(hipEventRecord(b, stream), "hipEventRecord");
+    hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+    if (v == 0 || ms < best) { best = ms; bv = v; }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  tuned[key] = bv;
 }
 
 }  // namespace adg

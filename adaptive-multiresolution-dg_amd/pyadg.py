@@ -93,6 +93,7 @@ def lib():
             "adg_run_linear": (i32, [vp, DP, dbl, i64, P(RunStats)]),
             "adg_run_hyperbolic": (i32, [vp, P(Problem), P(RunStats)]),
             "adg_stage": (i32, [vp, P(Problem), i32, i32, dbl]),
+            "adg_set_kernel_variant": (i32, [vp, i32]),
             "adg_state_save": (i32, [vp, i32]),
             "adg_state_restore": (i32, [vp, i32]),
             "adg_profile": (i32, [vp, i32]),
@@ -408,6 +409,10 @@ class Solution:
         st = RunStats()
         self._check(lib().adg_run_hyperbolic(self._h, ctypes.byref(problem), ctypes.byref(st)))
         return st
+
+    def set_kernel_variant(self, variant):
+        """Thread mapping of the fused kernels (-1: autotuned)."""
+        self._check(lib().adg_set_kernel_variant(self._h, variant))
 
     def stage(self, problem, scheme, stage, dt):
         self._check(lib().adg_stage(self._h, ctypes.byref(problem), scheme, stage, dt))

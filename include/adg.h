@@ -269,6 +269,14 @@ int adg_instance_range(adg_ctx* c, int inst, int64_t* first, int64_t* count);
 /* Per-instance refine / coarsen thresholds (arrays of adg_num_instances values). */
 int adg_set_instance_thresholds(adg_ctx* c, const double* eps, const double* eta);
 
+/* Thread mapping of the fused stage kernels (no counterpart in the
+ * reference): 0 one thread per output value, 1 one thread per (element, index
+ * over the non-swept dimensions), 2 one thread per (element, four such
+ * indices).  All mappings give identical results.  -1 (the default) times the
+ * three on the live grid the first time a kernel meets a grid-size class and
+ * keeps the fastest. */
+int adg_set_kernel_variant(adg_ctx* c, int variant);
+
 /* One fused RK stage of the same operator: point values, flux, surplus
  * transform, right-hand side (volume, flux and LF penalty terms) and the stage
  * update in 3*dim launches; bit-identical to adg_eval_up + adg_flux +

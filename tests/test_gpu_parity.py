@@ -180,7 +180,8 @@ def test_sincos_flux_tolerance():
 
 @pytest.mark.parametrize("dim,k,m,nmax,level_init,kind", [(2, 2, 3, 6, 4, 0), (2, 2, 3, 6, 4, 1), (3, 1, 2, 4, 3, 2),
                                                           (4, 1, 2, 4, 4, 2), (2, 3, 4, 5, 4, 2)])
-def test_fused_stage_matches_per_sweep_path(dim, k, m, nmax, level_init, kind):
+@pytest.mark.parametrize("variant", [0, 1, 2, -1])
+def test_fused_stage_matches_per_sweep_path(dim, k, m, nmax, level_init, kind, variant):
     # the fused stage must reproduce the per-sweep kernels bit for bit (same sums, same order)
     if kind == 0:
         p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.08, (0.3, 0.6))])
@@ -195,6 +196,7 @@ def test_fused_stage_matches_per_sweep_path(dim, k, m, nmax, level_init, kind):
     a, _ = adapted_pair(dim, k, m, nmax, level_init, p, 1e-4)
     b = pyadg.Solution(dim, k, m, nmax, True, level_init)
     b.init_adaptive_intp_builtin(p, 1e-4)
+    b.set_kernel_variant(variant)
     assert same_grid(a, b)
     a.rk_init(); b.rk_init()
     for st in range(3):
