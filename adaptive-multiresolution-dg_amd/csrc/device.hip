@@ -121,6 +121,9 @@ constexpr int kBlock = 256;
 // one term for consecutive outputs (same element, same fiber walk), the
 // terms of an output are then added in order from LDS.
 constexpr int kFinalBlock = 1024;
+// Per-element final kernels hold a block of accumulators per thread: at most
+// 512 threads per block leaves them 256 VGPRs.
+constexpr int kElemBlock = 512;
 
 
 inline unsigned nblocks(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
@@ -742,20 +745,21 @@ __global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntas
 // A block holds EB elements; threads are (term, element, chunk), term-major,
 // and write their values to LDS; the point values then add the terms in order.
 template <int PF1, int PT1>
-__global__ __launch_bounds__(kFinalBlock) void k_up_final(const TermDev* terms, int nterm, int EB, int nch, int rc,
+__global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, int nterm, int EB, int nch, int rc,
                                                            int64_t E, int P, int D, double* up, double* fp,
                                                            const double* vcoef, int kind, double s, int ncomp, int cap) {
   extern __shared__ double vals[];  // [nterm][EB][P]
   const int tid = threadIdx.x;
-  const int per_term = EB * nch;
-  const int t = tid / per_term, rem = tid - t * per_term;
+  const int per_term = EB * nch, nslot = blockDim.x / per_term;
+  const int slot = tid / per_term, rem = tid - slot * per_term;
   const int el = rem / nch, c = rem - (rem / nch) * nch;
   const int64_t e0 = static_cast<int64_t>(blockIdx.x) * EB;
   int so[kMaxDim], osz[kMaxDim];
   for (int d = 0; d < D; ++d) osz[d] = PT1;
   strides_dev(osz, D, so);
-  if (t < nterm && e0 + el < E)
-    term_chunk_store<PF1, PT1>(terms[t], e0 + el, c, rc, D, so, cap, vals + (static_cast<int64_t>(t) * EB + el) * P);
+  if (e0 + el < E)
+    for (int t = slot; t < nterm; t += nslot)
+      term_chunk_store<PF1, PT1>(terms[t], e0 + el, c, rc, D, so, cap, vals + (static_cast<int64_t>(t) * EB + el) * P);
   __syncthreads();
   const int64_t n = E * P;
   for (int o = tid; o < EB * P; o += blockDim.x) {
@@ -781,25 +785,26 @@ __global__ __launch_bounds__(kFinalBlock) void k_up_final(const TermDev* terms, 
 // u_next (u itself is still read by other threads' penalty sweeps).  Same
 // block structure as k_up_final.
 template <int PFM, int PFK>
-__global__ __launch_bounds__(kFinalBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens,
+__global__ __launch_bounds__(kElemBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens,
                                                             int npen, int EB, int nch, int rc, int64_t E, int K, int D,
                                                             double* rhs, const double* u, const double* tn,
                                                             double* u_next, int scheme, int stage, double dt,
                                                             const double* dtv, const int32_t* inst, int cap) {
   extern __shared__ double vals[];  // [nterm + npen][EB][K]
   const int tid = threadIdx.x;
-  const int per_term = EB * nch;
-  const int t = tid / per_term, rem = tid - t * per_term;
+  const int per_term = EB * nch, nslot = blockDim.x / per_term;
+  const int slot = tid / per_term, rem = tid - slot * per_term;
   const int el = rem / nch, c = rem - (rem / nch) * nch;
   const int64_t e0 = static_cast<int64_t>(blockIdx.x) * EB;
   int so[kMaxDim], osz[kMaxDim];
   for (int d = 0; d < D; ++d) osz[d] = PFK;
   strides_dev(osz, D, so);
-  if (e0 + el < E) {
-    double* out = vals + (static_cast<int64_t>(t) * EB + el) * K;
-    if (t < nterm) term_chunk_store<PFM, PFK>(terms[t], e0 + el, c, rc, D, so, cap, out);
-    else if (t < nterm + npen) term_chunk_store<PFK, PFK>(pens[t - nterm], e0 + el, c, rc, D, so, cap, out);
-  }
+  if (e0 + el < E)
+    for (int t = slot; t < nterm + npen; t += nslot) {
+      double* out = vals + (static_cast<int64_t>(t) * EB + el) * K;
+      if (t < nterm) term_chunk_store<PFM, PFK>(terms[t], e0 + el, c, rc, D, so, cap, out);
+      else term_chunk_store<PFK, PFK>(pens[t - nterm], e0 + el, c, rc, D, so, cap, out);
+    }
   __syncthreads();
   for (int o = tid; o < EB * K; o += blockDim.x) {
     const int ol = o / K, q = o - ol * K;
@@ -841,14 +846,18 @@ inline int final_outputs_per_block(int nterm) {
   return o;
 }
 
-// Elements per block of a final kernel with nt terms, nch chunks per element
-// and `outs` outputs per element: at most kFinalBlock threads and 64 KB of LDS.
+// Elements per block of a per-element final kernel with nt terms, nch chunks
+// per element and `outs` outputs per element (at most kElemBlock threads and
+// 64 KB of LDS), and the number of term slots: threads are (slot, element,
+// chunk) and a slot takes the terms slot, slot + nslot, ...
 inline int final_elements_per_block(int nt, int nch, int outs) {
-  int eb = std::min(kFinalBlock / (nt * nch), 65536 / (nt * outs * 8));
-  if (eb < 1) throw std::runtime_error("final kernel: too many terms for one block");
+  if (nch > kElemBlock || nt * outs * 8 > 65536) throw std::runtime_error("final kernel: block too large");
+  int eb = std::max(1, std::min(kElemBlock / (nt * nch), 65536 / (nt * outs * 8)));
   if (nch == 1 && eb >= 64) eb -= eb % 64;  // whole wavefronts per term
   return eb;
 }
+
+inline int final_term_slots(int nt, int nch, int eb) { return std::max(1, std::min(nt, kElemBlock / (eb * nch))); }
 
 template <int PF1, int PT1>
 void launch_sweep_t(const SweepArgs& a, hipStream_t s) {
@@ -1557,7 +1566,7 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
       const int nch = (R + rc - 1) / rc;
       const int EB = final_elements_per_block(pl.n_up_terms, nch, P);
       const unsigned nb = static_cast<unsigned>((E() + EB - 1) / EB);
-      const unsigned bs = static_cast<unsigned>(pl.n_up_terms * EB * nch);
+      const unsigned bs = static_cast<unsigned>(final_term_slots(pl.n_up_terms, nch, EB) * EB * nch);
       const size_t lds = sizeof(double) * pl.n_up_terms * EB * P;
 #define ADG_UF(K1, M1)                                                                                             \
   if (k + 1 == K1 && m + 1 == M1)                                                                                  \
@@ -1605,7 +1614,7 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
       const int nch = (R + rc - 1) / rc;
       const int EB = final_elements_per_block(nt, nch, K);
       const unsigned nb = static_cast<unsigned>((E() + EB - 1) / EB);
-      const unsigned bs = static_cast<unsigned>(nt * EB * nch);
+      const unsigned bs = static_cast<unsigned>(final_term_slots(nt, nch, EB) * EB * nch);
       const size_t lds = sizeof(double) * nt * EB * K;
 #define ADG_RF(M1, K1)                                                                                             \
   if (m + 1 == M1 && k + 1 == K1)                                                                                  \
@@ -1918,9 +1927,20 @@ void Context::eval_tensor(int nx, const double* x1d, double* out_host) {
 // a kernel class meets a grid-size class, every mapping is launched once on
 // the live data with events around it and the fastest is kept; the outputs of
 // the last launch stand.  A forced mapping (kernel_variant >= 0) skips this.
+// A mapping whose block does not fit (its shape check throws before any
+// launch) is skipped; mapping 0 always fits.
 void Context::tuned_launch(int cls, const std::function<void(int)>& launch) {
+  auto try_launch = [&](int v) {
+    try {
+      launch(v);
+      return true;
+    } catch (const std::runtime_error&) {
+      if (v == 0) throw;
+      return false;
+    }
+  };
   if (kernel_variant >= 0) {
-    launch(kernel_variant);
+    if (!try_launch(kernel_variant)) launch(0);
     return;
   }
   int bucket = 0;
@@ -1938,7 +1958,7 @@ void Context::tuned_launch(int cls, const std::function<void(int)>& launch) {
   int bv = 0;
   for (int v = 0; v < 3; ++v) {
     hip_check(hipEventRecord(a, stream), "hipEventRecord");
-    launch(v);
+    if (!try_launch(v)) continue;
     hip_check(hipEventRecord(b, stream), "hipEventRecord");
     hip_check(hipEventSynchronize(b), "hipEventSynchronize");
     float ms = 0.f;

@@ -20,6 +20,7 @@ import ctypes
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -99,7 +100,9 @@ def main():
     ap.add_argument("--no-events", action="store_true", help="time without per-kernel HIP events")
     ap.add_argument("--time-steps", type=int, default=0,
                     help="cap the time steps of every solve (0: run to the workload's final time)")
-    ap.add_argument("--batch", type=int, default=64,
+    ap.add_argument("--streams", type=int, default=2,
+                    help="contexts (HIP streams, host threads) the batch is split over in the chosen form")
+    ap.add_argument("--batch", type=int, default=256,
                     help="independent seeded instances solved together in the chosen form (c2, c5)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -117,52 +120,86 @@ def main():
     sol = workloads.make_solution(w)
     print(f"[bench] init done: {sol.num_elem} elements; warmup {args.warmup} steps", file=sys.stderr, flush=True)
     sol.save_state(0)
-    bsol = None
+    bsols = []
     if batched:
         ws = [workload(args.seed + i) for i in range(args.batch)]
-        print(f"[bench] batch of {args.batch} instances (seeds {args.seed}..{args.seed + args.batch - 1}): init",
-              file=sys.stderr, flush=True)
-        bsol = workloads.make_batch(ws)
-        bsol.save_state(0)
-        print(f"[bench] batch init done: {bsol.num_elem} elements", file=sys.stderr, flush=True)
+        print(f"[bench] batch of {args.batch} instances (seeds {args.seed}..{args.seed + args.batch - 1}) in "
+              f"{args.streams} context(s): init", file=sys.stderr, flush=True)
+        per = (args.batch + args.streams - 1) // args.streams
+        for c in range(args.streams):
+            part = ws[c * per:(c + 1) * per]
+            if part:
+                b = workloads.make_batch(part)
+                b.save_state(0)
+                bsols.append(b)
+        print(f"[bench] batch init done: {sum(b.num_elem for b in bsols)} elements", file=sys.stderr, flush=True)
 
-    def measure(s, form):
-        """Time --steps full solves (after --warmup) of one form; returns figures and HIP-event profile."""
-        prob = None if w.linear else pyadg.Problem.from_buffer_copy(w.problem)
+    def measure(sols, form):
+        """Time --steps full solves (after --warmup) of one form; returns figures and HIP-event profile.
+        Several contexts are solved concurrently, one host thread each (each context has its own
+        stream; the host work of one overlaps the kernels of the others)."""
+        if not isinstance(sols, list):
+            sols = [sols]
+        probs = [None if w.linear else pyadg.Problem.from_buffer_copy(w.problem) for _ in sols]
 
-        def solve():
+        def solve(i):
+            s, prob = sols[i], probs[i]
             s.restore_state(0)
             if w.linear:
                 return s.run_linear(w.coef, w.dt, min(w.nsteps, args.time_steps) if args.time_steps else w.nsteps)
             prob.t_start, prob.max_steps, prob.form = 0.0, args.time_steps, form
             return s.run_hyperbolic(prob)
 
-        for _ in range(args.warmup):
-            solve()
-        if not args.no_events:
-            s.profile(True)
-        s.sync()
-        dof_stages, time_steps, max_elem = 0, 0, 0
+        results = [[] for _ in sols]
+
+        def worker(i, n):
+            for _ in range(n):
+                results[i].append(solve(i))
+
+        def run_all(n):
+            th = [threading.Thread(target=worker, args=(i, n)) for i in range(len(sols))]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+
+        run_all(args.warmup)
+        for s in sols:
+            if not args.no_events:
+                s.profile(True)
+            s.sync()
+        for r in results:
+            r.clear()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            st = solve()
-            dof_stages += st.dof_stages
-            time_steps += st.steps
-            max_elem = max(max_elem, st.max_elem)
-        s.sync()
+        run_all(args.steps)
+        for s in sols:
+            s.sync()
         sec = time.perf_counter() - t0
-        prof = s.profile_get()
-        host = s.profile_host()
-        s.profile(False)
-        u = s.instance_get(0) if s.num_instances > 1 else s.get(pyadg.FIELD_UCOE)
-        print(f"[bench] form {form}, {s.num_instances} instance(s): {args.steps} solves ({time_steps} time steps) "
-              f"in {sec:.3f} s, {dof_stages / sec:.4g} DoF-stage/s", file=sys.stderr, flush=True)
+        dof_stages = sum(st.dof_stages for r in results for st in r)
+        time_steps = max(sum(st.steps for st in r) for r in results)
+        max_elem = sum(max(st.max_elem for st in r) for r in results)
+        prof = {k: {"ms": 0.0, "count": 0, "bytes": 0.0} for k in sols[0].profile_get()}
+        host = {}
+        for s in sols:
+            for k, v in s.profile_get().items():
+                for f in ("ms", "count", "bytes"):
+                    prof[k][f] += v[f]
+            for k, v in s.profile_host().items():
+                host[k] = host.get(k, 0.0) + v
+            s.profile(False)
+        s0 = sols[0]
+        u = s0.instance_get(0) if s0.num_instances > 1 else s0.get(pyadg.FIELD_UCOE)
+        ninst = sum(s.num_instances for s in sols)
+        print(f"[bench] form {form}, {ninst} instance(s) in {len(sols)} context(s): {args.steps} solves "
+              f"({time_steps} time steps) in {sec:.3f} s, {dof_stages / sec:.4g} DoF-stage/s", file=sys.stderr,
+              flush=True)
         return {"value": dof_stages / sec, "sec": sec, "dof_stages": dof_stages, "time_steps": time_steps,
-                "max_elem": max_elem, "prof": prof, "host": host, "u": u, "sim_time": st.final_time}
+                "max_elem": max_elem, "prof": prof, "host": host, "u": u,
+                "sim_time": results[0][-1].final_time}
 
     plain = measure(sol, 0)
     fused = plain if w.linear else measure(sol, 1)
-    chosen = measure(bsol, 1) if batched else fused
+    chosen = measure(bsols, 1) if batched else fused
     value = chosen["value"]
     # instance 0 of the batch is the plain problem itself: equal L2 error means equal coefficients
     identical = bool(np.array_equal(plain["u"], chosen["u"]))
@@ -196,8 +233,10 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": dict(w.config(), step="one full solve from the saved initial state",
                        form=("chosen: fused FP64 stages (3*dim launches per RK stage), %d seeded instances "
-                             "(seeds %d..%d) per launch" % (args.batch, args.seed, args.seed + args.batch - 1))
+                             "(seeds %d..%d) in %d context(s) solved concurrently"
+                             % (args.batch, args.seed, args.seed + args.batch - 1, len(bsols)))
                        if batched else "chosen: fused FP64 stages (3*dim launches per RK stage)",
+                       contexts=len(bsols) if batched else 1,
                        plain_form="plain FP64, one kernel per 1D sweep, one instance",
                        plain_value=plain["value"], fused_single_instance_value=fused["value"],
                        chosen_value=chosen["value"], batch=args.batch if batched else 1,
