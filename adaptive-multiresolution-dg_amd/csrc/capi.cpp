@@ -227,10 +227,14 @@ int adg_lagr_points(adg_ctx* h, double* x) {
         for (int d = c.D - 1; d >= 0; --d) {
           const int pd = rem % p1;
           rem /= p1;
-          x[(e * c.P + q) * c.D + d] = c.lagr.point(c.uslot[e * c.D + d] * p1 + pd);
+          x[(e * c.P + q) * c.D + d] = c.interp_basis().point(c.uslot[e * c.D + d] * p1 + pd);
         }
       }
   });
+}
+
+int adg_set_interpolation(adg_ctx* h, int family) {
+  return guard(h, [&](adg::Context& c) { c.set_interpolation(family); });
 }
 
 int adg_set_level_sum_cap(adg_ctx* h, int cap) {

@@ -34,6 +34,10 @@ struct DevBuf {
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { if (p) (void)hipFree(p); }
+  void swap(DevBuf& o) noexcept {
+    std::swap(p, o.p);
+    std::swap(cap, o.cap);
+  }
   void ensure(size_t n) {
     if (n <= cap) return;
     if (p) hip_check(hipFree(p), "hipFree");
@@ -92,6 +96,26 @@ class Context {
   DevBuf<double> t_pt, t_la_uv, t_la_uvx, t_la_flx, t_aa[5], t_lin[kMaxDim], t_lmass, t_wt;
   DevBuf<int32_t> t_anc_slot, t_anc_point;
   int na = 0, nl = 0;  // 1D basis counts
+
+  // Interpolation family of eval_up / flux / fp_to_coe / rhs_lagr and the
+  // fused stage: 0 Lagrange, 1 Hermite P3 (HermInterpolation with
+  // HyperbolicHermRHS).  The family in use has its point, operator and
+  // surplus tables in t_pt, t_la_*, t_wt, t_anc_*; the other family's wait in
+  // alt_*.  The Hermite tables are built on first use.
+  int interp = 0;
+  std::unique_ptr<Basis1D> herm;
+  DevBuf<double> alt_pt, alt_uv, alt_uvx, alt_flx, alt_wt;
+  DevBuf<int32_t> alt_anc_slot, alt_anc_point;
+  void set_interpolation(int family);
+  const Basis1D& interp_basis() const { return interp ? *herm : lagr; }
+  // The adaptive interpolation initialisation always runs on the Lagrange
+  // family (as DGAdaptIntp does); this restores the caller's family after it.
+  struct LagrangeScope {
+    Context& c;
+    int saved;
+    explicit LagrangeScope(Context& ctx) : c(ctx), saved(ctx.interp) { c.set_interpolation(0); }
+    ~LagrangeScope() { c.set_interpolation(saved); }
+  };
 
   // grid arrays
   DevBuf<int32_t> g_slot[kMaxDim], g_fbase[kMaxDim], g_fmaxl[kMaxDim], g_fmap[kMaxDim];

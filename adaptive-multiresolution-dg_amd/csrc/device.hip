@@ -224,6 +224,53 @@ __global__ __launch_bounds__(kBlock) void k_flux(const double* up, double* fp, c
   }
 }
 
+// f, f' and f'' of flux component c at u: Burgers u^2/2 (kind 0) or the
+// sin/cos pair scaled by s (kind 1) (FluxFunction, source/Interplation.cpp:5070-5113).
+__device__ inline double flux_der(int kind, int c, int order, double u, double s) {
+  if (kind == 0) return order == 0 ? u * u * 0.5 : (order == 1 ? u : 1.);
+  if (c == 0) return order == 0 ? sin(u) * s : (order == 1 ? cos(u) * s : -sin(u) * s);
+  return order == 0 ? cos(u) * s : (order == 1 ? -sin(u) * s : -cos(u) * s);
+}
+
+// Hermite P3 point flux (HermInterpolation::eval_fp_Her_1D / eval_fp_Her_2D,
+// source/Interplation.cpp:2921-3030,2045-2316).  Thread per (element, degree
+// of freedom).  Degree p of a dimension is (point p % 2, derivative p / 2);
+// a slope of f(u) is f'(u) times the slope of u at the same point, the cross
+// slope adds f''(u) u_x u_y, in the reference's order of operations.
+__global__ __launch_bounds__(kBlock) void k_flux_herm(const double* up, double* fp, int64_t E, int D, int kind,
+                                                       double s, int ncomp) {
+  const int P = D == 1 ? 4 : 16;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= E * P) return;
+  const int64_t e = gid / P;
+  const int q = static_cast<int>(gid - e * P);
+  const double* ue = up + e * P;
+  const int a = D == 1 ? q : q >> 2, b = D == 1 ? 0 : (q & 3);
+  const int k0 = a >> 1, k1 = b >> 1;
+  const int64_t n = E * P;
+  const double uq = ue[q];
+  for (int c = 0; c < ncomp; ++c) {
+    double v;
+    if (k0 == 0 && k1 == 0) {
+      v = flux_der(kind, c, 0, uq, s);
+    } else if (k1 == 0) {
+      const double u0 = ue[D == 1 ? a - 2 : (a - 2) * 4 + b];
+      v = 0.;
+      v += flux_der(kind, c, 1, u0, s) * uq;
+    } else if (k0 == 0) {
+      const double u0 = ue[a * 4 + b - 2];
+      v = 0.;
+      v += flux_der(kind, c, 1, u0, s) * uq;
+    } else {
+      const double u0 = ue[(a - 2) * 4 + b - 2], ux = ue[a * 4 + b - 2], uy = ue[(a - 2) * 4 + b];
+      v = 0.;
+      v += flux_der(kind, c, 2, u0, s) * ux * uy;
+      v += flux_der(kind, c, 1, u0, s) * uq;
+    }
+    fp[c * n + gid] = v;
+  }
+}
+
 // One dimension of the hierarchical-surplus transform
 // (LagrInterpolation::eval_fp_to_coe_D_Lag, source/Interplation.cpp:1279-1400).
 struct SurplusArgs {
@@ -583,7 +630,7 @@ __global__ __launch_bounds__(kFinalBlock) void k_up_final_pt(const TermDev* term
   } else if (kind == 1) {
     fp[gid] = sin(u) * s;
     fp[n + gid] = cos(u) * s;
-  } else {
+  } else if (kind == 2) {
     for (int c = 0; c < ncomp; ++c) fp[c * n + gid] = vcoef[c * n + gid] * u;
   }
 }
@@ -774,7 +821,7 @@ __global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, i
     } else if (kind == 1) {
       fp[gid] = sin(u) * s;
       fp[n + gid] = cos(u) * s;
-    } else {
+    } else if (kind == 2) {
       for (int cc = 0; cc < ncomp; ++cc) fp[cc * n + gid] = vcoef[cc * n + gid] * u;
     }
   }
@@ -953,8 +1000,40 @@ Context::Context(int dim, int k_, int m_, int nmax_)
   sync();
 }
 
+void Context::set_interpolation(int family) {
+  if (family != 0 && family != 1) throw std::runtime_error("unknown interpolation family");
+  if (family == interp) return;
+  if (family == 1 && !herm) {
+    if (m != 3) throw std::runtime_error("Hermite interpolation needs interpolation degree m = 3");
+    std::unique_ptr<Basis1D> hb(new Basis1D(BasisKind::Hermite, 3, nmax));
+    OperatorTables ha(*hb, alpt);
+    const int ns = 1 << nmax;
+    upload(alt_pt, to_blocks(alpt_at_herm_points(alpt, *hb), ns, k + 1, m + 1), stream);
+    upload(alt_uv, to_blocks(ha.u_v, ns, m + 1, k + 1), stream);
+    upload(alt_uvx, to_blocks(ha.u_vx, ns, m + 1, k + 1), stream);
+    std::vector<double> flx(ha.ulft_vjp.size());
+    for (size_t i = 0; i < flx.size(); ++i) flx[i] = ha.ulft_vjp[i] + ha.urgt_vjp[i];
+    upload(alt_flx, to_blocks(flx, ns, m + 1, k + 1), stream);
+    SurplusTables hs(*hb);
+    upload(alt_anc_slot, hs.anc_slot, stream);
+    upload(alt_anc_point, hs.anc_point, stream);
+    upload(alt_wt, hs.wt, stream);
+    sync();
+    herm = std::move(hb);
+  }
+  t_pt.swap(alt_pt);
+  t_la_uv.swap(alt_uv);
+  t_la_uvx.swap(alt_uvx);
+  t_la_flx.swap(alt_flx);
+  t_wt.swap(alt_wt);
+  t_anc_slot.swap(alt_anc_slot);
+  t_anc_point.swap(alt_anc_point);
+  interp = family;
+}
+
 struct Context::FusedPlan {
   int64_t version = -1;
+  int interp = -1;
   const double *u = nullptr, *tn = nullptr, *fucoe = nullptr, *tmp = nullptr;
   double* unext = nullptr;
   int flux_kind = -1;
@@ -1262,8 +1341,29 @@ void Context::compute_vcoef() {
   ensure_fields();
 }
 
+namespace {
+void check_herm_flux(int kind, int D) {
+  if (D > 2) throw std::runtime_error("Hermite flux interpolation is one- or two-dimensional");
+  if (kind != 0 && kind != 1) throw std::runtime_error("Hermite flux interpolation needs a flux of u alone (kinds 0, 1)");
+}
+}  // namespace
+
 void Context::flux(int kind, const double* params) {
   const double s = params ? params[0] : 0.;
+  if (interp == 1) {
+    check_herm_flux(kind, D);
+    if (kind == 1 && D != 2) throw std::runtime_error("sin/cos flux is two-dimensional");
+    flux_kind = kind;
+    flux_param = s;
+    same_flux = (kind == 0);
+    const int ncomp = same_flux ? 1 : D;
+    const int64_t n = E() * P;
+    if (n == 0) return;
+    prof_begin(PROF_FLUX, 8.0 * static_cast<double>(n) * (1 + ncomp));
+    hipLaunchKernelGGL(k_flux_herm, dim3(nblocks(n)), dim3(kBlock), 0, stream, up.p, fp.p, E(), D, kind, s, ncomp);
+    prof_end();
+    return;
+  }
   if (kind == ADG_FLUX_VARCOEF_ID && (flux_kind != kind || flux_param != s)) {
     flux_kind = kind;
     flux_param = s;
@@ -1388,7 +1488,7 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     FusedPlan& pl = *plans[slot];
     const bool same_alpha = std::equal(pl.alpha, pl.alpha + D, p.alpha);
     if (pl.version == grid_version && pl.tn == tn.p && pl.fucoe == fucoe.p && pl.unext == u_next.p &&
-        pl.tmp == fused_tmp.p && pl.flux_kind == p.flux_kind && same_alpha)
+        pl.tmp == fused_tmp.p && pl.flux_kind == p.flux_kind && pl.interp == interp && same_alpha)
       return pl;
   } else {
     slot = !plans[0] ? 0 : (!plans[1] ? 1 : 0);
@@ -1408,6 +1508,7 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
   if (!plans[slot]) plans[slot].reset(new FusedPlan());
   FusedPlan& pl = *plans[slot];
   pl.version = grid_version;
+  pl.interp = interp;
   pl.u = u.p; pl.tn = tn.p; pl.fucoe = fucoe.p; pl.unext = u_next.p; pl.tmp = fused_tmp.p;
   pl.flux_kind = p.flux_kind;
   std::copy(p.alpha, p.alpha + D, pl.alpha);
@@ -1532,6 +1633,10 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   flux_param = p.flux_param;
   same_flux = (p.flux_kind == 0);
   if (p.flux_kind == 1 && D != 2) throw std::runtime_error("sin/cos flux is two-dimensional");
+  if (interp == 1) check_herm_flux(p.flux_kind, D);
+  // Hermite: the final eval_up kernel writes point values only (kind -1); the
+  // point flux needs the element's other degrees of freedom and runs after it
+  const int up_kind = interp == 1 ? -1 : p.flux_kind;
   FusedPlan& pl = fused_plan(p);
   const int ncomp = same_flux ? 1 : D;
   // point values and flux
@@ -1556,7 +1661,7 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
 #define ADG_UP(K1)                                                                                                 \
   if (k + 1 == K1)                                                                                                 \
     hipLaunchKernelGGL((k_up_final_pt<K1>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, \
-                       E(), P, m + 1, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap);
+                       E(), P, m + 1, D, up.p, fp.p, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);
         ADG_UP(2) ADG_UP(3) ADG_UP(4)
 #undef ADG_UP
         return;
@@ -1571,11 +1676,14 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
 #define ADG_UF(K1, M1)                                                                                             \
   if (k + 1 == K1 && m + 1 == M1)                                                                                  \
     hipLaunchKernelGGL((k_up_final<K1, M1>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p, pl.n_up_terms, EB, nch, \
-                       rc, E(), P, D, up.p, fp.p, vcoef.p, p.flux_kind, p.flux_param, ncomp, lsum_cap);
+                       rc, E(), P, D, up.p, fp.p, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);
       ADG_UF(2, 2) ADG_UF(2, 3) ADG_UF(2, 4) ADG_UF(2, 5) ADG_UF(3, 2) ADG_UF(3, 3) ADG_UF(3, 4) ADG_UF(3, 5)
       ADG_UF(4, 2) ADG_UF(4, 3) ADG_UF(4, 4) ADG_UF(4, 5)
 #undef ADG_UF
     });
+    if (interp == 1)
+      hipLaunchKernelGGL(k_flux_herm, dim3(nblocks(E() * P)), dim3(kBlock), 0, stream, up.p, fp.p, E(), D,
+                         p.flux_kind, p.flux_param, ncomp);
     prof_end();
   }
   surplus_transform(fp.p, fucoe.p, interA.p, interB.p, ncomp);
@@ -1724,6 +1832,7 @@ void Context::init_adaptive_intp_batch(double (*f)(const double*, int, void*), v
     explicit CapGuard(int& cap) : c(cap), saved(cap) { c = -1; }
     ~CapGuard() { c = saved; }
   } cap_guard(lsum_cap);
+  LagrangeScope lagrange_scope(*this);
   if (up_host.size() != static_cast<size_t>(E()) * P) {
     up_host.assign(static_cast<size_t>(E()) * P, 0.);
     new_add.assign(E(), 1);

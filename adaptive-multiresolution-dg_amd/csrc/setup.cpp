@@ -221,6 +221,51 @@ static double lagr_phi(double xt, int msh, int P, int p) {
   throw std::runtime_error("Lagrange basis: only degree 1..4, msh_case 1 are provided");
 }
 
+// ---------------------------------------------------------------------------
+// Hermite P3 basis, msh_case 1 (HermBasis::phi_P1_L1 / phi_P1_L1_D1,
+// source/HermBasis.cpp:347-656): two points per cell, value and slope at each.
+
+static double herm_phi(double x, int msh, int pp) {
+  if (x < 0 || x > 1) return 0.;
+  const int p = pp % 2, l = pp / 2;
+  if (msh == 0) {
+    if (p == 0) return l == 0 ? (-1 + x) * (-1 + x) * (1 + 2 * x) : (1 - x) * (1 - x) * x;
+    return l == 0 ? -x * x * (-3 + 2 * x) : (-1 + x) * x * x;
+  }
+  if (p == 0) {
+    if (x > 0.5) return 0.;
+    return l == 0 ? -4 * x * x * (4 * x - 3) : 2 * x * x * (2 * x - 1);
+  }
+  if (x < 0.5) return 0.;
+  return l == 0 ? 4 * (x - 1) * (x - 1) * (4 * x - 1) : 2 * (x - 1) * (x - 1) * (2 * x - 1);
+}
+
+static double herm_phi_d1(double x, int msh, int pp) {
+  if (x < 0 || x > 1) return 0.;
+  const int p = pp % 2, l = pp / 2;
+  if (msh == 0) {
+    if (p == 0) return l == 0 ? 6 * x * (-1 + x) : (x - 1) * (3 * x - 1);
+    return l == 0 ? -6 * x * (x - 1) : x * (3 * x - 2);
+  }
+  if (p == 0) {
+    if (x > 0.5) return 0.;
+    return l == 0 ? 24 * x * (1 - 2 * x) : 4 * x * (3 * x - 1);
+  }
+  if (x < 0.5) return 0.;
+  return l == 0 ? 24 * (x - 1) * (2 * x - 1) : 4 * (x - 1) * (3 * x - 2);
+}
+
+// HermBasis::val0 / val1 (source/HermBasis.cpp:157-250): levels >= 2 are the
+// level-1 functions shifted and scaled by 2^(-(n-1) * (l - derivative))
+static double herm_val(double x, int n, int j, int p, int deriv) {
+  if (n <= 1) return deriv == 0 ? herm_phi(x, n, p) : herm_phi_d1(x, n, p);
+  const int odd_j = (j - 1) / 2;
+  const double xtrans = std::pow(2, n - 1) * x - odd_j;
+  const int l = p / 2;
+  if (deriv == 0) return herm_phi(xtrans, 1, p) * std::pow(2, -(n - 1) * l);
+  return herm_phi_d1(xtrans, 1, p) * std::pow(2, -(n - 1) * (l - 1));
+}
+
 static double lagr_val0(double x, int n, int j, int p, int P) {
   if (n <= 1) return lagr_phi(x, n, P, p);
   const int odd_j = (j - 1) / 2;
@@ -232,7 +277,16 @@ static double lagr_val0(double x, int n, int j, int p, int P) {
 
 Basis1D::Basis1D(BasisKind k, int pmax_, int nmax_, int msh_case_)
     : kind(k), pmax(pmax_), msh_case(msh_case_), nmax(nmax_) {
-  if (kind == BasisKind::Lagrange) {
+  if (kind == BasisKind::Hermite) {
+    // HermBasis::set_interp_msh01 (source/HermBasis.cpp:40-68), P3 only
+    if (pmax != 3 || msh_case != 1) throw std::runtime_error("Hermite basis: only degree 3, msh_case 1 is provided");
+    msh0 = {-1.0 + ROUND_OFF, 1.0 - ROUND_OFF, -1.0 + ROUND_OFF, 1.0 - ROUND_OFF};
+    msh1 = {0.0 - ROUND_OFF, 0.0 + ROUND_OFF, 0.0 - ROUND_OFF, 0.0 + ROUND_OFF};
+    for (int i = 0; i < pmax + 1; ++i) {
+      msh0[i] = (msh0[i] + 1) / 2.;
+      msh1[i] = (msh1[i] + 1) / 2.;
+    }
+  } else if (kind == BasisKind::Lagrange) {
     if (msh_case != 1) throw std::runtime_error("Lagrange basis: only msh_case 1 is provided");
     // LagrBasis::set_interp_msh01 (source/LagrBasis.cpp:33-154), points on [-1,1]
     if (pmax == 1) { msh0 = {-1. / 3., 1. / 3.}; msh1 = {-2. / 3., 2. / 3.}; }
@@ -274,6 +328,10 @@ double Basis1D::val(int order, double x, int deriv, int sgn) const {
     if (deriv == 0) return alpt_val0(xlim, n, j, p, pmax);
     if (deriv == 1) return alpt_val1(xlim, n, j, p, pmax);
     throw std::runtime_error("Alpert basis: derivative order > 1 is not provided");
+  }
+  if (kind == BasisKind::Hermite) {
+    if (deriv > 1) throw std::runtime_error("Hermite basis: derivative order > 1 is not provided");
+    return herm_val(xlim, n, j, p, deriv);
   }
   if (deriv != 0) throw std::runtime_error("Lagrange basis: derivatives are not provided");
   return lagr_val0(xlim, n, j, p, pmax);
@@ -391,6 +449,23 @@ std::vector<double> alpt_at_lagr_points(const Basis1D& alpt, const Basis1D& lagr
   return t;
 }
 
+std::vector<double> alpt_at_herm_points(const Basis1D& alpt, const Basis1D& herm) {
+  const int na = alpt.size(), nh = herm.size();
+  std::vector<double> t(static_cast<size_t>(na) * nh, 0.);
+  for (int m1 = 0; m1 < nh; ++m1) {
+    const double pos = herm.point(m1);
+    const int m = Basis1D::herm_deriv(m1 % (herm.pmax + 1));
+    for (int m2 = 0; m2 < na; ++m2) {
+      int n, j, p;
+      alpt.decode(m2, n, j, p);
+      double xl, xr;
+      cell_support(n, j, xl, xr);
+      t[static_cast<size_t>(m2) * nh + m1] = (pos < xl || pos > xr) ? 0. : alpt.val(m2, pos, m, 0);
+    }
+  }
+  return t;
+}
+
 std::vector<double> lagr_mass_diag(const Basis1D& lagr) {
   std::vector<double> d(lagr.size());
   for (int i = 0; i < lagr.size(); ++i) d[i] = product_volume(lagr, i, lagr, i, 0, 0);
@@ -435,6 +510,24 @@ SurplusTables::SurplusTables(const Basis1D& lagr) {
     for (int c = 0; c < p1; ++c) {
       anc_slot[static_cast<size_t>(s) * p1 + c] = p_slot[c];
       anc_point[static_cast<size_t>(s) * p1 + c] = p_num[c];
+    }
+    if (lagr.kind == BasisKind::Hermite) {
+      // ancestors at the leftmost position take the level-0 left-point
+      // degrees of freedom 0, 2, the others 1, 3, in scan order
+      const double pl = sorted[0];
+      for (int p0 = 0; p0 < p1; ++p0) {
+        int i10 = 0, i20 = 1;
+        for (int c = 0; c < p1; ++c) {
+          int ic0;
+          if (std::fabs(p_pos[c] - pl) < 1.0e-14) { ic0 = i10; i10 += 2; }
+          else { ic0 = i20; i20 += 2; }
+          const int l1 = Basis1D::herm_deriv(p0), l2 = Basis1D::herm_deriv(ic0);
+          const double xt = lagr.msh1[p0];
+          wt[(static_cast<size_t>(s) * p1 + p0) * p1 + c] =
+              -1.0 * std::pow(2, (l - 1) * (l1 - l2)) * lagr.val(ic0, xt, l1, 0);
+        }
+      }
+      continue;
     }
     for (int p0 = 0; p0 < p1; ++p0) {
       for (int c = 0; c < p1; ++c) {

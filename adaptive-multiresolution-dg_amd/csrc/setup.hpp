@@ -27,7 +27,7 @@ namespace adg {
 constexpr double ROUND_OFF = 1e-13;  // include/libs.h:28
 constexpr double PI = 3.1415926535897932384626433832795028841971693993751058209749445923078164062;
 
-enum class BasisKind : int { Alpert = 0, Lagrange = 1 };
+enum class BasisKind : int { Alpert = 0, Lagrange = 1, Hermite = 2 };
 
 // 1D hierarchical index helpers.  "slot" is the level-ordered position of a
 // 1D cell: level 0 -> 0, level n>=1 support j (odd) -> 2^(n-1) + (j-1)/2
@@ -54,6 +54,10 @@ struct Basis1D {
   double val(int order, double x, int deriv = 0, int sgn = 0) const;
   // interpolation point of Lagrange basis `order` (LagrBasis::intep_pt)
   double point(int order) const;
+  // Hermite degrees of freedom: order's p -> (point 0/1, derivative 0/1)
+  // (HermBasis::deg_pt_deri_1d): 0 value at left, 1 value at right, 2 slope
+  // at left, 3 slope at right
+  static int herm_deriv(int p) { return p / 2; }
 };
 
 // GL 10-point rule on [tl,tr] as Quad::GL_1D evaluates it.
@@ -76,6 +80,10 @@ struct OperatorTables {
 // point (0 outside the closed support); the transpose that FastLagrIntp builds
 // from LagrInterpolation::Lag_pt_Alpt_1D (source/FastMultiplyLU.cpp:1339-1360).
 std::vector<double> alpt_at_lagr_points(const Basis1D& alpt, const Basis1D& lagr);
+// The same for the Hermite degrees of freedom: value or first derivative of
+// the Alpert basis at the Hermite point (HermInterpolation::eval_Her_pt_at_Alpt_1D,
+// source/Interplation.cpp:1886-1976).
+std::vector<double> alpt_at_herm_points(const Basis1D& alpt, const Basis1D& herm);
 
 // Diagonal of the Lagrange-Lagrange mass matrix (u_v(num,num)), used by the
 // interpolation refinement indicator (source/DGAdaptIntp.cpp:236-272).
@@ -89,7 +97,9 @@ struct SurplusTables {
   std::vector<int> anc_slot;   // [nslots * p1]
   std::vector<int> anc_point;  // [nslots * p1]
   std::vector<double> wt;      // [nslots * p1 * p1]
-  explicit SurplusTables(const Basis1D& lagr);
+  // Lagrange (LagrInterpolation::set_pts_wts_1d_ada_Lag) or Hermite
+  // (HermInterpolation::set_pts_wts_1d_ada_Her, source/Interplation.cpp:3166-3316)
+  explicit SurplusTables(const Basis1D& intp);
 };
 
 // L2 projection of func onto every basis function of B (AllBasis::projection

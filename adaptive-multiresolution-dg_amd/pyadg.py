@@ -18,6 +18,7 @@ TAB_ALPT_AT_LAGR, TAB_LA_UV, TAB_LA_UVX, TAB_LA_FLX = 0, 1, 2, 3
 TAB_AA_UV, TAB_AA_UVX, TAB_AA_ULFT, TAB_AA_URGT, TAB_AA_UJP = 4, 5, 6, 7, 8
 LU_FULL, LU_L, LU_U = 0, 1, 2
 FLUX_BURGERS, FLUX_SINCOS, FLUX_VARCOEF = 0, 1, 2
+INTERP_LAGRANGE, INTERP_HERMITE = 0, 1
 RK_EULER, RK2, RK3 = 0, 2, 3
 
 POINT_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p)
@@ -72,6 +73,7 @@ def lib():
             "adg_sweep": (i32, [vp, i32, i32, i32, i32, I32P, I32P, dbl, DP, DP]),
             "adg_eval_up": (i32, [vp]),
             "adg_set_level_sum_cap": (i32, [vp, i32]),
+            "adg_set_interpolation": (i32, [vp, i32]),
             "adg_lagr_points": (i32, [vp, DP]),
             "adg_flux": (i32, [vp, i32, DP]),
             "adg_flux_set": (i32, [vp, i32, DP]),
@@ -225,6 +227,11 @@ class Solution:
         x = np.zeros(self.num_elem * self.P * self.dim)
         self._check(lib().adg_lagr_points(self._h, _dp(x)))
         return x.reshape(self.num_elem, self.P, self.dim)
+
+    def set_interpolation(self, family):
+        """INTERP_LAGRANGE or INTERP_HERMITE (P3, needs m = 3) for eval_up, flux,
+        fp_to_coe, rhs_lagr_*, stage and run_hyperbolic."""
+        self._check(lib().adg_set_interpolation(self._h, family))
 
     def set_level_sum_cap(self, cap):
         """Coarse grid: sweeps ignore elements with level sum above cap (cap < 0: none)."""

@@ -103,10 +103,24 @@ int adg_field_get(adg_ctx* c, int field, double* host);
 int adg_sweep(adg_ctx* c, int table, int flx, int lu, int dim, const int32_t* size_in,
               const int32_t* size_out, double coef, const double* in, double* out);
 
-/* Coordinates of every element's Lagrange points (LagrBasis::intep_pt,
- * source/LagrBasis.cpp:19-28): x[(e*P + q)*dim + d], points in row-major
+/* Coordinates of every element's interpolation points of the current family
+ * (LagrBasis::intep_pt, source/LagrBasis.cpp:19-28; HermBasis::intep_pt,
+ * source/HermBasis.cpp:9-30): x[(e*P + q)*dim + d], points in row-major
  * order.  For host flux / coefficient closures. */
 int adg_lagr_points(adg_ctx* c, double* x);
+
+/* Interpolation family of adg_eval_up, adg_flux, adg_fp_to_coe,
+ * adg_rhs_lagr_vol/flx, adg_sweep tables 0-3, adg_stage and
+ * adg_run_hyperbolic.  ADG_INTERP_HERMITE is the Hermite P3 flux
+ * interpolation (HermInterpolation::nonlinear_Herm_2D_fast with FastHermIntp
+ * and HyperbolicSameFluxHermRHS / HyperbolicDiffFluxHermRHS,
+ * source/Interplation.cpp:1886-2316,3166-3316,3651-3850,4609-4621;
+ * source/FastMultiplyLU.cpp:970-1069,1316-1336): degrees of freedom are the
+ * value and slope of u at two points per cell; it needs m = 3, dim <= 2 and a
+ * flux of u alone (ADG_FLUX_BURGERS, ADG_FLUX_SINCOS).  The adaptive
+ * interpolation initialisation always uses Lagrange, as in the reference. */
+enum { ADG_INTERP_LAGRANGE = 0, ADG_INTERP_HERMITE = 1 };
+int adg_set_interpolation(adg_ctx* c, int family);
 
 /* Coarse grid: the _coarse_grid variants of the reference
  * (FastMultiplyLU::transform_1D_coarse_grid, source/FastMultiplyLU.cpp:514-594;

@@ -39,6 +39,7 @@
 #include <cstdio>
 #include <functional>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <set>
 #include <stdexcept>
@@ -75,6 +76,15 @@ struct or_ctx {
   adg::OperatorTables aa, la;
   std::vector<double> pt, la_flx, lmass;
   adg::SurplusTables sur;
+  // Hermite P3 family (HermInterpolation, source/Interplation.cpp:1886-3850;
+  // HyperbolicHermRHS, source/FastMultiplyLU.cpp:1221-1267), built on first use.
+  // interp selects the family of eval_up / flux / fp_to_coe / rhs_lagr; the
+  // adaptive interpolation initialisation always uses Lagrange, as the reference.
+  int interp = 0;
+  std::unique_ptr<adg::Basis1D> herm;
+  std::unique_ptr<adg::OperatorTables> ha;
+  std::unique_ptr<adg::SurplusTables> hsur;
+  std::vector<double> hpt, ha_flx;
   adg::HashKey hk;
   std::unordered_map<int, OElem> dg;
   std::unordered_map<int, OElem*> leaf, leaf0;
@@ -92,6 +102,27 @@ struct or_ctx {
   }
 
   int key(const std::vector<int>& l, const std::vector<int>& j) const { return hk.key(l.data(), j.data(), D); }
+
+  void build_hermite() {
+    if (herm) return;
+    if (m != 3) throw std::runtime_error("oracle: the Hermite family needs interpolation degree 3");
+    herm.reset(new adg::Basis1D(adg::BasisKind::Hermite, 3, nmax));
+    ha.reset(new adg::OperatorTables(*herm, alpt));
+    hsur.reset(new adg::SurplusTables(*herm));
+    hpt = adg::alpt_at_herm_points(alpt, *herm);
+    ha_flx.resize(ha->ulft_vjp.size());
+    for (size_t i = 0; i < ha_flx.size(); ++i) ha_flx[i] = ha->ulft_vjp[i] + ha->urgt_vjp[i];
+  }
+  void set_interp(int family) {
+    if (family != 0 && family != 1) throw std::runtime_error("oracle: unknown interpolation family");
+    if (family == 1) build_hermite();
+    interp = family;
+  }
+  const adg::Basis1D& ib() const { return interp ? *herm : lagr; }
+  const std::vector<double>& ipt() const { return interp ? hpt : pt; }
+  const adg::OperatorTables& ila() const { return interp ? *ha : la; }
+  const std::vector<double>& iflx() const { return interp ? ha_flx : la_flx; }
+  const adg::SurplusTables& isur() const { return interp ? *hsur : sur; }
 
   // ---- element construction and predicates (source/Element.cpp) ----
   OElem make_elem(const std::vector<int>& l, const std::vector<int>& j) const {
@@ -523,8 +554,8 @@ struct or_ctx {
 
   void eval_up() {
     for (auto& it : dg) std::fill(it.second.up.begin(), it.second.up.end(), 0.);
-    std::vector<const double*> mats(D, pt.data());
-    std::vector<int> cols(D, lagr.size()), flx(D, 0);
+    std::vector<const double*> mats(D, ipt().data());
+    std::vector<int> cols(D, ib().size()), flx(D, 0);
     partial_sums(mats, cols, flx, k, m, 1.0, [](OElem& e) { return e.ucoe; }, [](OElem& e) -> std::vector<double>& { return e.up; });
   }
 
@@ -533,8 +564,8 @@ struct or_ctx {
       std::vector<const double*> mats(D);
       std::vector<int> cols(D, alpt.size()), flx(D, 0);
       for (int dd = 0; dd < D; ++dd) {
-        if (dd == d) { mats[dd] = flux_term ? la_flx.data() : la.u_vx.data(); flx[dd] = flux_term ? 1 : 0; }
-        else mats[dd] = la.u_v.data();
+        if (dd == d) { mats[dd] = flux_term ? iflx().data() : ila().u_vx.data(); flx[dd] = flux_term ? 1 : 0; }
+        else mats[dd] = ila().u_v.data();
       }
       partial_sums(mats, cols, flx, m, k, flux_term ? 0.5 : 1.0,
                    [d](OElem& e) { return e.fucoe[d]; }, [](OElem& e) -> std::vector<double>& { return e.rhs; });
@@ -551,7 +582,56 @@ struct or_ctx {
       for (int i = 0; i < K; ++i) it.second.rhs[i] += it.second.to[i];
   }
 
+  // f, f' and f'' of flux component d at u (FluxFunction::burgers_flux_*,
+  // sin_flux_*, cos_flux_*, source/Interplation.cpp:5070-5113; the sin/cos pair
+  // is scaled by s)
+  static double flux_der(int kind, int d, int order, double u, double s) {
+    if (kind == 0) return order == 0 ? u * u * 0.5 : (order == 1 ? u : 1.);
+    if (d == 0) return order == 0 ? std::sin(u) * s : (order == 1 ? std::cos(u) * s : -std::sin(u) * s);
+    return order == 0 ? std::cos(u) * s : (order == 1 ? -std::sin(u) * s : -std::cos(u) * s);
+  }
+
+  // HermInterpolation::eval_fp_Her_1D / eval_fp_Her_2D
+  // (source/Interplation.cpp:2921-3030,2045-2316): the degree of freedom of a
+  // point is a value or a slope per dimension; slopes of f(u) follow from the
+  // chain rule with the values, slopes and cross slope of u at the same point.
+  void flux_herm(int kind, const double* p) {
+    if (D > 2) throw std::runtime_error("oracle: Hermite flux is one- or two-dimensional");
+    if (kind != 0 && kind != 1) throw std::runtime_error("oracle: Hermite flux needs a flux of u alone");
+    const double s = p ? p[0] : 0.;
+    for (auto& it : dg) {
+      OElem& e = it.second;
+      const std::vector<double>& up = e.up;
+      for (int q = 0; q < P; ++q) {
+        const int a = D == 1 ? q : q / 4, b = D == 1 ? 0 : q % 4;
+        auto at = [&](int i0, int i1) { return up[D == 1 ? i0 : i0 * 4 + i1]; };
+        const int k0 = a / 2, k1 = b / 2;
+        for (int d = 0; d < D; ++d) {
+          double v;
+          if (k0 == 0 && k1 == 0) {
+            v = flux_der(kind, d, 0, up[q], s);
+          } else if (k0 == 1 && k1 == 0) {
+            const double u = at(a - 2, b);
+            v = 0.;
+            v += flux_der(kind, d, 1, u, s) * up[q];
+          } else if (k0 == 0 && k1 == 1) {
+            const double u = at(a, b - 2);
+            v = 0.;
+            v += flux_der(kind, d, 1, u, s) * up[q];
+          } else {
+            const double u = at(a - 2, b - 2), ux = at(a, b - 2), uy = at(a - 2, b);
+            v = 0.;
+            v += flux_der(kind, d, 2, u, s) * ux * uy;
+            v += flux_der(kind, d, 1, u, s) * up[q];
+          }
+          e.fp[d][q] = v;
+        }
+      }
+    }
+  }
+
   void flux(int kind, const double* p) {
+    if (interp == 1) { flux_herm(kind, p); return; }
     for (auto& it : dg) {
       OElem& e = it.second;
       std::vector<int> idx(D, 0);
@@ -586,6 +666,7 @@ struct or_ctx {
   // LagrInterpolation::eval_fp_to_coe_D_Lag (all flux components, all elements)
   void fp_to_coe() {
     const int p1 = m + 1;
+    const adg::SurplusTables& sur = isur();
     for (auto& it : dg)
       for (int c = 0; c < D; ++c) it.second.finter[c][0] = it.second.fp[c];
     for (int d = 0; d < D; ++d) {
@@ -748,6 +829,7 @@ static std::vector<double>& field_ref(or_ctx* c, OElem& e, int field) {
     case 4: return e.ucoe_intp;
     default:
       if (field >= 10 && field < 10 + c->D) return e.fucoe[field - 10];
+      if (field >= 20 && field < 20 + c->D) return e.fp[field - 20];
   }
   throw std::runtime_error("oracle: unknown field");
 }
@@ -772,10 +854,10 @@ void or_set_field(or_ctx* c, int field, const double* in) {
 
 static const double* table_ptr(or_ctx* c, int table, int& cols) {
   switch (table) {
-    case 0: cols = c->lagr.size(); return c->pt.data();
-    case 1: cols = c->alpt.size(); return c->la.u_v.data();
-    case 2: cols = c->alpt.size(); return c->la.u_vx.data();
-    case 3: cols = c->alpt.size(); return c->la_flx.data();
+    case 0: cols = c->ib().size(); return c->ipt().data();
+    case 1: cols = c->alpt.size(); return c->ila().u_v.data();
+    case 2: cols = c->alpt.size(); return c->ila().u_vx.data();
+    case 3: cols = c->alpt.size(); return c->iflx().data();
     case 4: cols = c->alpt.size(); return c->aa.u_v.data();
     case 5: cols = c->alpt.size(); return c->aa.u_vx.data();
     case 6: cols = c->alpt.size(); return c->aa.ulft_vjp.data();
@@ -800,6 +882,7 @@ void or_sweep(or_ctx* c, int table, int flx, int lu, int dim, const int32_t* siz
   for (size_t i = 0; i < v.size(); ++i) std::copy(v[i]->to.begin(), v[i]->to.end(), out + i * nout);
 }
 
+void or_set_interpolation(or_ctx* c, int family) { c->set_interp(family); }
 void or_eval_up(or_ctx* c) { c->eval_up(); }
 void or_set_level_sum_cap(or_ctx* c, int cap) { c->cap = cap; }
 void or_flux(or_ctx* c, int kind, const double* params) { c->flux(kind, params); }
@@ -1030,7 +1113,7 @@ double or_l2_error_split(or_ctx* num, or_ctx* exact) {
 int64_t or_table_size(or_ctx* c, int table, int32_t* rows, int32_t* cols) {
   int cl = 0;
   table_ptr(c, table, cl);
-  const int r = (table == 0 || table >= 4) ? c->alpt.size() : c->lagr.size();
+  const int r = (table == 0 || table >= 4) ? c->alpt.size() : c->ib().size();
   if (rows) *rows = r;
   if (cols) *cols = cl;
   return static_cast<int64_t>(r) * cl;
@@ -1052,14 +1135,19 @@ int64_t or_neighbors(or_ctx* c, int64_t elem, int dim, int flx, int32_t* out, in
   return static_cast<int64_t>(rel.size());
 }
 
-double or_basis_val(or_ctx* c, int kind, int order, double x, int deriv, int sgn) {
-  return (kind == 0 ? c->alpt : c->lagr).val(order, x, deriv, sgn);
+static const adg::Basis1D& basis_of(or_ctx* c, int kind) {
+  if (kind == 2) { c->build_hermite(); return *c->herm; }
+  return kind == 0 ? c->alpt : c->lagr;
 }
 
-double or_lagr_point(or_ctx* c, int order) { return c->lagr.point(order); }
+double or_basis_val(or_ctx* c, int kind, int order, double x, int deriv, int sgn) {
+  return basis_of(c, kind).val(order, x, deriv, sgn);
+}
+
+double or_lagr_point(or_ctx* c, int order) { return c->ib().point(order); }
 
 double or_product_volume(or_ctx* c, int kind_u, int ru, int kind_v, int cv, int du, int dv) {
-  return adg::product_volume(kind_u == 0 ? c->alpt : c->lagr, ru, kind_v == 0 ? c->alpt : c->lagr, cv, du, dv);
+  return adg::product_volume(basis_of(c, kind_u), ru, basis_of(c, kind_v), cv, du, dv);
 }
 
 int32_t or_hash_key(const int32_t* level, const int32_t* suppt, int dim) {
@@ -1068,9 +1156,10 @@ int32_t or_hash_key(const int32_t* level, const int32_t* suppt, int dim) {
 }
 
 void or_surplus_tables(or_ctx* c, int32_t* anc_slot, int32_t* anc_point, double* wt) {
-  std::copy(c->sur.anc_slot.begin(), c->sur.anc_slot.end(), anc_slot);
-  std::copy(c->sur.anc_point.begin(), c->sur.anc_point.end(), anc_point);
-  std::copy(c->sur.wt.begin(), c->sur.wt.end(), wt);
+  const adg::SurplusTables& sur = c->isur();
+  std::copy(sur.anc_slot.begin(), sur.anc_slot.end(), anc_slot);
+  std::copy(sur.anc_point.begin(), sur.anc_point.end(), anc_point);
+  std::copy(sur.wt.begin(), sur.wt.end(), wt);
 }
 
 void or_lagr_mass(or_ctx* c, double* out) { std::copy(c->lmass.begin(), c->lmass.end(), out); }

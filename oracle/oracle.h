@@ -21,7 +21,7 @@ void or_destroy(or_ctx* c);
 int64_t or_num_elem(or_ctx* c);
 void or_get_grid(or_ctx* c, int32_t* level, int32_t* suppt, int32_t* hash);
 
-/* fields: 0 ucoe_alpt, 1 rhs, 2 up_intp, 3 ucoe_tn, 4 ucoe_intp, 10+d fucoe_intp[d] */
+/* fields: 0 ucoe_alpt, 1 rhs, 2 up_intp, 3 ucoe_tn, 4 ucoe_intp, 10+d fucoe_intp[d], 20+d fp_intp[d] */
 void or_get_field(or_ctx* c, int field, double* out);
 void or_set_field(or_ctx* c, int field, const double* in);
 
@@ -31,6 +31,10 @@ void or_set_field(or_ctx* c, int field, const double* in);
 void or_sweep(or_ctx* c, int table, int flx, int lu, int dim, const int32_t* size_in,
               const int32_t* size_out, double coef, const double* in, double* out);
 
+/* interpolation family of eval_up / flux / fp_to_coe / rhs_lagr_*: 0 Lagrange,
+ * 1 Hermite P3 (HermInterpolation + HyperbolicHermRHS; needs m = 3, flux kinds 0
+ * and 1, dim <= 2 for the flux) */
+void or_set_interpolation(or_ctx* c, int family);
 void or_eval_up(or_ctx* c);                                   /* FastLagrIntp::eval_up_Lagr */
 /* _coarse_grid variants: sweeps and assembled operators skip elements whose
  * level sum exceeds cap (negative: no cap). */
@@ -74,9 +78,9 @@ void or_table(or_ctx* c, int table, double* out);
 /* related elements (vol flx=0 / flx flx=1) of element `elem` (hash order) in dimension dim, in
  * summation order; returns the count, writes up to `cap` element indices (hash order) */
 int64_t or_neighbors(or_ctx* c, int64_t elem, int dim, int flx, int32_t* out, int64_t cap);
-/* Basis1D values: kind 0 Alpert (degree k), 1 Lagrange (degree m) */
+/* Basis1D values: kind 0 Alpert (degree k), 1 Lagrange (degree m), 2 Hermite (degree 3) */
 double or_basis_val(or_ctx* c, int kind, int order, double x, int deriv, int sgn);
-double or_lagr_point(or_ctx* c, int order);
+double or_lagr_point(or_ctx* c, int order);  /* point of the current interpolation family */
 double or_product_volume(or_ctx* c, int kind_u, int ru, int kind_v, int cv, int du, int dv);
 int32_t or_hash_key(const int32_t* level, const int32_t* suppt, int dim);
 void or_surplus_tables(or_ctx* c, int32_t* anc_slot, int32_t* anc_point, double* wt);

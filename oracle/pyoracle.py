@@ -12,6 +12,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
 FIELD_UCOE, FIELD_RHS, FIELD_UP, FIELD_TN, FIELD_UINTP = 0, 1, 2, 3, 4
+FIELD_FUCOE, FIELD_FP = 10, 20  # + flux component
 FUNC_T = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p)
 SEP_T = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
 COEF_T = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p)
@@ -35,6 +36,7 @@ def lib():
             "or_set_field": (None, [vp, i32, P(dbl)]),
             "or_sweep": (None, [vp, i32, i32, i32, i32, P(ctypes.c_int32), P(ctypes.c_int32), dbl, P(dbl), P(dbl)]),
             "or_eval_up": (None, [vp]),
+            "or_set_interpolation": (None, [vp, i32]),
             "or_set_level_sum_cap": (None, [vp, ctypes.c_int]),
             "or_flux": (None, [vp, i32, P(dbl)]),
             "or_fp_to_coe": (None, [vp]),
@@ -142,6 +144,10 @@ class Oracle:
         out = np.zeros(self.num_elem * int(np.prod(so)))
         lib().or_sweep(self._h, table, int(flx), lu, dim, _ip(si), _ip(so), coef, _dp(x), _dp(out))
         return out
+
+    def set_interpolation(self, family):
+        """0 Lagrange, 1 Hermite P3 (needs m = 3)."""
+        lib().or_set_interpolation(self._h, family)
 
     def eval_up(self):
         lib().or_eval_up(self._h)
