@@ -303,7 +303,11 @@ FastLagrIntp::FastLagrIntp(DGSolution& dg, const std::vector<std::vector<double>
                            const std::vector<std::vector<double>>&)
     : dgsolution_ptr(&dg) {}
 
-void FastLagrIntp::eval_up_Lagr() { adg_facade_check(dgsolution_ptr->ctx(), adg_eval_up(dgsolution_ptr->ctx()), "eval_up_Lagr"); }
+void FastLagrIntp::eval_up_Lagr() {
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_set_interpolation(c, ADG_INTERP_LAGRANGE), "eval_up_Lagr");
+  adg_facade_check(c, adg_eval_up(c), "eval_up_Lagr");
+}
 
 void FastLagrIntp::eval_up_Lagr_coarse_grid(const int mesh_nmax) {
   CoarseGrid cg(dgsolution_ptr->ctx(), mesh_nmax);
@@ -311,8 +315,126 @@ void FastLagrIntp::eval_up_Lagr_coarse_grid(const int mesh_nmax) {
 }
 
 double FluxFunction::burgers_flux_scalar(const double u) { return u * u * 0.5; }
+double FluxFunction::burgers_flux_1st_derivative_scalar(const double u) { return u; }
+double FluxFunction::burgers_flux_2nd_derivative_scalar(const double) { return 1.; }
 double FluxFunction::sin_flux(const double u) { return sin(u); }
+double FluxFunction::sin_flux_1st_derivative(const double u) { return cos(u); }
+double FluxFunction::sin_flux_2nd_derivative(const double u) { return -sin(u); }
 double FluxFunction::cos_flux(const double u) { return cos(u); }
+double FluxFunction::cos_flux_1st_derivative(const double u) { return -sin(u); }
+double FluxFunction::cos_flux_2nd_derivative(const double u) { return -cos(u); }
+
+FastHermIntp::FastHermIntp(DGSolution& dg, const std::vector<std::vector<double>>&) : dgsolution_ptr(&dg) {}
+
+void FastHermIntp::eval_up_Herm() {
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_set_interpolation(c, ADG_INTERP_HERMITE), "eval_up_Herm");
+  adg_facade_check(c, adg_eval_up(c), "eval_up_Herm");
+}
+
+HermInterpolation::HermInterpolation(DGSolution& dg) : dgsolution_ptr(&dg) {
+  if (DGSolution::DIM != 2 || HermBasis::PMAX != 3) {
+    std::cout << "HermInterpolation: the GPU library provides P3 Hermite interpolation in two dimensions" << std::endl;
+    std::exit(1);
+  }
+}
+
+void HermInterpolation::nonlinear_Herm_2D_fast(std::function<double(std::vector<double>, int, int)> func,
+                                               std::function<double(std::vector<double>, int, int, int)> func_d1,
+                                               std::function<double(std::vector<double>, int, int, int, int)> func_d2,
+                                               std::vector<std::vector<bool>> is_intp, FastHermIntp& fastHerm) {
+  adg_ctx* c = dgsolution_ptr->ctx();
+  fastHerm.eval_up_Herm();
+  const int D = 2, P = 16;
+  const int64_t E = adg_num_elem(c), n = E * P;
+  std::vector<double> up(n);
+  adg_facade_check(c, adg_field_get(c, ADG_FIELD_UP, up.data()), "nonlinear_Herm_2D_fast");
+  bool intp[2] = {true, true};
+  if (!is_intp.empty())
+    for (int d = 0; d < D; ++d) intp[d] = is_intp[0][d];
+  const int ncomp = intp[1] ? D : 1;
+  std::vector<double> fp(n * ncomp, 0.);
+  std::vector<double> u(1);
+  // eval_fp_Her_2D (source/Interplation.cpp:2045-2316), one scalar unknown
+  for (int64_t e = 0; e < E; ++e) {
+    const double* ue = &up[e * P];
+    for (int q = 0; q < P; ++q) {
+      const int a = q / 4, b = q % 4, k0 = a / 2, k1 = b / 2;
+      for (int d = 0; d < ncomp; ++d) {
+        if (!intp[d]) continue;
+        double v;
+        if (k0 == 0 && k1 == 0) {
+          u[0] = ue[q];
+          v = func(u, 0, d);
+        } else if (k1 == 0) {
+          u[0] = ue[(a - 2) * 4 + b];
+          v = 0.;
+          v += func_d1(u, 0, d, 0) * ue[q];
+        } else if (k0 == 0) {
+          u[0] = ue[a * 4 + b - 2];
+          v = 0.;
+          v += func_d1(u, 0, d, 0) * ue[q];
+        } else {
+          u[0] = ue[(a - 2) * 4 + b - 2];
+          const double ux = ue[a * 4 + b - 2], uy = ue[(a - 2) * 4 + b];
+          v = 0.;
+          v += func_d2(u, 0, d, 0, 0) * ux * uy;
+          v += func_d1(u, 0, d, 0) * ue[q];
+        }
+        fp[d * n + e * P + q] = v;
+      }
+    }
+  }
+  adg_facade_check(c, adg_flux_set(c, ncomp, fp.data()), "nonlinear_Herm_2D_fast");
+  adg_facade_check(c, adg_fp_to_coe(c), "nonlinear_Herm_2D_fast");
+}
+
+void HermInterpolation::nonlinear_Herm_2D_fast(int flux_id, double flux_param, std::vector<std::vector<bool>>,
+                                               FastHermIntp& fastHerm) {
+  adg_ctx* c = dgsolution_ptr->ctx();
+  fastHerm.eval_up_Herm();
+  adg_facade_check(c, adg_flux(c, flux_id, &flux_param), "nonlinear_Herm_2D_fast");
+  adg_facade_check(c, adg_fp_to_coe(c), "nonlinear_Herm_2D_fast");
+}
+
+HyperbolicHermRHS::HyperbolicHermRHS(DGSolution& dg, const OperatorMatrix1D<HermBasis, AlptBasis>&)
+    : dgsolution_ptr(&dg) {}
+void HyperbolicHermRHS::rhs_vol_scalar() {
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_set_interpolation(c, ADG_INTERP_HERMITE), "rhs_vol_scalar");
+  adg_facade_check(c, adg_rhs_lagr_vol(c), "rhs_vol_scalar");
+}
+void HyperbolicHermRHS::rhs_flx_intp_scalar() {
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_set_interpolation(c, ADG_INTERP_HERMITE), "rhs_flx_intp_scalar");
+  adg_facade_check(c, adg_rhs_lagr_flx(c), "rhs_flx_intp_scalar");
+}
+
+// The characteristic through x carries v = sin(2 pi x0) from x0 = x - v t.
+// Before the shock, g(v) = v - sin(2 pi (x - v t)) is increasing on [-1, 1]
+// with g(-1) <= 0 <= g(1): Newton steps safeguarded by bisection.
+double BurgersExact::norm_1d(const double x, const double t) const {
+  const double pi = Const::PI;
+  double lo = -1., hi = 1., v = std::sin(2. * pi * x);
+  for (int it = 0; it < 200; ++it) {
+    const double s = std::sin(2. * pi * (x - v * t));
+    const double g = v - s;
+    if (g > 0) hi = v; else lo = v;
+    const double dg = 1. + 2. * pi * t * std::cos(2. * pi * (x - v * t));
+    double next = v - g / dg;
+    if (!(next > lo && next < hi)) next = 0.5 * (lo + hi);
+    if (std::abs(next - v) < 1e-15 || hi - lo < 1e-15) return next;
+    v = next;
+  }
+  return v;
+}
+
+double BurgersExact::exact_1d(const double x, const double t) const { return a + b * norm_1d(x - a * t + c, b * t); }
+
+// u = a + b v(x + y - 2 a t + c, 2 b t): the 2D problem is the 1D one along x + y
+double BurgersExact::exact_2d(const double x, const double y, const double t) const {
+  return a + b * norm_1d(x + y - 2. * a * t + c, 2. * b * t);
+}
 
 LagrInterpolation::LagrInterpolation(DGSolution& dg) : dgsolution_ptr(&dg) {}
 
@@ -390,8 +512,16 @@ void LagrInterpolation::var_coeff_u_Lagr_fast(double s, std::vector<std::vector<
 
 HyperbolicLagrRHS::HyperbolicLagrRHS(DGSolution& dg, const OperatorMatrix1D<LagrBasis, AlptBasis>&)
     : dgsolution_ptr(&dg) {}
-void HyperbolicLagrRHS::rhs_vol_scalar() { adg_facade_check(dgsolution_ptr->ctx(), adg_rhs_lagr_vol(dgsolution_ptr->ctx()), "rhs_vol_scalar"); }
-void HyperbolicLagrRHS::rhs_flx_intp_scalar() { adg_facade_check(dgsolution_ptr->ctx(), adg_rhs_lagr_flx(dgsolution_ptr->ctx()), "rhs_flx_intp_scalar"); }
+void HyperbolicLagrRHS::rhs_vol_scalar() {
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_set_interpolation(c, ADG_INTERP_LAGRANGE), "rhs_vol_scalar");
+  adg_facade_check(c, adg_rhs_lagr_vol(c), "rhs_vol_scalar");
+}
+void HyperbolicLagrRHS::rhs_flx_intp_scalar() {
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_set_interpolation(c, ADG_INTERP_LAGRANGE), "rhs_flx_intp_scalar");
+  adg_facade_check(c, adg_rhs_lagr_flx(c), "rhs_flx_intp_scalar");
+}
 void HyperbolicLagrRHS::rhs_vol_scalar_coarse_grid(const int mesh_nmax) {
   CoarseGrid cg(dgsolution_ptr->ctx(), mesh_nmax);
   rhs_vol_scalar();

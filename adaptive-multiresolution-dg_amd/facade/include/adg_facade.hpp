@@ -197,9 +197,86 @@ class FastLagrIntp {
 
 namespace FluxFunction {
 double burgers_flux_scalar(const double u);
+double burgers_flux_1st_derivative_scalar(const double u);
+double burgers_flux_2nd_derivative_scalar(const double u);
 double sin_flux(const double u);
+double sin_flux_1st_derivative(const double u);
+double sin_flux_2nd_derivative(const double u);
 double cos_flux(const double u);
+double cos_flux_1st_derivative(const double u);
+double cos_flux_2nd_derivative(const double u);
 }  // namespace FluxFunction
+
+// FastHermIntp (source/FastMultiplyLU.cpp:1316-1336): values and slopes of u
+// at the Hermite points, by the library's fast transform.
+class FastHermIntp {
+ public:
+  FastHermIntp(DGSolution& dgsolution, const std::vector<std::vector<double>>& Her_pt_Alpt_1D);
+  void eval_up_Herm();
+  DGSolution* const dgsolution_ptr;
+};
+
+// HermInterpolation (include/Interpolation.h:251-390), P3: the flux f(u) is
+// interpolated from its values, slopes and cross slope at two points per cell.
+class HermInterpolation {
+ public:
+  explicit HermInterpolation(DGSolution& dgsolution);
+  // point tables are held by the library context; this stays empty
+  std::vector<std::vector<double>> Her_pt_Alpt_1D;
+
+  // HermInterpolation::nonlinear_Herm_2D_fast (source/Interplation.cpp:4609-4621)
+  // with host closures of the reference's signatures: point values come back
+  // to the host, the chain rule of eval_fp_Her_2D runs per point, the flux
+  // values go back to the device.  With is_intp = {{true, false}} only the
+  // first component is interpolated (the same-flux form).
+  void nonlinear_Herm_2D_fast(std::function<double(std::vector<double>, int, int)> func,
+                              std::function<double(std::vector<double>, int, int, int)> func_d1,
+                              std::function<double(std::vector<double>, int, int, int, int)> func_d2,
+                              std::vector<std::vector<bool>> is_intp, FastHermIntp& fastHerm);
+  // Device flux by id (ADG_FLUX_BURGERS / ADG_FLUX_SINCOS): no host round trip.
+  void nonlinear_Herm_2D_fast(int flux_id, double flux_param, std::vector<std::vector<bool>> is_intp,
+                              FastHermIntp& fastHerm);
+  DGSolution* const dgsolution_ptr;
+};
+
+// HyperbolicHermRHS / HyperbolicSameFluxHermRHS / HyperbolicDiffFluxHermRHS
+// (source/FastMultiplyLU.cpp:970-1069,1221-1267): volume and flux integrals of
+// the Hermite-interpolated flux.  The same-flux form reads the first flux
+// component in every dimension: interpolate with is_intp = {{true, false}}
+// or a device flux id whose components coincide (ADG_FLUX_BURGERS).
+class HyperbolicHermRHS {
+ public:
+  HyperbolicHermRHS(DGSolution& dgsolution, const OperatorMatrix1D<HermBasis, AlptBasis>& oper_matx_herm);
+  void rhs_vol_scalar();
+  void rhs_flx_intp_scalar();
+  DGSolution* const dgsolution_ptr;
+};
+
+class HyperbolicSameFluxHermRHS : public HyperbolicHermRHS {
+ public:
+  using HyperbolicHermRHS::HyperbolicHermRHS;
+};
+
+class HyperbolicDiffFluxHermRHS : public HyperbolicHermRHS {
+ public:
+  using HyperbolicHermRHS::HyperbolicHermRHS;
+};
+
+// BurgersExact (include/ExactSolution.h:6-40): u(x, y, t) of
+// u_t + (u^2/2)_x + (u^2/2)_y = 0 with u(x, y, 0) = a + b sin(2 pi (x + y + c)),
+// before the shock (2 b t < 1 / (2 pi)).
+class BurgersExact {
+ public:
+  BurgersExact() : a(0.), b(1.), c(0.) {}
+  BurgersExact(const double a_, const double b_, const double c_) : a(a_), b(b_), c(c_) {}
+  double exact_1d(const double x, const double t) const;
+  double exact_2d(const double x, const double y, const double t) const;
+
+ private:
+  const double a, b, c;
+  // v = sin(2 pi (x - v t)) on the characteristic through x
+  double norm_1d(const double x, const double t) const;
+};
 
 class LagrInterpolation {
  public:

@@ -49,3 +49,25 @@ def test_var_coeff_coarse_grid_driver(tmp_path):
         assert open(prefix + "error.txt").read().startswith("5 ")
     # both initialisations approximate the same bump
     assert abs(errs[0] - errs[1]) < 0.5 * max(errs)
+
+
+def test_burgers_hermite_driver_host_and_device_flux_identical():
+    # host closures (func, func_d1, func_d2) and the device flux id run the same chain rule
+    a = run("burgers_adapt_herm", "-NM", "6", "-N0", "4", "-r", "1e-4", "-ms", "5")
+    b = run("burgers_adapt_herm", "-NM", "6", "-N0", "4", "-r", "1e-4", "-ms", "5", "-hf", "1")
+    la = [l for l in a.splitlines() if l.startswith("steps") or l.startswith("L1")]
+    lb = [l for l in b.splitlines() if l.startswith("steps") or l.startswith("L1")]
+    assert la == lb and len(la) == 2
+    assert " steps 5" in " " + la[0]
+
+
+def test_burgers_hermite_driver_converges_to_the_exact_solution():
+    # sin(2 pi (x + y)) to t = 0.03, before the shock; sparse grids without adaptation
+    errs = []
+    for n in (4, 5, 6):
+        out = run("burgers_adapt_herm", "-NM", str(n), "-N0", str(n), "-r", "1e10", "-c", "0", "-g", "3")
+        e = [float(v) for v in re.search(r"error at final time: ([^\n]+)", out).group(1).split(",")]
+        assert e[0] <= e[1] <= e[2], e
+        errs.append(e[1])
+    print("[hermite burgers L2 errors]", errs, flush=True)
+    assert errs[2] < errs[1] < errs[0] and errs[2] < errs[0] / 8, errs
