@@ -5,6 +5,9 @@
 #include <cstring>
 #include <exception>
 #include <string>
+#include <unordered_map>
+#include <memory>
+#include <cstdio>
 
 #include "../../include/adg.h"
 #include "context.hpp"
@@ -658,6 +661,116 @@ int adg_stage(adg_ctx* h, const adg_problem* p, int scheme, int stage, double dt
           (scheme == ADG_RK3 && stage >= 0 && stage <= 2)))
       throw std::runtime_error("invalid RK scheme/stage");
     c.stage_fused(stage_problem(p, c.D), scheme, stage, dt);
+  });
+}
+
+namespace {
+const char kStateMagic[8] = {'A', 'D', 'G', 'S', 'T', 'A', 'T', '1'};
+
+struct FileCloser {
+  void operator()(std::FILE* f) const { if (f) std::fclose(f); }
+};
+using FilePtr = std::unique_ptr<std::FILE, FileCloser>;
+
+void write_all(std::FILE* f, const void* p, size_t n) {
+  if (n && std::fwrite(p, 1, n, f) != n) throw std::runtime_error("state file: write failed");
+}
+void read_all(std::FILE* f, void* p, size_t n) {
+  if (n && std::fread(p, 1, n, f) != n) throw std::runtime_error("state file: truncated");
+}
+}  // namespace
+
+int adg_state_write(adg_ctx* h, const char* path) {
+  return guard(h, [&](adg::Context& c) {
+    if (!path) throw std::runtime_error("null path");
+    std::vector<double> u(static_cast<size_t>(c.E()) * c.K);
+    adg::hip_check(hipMemcpyAsync(u.data(), c.u.p, u.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream),
+                   "state download");
+    c.sync();
+    FilePtr f(std::fopen(path, "wb"));
+    if (!f) throw std::runtime_error(std::string("state file: cannot open ") + path);
+    const int32_t hdr[6] = {c.D, c.k, c.m, c.nmax, c.B(), 0};
+    write_all(f.get(), kStateMagic, sizeof(kStateMagic));
+    write_all(f.get(), hdr, sizeof(hdr));
+    for (int i = 0; i < c.B(); ++i) {
+      const int64_t n = c.off[i + 1] - c.off[i];
+      write_all(f.get(), &n, sizeof(n));
+    }
+    std::vector<int32_t> ls(static_cast<size_t>(c.E()) * 2 * c.D);
+    for (int64_t e = 0; e < c.E(); ++e)
+      for (int d = 0; d < c.D; ++d) {
+        ls[(e * 2) * c.D + d] = c.level(e, d);
+        ls[(e * 2 + 1) * c.D + d] = adg::suppt_of_slot(c.uslot[e * c.D + d]);
+      }
+    write_all(f.get(), ls.data(), ls.size() * sizeof(int32_t));
+    write_all(f.get(), u.data(), u.size() * sizeof(double));
+    if (std::fflush(f.get()) != 0) throw std::runtime_error("state file: write failed");
+  });
+}
+
+int adg_state_read(adg_ctx* h, const char* path) {
+  return guard(h, [&](adg::Context& c) {
+    if (!path) throw std::runtime_error("null path");
+    FilePtr f(std::fopen(path, "rb"));
+    if (!f) throw std::runtime_error(std::string("state file: cannot open ") + path);
+    char magic[8];
+    int32_t hdr[6];
+    read_all(f.get(), magic, sizeof(magic));
+    if (std::memcmp(magic, kStateMagic, sizeof(magic)) != 0) throw std::runtime_error("state file: bad magic");
+    read_all(f.get(), hdr, sizeof(hdr));
+    if (hdr[0] != c.D || hdr[1] != c.k || hdr[2] != c.m || hdr[3] != c.nmax)
+      throw std::runtime_error("state file: dimension, degrees or maximum level differ from the context");
+    const int B = hdr[4];
+    if (B < 1 || B > (1 << 20)) throw std::runtime_error("state file: bad instance count");
+    std::vector<int64_t> cnt(B);
+    read_all(f.get(), cnt.data(), cnt.size() * sizeof(int64_t));
+    std::vector<std::vector<int32_t>> slots(B);
+    int64_t tot = 0;
+    for (int i = 0; i < B; ++i) {
+      if (cnt[i] < 1 || cnt[i] > (int64_t(1) << 31)) throw std::runtime_error("state file: bad element count");
+      std::vector<int32_t> ls(static_cast<size_t>(cnt[i]) * 2 * c.D);
+      read_all(f.get(), ls.data(), ls.size() * sizeof(int32_t));
+      slots[i].resize(static_cast<size_t>(cnt[i]) * c.D);
+      for (int64_t e = 0; e < cnt[i]; ++e)
+        for (int d = 0; d < c.D; ++d) {
+          const int32_t l = ls[(e * 2) * c.D + d], j = ls[(e * 2 + 1) * c.D + d];
+          if (l < 0 || l > c.nmax) throw std::runtime_error("state file: level out of range");
+          const int jmax = l == 0 ? 1 : (1 << l) - 1;
+          if (j < 1 || j > jmax || j % 2 == 0) throw std::runtime_error("state file: support index out of range");
+          slots[i][e * c.D + d] = adg::slot_of(l, j);
+        }
+      tot += cnt[i];
+    }
+    std::vector<double> u(static_cast<size_t>(tot) * c.K);
+    read_all(f.get(), u.data(), u.size() * sizeof(double));
+    char extra;
+    if (std::fread(&extra, 1, 1, f.get()) != 0) throw std::runtime_error("state file: trailing bytes");
+    // the file's element order is hash-key order, which the grid rebuild restores;
+    // map the coefficients by key
+    std::vector<std::vector<int32_t>> keyed = slots;
+    c.set_grids(slots, false);
+    c.inst_eps.clear();
+    c.inst_eta.clear();
+    if (c.E() != tot) throw std::runtime_error("state file: duplicate elements");
+    std::vector<double> ud(u.size());
+    int64_t src = 0;
+    for (int i = 0; i < B; ++i) {
+      std::unordered_map<int32_t, int64_t> pos;
+      for (int64_t e = c.off[i]; e < c.off[i + 1]; ++e) pos[c.uhash[e]] = e;
+      for (int64_t e = 0; e < cnt[i]; ++e, ++src) {
+        int32_t l[adg::kMaxDim], j[adg::kMaxDim];
+        for (int d = 0; d < c.D; ++d) {
+          l[d] = adg::level_of_slot(keyed[i][e * c.D + d]);
+          j[d] = adg::suppt_of_slot(keyed[i][e * c.D + d]);
+        }
+        auto it = pos.find(c.hk.key(l, j, c.D));
+        if (it == pos.end()) throw std::runtime_error("state file: element lost in the grid rebuild");
+        std::copy(&u[src * c.K], &u[src * c.K] + c.K, &ud[it->second * c.K]);
+      }
+    }
+    adg::hip_check(hipMemcpyAsync(c.u.p, ud.data(), ud.size() * sizeof(double), hipMemcpyHostToDevice, c.stream),
+                   "state upload");
+    c.sync();
   });
 }
 

@@ -13,6 +13,7 @@
 #include "FastMultiplyLU.h"
 #include "Hash.h"
 #include "HermBasis.h"
+#include "IO.h"
 #include "Interpolation.h"
 #include "LagrBasis.h"
 #include "ODESolver.h"
@@ -52,6 +53,8 @@ int main(int argc, char* argv[]) {
   args.AddOption(&num_gauss_pt_compute_error, "-g", "--num-gauss-pt-compute-error", "number of gauss points in computing error");
   args.AddOption(&host_flux, "-hf", "--host-flux", "flux by host closures (1) or by device id (0)");
   args.AddOption(&max_steps, "-ms", "--max-steps", "stop after this many steps (0: none)");
+  std::string state_file;
+  args.AddOption(&state_file, "-st", "--state-file", "write the final state to this file (empty: none)");
   args.Parse();
   if (!args.Good()) { args.PrintUsage(std::cout); return 1; }
   if (N_init > NMAX) { std::cout << "Mesh level in initialization should not be larger than Maximum mesh level" << std::endl; return 1; }
@@ -151,6 +154,10 @@ int main(int argc, char* argv[]) {
   std::cout.precision(17);
   std::cout << "steps " << num_time_step << " time " << curr_time << " elements " << dg_solu.size_elem()
             << " dof " << dg_solu.get_dof() << " checksum " << checksum << std::endl;
+  if (!state_file.empty()) {
+    IO inout(dg_solu);
+    inout.output_state(state_file);
+  }
   auto final_func = [&](std::vector<double> x) { return burgers.exact_2d(x[0], x[1], curr_time); };
   const std::vector<double> err = dg_solu.get_error_no_separable_scalar(final_func, num_gauss_pt_compute_error);
   std::cout << "L1, L2 and Linf error at final time: " << err[0] << ", " << err[1] << ", " << err[2] << std::endl;

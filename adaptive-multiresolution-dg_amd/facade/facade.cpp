@@ -220,6 +220,14 @@ void IO::write_error(const int NMAX, const std::vector<double>& err, const std::
   output << std::endl;
 }
 
+void IO::output_state(const std::string& file_name) const {
+  adg_facade_check(dgsolution_ptr->ctx(), adg_state_write(dgsolution_ptr->ctx(), file_name.c_str()), "output_state");
+}
+
+void IO::input_state(const std::string& file_name) {
+  adg_facade_check(dgsolution_ptr->ctx(), adg_state_read(dgsolution_ptr->ctx(), file_name.c_str()), "input_state");
+}
+
 DGAdapt::DGAdapt(const bool sparse_, const int level_init_, const int NMAX_, AllBasis<AlptBasis>& a,
                  AllBasis<LagrBasis>& l, AllBasis<HermBasis>& h, Hash& hash, const double eps_, const double eta_,
                  const bool, const bool, const bool)

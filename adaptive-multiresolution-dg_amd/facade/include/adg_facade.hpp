@@ -410,6 +410,10 @@ class IO {
   void write_error_time(const double tn, const std::vector<double>& err, const int init,
                         const std::string file_name) const;
   void write_error(const int NMAX, const std::vector<double>& err, const std::string file_name) const;
+  // Binary state files (element set + Alpert coefficients; adg_state_write /
+  // adg_state_read).  No reference counterpart: checkpoints and golden fixtures.
+  void output_state(const std::string& file_name) const;
+  void input_state(const std::string& file_name);
 
  private:
   DGSolution* const dgsolution_ptr;

@@ -74,6 +74,8 @@ def lib():
             "adg_eval_up": (i32, [vp]),
             "adg_set_level_sum_cap": (i32, [vp, i32]),
             "adg_set_interpolation": (i32, [vp, i32]),
+            "adg_state_write": (i32, [vp, ctypes.c_char_p]),
+            "adg_state_read": (i32, [vp, ctypes.c_char_p]),
             "adg_lagr_points": (i32, [vp, DP]),
             "adg_flux": (i32, [vp, i32, DP]),
             "adg_flux_set": (i32, [vp, i32, DP]),
@@ -232,6 +234,14 @@ class Solution:
         """INTERP_LAGRANGE or INTERP_HERMITE (P3, needs m = 3) for eval_up, flux,
         fp_to_coe, rhs_lagr_*, stage and run_hyperbolic."""
         self._check(lib().adg_set_interpolation(self._h, family))
+
+    def state_write(self, path):
+        """Binary state file: element sets and Alpert coefficients of every instance."""
+        self._check(lib().adg_state_write(self._h, os.fsencode(path)))
+
+    def state_read(self, path):
+        """Replace the grids and coefficients by those of a state file."""
+        self._check(lib().adg_state_read(self._h, os.fsencode(path)))
 
     def set_level_sum_cap(self, cap):
         """Coarse grid: sweeps ignore elements with level sum above cap (cap < 0: none)."""

@@ -303,6 +303,19 @@ int adg_stage(adg_ctx* c, const adg_problem* p, int scheme, int stage, double dt
 int adg_state_save(adg_ctx* c, int slot);
 int adg_state_restore(adg_ctx* c, int slot);
 
+/* State files: the element sets and Alpert coefficients of every instance.
+ * No reference counterpart (the reference's IO writes text only,
+ * source/IO.cpp:18-616, and reads nothing back); they carry golden fixtures
+ * and checkpoints across machines.  Little-endian layout:
+ *   char magic[8] = "ADGSTAT1"; int32 dim, k, m, nmax, ninst, 0;
+ *   int64 count[ninst];
+ *   per instance, per element in hash-key order: int32 level[dim], int32 suppt[dim];
+ *   float64 ucoe[sum(count) * (k+1)^dim] (per element, row-major).
+ * Reading checks magic, dimensions and every (level, support) and replaces the
+ * context's grids and coefficients (user thresholds are cleared). */
+int adg_state_write(adg_ctx* c, const char* path);
+int adg_state_read(adg_ctx* c, const char* path);
+
 /* Kernel timing: HIP events around every launch on the context's stream
  * (enable = 1 starts a fresh record).  adg_profile_get fills, per category
  * (0 sweep, 1 surplus transform, 2 flux, 3 RK stage, 4 other), the summed

@@ -7,6 +7,8 @@ import subprocess
 
 import pytest
 
+from state_file import unpack
+
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -51,14 +53,20 @@ def test_var_coeff_coarse_grid_driver(tmp_path):
     assert abs(errs[0] - errs[1]) < 0.5 * max(errs)
 
 
-def test_burgers_hermite_driver_host_and_device_flux_identical():
-    # host closures (func, func_d1, func_d2) and the device flux id run the same chain rule
-    a = run("burgers_adapt_herm", "-NM", "6", "-N0", "4", "-r", "1e-4", "-ms", "5")
-    b = run("burgers_adapt_herm", "-NM", "6", "-N0", "4", "-r", "1e-4", "-ms", "5", "-hf", "1")
+def test_burgers_hermite_driver_host_and_device_flux_identical(tmp_path):
+    # host closures (func, func_d1, func_d2) and the device flux id run the same chain rule;
+    # the final states written through IO::output_state are identical files
+    sa, sb = str(tmp_path / "dev.adgstate"), str(tmp_path / "host.adgstate")
+    a = run("burgers_adapt_herm", "-NM", "6", "-N0", "4", "-r", "1e-4", "-ms", "5", "-st", sa)
+    b = run("burgers_adapt_herm", "-NM", "6", "-N0", "4", "-r", "1e-4", "-ms", "5", "-hf", "1", "-st", sb)
     la = [l for l in a.splitlines() if l.startswith("steps") or l.startswith("L1")]
     lb = [l for l in b.splitlines() if l.startswith("steps") or l.startswith("L1")]
     assert la == lb and len(la) == 2
     assert " steps 5" in " " + la[0]
+    da, db = open(sa, "rb").read(), open(sb, "rb").read()
+    assert da == db
+    _, insts = unpack(da)
+    assert len(insts[0][0]) == int(re.search(r"elements (\d+)", la[0]).group(1))
 
 
 def test_burgers_hermite_driver_converges_to_the_exact_solution():
