@@ -236,6 +236,13 @@ int adg_lagr_points(adg_ctx* h, double* x) {
   });
 }
 
+int adg_set_precision(adg_ctx* h, int precision) {
+  return guard(h, [&](adg::Context& c) {
+    if (precision != ADG_PREC_FP64 && precision != ADG_PREC_FP32_SPLIT) throw std::runtime_error("unknown precision");
+    c.precision = precision;
+  });
+}
+
 int adg_set_interpolation(adg_ctx* h, int family) {
   return guard(h, [&](adg::Context& c) { c.set_interpolation(family); });
 }

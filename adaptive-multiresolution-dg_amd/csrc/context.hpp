@@ -191,6 +191,19 @@ class Context {
   void stage_fused(const StageProblem& p, int scheme, int stage, double dt);
   struct FusedPlan;
   FusedPlan& fused_plan(const StageProblem& p);
+  template <class T>
+  void stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int stage, double dt);
+
+  // Storage precision of the fused stage: 0 FP64; 1 FP32 fiber transforms,
+  // point values, flux and surplus transform, with every 1D table and surplus
+  // weight split into two FP32 parts (M = hi + lo).  The RK combination, the
+  // accumulation of the right-hand side, the LF penalty and the refine /
+  // coarsen norms stay FP64.  The per-sweep path is always FP64.
+  int precision = 0;
+  DevBuf<float> up32, fp32, fucoe32, interA32, interB32;
+  struct SplitTable { DevBuf<float> hi, lo; };
+  std::map<const double*, std::unique_ptr<SplitTable>> split_tabs;
+  const SplitTable& split_table(const double* dev, size_t n);
   // thread mapping of the fused kernels: 0 per output, 1 per (element, rest
   // index), 2 per (element, 4 rest indices); -1 lets the autotuner choose
   // per kernel class and grid-size class
@@ -238,6 +251,7 @@ class Context {
  private:
   void remap_field(DevBuf<double>& f, int per_elem, const std::vector<int32_t>& old_of_new, int64_t old_E);
   void surplus_transform(const double* in, double* out, double* scratch_a, double* scratch_b, int ncomp);
+  void surplus_transform32(const float* in, float* out, float* scratch_a, float* scratch_b, int ncomp);
   void compute_vcoef();
 };
 

@@ -226,8 +226,9 @@ __global__ __launch_bounds__(kBlock) void k_flux(const double* up, double* fp, c
 
 // f, f' and f'' of flux component c at u: Burgers u^2/2 (kind 0) or the
 // sin/cos pair scaled by s (kind 1) (FluxFunction, source/Interplation.cpp:5070-5113).
-__device__ inline double flux_der(int kind, int c, int order, double u, double s) {
-  if (kind == 0) return order == 0 ? u * u * 0.5 : (order == 1 ? u : 1.);
+template <class T>
+__device__ inline T flux_der(int kind, int c, int order, T u, T s) {
+  if (kind == 0) return order == 0 ? u * u * T(0.5) : (order == 1 ? u : T(1));
   if (c == 0) return order == 0 ? sin(u) * s : (order == 1 ? cos(u) * s : -sin(u) * s);
   return order == 0 ? cos(u) * s : (order == 1 ? -sin(u) * s : -cos(u) * s);
 }
@@ -237,33 +238,34 @@ __device__ inline double flux_der(int kind, int c, int order, double u, double s
 // of freedom).  Degree p of a dimension is (point p % 2, derivative p / 2);
 // a slope of f(u) is f'(u) times the slope of u at the same point, the cross
 // slope adds f''(u) u_x u_y, in the reference's order of operations.
-__global__ __launch_bounds__(kBlock) void k_flux_herm(const double* up, double* fp, int64_t E, int D, int kind,
-                                                       double s, int ncomp) {
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_flux_herm(const T* up, T* fp, int64_t E, int D, int kind, T s,
+                                                       int ncomp) {
   const int P = D == 1 ? 4 : 16;
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= E * P) return;
   const int64_t e = gid / P;
   const int q = static_cast<int>(gid - e * P);
-  const double* ue = up + e * P;
+  const T* ue = up + e * P;
   const int a = D == 1 ? q : q >> 2, b = D == 1 ? 0 : (q & 3);
   const int k0 = a >> 1, k1 = b >> 1;
   const int64_t n = E * P;
-  const double uq = ue[q];
+  const T uq = ue[q];
   for (int c = 0; c < ncomp; ++c) {
-    double v;
+    T v;
     if (k0 == 0 && k1 == 0) {
       v = flux_der(kind, c, 0, uq, s);
     } else if (k1 == 0) {
-      const double u0 = ue[D == 1 ? a - 2 : (a - 2) * 4 + b];
-      v = 0.;
+      const T u0 = ue[D == 1 ? a - 2 : (a - 2) * 4 + b];
+      v = T(0);
       v += flux_der(kind, c, 1, u0, s) * uq;
     } else if (k0 == 0) {
-      const double u0 = ue[a * 4 + b - 2];
-      v = 0.;
+      const T u0 = ue[a * 4 + b - 2];
+      v = T(0);
       v += flux_der(kind, c, 1, u0, s) * uq;
     } else {
-      const double u0 = ue[(a - 2) * 4 + b - 2], ux = ue[a * 4 + b - 2], uy = ue[(a - 2) * 4 + b];
-      v = 0.;
+      const T u0 = ue[(a - 2) * 4 + b - 2], ux = ue[a * 4 + b - 2], uy = ue[(a - 2) * 4 + b];
+      v = T(0);
       v += flux_der(kind, c, 2, u0, s) * ux * uy;
       v += flux_der(kind, c, 1, u0, s) * uq;
     }
@@ -276,6 +278,10 @@ __global__ __launch_bounds__(kBlock) void k_flux_herm(const double* up, double* 
 struct SurplusArgs {
   const double* in;
   double* out;
+  const float* in32;  // FP32 storage (precision mode): in32 / out32 and the split weights
+  float* out32;
+  const float* wh;
+  const float* wl;
   const int32_t* slot;
   const int32_t* fbase;
   const int32_t* fmap;
@@ -287,6 +293,22 @@ struct SurplusArgs {
   int P, p1, stride_d, ncomp;
 };
 
+template <class T>
+__device__ __forceinline__ const T* surplus_in(const SurplusArgs& a);
+template <>
+__device__ __forceinline__ const double* surplus_in<double>(const SurplusArgs& a) { return a.in; }
+template <>
+__device__ __forceinline__ const float* surplus_in<float>(const SurplusArgs& a) { return a.in32; }
+
+// FP64: coe += wt * anc.  FP32: the weight is split, wt = wh + wl, and both
+// parts are applied with fused multiply-adds, the small part first.
+__device__ __forceinline__ void surplus_madd(double& coe, const SurplusArgs& a, int w, double x) { coe += a.wt[w] * x; }
+__device__ __forceinline__ void surplus_madd(float& coe, const SurplusArgs& a, int w, float x) {
+  coe = fmaf(a.wl[w], x, coe);
+  coe = fmaf(a.wh[w], x, coe);
+}
+
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t per = a.E * a.P;
@@ -295,8 +317,8 @@ __global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
   const int64_t rem = gid - c * per;
   const int64_t e = rem / a.P;
   const int q = static_cast<int>(rem - e * a.P);
-  const double* in = a.in + c * a.comp_stride;
-  double coe = in[e * a.P + q];
+  const T* in = surplus_in<T>(a) + c * a.comp_stride;
+  T coe = in[e * a.P + q];
   const int s = a.slot[e];
   if (s != 0) {
     const int pd = (q / a.stride_d) % a.p1;
@@ -304,10 +326,11 @@ __global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
       const int anc = a.fmap[a.fbase[e] + a.anc_slot[s * a.p1 + ic]];
       if (anc < 0) { a.err[0] = 1; continue; }
       const int mm = q + (a.anc_point[s * a.p1 + ic] - pd) * a.stride_d;
-      coe += a.wt[(s * a.p1 + pd) * a.p1 + ic] * in[static_cast<int64_t>(anc) * a.P + mm];
+      surplus_madd(coe, a, (s * a.p1 + pd) * a.p1 + ic, in[static_cast<int64_t>(anc) * a.P + mm]);
     }
   }
-  a.out[c * a.comp_stride + e * a.P + q] = coe;
+  if constexpr (sizeof(T) == 8) a.out[c * a.comp_stride + e * a.P + q] = coe;
+  else a.out32[c * a.comp_stride + e * a.P + q] = coe;
 }
 
 // One explicit RK stage update from u_n (t), the current stage value (uu) and
@@ -497,6 +520,11 @@ __global__ __launch_bounds__(kBlock) void k_remap(double* dst, const double* src
 struct TermDev {
   const double* in;
   const double* M;
+  // FP32 precision mode: input in32 (or in, rounded on load, when in32 is
+  // null) and the table split as M = Mh + Ml
+  const float* in32;
+  const float* Mh;
+  const float* Ml;
   const int32_t* slot;
   const int32_t* fbase;
   const int32_t* fmaxl;
@@ -512,31 +540,62 @@ struct TermDev {
 struct TaskDev {  // a term written to an output buffer
   TermDev t;
   double* out;
+  float* out32;  // FP32 precision mode
   int Tout;
   int osz[kMaxDim];
   int R;              // rest indices per element (Tin / input size in the swept dimension)
   int64_t begin_v[3];  // first work item of this task in the phase, per thread mapping
 };
 
+// Arithmetic of a sweep in the storage type T: FP64 (x * M added in order)
+// or FP32 with the 1D table split as M = Mh + Ml in FP32, both parts applied
+// by fused multiply-adds, the small part first.
+template <class T>
+struct SweepOps;
+template <>
+struct SweepOps<double> {
+  struct M { double m; };
+  static __device__ __forceinline__ double in(const TermDev& t, int64_t o) { return t.in[o]; }
+  static __device__ __forceinline__ M mat(const TermDev& t, int64_t o) { return {t.M[o]}; }
+  static __device__ __forceinline__ M zero() { return {0.}; }
+  static __device__ __forceinline__ void madd(double& acc, double x, M m) { acc += x * m.m; }
+  static __device__ __forceinline__ double* out(const TaskDev& tk) { return tk.out; }
+};
+template <>
+struct SweepOps<float> {
+  struct M { float h, l; };
+  static __device__ __forceinline__ float in(const TermDev& t, int64_t o) {
+    return t.in32 ? t.in32[o] : static_cast<float>(t.in[o]);
+  }
+  static __device__ __forceinline__ M mat(const TermDev& t, int64_t o) { return {t.Mh[o], t.Ml[o]}; }
+  static __device__ __forceinline__ M zero() { return {0.f, 0.f}; }
+  static __device__ __forceinline__ void madd(float& acc, float x, M m) {
+    acc = fmaf(x, m.l, acc);
+    acc = fmaf(x, m.h, acc);
+  }
+  static __device__ __forceinline__ float* out(const TaskDev& tk) { return tk.out32; }
+};
+
 // Value of one sweep at output (e, idx): coef * sum over related sources, in
 // ascending slot order, of sum_k in[src][idx with dd <- k] * M[s*PF1+k][st*pt1+idx[dd]].
 // Loads of up to four sources are issued before their terms are added in order.
-template <int PF1>
-__device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const int* idx, int D, int cap) {
+template <int PF1, class T = double>
+__device__ __forceinline__ T term_value(const TermDev& t, int64_t e, const int* idx, int D, int cap) {
+  using Ops = SweepOps<T>;
   int base = 0;
   for (int d = 0; d < D; ++d)
     if (d != t.dd) base += idx[d] * t.sin[d];
   const int sdd = t.sin[t.dd];
   const int st = t.slot[e];
   const int fb = t.fbase[e];
-  const double* Mc = t.M + static_cast<int64_t>(st) * t.ns * PF1 * t.pt1 + idx[t.dd];
+  const int64_t Mc = static_cast<int64_t>(st) * t.ns * PF1 * t.pt1 + idx[t.dd];
   int maxl = t.fmaxl[e];
   if (cap >= 0) {  // coarse grid, as in k_sweep
     const int ls = t.lsum[e];
-    if (ls > cap) return 0.;
+    if (ls > cap) return T(0);
     maxl = min(maxl, cap - ls + slot_level(st));
   }
-  double acc = 0.;
+  T acc = T(0);
   // sources are gathered NB at a time so that their loads are in flight together
   constexpr int NB = (PF1 <= 4) ? 4 : 2;
   int pend[NB];
@@ -545,22 +604,23 @@ __device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const 
     int src[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) src[j] = (j < np) ? t.fmap[fb + pend[j]] : -1;
-    double xv[NB][PF1], mv[NB][PF1];
+    T xv[NB][PF1];
+    typename Ops::M mv[NB][PF1];
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-      const double* x = t.in + static_cast<int64_t>(src[j] < 0 ? 0 : src[j]) * t.Tin + base;
-      const double* Mr = Mc + (j < np ? pend[j] : 0) * PF1 * t.pt1;
+      const int64_t x = static_cast<int64_t>(src[j] < 0 ? 0 : src[j]) * t.Tin + base;
+      const int64_t Mr = Mc + (j < np ? pend[j] : 0) * PF1 * t.pt1;
 #pragma unroll
       for (int kk = 0; kk < PF1; ++kk) {
-        xv[j][kk] = (src[j] >= 0) ? x[kk * sdd] : 0.;
-        mv[j][kk] = (src[j] >= 0) ? Mr[kk * t.pt1] : 0.;
+        xv[j][kk] = (src[j] >= 0) ? Ops::in(t, x + kk * sdd) : T(0);
+        mv[j][kk] = (src[j] >= 0) ? Ops::mat(t, Mr + kk * t.pt1) : Ops::zero();
       }
     }
 #pragma unroll
     for (int j = 0; j < NB; ++j)
       if (src[j] >= 0) {
 #pragma unroll
-        for (int kk = 0; kk < PF1; ++kk) acc += xv[j][kk] * mv[j][kk];
+        for (int kk = 0; kk < PF1; ++kk) Ops::madd(acc, xv[j][kk], mv[j][kk]);
       }
     np = 0;
   };
@@ -569,7 +629,7 @@ __device__ __forceinline__ double term_value(const TermDev& t, int64_t e, const 
     if (np == NB) flush();
   });
   if (np > 0) flush();
-  return acc * t.coef;
+  return acc * T(t.coef);
 }
 
 __device__ __forceinline__ void unflatten(int o, const int* sz, int D, int* idx) {
@@ -580,7 +640,7 @@ __device__ __forceinline__ void unflatten(int o, const int* sz, int D, int* idx)
 }
 
 // Per-output form of k_phase: thread per (task, element, output).
-template <int PF1>
+template <int PF1, class T>
 __global__ __launch_bounds__(kBlock) void k_phase_pt(const TaskDev* tasks, int ntask, int64_t total, int D, int cap) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (gid >= total) return;
@@ -592,14 +652,28 @@ __global__ __launch_bounds__(kBlock) void k_phase_pt(const TaskDev* tasks, int n
   const int o = static_cast<int>(local - e * tk.Tout);
   int idx[kMaxDim];
   unflatten(o, tk.osz, D, idx);
-  tk.out[local] = term_value<PF1>(tk.t, e, idx, D, cap);
+  SweepOps<T>::out(tk)[local] = term_value<PF1, T>(tk.t, e, idx, D, cap);
+}
+
+// Point flux of the fused eval_up kernels (kind -1: none, the Hermite flux runs after).
+template <class T>
+__device__ __forceinline__ void flux_point(int kind, T u, double s, const double* vcoef, T* fp, int64_t gid, int64_t n,
+                                           int ncomp) {
+  if (kind == 0) {
+    fp[gid] = u * u * T(0.5);
+  } else if (kind == 1) {
+    fp[gid] = sin(u) * T(s);
+    fp[n + gid] = cos(u) * T(s);
+  } else if (kind == 2) {
+    for (int c = 0; c < ncomp; ++c) fp[c * n + gid] = T(vcoef[c * n + gid]) * u;
+  }
 }
 
 // Per-output form of k_up_final: threads are (point, term) pairs, one term per
 // wavefront, and the point's first thread adds the terms in order.
-template <int PF1>
+template <int PF1, class T>
 __global__ __launch_bounds__(kFinalBlock) void k_up_final_pt(const TermDev* terms, int nterm, int per_block, int64_t E,
-                                                           int P, int p1, int D, double* up, double* fp,
+                                                           int P, int p1, int D, T* up, T* fp,
                                                            const double* vcoef, int kind, double s, int ncomp, int cap) {
   __shared__ double vals[kFinalBlock];
   const int t = threadIdx.x / per_block, lo = threadIdx.x - t * per_block;
@@ -616,27 +690,20 @@ __global__ __launch_bounds__(kFinalBlock) void k_up_final_pt(const TermDev* term
     int sz[kMaxDim], idx[kMaxDim];
     for (int d = 0; d < D; ++d) sz[d] = p1;
     unflatten(q, sz, D, idx);
-    v = term_value<PF1>(terms[t], e, idx, D, cap);
+    v = term_value<PF1, T>(terms[t], e, idx, D, cap);
   }
   vals[threadIdx.x] = v;
   __syncthreads();
   if (!live || t != 0) return;
   double u = 0.;
   for (int j = 0; j < nterm; ++j) u += vals[j * per_block + lo];
-  up[gid] = u;
-  const int64_t n = E * P;
-  if (kind == 0) {
-    fp[gid] = u * u * 0.5;
-  } else if (kind == 1) {
-    fp[gid] = sin(u) * s;
-    fp[n + gid] = cos(u) * s;
-  } else if (kind == 2) {
-    for (int c = 0; c < ncomp; ++c) fp[c * n + gid] = vcoef[c * n + gid] * u;
-  }
+  const T uu = T(u);
+  up[gid] = uu;
+  flux_point<T>(kind, uu, s, vcoef, fp, gid, E * P, ncomp);
 }
 
 // Per-output form of k_rhs_final: threads are (DoF, term) pairs.
-template <int PFM, int PFK>
+template <int PFM, int PFK, class T>
 __global__ __launch_bounds__(kFinalBlock) void k_rhs_final_pt(const TermDev* terms, int nterm, const TermDev* pens,
                                                             int npen, int per_block, int64_t E, int K, int k1, int D,
                                                             double* rhs, const double* u, const double* tn,
@@ -657,7 +724,8 @@ __global__ __launch_bounds__(kFinalBlock) void k_rhs_final_pt(const TermDev* ter
     int sz[kMaxDim], idx[kMaxDim];
     for (int d = 0; d < D; ++d) sz[d] = k1;
     unflatten(q, sz, D, idx);
-    v = (t < nterm) ? term_value<PFM>(terms[t], e, idx, D, cap) : term_value<PFK>(pens[t - nterm], e, idx, D, cap);
+    v = (t < nterm) ? double(term_value<PFM, T>(terms[t], e, idx, D, cap))
+                    : term_value<PFK, double>(pens[t - nterm], e, idx, D, cap);
   }
   vals[threadIdx.x] = v;
   __syncthreads();
@@ -699,13 +767,14 @@ __device__ __forceinline__ void strides_dev(const int* sz, int D, int* st) {
 // acc[r][i] = sum over related sources of element e, in ascending slot order,
 // of sum_k in[src][bin[r] + k*sdd] * M(s*PF1+k, st*PF1+i), for r < nr.
 // Returns false when e lies above the coarse-grid cap (its outputs are zero).
-template <int PF1, int PT1>
+template <int PF1, int PT1, class T>
 __device__ __forceinline__ bool term_block(const TermDev& t, int64_t e, const int* bin, int nr, int cap,
-                                           double (&acc)[kRC][PT1]) {
+                                           T (&acc)[kRC][PT1]) {
+  using Ops = SweepOps<T>;
 #pragma unroll
   for (int r = 0; r < kRC; ++r)
 #pragma unroll
-    for (int i = 0; i < PT1; ++i) acc[r][i] = 0.;
+    for (int i = 0; i < PT1; ++i) acc[r][i] = T(0);
   const int st = t.slot[e];
   const int fb = t.fbase[e];
   int maxl = t.fmaxl[e];
@@ -715,27 +784,27 @@ __device__ __forceinline__ bool term_block(const TermDev& t, int64_t e, const in
     maxl = min(maxl, cap - ls + slot_level(st));
   }
   const int sdd = t.sin[t.dd];
-  const double* Mst = t.M + static_cast<int64_t>(st) * t.ns * PF1 * PT1;
+  const int64_t Mst = static_cast<int64_t>(st) * t.ns * PF1 * PT1;
   for_each_present(st, maxl, t.flx != 0, t.lu, t.fbits, fb, [&](int s) {
     const int src = t.fmap[fb + s];
     if (src < 0) return;
-    const double* Mr = Mst + s * PF1 * PT1;
-    double m[PF1][PT1];
+    const int64_t Mr = Mst + s * PF1 * PT1;
+    typename Ops::M m[PF1][PT1];
 #pragma unroll
     for (int k = 0; k < PF1; ++k)
 #pragma unroll
-      for (int i = 0; i < PT1; ++i) m[k][i] = Mr[k * PT1 + i];
-    const double* xs = t.in + static_cast<int64_t>(src) * t.Tin;
+      for (int i = 0; i < PT1; ++i) m[k][i] = Ops::mat(t, Mr + k * PT1 + i);
+    const int64_t xs = static_cast<int64_t>(src) * t.Tin;
 #pragma unroll
     for (int r = 0; r < kRC; ++r) {
       if (r >= nr) break;
-      double xv[PF1];
+      T xv[PF1];
 #pragma unroll
-      for (int k = 0; k < PF1; ++k) xv[k] = xs[bin[r] + k * sdd];
+      for (int k = 0; k < PF1; ++k) xv[k] = Ops::in(t, xs + bin[r] + k * sdd);
 #pragma unroll
       for (int i = 0; i < PT1; ++i)
 #pragma unroll
-        for (int k = 0; k < PF1; ++k) acc[r][i] += xv[k] * m[k][i];
+        for (int k = 0; k < PF1; ++k) Ops::madd(acc[r][i], xv[k], m[k][i]);
     }
   });
   return true;
@@ -743,9 +812,9 @@ __device__ __forceinline__ bool term_block(const TermDev& t, int64_t e, const in
 
 // Evaluates term t for (element e, chunk c) and stores coef * value at
 // out[bout + i*so_dd] (zero above the coarse-grid cap).
-template <int PF1, int PT1>
+template <int PF1, int PT1, class T, class O>
 __device__ __forceinline__ void term_chunk_store(const TermDev& t, int64_t e, int c, int rc, int D, const int* so,
-                                                 int cap, double* out) {
+                                                 int cap, O* out) {
   const int R = t.Tin / t.isz[t.dd];
   const int r0 = c * rc;
   const int nr = min(rc, R - r0);
@@ -756,21 +825,21 @@ __device__ __forceinline__ void term_chunk_store(const TermDev& t, int64_t e, in
     bout[r] = 0;
     if (r < nr) rest_offsets(r0 + r, t.dd, D, t.isz, t.sin, so, bin[r], bout[r]);
   }
-  double acc[kRC][PT1];
-  const bool live = term_block<PF1, PT1>(t, e, bin, nr, cap, acc);
+  T acc[kRC][PT1];
+  const bool live = term_block<PF1, PT1, T>(t, e, bin, nr, cap, acc);
   const int sod = so[t.dd];
 #pragma unroll
   for (int r = 0; r < kRC; ++r) {
     if (r >= nr) break;
 #pragma unroll
-    for (int i = 0; i < PT1; ++i) out[bout[r] + i * sod] = live ? acc[r][i] * t.coef : 0.;
+    for (int i = 0; i < PT1; ++i) out[bout[r] + i * sod] = live ? O(acc[r][i] * T(t.coef)) : O(0);
   }
 }
 
 // First sweeps of the partial sums of one phase: thread per (task, element,
 // chunk of rc rest indices); variant v (1: rc = 1, 2: rc = kRC) selects the
 // work-item offsets.
-template <int PF1, int PT1>
+template <int PF1, int PT1, class T>
 __global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntask, int64_t total, int D, int cap,
                                                   int v, int rc) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -784,16 +853,16 @@ __global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntas
   const int c = static_cast<int>(local - e * nch);
   int so[kMaxDim];
   strides_dev(tk.osz, D, so);
-  term_chunk_store<PF1, PT1>(tk.t, e, c, rc, D, so, cap, tk.out + e * tk.Tout);
+  term_chunk_store<PF1, PT1, T, T>(tk.t, e, c, rc, D, so, cap, SweepOps<T>::out(tk) + e * tk.Tout);
 }
 
 // Last sweep of every eval_up partial sum, accumulated in ordering order into
 // the point value, followed by the point flux (LagrInterpolation::eval_fp_Lag).
 // A block holds EB elements; threads are (term, element, chunk), term-major,
 // and write their values to LDS; the point values then add the terms in order.
-template <int PF1, int PT1>
+template <int PF1, int PT1, class T>
 __global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, int nterm, int EB, int nch, int rc,
-                                                           int64_t E, int P, int D, double* up, double* fp,
+                                                           int64_t E, int P, int D, T* up, T* fp,
                                                            const double* vcoef, int kind, double s, int ncomp, int cap) {
   extern __shared__ double vals[];  // [nterm][EB][P]
   const int tid = threadIdx.x;
@@ -806,7 +875,8 @@ __global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, i
   strides_dev(osz, D, so);
   if (e0 + el < E)
     for (int t = slot; t < nterm; t += nslot)
-      term_chunk_store<PF1, PT1>(terms[t], e0 + el, c, rc, D, so, cap, vals + (static_cast<int64_t>(t) * EB + el) * P);
+      term_chunk_store<PF1, PT1, T, double>(terms[t], e0 + el, c, rc, D, so, cap,
+                                            vals + (static_cast<int64_t>(t) * EB + el) * P);
   __syncthreads();
   const int64_t n = E * P;
   for (int o = tid; o < EB * P; o += blockDim.x) {
@@ -815,15 +885,9 @@ __global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, i
     double u = 0.;
     for (int j = 0; j < nterm; ++j) u += vals[(static_cast<int64_t>(j) * EB + ol) * P + q];
     const int64_t gid = (e0 + ol) * P + q;
-    up[gid] = u;
-    if (kind == 0) {
-      fp[gid] = u * u * 0.5;
-    } else if (kind == 1) {
-      fp[gid] = sin(u) * s;
-      fp[n + gid] = cos(u) * s;
-    } else if (kind == 2) {
-      for (int cc = 0; cc < ncomp; ++cc) fp[cc * n + gid] = vcoef[cc * n + gid] * u;
-    }
+    const T uu = T(u);
+    up[gid] = uu;
+    flux_point<T>(kind, uu, s, vcoef, fp, gid, n, ncomp);
   }
 }
 
@@ -831,7 +895,7 @@ __global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, i
 // accumulated in the reference order, then the RK stage update written to
 // u_next (u itself is still read by other threads' penalty sweeps).  Same
 // block structure as k_up_final.
-template <int PFM, int PFK>
+template <int PFM, int PFK, class T>
 __global__ __launch_bounds__(kElemBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens,
                                                             int npen, int EB, int nch, int rc, int64_t E, int K, int D,
                                                             double* rhs, const double* u, const double* tn,
@@ -849,8 +913,8 @@ __global__ __launch_bounds__(kElemBlock) void k_rhs_final(const TermDev* terms, 
   if (e0 + el < E)
     for (int t = slot; t < nterm + npen; t += nslot) {
       double* out = vals + (static_cast<int64_t>(t) * EB + el) * K;
-      if (t < nterm) term_chunk_store<PFM, PFK>(terms[t], e0 + el, c, rc, D, so, cap, out);
-      else term_chunk_store<PFK, PFK>(pens[t - nterm], e0 + el, c, rc, D, so, cap, out);
+      if (t < nterm) term_chunk_store<PFM, PFK, T, double>(terms[t], e0 + el, c, rc, D, so, cap, out);
+      else term_chunk_store<PFK, PFK, double, double>(pens[t - nterm], e0 + el, c, rc, D, so, cap, out);
     }
   __syncthreads();
   for (int o = tid; o < EB * K; o += blockDim.x) {
@@ -865,20 +929,21 @@ __global__ __launch_bounds__(kElemBlock) void k_rhs_final(const TermDev* terms, 
   }
 }
 
-template <int PF1, int PT1>
+template <int PF1, int PT1, class T>
 void launch_phase_t(const TaskDev* tasks, int ntask, int64_t total, int D, int cap, int v, hipStream_t s) {
   if (v == 0)
-    hipLaunchKernelGGL((k_phase_pt<PF1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap);
+    hipLaunchKernelGGL((k_phase_pt<PF1, T>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap);
   else
-    hipLaunchKernelGGL((k_phase<PF1, PT1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap, v,
-                       v == 1 ? 1 : kRC);
+    hipLaunchKernelGGL((k_phase<PF1, PT1, T>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap,
+                       v, v == 1 ? 1 : kRC);
 }
 
 // v: 0 thread per output, 1 thread per (element, rest index), 2 thread per
 // (element, kRC rest indices); total counts the work items of that mapping.
+template <class R>
 void launch_phase(int pf1, int pt1, const TaskDev* tasks, int ntask, int64_t total, int D, int cap, int v,
                   hipStream_t s) {
-#define ADG_PH(F, T) if (pf1 == F && pt1 == T) { launch_phase_t<F, T>(tasks, ntask, total, D, cap, v, s); return; }
+#define ADG_PH(F, T) if (pf1 == F && pt1 == T) { launch_phase_t<F, T, R>(tasks, ntask, total, D, cap, v, s); return; }
   ADG_PH(2, 2) ADG_PH(2, 3) ADG_PH(2, 4) ADG_PH(2, 5) ADG_PH(3, 2) ADG_PH(3, 3) ADG_PH(3, 4) ADG_PH(3, 5)
   ADG_PH(4, 2) ADG_PH(4, 3) ADG_PH(4, 4) ADG_PH(4, 5) ADG_PH(5, 2) ADG_PH(5, 3) ADG_PH(5, 4)
 #undef ADG_PH
@@ -1034,6 +1099,8 @@ void Context::set_interpolation(int family) {
 struct Context::FusedPlan {
   int64_t version = -1;
   int interp = -1;
+  int precision = -1;
+  const float* fucoe32 = nullptr;
   const double *u = nullptr, *tn = nullptr, *fucoe = nullptr, *tmp = nullptr;
   double* unext = nullptr;
   int flux_kind = -1;
@@ -1360,7 +1427,8 @@ void Context::flux(int kind, const double* params) {
     const int64_t n = E() * P;
     if (n == 0) return;
     prof_begin(PROF_FLUX, 8.0 * static_cast<double>(n) * (1 + ncomp));
-    hipLaunchKernelGGL(k_flux_herm, dim3(nblocks(n)), dim3(kBlock), 0, stream, up.p, fp.p, E(), D, kind, s, ncomp);
+    hipLaunchKernelGGL(k_flux_herm<double>, dim3(nblocks(n)), dim3(kBlock), 0, stream, up.p, fp.p, E(), D, kind, s,
+                       ncomp);
     prof_end();
     return;
   }
@@ -1395,11 +1463,55 @@ void Context::surplus_transform(const double* in, double* out, double* sa, doubl
     a.E = E(); a.comp_stride = E() * P; a.P = P; a.p1 = p1; a.stride_d = stride; a.ncomp = ncomp;
     if (E() > 0) {
       prof_begin(PROF_SURPLUS, 8.0 * static_cast<double>(E()) * P * ncomp * 2);
-      hipLaunchKernelGGL(k_surplus, dim3(nblocks(E() * P * ncomp)), dim3(kBlock), 0, stream, a);
+      hipLaunchKernelGGL(k_surplus<double>, dim3(nblocks(E() * P * ncomp)), dim3(kBlock), 0, stream, a);
       prof_end();
     }
     cur = dst;
   }
+}
+
+void Context::surplus_transform32(const float* in, float* out, float* sa, float* sb, int ncomp) {
+  const int p1 = m + 1;
+  const SplitTable& w = split_table(t_wt.p, static_cast<size_t>(1 << nmax) * p1 * p1);
+  const float* cur = in;
+  int stride = P;
+  for (int d = 0; d < D; ++d) {
+    stride /= p1;
+    float* dst = (d == D - 1) ? out : ((d % 2 == 0) ? sa : sb);
+    SurplusArgs a{};
+    a.in32 = cur; a.out32 = dst; a.wh = w.hi.p; a.wl = w.lo.p;
+    a.slot = g_slot[d].p; a.fbase = g_fbase[d].p; a.fmap = g_fmap[d].p;
+    a.anc_slot = t_anc_slot.p; a.anc_point = t_anc_point.p; a.err = err_flag.p;
+    a.E = E(); a.comp_stride = E() * P; a.P = P; a.p1 = p1; a.stride_d = stride; a.ncomp = ncomp;
+    if (E() > 0) {
+      prof_begin(PROF_SURPLUS, 4.0 * static_cast<double>(E()) * P * ncomp * 2);
+      hipLaunchKernelGGL(k_surplus<float>, dim3(nblocks(E() * P * ncomp)), dim3(kBlock), 0, stream, a);
+      prof_end();
+    }
+    cur = dst;
+  }
+}
+
+// hi = fl32(M), lo = fl32(M - hi)
+__global__ __launch_bounds__(kBlock) void k_split(const double* M, float* hi, float* lo, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float h = static_cast<float>(M[i]);
+  hi[i] = h;
+  lo[i] = static_cast<float>(M[i] - static_cast<double>(h));
+}
+
+const Context::SplitTable& Context::split_table(const double* dev, size_t n) {
+  auto it = split_tabs.find(dev);
+  if (it != split_tabs.end()) return *it->second;
+  std::unique_ptr<SplitTable> t(new SplitTable());
+  t->hi.ensure(n);
+  t->lo.ensure(n);
+  hipLaunchKernelGGL(k_split, dim3(nblocks(static_cast<int64_t>(n))), dim3(kBlock), 0, stream, dev, t->hi.p, t->lo.p,
+                     static_cast<int64_t>(n));
+  const SplitTable& r = *t;
+  split_tabs[dev] = std::move(t);
+  return r;
 }
 
 void Context::fp_to_coe() {
@@ -1488,7 +1600,8 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     FusedPlan& pl = *plans[slot];
     const bool same_alpha = std::equal(pl.alpha, pl.alpha + D, p.alpha);
     if (pl.version == grid_version && pl.tn == tn.p && pl.fucoe == fucoe.p && pl.unext == u_next.p &&
-        pl.tmp == fused_tmp.p && pl.flux_kind == p.flux_kind && pl.interp == interp && same_alpha)
+        pl.tmp == fused_tmp.p && pl.flux_kind == p.flux_kind && pl.interp == interp && same_alpha &&
+        pl.precision == precision && pl.fucoe32 == fucoe32.p)
       return pl;
   } else {
     slot = !plans[0] ? 0 : (!plans[1] ? 1 : 0);
@@ -1509,14 +1622,25 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
   FusedPlan& pl = *plans[slot];
   pl.version = grid_version;
   pl.interp = interp;
+  pl.precision = precision;
+  pl.fucoe32 = fucoe32.p;
+  const bool f32 = precision == 1;
+  // algorithmic bytes of a term input: FP64 coefficients, else the storage word
+  const double w = f32 ? 4.0 : 8.0;
+  auto in_bytes = [&](const TermDev& t) { return (t.in32 || !f32) ? w : 8.0; };
   pl.u = u.p; pl.tn = tn.p; pl.fucoe = fucoe.p; pl.unext = u_next.p; pl.tmp = fused_tmp.p;
   pl.flux_kind = p.flux_kind;
   std::copy(p.alpha, p.alpha + D, pl.alpha);
   auto tmp = [&](int chain, int step) { return fused_tmp.p + (static_cast<size_t>(step) * nchains + chain) * per; };
-  auto make_term = [&](const double* in, const int* sin_sz, int dd, SweepTable tab, int pt1, bool flx, int lu,
-                       double coef) {
+  auto make_term = [&](const double* in, const float* in32, const int* sin_sz, int dd, SweepTable tab, int pt1,
+                       bool flx, int lu, double coef, bool split) {
     TermDev t{};
-    t.in = in; t.M = tab.dev; t.ns = tab.ns; t.pt1 = pt1; t.dd = dd;
+    t.in = in; t.in32 = in32; t.M = tab.dev; t.ns = tab.ns; t.pt1 = pt1; t.dd = dd;
+    if (split) {  // FP32 sweep: the table split in two FP32 parts
+      const SplitTable& sp = split_table(tab.dev, static_cast<size_t>(tab.ns) * tab.ns * sin_sz[dd] * pt1);
+      t.Mh = sp.hi.p;
+      t.Ml = sp.lo.p;
+    }
     t.slot = g_slot[dd].p; t.fbase = g_fbase[dd].p; t.fmaxl = g_fmaxl[dd].p; t.fmap = g_fmap[dd].p;
     t.fbits = g_fbits[dd].p;
     t.lsum = g_lsum.p;
@@ -1527,26 +1651,30 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     return t;
   };
   // one chain of partial sums: returns its last term; appends first-step tasks per phase
-  auto chain = [&](int cid, const double* src, int pf, int pt, const SweepTable* tabs, const int* flx, int o,
-                   double coef, std::vector<std::vector<TaskDev>>& phase_tasks) {
+  auto chain = [&](int cid, const double* src, const float* src32, int pf, int pt, const SweepTable* tabs,
+                   const int* flx, int o, double coef, std::vector<std::vector<TaskDev>>& phase_tasks) {
     int sz[kMaxDim];
     for (int d = 0; d < D; ++d) sz[d] = pf + 1;
     const double* cur = src;
+    const float* cur32 = src32;
     for (int st = 0; st < D; ++st) {
       const int dd = dims[o][st];
       int sin_sz[kMaxDim];
       std::copy(sz, sz + D, sin_sz);
       sz[dd] = pt + 1;
-      TermDev t = make_term(cur, sin_sz, dd, tabs[dd], pt + 1, flx[dd] != 0, lus[o][st], st == 0 ? coef : 1.0);
+      TermDev t = make_term(cur, cur32, sin_sz, dd, tabs[dd], pt + 1, flx[dd] != 0, lus[o][st], st == 0 ? coef : 1.0,
+                            f32);
       if (st == D - 1) return t;
       TaskDev tk{};
       tk.t = t;
       tk.out = tmp(cid, st);
+      tk.out32 = reinterpret_cast<float*>(tk.out);  // FP32 storage uses the first half of the buffer
       tk.Tout = prod_of(sz, D);
       std::copy(sz, sz + D, tk.osz);
       tk.R = t.Tin / sin_sz[dd];
       phase_tasks[st].push_back(tk);
       cur = tk.out;
+      cur32 = tk.out32;
     }
     return TermDev{};
   };
@@ -1564,7 +1692,7 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
         begin[0] += E() * tk.Tout;
         begin[1] += E() * tk.R;
         begin[2] += E() * ((tk.R + kRC - 1) / kRC);
-        b += 8.0 * E() * (tk.t.Tin + tk.Tout);
+        b += E() * (in_bytes(tk.t) * tk.t.Tin + w * tk.Tout);
       }
       bytes.push_back(b);
       ntask.push_back(static_cast<int>(tasks.size()));
@@ -1580,13 +1708,13 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     SweepTable tabs[kMaxDim];
     int flx[kMaxDim];
     for (int d = 0; d < D; ++d) { tabs[d] = table(0); flx[d] = 0; }
-    for (int o = 0; o < nord; ++o) terms.push_back(chain(o, u.p, k, m, tabs, flx, o, 1.0, ph));
+    for (int o = 0; o < nord; ++o) terms.push_back(chain(o, u.p, nullptr, k, m, tabs, flx, o, 1.0, ph));
     finish_phases(ph, pl.up_tasks, pl.up_phase_ntask, pl.up_phase_total, pl.up_phase_bytes);
     upload(pl.up_terms, terms, stream);
     pl.n_up_terms = static_cast<int>(terms.size());
     const int nc = (p.flux_kind == 0) ? 1 : D;
-    double b = 8.0 * E() * P * (1 + nc + (p.flux_kind == ADG_FLUX_VARCOEF_ID ? nc : 0));
-    for (const TermDev& t : terms) b += 8.0 * E() * t.Tin;
+    double b = w * E() * P * (1 + nc) + (p.flux_kind == ADG_FLUX_VARCOEF_ID ? 8.0 * E() * P * nc : 0.);
+    for (const TermDev& t : terms) b += in_bytes(t) * E() * t.Tin;
     pl.up_final_bytes = b;
   }
   // rhs chains, in the reference order: volume terms for d = 0..D-1, then flux terms
@@ -1602,20 +1730,25 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
           if (dd == d) { tabs[dd] = ft ? table(3) : table(2); flx[dd] = ft; }
           else { tabs[dd] = table(1); flx[dd] = 0; }
         }
-        const double* src = fucoe.p + static_cast<int64_t>(same_flux ? 0 : d) * E() * P;
-        for (int o = 0; o < nord; ++o) terms.push_back(chain(cid++, src, m, k, tabs, flx, o, ft ? 0.5 : 1.0, ph));
+        const int64_t coff = static_cast<int64_t>(same_flux ? 0 : d) * E() * P;
+        const double* src = fucoe.p + coff;
+        const float* src32 = f32 ? fucoe32.p + coff : nullptr;
+        for (int o = 0; o < nord; ++o)
+          terms.push_back(chain(cid++, src, src32, m, k, tabs, flx, o, ft ? 0.5 : 1.0, ph));
       }
     }
     finish_phases(ph, pl.rhs_tasks, pl.rhs_phase_ntask, pl.rhs_phase_total, pl.rhs_phase_bytes);
     upload(pl.rhs_terms, terms, stream);
     pl.n_rhs_terms = static_cast<int>(terms.size());
     double b = 8.0 * E() * K * (1 + 3 + D);  // rhs out, u, tn and u_next, penalty inputs
-    for (const TermDev& t : terms) b += 8.0 * E() * t.Tin;
+    for (const TermDev& t : terms) b += in_bytes(t) * E() * t.Tin;
     pl.rhs_final_bytes = b;
     std::vector<TermDev> pens;
     int ksz[kMaxDim];
     for (int d = 0; d < D; ++d) ksz[d] = k + 1;
-    for (int d = 0; d < D; ++d) pens.push_back(make_term(u.p, ksz, d, table(8), k + 1, true, LU_FULL, -p.alpha[d] / 2.));
+    // the LF penalty stays FP64 in every precision
+    for (int d = 0; d < D; ++d)
+      pens.push_back(make_term(u.p, nullptr, ksz, d, table(8), k + 1, true, LU_FULL, -p.alpha[d] / 2., false));
     upload(pl.pens, pens, stream);
     pl.n_pen = D;
   }
@@ -1634,10 +1767,28 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   same_flux = (p.flux_kind == 0);
   if (p.flux_kind == 1 && D != 2) throw std::runtime_error("sin/cos flux is two-dimensional");
   if (interp == 1) check_herm_flux(p.flux_kind, D);
+  if (precision == 1) {
+    const size_t e = static_cast<size_t>(std::max<int64_t>(E(), 1));
+    up32.ensure(e * P);
+    for (DevBuf<float>* b : {&fp32, &fucoe32, &interA32, &interB32}) b->ensure(e * P * D);
+  }
+  FusedPlan& pl = fused_plan(p);
+  if (precision == 1) stage_body<float>(p, pl, scheme, stage, dt);
+  else stage_body<double>(p, pl, scheme, stage, dt);
+}
+
+// One fused stage with intermediates stored in T (double, or float with split tables).
+template <class T>
+void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int stage, double dt) {
   // Hermite: the final eval_up kernel writes point values only (kind -1); the
   // point flux needs the element's other degrees of freedom and runs after it
   const int up_kind = interp == 1 ? -1 : p.flux_kind;
-  FusedPlan& pl = fused_plan(p);
+  constexpr bool f32 = sizeof(T) == 4;
+  T* upp;
+  T* fpp;
+  if constexpr (f32) { upp = up32.p; fpp = fp32.p; }
+  else { upp = up.p; fpp = fp.p; }
+  const int tc = f32 ? 4 : 0;  // autotuner classes per precision
   const int ncomp = same_flux ? 1 : D;
   // point values and flux
   int64_t off = 0;
@@ -1646,13 +1797,13 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
     const TaskDev* tasks = pl.up_tasks.p + off;
     const int nt = pl.up_phase_ntask[s];
     const std::array<int64_t, 3> tot = pl.up_phase_total[s];
-    tuned_launch(TUNE_UP_PHASE, [&](int v) { launch_phase(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
+    tuned_launch(TUNE_UP_PHASE + tc, [&](int v) { launch_phase<T>(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
     prof_end();
     off += pl.up_phase_ntask[s];
   }
   {
     prof_begin(PROF_FLUX, pl.up_final_bytes);
-    tuned_launch(TUNE_UP_FINAL, [&](int v) {
+    tuned_launch(TUNE_UP_FINAL + tc, [&](int v) {
       if (v == 0) {
         const int64_t n = E() * P;
         const int per_block = final_outputs_per_block(pl.n_up_terms);
@@ -1660,8 +1811,8 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
         const unsigned bs = static_cast<unsigned>(per_block * pl.n_up_terms);
 #define ADG_UP(K1)                                                                                                 \
   if (k + 1 == K1)                                                                                                 \
-    hipLaunchKernelGGL((k_up_final_pt<K1>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, \
-                       E(), P, m + 1, D, up.p, fp.p, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);
+    hipLaunchKernelGGL((k_up_final_pt<K1, T>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, \
+                       E(), P, m + 1, D, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);
         ADG_UP(2) ADG_UP(3) ADG_UP(4)
 #undef ADG_UP
         return;
@@ -1675,25 +1826,26 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
       const size_t lds = sizeof(double) * pl.n_up_terms * EB * P;
 #define ADG_UF(K1, M1)                                                                                             \
   if (k + 1 == K1 && m + 1 == M1)                                                                                  \
-    hipLaunchKernelGGL((k_up_final<K1, M1>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p, pl.n_up_terms, EB, nch, \
-                       rc, E(), P, D, up.p, fp.p, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);
+    hipLaunchKernelGGL((k_up_final<K1, M1, T>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p, pl.n_up_terms, EB, nch, \
+                       rc, E(), P, D, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);
       ADG_UF(2, 2) ADG_UF(2, 3) ADG_UF(2, 4) ADG_UF(2, 5) ADG_UF(3, 2) ADG_UF(3, 3) ADG_UF(3, 4) ADG_UF(3, 5)
       ADG_UF(4, 2) ADG_UF(4, 3) ADG_UF(4, 4) ADG_UF(4, 5)
 #undef ADG_UF
     });
     if (interp == 1)
-      hipLaunchKernelGGL(k_flux_herm, dim3(nblocks(E() * P)), dim3(kBlock), 0, stream, up.p, fp.p, E(), D,
-                         p.flux_kind, p.flux_param, ncomp);
+      hipLaunchKernelGGL(k_flux_herm<T>, dim3(nblocks(E() * P)), dim3(kBlock), 0, stream, upp, fpp, E(), D,
+                         p.flux_kind, T(p.flux_param), ncomp);
     prof_end();
   }
-  surplus_transform(fp.p, fucoe.p, interA.p, interB.p, ncomp);
+  if constexpr (f32) surplus_transform32(fp32.p, fucoe32.p, interA32.p, interB32.p, ncomp);
+  else surplus_transform(fp.p, fucoe.p, interA.p, interB.p, ncomp);
   off = 0;
   for (size_t s = 0; s < pl.rhs_phase_ntask.size(); ++s) {
     prof_begin(PROF_SWEEP, pl.rhs_phase_bytes[s]);
     const TaskDev* tasks = pl.rhs_tasks.p + off;
     const int nt = pl.rhs_phase_ntask[s];
     const std::array<int64_t, 3> tot = pl.rhs_phase_total[s];
-    tuned_launch(TUNE_RHS_PHASE, [&](int v) { launch_phase(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
+    tuned_launch(TUNE_RHS_PHASE + tc, [&](int v) { launch_phase<T>(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
     prof_end();
     off += pl.rhs_phase_ntask[s];
   }
@@ -1701,7 +1853,7 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
     prof_begin(PROF_RK, pl.rhs_final_bytes);
     const int nt = pl.n_rhs_terms + pl.n_pen;
     const double* dtv = use_inst_dt ? d_inst_dt.p : nullptr;
-    tuned_launch(TUNE_RHS_FINAL, [&](int v) {
+    tuned_launch(TUNE_RHS_FINAL + tc, [&](int v) {
       if (v == 0) {
         const int64_t n = E() * K;
         const int per_block = final_outputs_per_block(nt);
@@ -1709,7 +1861,7 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
         const unsigned bs = static_cast<unsigned>(per_block * nt);
 #define ADG_RP(M1, K1)                                                                                             \
   if (m + 1 == M1 && k + 1 == K1)                                                                                  \
-    hipLaunchKernelGGL((k_rhs_final_pt<M1, K1>), dim3(nb), dim3(bs), 0, stream, pl.rhs_terms.p, pl.n_rhs_terms,    \
+    hipLaunchKernelGGL((k_rhs_final_pt<M1, K1, T>), dim3(nb), dim3(bs), 0, stream, pl.rhs_terms.p, pl.n_rhs_terms,    \
                        pl.pens.p, pl.n_pen, per_block, E(), K, k + 1, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, \
                        dt, dtv, g_inst.p, lsum_cap);
         ADG_RP(2, 2) ADG_RP(2, 3) ADG_RP(2, 4) ADG_RP(3, 2) ADG_RP(3, 3) ADG_RP(3, 4) ADG_RP(4, 2) ADG_RP(4, 3)
@@ -1726,7 +1878,7 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
       const size_t lds = sizeof(double) * nt * EB * K;
 #define ADG_RF(M1, K1)                                                                                             \
   if (m + 1 == M1 && k + 1 == K1)                                                                                  \
-    hipLaunchKernelGGL((k_rhs_final<M1, K1>), dim3(nb), dim3(bs), lds, stream, pl.rhs_terms.p, pl.n_rhs_terms,     \
+    hipLaunchKernelGGL((k_rhs_final<M1, K1, T>), dim3(nb), dim3(bs), lds, stream, pl.rhs_terms.p, pl.n_rhs_terms,     \
                        pl.pens.p, pl.n_pen, EB, nch, rc, E(), K, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt,  \
                        dtv, g_inst.p, lsum_cap);
       ADG_RF(2, 2) ADG_RF(2, 3) ADG_RF(2, 4) ADG_RF(3, 2) ADG_RF(3, 3) ADG_RF(3, 4) ADG_RF(4, 2) ADG_RF(4, 3)

@@ -19,6 +19,7 @@ TAB_AA_UV, TAB_AA_UVX, TAB_AA_ULFT, TAB_AA_URGT, TAB_AA_UJP = 4, 5, 6, 7, 8
 LU_FULL, LU_L, LU_U = 0, 1, 2
 FLUX_BURGERS, FLUX_SINCOS, FLUX_VARCOEF = 0, 1, 2
 INTERP_LAGRANGE, INTERP_HERMITE = 0, 1
+PREC_FP64, PREC_FP32_SPLIT = 0, 1
 RK_EULER, RK2, RK3 = 0, 2, 3
 
 POINT_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p)
@@ -74,6 +75,7 @@ def lib():
             "adg_eval_up": (i32, [vp]),
             "adg_set_level_sum_cap": (i32, [vp, i32]),
             "adg_set_interpolation": (i32, [vp, i32]),
+            "adg_set_precision": (i32, [vp, i32]),
             "adg_state_write": (i32, [vp, ctypes.c_char_p]),
             "adg_state_read": (i32, [vp, ctypes.c_char_p]),
             "adg_lagr_points": (i32, [vp, DP]),
@@ -234,6 +236,10 @@ class Solution:
         """INTERP_LAGRANGE or INTERP_HERMITE (P3, needs m = 3) for eval_up, flux,
         fp_to_coe, rhs_lagr_*, stage and run_hyperbolic."""
         self._check(lib().adg_set_interpolation(self._h, family))
+
+    def set_precision(self, precision):
+        """PREC_FP64 or PREC_FP32_SPLIT for stage() and the fused form of run_hyperbolic."""
+        self._check(lib().adg_set_precision(self._h, precision))
 
     def state_write(self, path):
         """Binary state file: element sets and Alpert coefficients of every instance."""

@@ -291,6 +291,16 @@ int adg_set_instance_thresholds(adg_ctx* c, const double* eps, const double* eta
  * keeps the fastest. */
 int adg_set_kernel_variant(adg_ctx* c, int variant);
 
+/* Storage precision of adg_stage and of adg_run_hyperbolic's fused form
+ * (north-star precision form; no reference counterpart, the reference is FP64
+ * throughout).  ADG_PREC_FP32_SPLIT runs the fiber transforms, point values,
+ * flux and surplus transform in FP32, every 1D table and surplus weight split
+ * into two FP32 parts (M = hi + lo, both applied by fused multiply-adds); the
+ * right-hand side accumulation, the LF penalty, the RK combination and the
+ * refine / coarsen norms stay FP64.  The per-sweep entry points are FP64. */
+enum { ADG_PREC_FP64 = 0, ADG_PREC_FP32_SPLIT = 1 };
+int adg_set_precision(adg_ctx* c, int precision);
+
 /* One fused RK stage of the same operator: point values, flux, surplus
  * transform, right-hand side (volume, flux and LF penalty terms) and the stage
  * update in 3*dim launches; bit-identical to adg_eval_up + adg_flux +

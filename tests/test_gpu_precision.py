@@ -1,0 +1,105 @@
+"""FP32 precision form of the fused stage (split 1D tables, FP64 accumulation
+of the right-hand side and RK combination) against the FP64 form.
+
+Tolerances: the FP32 intermediates carry ~6e-8 relative rounding per
+operation, so after a few stages the coefficients differ from FP64 by well
+under 1e-5 relative; the L2 error to the exact solution must stay within 1%
+of the FP64 error (the criterion is 1.1x)."""
+import numpy as np
+import pytest
+
+import pyadg
+import workloads
+from common import ic_params, problem, same_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, -1])
+@pytest.mark.parametrize("case", ["burgers", "sincos", "varcoef3d", "hermite"])
+def test_fp32_stage_close_to_fp64(case, variant):
+    print(f"[case] fp32 stage {case} variant={variant}", flush=True)
+    if case == "varcoef3d":
+        dim, k, m, nmax, li = 3, 1, 2, 5, 4
+        p = ic_params(1, shift=0.1)
+        prob = problem(flux_kind=2, flux_param=0.5, alpha=[1.5] * 3, wave_speed=[1.5] * 3, cfl=0.2, t_final=1.0)
+    else:
+        dim, k, m, nmax, li = 2, 2, 3, 6, 4
+        p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.08, (0.3, 0.6))])
+        kind = 1 if case == "sincos" else 0
+        prob = problem(flux_kind=kind, flux_param=1.0, alpha=[1.2] * 2, wave_speed=[1.0] * 2, cfl=0.2, t_final=1.0)
+    sols = []
+    for prec in (pyadg.PREC_FP64, pyadg.PREC_FP32_SPLIT):
+        s = pyadg.Solution(dim, k, m, nmax, True, li)
+        s.init_adaptive_intp_builtin(p, 1e-4)
+        if case == "hermite":
+            s.set_interpolation(pyadg.INTERP_HERMITE)
+        s.set_precision(prec)
+        s.set_kernel_variant(variant)
+        s.rk_init()
+        for st in range(3):
+            s.stage(prob, pyadg.RK3, st, 1e-3)
+        sols.append(s)
+    a, b = sols[1].get(pyadg.FIELD_UCOE), sols[0].get(pyadg.FIELD_UCOE)
+    r = rel(a, b)
+    assert 0 < r < 1e-6, r
+    # the right-hand side is a difference of much larger flux terms (cancellation), so its
+    # FP32 error relative to its own norm is larger than the coefficients'
+    x, y = sols[1].get(pyadg.FIELD_RHS), sols[0].get(pyadg.FIELD_RHS)
+    assert rel(x, y) < 5e-4
+
+
+def test_fp32_error_to_exact_solution_within_one_percent():
+    # u_t + u_x + u_y = 0 through the interpolated flux (a = 1), exact sin(2 pi (x + y - 2t))
+    t_final = 0.05
+    errs = []
+    for prec in (pyadg.PREC_FP64, pyadg.PREC_FP32_SPLIT):
+        print(f"[case] fp32 linear exact precision={prec}", flush=True)
+        s = pyadg.Solution(2, 2, 3, 6, True, 6)
+        s.init_adaptive_intp_builtin(ic_params(0, shift=0.0), 1e10)
+        s.set_precision(prec)
+        prob = problem(flux_kind=2, flux_param=0.0, alpha=[1.0, 1.0], wave_speed=[1.0, 1.0], cfl=0.1,
+                       t_final=t_final, adaptive=0, eps=0.0, eta=0.0, max_steps=0, form=1)
+        s.run_hyperbolic(prob)
+        errs.append(s.error_no_separable_builtin(ic_params(0, shift=2 * t_final), 3))
+    e64, e32 = errs
+    assert e64[1] > 0 and abs(e32[1] / e64[1] - 1) < 0.01, (e64, e32)
+
+
+def test_fp32_adaptive_batch_runs_and_stays_close():
+    # the chosen form (batch + fused + FP32); element sets are reported, coefficients compared where they agree
+    ws = [workloads.c2_burgers(seed=s, nmax=6, level_init=4, t_final=0.02) for s in (3, 5)]
+    out = []
+    for prec in (pyadg.PREC_FP64, pyadg.PREC_FP32_SPLIT):
+        print(f"[case] fp32 batch precision={prec}", flush=True)
+        b = workloads.make_batch(ws)
+        b.set_precision(prec)
+        prob = ws[0].problem
+        prob.form = 1
+        b.run_hyperbolic(prob)
+        out.append(b)
+    for i in range(2):
+        g64, g32 = out[0].instance_grid(i), out[1].instance_grid(i)
+        if np.array_equal(g64[2], g32[2]):
+            assert rel(out[1].instance_get(i), out[0].instance_get(i)) < 1e-5
+        else:
+            print(f"[fp32 batch] instance {i}: element sets differ ({len(g64[2])} vs {len(g32[2])})", flush=True)
+
+
+def test_fp32_runs_are_deterministic():
+    outs = []
+    for _ in range(2):
+        print("[case] fp32 determinism", flush=True)
+        s = pyadg.Solution(2, 2, 3, 6, True, 4)
+        s.init_adaptive_intp_builtin(ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.08, (0.3, 0.6))]), 1e-4)
+        s.set_precision(pyadg.PREC_FP32_SPLIT)
+        prob = problem(flux_kind=0, alpha=[1.2, 1.2], wave_speed=[1.0, 1.0], cfl=0.2, t_final=0.02, adaptive=1,
+                       eps=1e-4, eta=1e-5, max_steps=3, form=1)
+        s.run_hyperbolic(prob)
+        outs.append(s)
+    assert same_grid(outs[0], outs[1])
+    assert np.array_equal(outs[0].get(pyadg.FIELD_UCOE), outs[1].get(pyadg.FIELD_UCOE))
