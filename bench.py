@@ -104,6 +104,9 @@ def main():
                     help="contexts (HIP streams, host threads) the batch is split over in the chosen form")
     ap.add_argument("--batch", type=int, default=256,
                     help="independent seeded instances solved together in the chosen form (c2, c5)")
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"),
+                    help="storage precision of the chosen form's fused stages (fp32: split FP32 tables, FP64 "
+                         "accumulation, RK and refine/coarsen norms)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -132,6 +135,12 @@ def main():
                 b = workloads.make_batch(part)
                 b.save_state(0)
                 bsols.append(b)
+    prec = pyadg.PREC_FP32_SPLIT if args.precision == "fp32" else pyadg.PREC_FP64
+    chosen_sols = bsols if batched else ([] if w.linear else [workloads.make_solution(w)])
+    for c in chosen_sols:
+        c.set_precision(prec)
+        if c is not sol:
+            c.save_state(0)
         print(f"[bench] batch init done: {sum(b.num_elem for b in bsols)} elements", file=sys.stderr, flush=True)
 
     def measure(sols, form):
@@ -189,20 +198,32 @@ def main():
             s.profile(False)
         s0 = sols[0]
         u = s0.instance_get(0) if s0.num_instances > 1 else s0.get(pyadg.FIELD_UCOE)
+        keys = s0.instance_grid(0)[2] if s0.num_instances > 1 else s0.grid()[2]
         ninst = sum(s.num_instances for s in sols)
         print(f"[bench] form {form}, {ninst} instance(s) in {len(sols)} context(s): {args.steps} solves "
               f"({time_steps} time steps) in {sec:.3f} s, {dof_stages / sec:.4g} DoF-stage/s", file=sys.stderr,
               flush=True)
         return {"value": dof_stages / sec, "sec": sec, "dof_stages": dof_stages, "time_steps": time_steps,
-                "max_elem": max_elem, "prof": prof, "host": host, "u": u,
+                "max_elem": max_elem, "prof": prof, "host": host, "u": u, "keys": keys,
                 "sim_time": results[0][-1].final_time}
 
     plain = measure(sol, 0)
     fused = plain if w.linear else measure(sol, 1)
-    chosen = measure(bsols, 1) if batched else fused
+    if batched:
+        chosen = measure(bsols, 1)
+    elif w.linear or prec == pyadg.PREC_FP64:
+        chosen = fused
+    else:
+        chosen = measure(chosen_sols, 1)
     value = chosen["value"]
     # instance 0 of the batch is the plain problem itself: equal L2 error means equal coefficients
-    identical = bool(np.array_equal(plain["u"], chosen["u"]))
+    same_set = bool(np.array_equal(plain["keys"], chosen["keys"]))
+    identical = same_set and bool(np.array_equal(plain["u"], chosen["u"]))
+    if same_set:
+        l2_diff = float(np.sqrt(np.sum((plain["u"] - chosen["u"]) ** 2)))
+        l2_rel = l2_diff / float(np.sqrt(np.sum(plain["u"] ** 2)))
+    else:
+        l2_diff = l2_rel = None
 
     roofline = None
     prof = chosen["prof"]
@@ -227,22 +248,26 @@ def main():
         one = cpu_baseline(w, args.cpu_seconds / 4, 1)
         cpu["one_thread_value"] = one["value"]
 
+    pname = ("FP32 (split FP32 tables; FP64 right-hand-side accumulation, RK and norms)"
+             if prec == pyadg.PREC_FP32_SPLIT and not w.linear else "FP64")
     line = {
         "metric": METRIC, "value": value, "unit": "DoF-stage updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": chosen["sec"] / max(args.steps, 1) * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == pyadg.PREC_FP64 or w.linear else "f32",
+        "data": "synthetic",
         "config": dict(w.config(), step="one full solve from the saved initial state",
-                       form=("chosen: fused FP64 stages (3*dim launches per RK stage), %d seeded instances "
+                       form=("chosen: fused %s stages (3*dim launches per RK stage), %d seeded instances "
                              "(seeds %d..%d) in %d context(s) solved concurrently"
-                             % (args.batch, args.seed, args.seed + args.batch - 1, len(bsols)))
-                       if batched else "chosen: fused FP64 stages (3*dim launches per RK stage)",
+                             % (pname, args.batch, args.seed, args.seed + args.batch - 1, len(bsols)))
+                       if batched else "chosen: fused %s stages (3*dim launches per RK stage)" % pname,
+                       precision=args.precision if not w.linear else "fp64",
                        contexts=len(bsols) if batched else 1,
                        plain_form="plain FP64, one kernel per 1D sweep, one instance",
                        plain_value=plain["value"], fused_single_instance_value=fused["value"],
                        chosen_value=chosen["value"], batch=args.batch if batched else 1,
                        speedup_vs_plain=chosen["value"] / plain["value"],
-                       forms_bit_identical=identical, l2_difference_between_forms=float(
-                           np.sqrt(np.sum((plain["u"] - chosen["u"]) ** 2))),
+                       forms_bit_identical=identical, element_sets_identical=same_set,
+                       l2_difference_between_forms=l2_diff, relative_l2_difference_between_forms=l2_rel,
                        time_steps_per_solve=plain["time_steps"] // max(args.steps, 1),
                        time_step_cap=args.time_steps or None,
                        max_elements=int(chosen["max_elem"]), dof_stage_updates=int(chosen["dof_stages"]),
