@@ -73,6 +73,38 @@ adg::Context::StageProblem stage_problem(const adg_problem* p, int D) {
   return sp;
 }
 
+// Decision log of the predict-and-refine step: features of every element at
+// the start of the step (kept for the leaves) ...
+void log_features(adg::Context& c, const std::vector<double>& eps, std::vector<float>& lf, std::vector<int32_t>& ll) {
+  const int64_t n = c.E();
+  c.prop_feats.ensure(static_cast<size_t>(std::max<int64_t>(n, 1)) * adg::kPropIn);
+  c.prop_leaf.ensure(std::max<int64_t>(n, 1));
+  c.proposer_features(eps, c.prop_feats.p, c.prop_leaf.p);
+  lf.resize(static_cast<size_t>(n) * adg::kPropIn);
+  ll.resize(n);
+  if (n > 0) {
+    adg::hip_check(hipMemcpyAsync(lf.data(), c.prop_feats.p, lf.size() * sizeof(float), hipMemcpyDeviceToHost,
+                                  c.stream), "download");
+    adg::hip_check(hipMemcpyAsync(ll.data(), c.prop_leaf.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream),
+                   "download");
+  }
+  c.sync();
+}
+
+// ... and whether the refinement that follows the prediction refines them
+// (leaf with predicted norm above eps, DGAdapt::refine).
+void log_labels(adg::Context& c, const std::vector<double>& eps, const std::vector<float>& lf,
+                const std::vector<int32_t>& ll) {
+  std::vector<double> s(c.E());
+  c.indicator_sumsq(s.data());
+  for (int i = 0; i < c.B(); ++i)
+    for (int64_t e = c.off[i]; e < c.off[i + 1]; ++e) {
+      if (!ll[e]) continue;
+      c.log_feats.insert(c.log_feats.end(), &lf[e * adg::kPropIn], &lf[e * adg::kPropIn] + adg::kPropIn);
+      c.log_labels.push_back(std::pow(s[e], 1. / 2.) > eps[i] ? 1 : 0);
+    }
+}
+
 // one RK3 stage of the Lagrange-interpolated hyperbolic operator
 void hyperbolic_rhs(adg::Context& c, const adg_problem* p) {
   const double fp = p->flux_param;
@@ -85,6 +117,22 @@ void hyperbolic_rhs(adg::Context& c, const adg_problem* p) {
   c.rhs_penalty(p->alpha);
 }
 
+}  // namespace
+
+namespace {
+const char kStateMagic[8] = {'A', 'D', 'G', 'S', 'T', 'A', 'T', '1'};
+
+struct FileCloser {
+  void operator()(std::FILE* f) const { if (f) std::fclose(f); }
+};
+using FilePtr = std::unique_ptr<std::FILE, FileCloser>;
+
+void write_all(std::FILE* f, const void* p, size_t n) {
+  if (n && std::fwrite(p, 1, n, f) != n) throw std::runtime_error("state file: write failed");
+}
+void read_all(std::FILE* f, void* p, size_t n) {
+  if (n && std::fread(p, 1, n, f) != n) throw std::runtime_error("state file: truncated");
+}
 }  // namespace
 
 extern "C" {
@@ -233,6 +281,115 @@ int adg_lagr_points(adg_ctx* h, double* x) {
           x[(e * c.P + q) * c.D + d] = c.interp_basis().point(c.uslot[e * c.D + d] * p1 + pd);
         }
       }
+  });
+}
+
+int adg_set_refine_mode(adg_ctx* h, int mode, double threshold) {
+  return guard(h, [&](adg::Context& c) {
+    if (mode != ADG_REFINE_PREDICT && mode != ADG_REFINE_LEARNED && mode != ADG_REFINE_CURRENT)
+      throw std::runtime_error("unknown refine mode");
+    if (!(threshold > 0. && threshold < 1.)) throw std::runtime_error("proposer threshold must lie in (0, 1)");
+    c.refine_mode = mode;
+    c.prop_threshold = threshold;
+  });
+}
+
+int adg_proposer_init(adg_ctx* h, uint64_t seed) { return guard(h, [&](adg::Context& c) { c.proposer_init(seed); }); }
+
+int adg_proposer_set_params(adg_ctx* h, const float* params) {
+  return guard(h, [&](adg::Context& c) {
+    if (!params) throw std::runtime_error("null parameters");
+    c.proposer_set_params(params);
+  });
+}
+
+int adg_proposer_get_params(adg_ctx* h, float* params) {
+  return guard(h, [&](adg::Context& c) {
+    if (!params) throw std::runtime_error("null output");
+    c.proposer_get_params(params);
+  });
+}
+
+int adg_proposer_gradient(adg_ctx* h, const float* feats, const uint8_t* labels, const float* weights, int64_t n,
+                          float* grad, double* loss) {
+  return guard(h, [&](adg::Context& c) {
+    if (!feats || !labels || !grad) throw std::runtime_error("null array");
+    const double l = c.proposer_gradient(feats, labels, weights, n, grad);
+    if (loss) *loss = l;
+  });
+}
+
+int adg_proposer_train(adg_ctx* h, const float* feats, const uint8_t* labels, int64_t n, int epochs, int batch,
+                       float lr, uint64_t seed, double* loss) {
+  return guard(h, [&](adg::Context& c) {
+    if (!feats || !labels) throw std::runtime_error("null array");
+    const double l = c.proposer_train(feats, labels, n, epochs, batch, lr, seed);
+    if (loss) *loss = l;
+  });
+}
+
+int adg_proposer_predict(adg_ctx* h, const float* feats, int64_t n, float* prob) {
+  return guard(h, [&](adg::Context& c) {
+    if (!feats || !prob || n < 0) throw std::runtime_error("null array or negative count");
+    c.prop_x.ensure(static_cast<size_t>(std::max<int64_t>(n, 1)) * adg::kPropIn);
+    c.prop_prob.ensure(std::max<int64_t>(n, 1));
+    if (n > 0)
+      adg::hip_check(hipMemcpyAsync(c.prop_x.p, feats, n * adg::kPropIn * sizeof(float), hipMemcpyHostToDevice,
+                                    c.stream), "upload");
+    c.proposer_predict(c.prop_x.p, n, c.prop_prob.p);
+    if (n > 0)
+      adg::hip_check(hipMemcpyAsync(prob, c.prop_prob.p, n * sizeof(float), hipMemcpyDeviceToHost, c.stream),
+                     "download");
+    c.sync();
+  });
+}
+
+int adg_proposer_features(adg_ctx* h, const double* eps, float* feats, int32_t* leaf) {
+  return guard(h, [&](adg::Context& c) {
+    if (!eps || !feats || !leaf) throw std::runtime_error("null array");
+    std::vector<float> lf;
+    std::vector<int32_t> ll;
+    log_features(c, std::vector<double>(eps, eps + c.B()), lf, ll);
+    std::copy(lf.begin(), lf.end(), feats);
+    std::copy(ll.begin(), ll.end(), leaf);
+  });
+}
+
+int adg_decision_log(adg_ctx* h, int enable) {
+  return guard(h, [&](adg::Context& c) {
+    c.log_decisions = enable != 0;
+    if (enable) {
+      c.log_feats.clear();
+      c.log_labels.clear();
+    }
+  });
+}
+
+int64_t adg_decision_log_size(adg_ctx* h) {
+  return (h && h->c) ? static_cast<int64_t>(h->c->log_labels.size()) : -1;
+}
+
+int adg_decision_log_get(adg_ctx* h, float* feats, uint8_t* labels) {
+  return guard(h, [&](adg::Context& c) {
+    if (feats) std::copy(c.log_feats.begin(), c.log_feats.end(), feats);
+    if (labels) std::copy(c.log_labels.begin(), c.log_labels.end(), labels);
+  });
+}
+
+int adg_decision_log_write(adg_ctx* h, const char* path) {
+  return guard(h, [&](adg::Context& c) {
+    if (!path) throw std::runtime_error("null path");
+    FilePtr f(std::fopen(path, "wb"));
+    if (!f) throw std::runtime_error(std::string("decision log: cannot open ") + path);
+    const char magic[8] = {'A', 'D', 'G', 'D', 'L', 'O', 'G', '1'};
+    const int32_t hdr[2] = {adg::kPropIn, 0};
+    const int64_t n = static_cast<int64_t>(c.log_labels.size());
+    write_all(f.get(), magic, sizeof(magic));
+    write_all(f.get(), hdr, sizeof(hdr));
+    write_all(f.get(), &n, sizeof(n));
+    write_all(f.get(), c.log_feats.data(), c.log_feats.size() * sizeof(float));
+    write_all(f.get(), c.log_labels.data(), c.log_labels.size());
+    if (std::fflush(f.get()) != 0) throw std::runtime_error("decision log: write failed");
   });
 }
 
@@ -492,6 +649,9 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
   return guard(h, [&](adg::Context& c) {
     c.sync();
     const auto t0 = std::chrono::steady_clock::now();
+    const int64_t redo0 = c.stat_redo, proposed0 = c.stat_proposed;
+    if (p->adaptive && c.refine_mode == ADG_REFINE_LEARNED && !c.prop_ready)
+      throw std::runtime_error("learned refinement: the proposer has no parameters (train or set them first)");
     const int nb = c.B();
     const std::vector<double> eps =
         static_cast<int>(c.inst_eps.size()) == nb ? c.inst_eps : std::vector<double>(nb, p->eps);
@@ -537,16 +697,26 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
           c.rk_stage(scheme, s, dt1);
         }
       };
-      if (p->adaptive) {
-        c.copy_to_predict();
-        c.rk_init();
-        stage(ADG_RK_EULER, 0);
-        c.refine_instances(eps);
-        c.copy_predict_to_ucoe();
+      if (p->adaptive && c.refine_mode != 0) {
+        // proposer, RK3 step and threshold judge (redoes the instances it refines)
+        c.proposer_step([&](int scheme, int s, double) { stage(scheme, s); }, eps, c.inst_dt.data());
         max_elem = std::max(max_elem, c.E());
+      } else {
+        if (p->adaptive) {
+          std::vector<float> lf;
+          std::vector<int32_t> ll;
+          if (c.log_decisions) log_features(c, eps, lf, ll);
+          c.copy_to_predict();
+          c.rk_init();
+          stage(ADG_RK_EULER, 0);
+          if (c.log_decisions) log_labels(c, eps, lf, ll);
+          c.refine_instances(eps);
+          c.copy_predict_to_ucoe();
+          max_elem = std::max(max_elem, c.E());
+        }
+        c.rk_init();
+        for (int s = 0; s < 3; ++s) stage(ADG_RK3, s);
       }
-      c.rk_init();
-      for (int s = 0; s < 3; ++s) stage(ADG_RK3, s);
       dof_stages += 3 * c.E() * c.K;
       if (p->adaptive) c.coarsen_instances(eta);
       ++steps;
@@ -613,6 +783,8 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
       double tmin = p->t_final;
       for (double v : t) tmin = std::min(tmin, v);
       st->final_time = tmin;
+      st->redone = c.stat_redo - redo0;
+      st->proposed = c.stat_proposed - proposed0;
     }
   });
 }
@@ -671,21 +843,6 @@ int adg_stage(adg_ctx* h, const adg_problem* p, int scheme, int stage, double dt
   });
 }
 
-namespace {
-const char kStateMagic[8] = {'A', 'D', 'G', 'S', 'T', 'A', 'T', '1'};
-
-struct FileCloser {
-  void operator()(std::FILE* f) const { if (f) std::fclose(f); }
-};
-using FilePtr = std::unique_ptr<std::FILE, FileCloser>;
-
-void write_all(std::FILE* f, const void* p, size_t n) {
-  if (n && std::fwrite(p, 1, n, f) != n) throw std::runtime_error("state file: write failed");
-}
-void read_all(std::FILE* f, void* p, size_t n) {
-  if (n && std::fread(p, 1, n, f) != n) throw std::runtime_error("state file: truncated");
-}
-}  // namespace
 
 int adg_state_write(adg_ctx* h, const char* path) {
   return guard(h, [&](adg::Context& c) {

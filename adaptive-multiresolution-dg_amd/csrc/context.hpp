@@ -22,6 +22,8 @@ struct HipError : std::runtime_error {
 };
 
 void hip_check(hipError_t e, const char* what);
+// runs fn(0..n-1) on the library's host thread pool (serially for small n)
+void parallel_for(int n, const std::function<void(int)>& fn);
 
 constexpr int ADG_FLUX_VARCOEF_ID = 2;  // ADG_FLUX_VARCOEF of include/adg.h
 
@@ -48,6 +50,16 @@ struct DevBuf {
     cap = c;
   }
 };
+
+// Learned refinement proposer: a multilayer perceptron with kPropIn inputs,
+// two hidden layers of kPropHidden ReLU units and one logistic output, the
+// probability that the plain predict-and-refine step would give a leaf
+// element its children.  Parameters (FP32, row-major): W1[H][F], b1[H],
+// W2[H][H], b2[H], W3[H], b3.
+constexpr int kPropIn = 12;
+constexpr int kPropHidden = 64;
+constexpr int kPropParams =
+    kPropHidden * kPropIn + kPropHidden + kPropHidden * kPropHidden + kPropHidden + kPropHidden + 1;
 
 // A 1D operator table on the device, stored target-major in cell blocks:
 // blk[((st * ns + s) * pf1 + k) * pt1 + i] = M(row s*pf1 + k, column st*pt1 + i)
@@ -214,6 +226,48 @@ class Context {
   std::unique_ptr<FusedPlan> plans[2];
   DevBuf<double> u_next, fused_tmp;
   int64_t grid_version = 0;
+
+  // ---- learned refinement proposer (proposer.hip) ----
+  // Features of every element for the proposer, from the current u:
+  // [0] log10(norm / eps) / 4 of the element, [1 + 2d], [2 + 2d] the same for
+  // its same-level neighbours at j - 2 and j + 2 in dimension d (periodic; a
+  // missing neighbour has norm 0, levels 0 and 1 are their own neighbours),
+  // [1 + 2D] sum of levels / (D nmax), [2 + 2D] max level / nmax, then zeros;
+  // clamped to [-4, 4].  leaf: fewer children present than possible.
+  void proposer_features(const std::vector<double>& eps, float* feats_dev, int32_t* leaf_dev);
+  DevBuf<float> prop_params, prop_grad, prop_m1, prop_m2, prop_norm, prop_feats, prop_prob;
+  DevBuf<float> prop_h1, prop_h2, prop_dh1, prop_dh2, prop_dz, prop_x, prop_y, prop_w, prop_loss;
+  DevBuf<uint16_t> prop_params16;  // FP16 copy for inference
+  DevBuf<int32_t> prop_leaf, prop_idx;
+  DevBuf<double> prop_eps;
+  bool prop_ready = false, prop16_valid = false;
+  int64_t prop_adam_t = 0;
+  void proposer_init(uint64_t seed);
+  void proposer_set_params(const float* host);
+  void proposer_get_params(float* host);
+  // gradient of the weighted logistic loss over n samples (host arrays)
+  double proposer_gradient(const float* feats, const uint8_t* labels, const float* weights, int64_t n, float* grad);
+  // Adam on minibatches; returns the mean weighted loss of the last epoch
+  double proposer_train(const float* feats, const uint8_t* labels, int64_t n, int epochs, int batch, float lr,
+                        uint64_t seed);
+  void proposer_predict(const float* feats_dev, int64_t n, float* prob_dev);  // FP16 inference
+  // refine modes of the adaptive step: 0 predict by forward Euler and refine
+  // (the reference), 1 learned proposer + threshold judge, 2 current norms +
+  // threshold judge
+  int refine_mode = 0;
+  double prop_threshold = 0.5;
+  int64_t stat_redo = 0, stat_proposed = 0;
+  // decision log of the predict-and-refine step: features of every leaf at
+  // the start of a step and whether the step refined it
+  bool log_decisions = false;
+  std::vector<float> log_feats;
+  std::vector<uint8_t> log_labels;
+  // one adaptive step with a proposer (modes 1, 2): propose, refine, RK3,
+  // judge (leaves above eps without children are refined and their
+  // instances redo the step from u^n); returns the number of redone steps
+  int64_t proposer_step(const std::function<void(int, int, double)>& stage, const std::vector<double>& eps,
+                        double* dt_inst);
+  DevBuf<int32_t> inst_mask;
 
   // In-memory snapshots of the solution state (grid + Alpert coefficients).
   struct Snapshot { std::vector<std::vector<int32_t>> slots; DevBuf<double> u; int64_t E = 0; };

@@ -337,6 +337,9 @@ __global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
 // the right-hand side (rr): forward Euler (source/ODESolver.cpp:209-213),
 // RK2 midpoint (:247-259) and SSP-RK3 (:285-301).
 __device__ __forceinline__ double rk_combine(int scheme, int stage, double dt, double t, double uu, double rr) {
+  // dt = 0 freezes an instance of a batch (its u and u_n both hold the
+  // accepted solution while other instances redo their step)
+  if (dt == 0.) return stage == 0 ? t : uu;
   if (scheme == 2) return stage == 0 ? t + 0.5 * dt * rr : t + dt * rr;
   if (scheme == 0 || stage == 0) return t + dt * rr;
   if (stage == 1) return 3. / 4. * t + 1. / 4. * (uu + dt * rr);

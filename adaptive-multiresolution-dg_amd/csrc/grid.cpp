@@ -115,6 +115,7 @@ std::vector<int32_t> refine_slots(const HostGrid& g, const std::vector<double>& 
   };
   int32_t s[kMaxDim];
   for (int64_t e = 0; e < g.E; ++e) {
+    if (!(norm[e] > eps)) continue;
     // leaf: fewer existing children than possible children
     const int total = g.num_total_children(e);
     int have = 0;
@@ -127,7 +128,7 @@ std::vector<int32_t> refine_slots(const HostGrid& g, const std::vector<double>& 
         s[d] = 2 * sd + 1; have += g.find(s) >= 0;
       }
     }
-    if (have >= total || !(norm[e] > eps)) continue;
+    if (have >= total) continue;
     for (int d = 0; d < D; ++d) {
       for (int q = 0; q < D; ++q) s[q] = g.slot[e * D + q];
       const int sd = s[d], l = level_of_slot(sd);

@@ -19,6 +19,8 @@ TAB_AA_UV, TAB_AA_UVX, TAB_AA_ULFT, TAB_AA_URGT, TAB_AA_UJP = 4, 5, 6, 7, 8
 LU_FULL, LU_L, LU_U = 0, 1, 2
 FLUX_BURGERS, FLUX_SINCOS, FLUX_VARCOEF = 0, 1, 2
 INTERP_LAGRANGE, INTERP_HERMITE = 0, 1
+REFINE_PREDICT, REFINE_LEARNED, REFINE_CURRENT = 0, 1, 2
+PROPOSER_INPUTS, PROPOSER_PARAMS = 12, 5057
 PREC_FP64, PREC_FP32_SPLIT = 0, 1
 RK_EULER, RK2, RK3 = 0, 2, 3
 
@@ -27,7 +29,8 @@ POINT_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ct
 
 class RunStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("dof_stages", ctypes.c_int64), ("max_elem", ctypes.c_int64),
-                ("seconds", ctypes.c_double), ("final_time", ctypes.c_double)]
+                ("seconds", ctypes.c_double), ("final_time", ctypes.c_double), ("redone", ctypes.c_int64),
+                ("proposed", ctypes.c_int64)]
 
 
 class Problem(ctypes.Structure):
@@ -58,6 +61,7 @@ def lib():
         vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
         P = ctypes.POINTER
         I32P, DP = P(ctypes.c_int32), P(dbl)
+        FP, U8P = P(ctypes.c_float), P(ctypes.c_uint8)
         sig = {
             "adg_create": (i32, [i32, i32, i32, i32, P(vp)]),
             "adg_destroy": (i32, [vp]),
@@ -76,6 +80,18 @@ def lib():
             "adg_set_level_sum_cap": (i32, [vp, i32]),
             "adg_set_interpolation": (i32, [vp, i32]),
             "adg_set_precision": (i32, [vp, i32]),
+            "adg_set_refine_mode": (i32, [vp, i32, dbl]),
+            "adg_proposer_init": (i32, [vp, ctypes.c_uint64]),
+            "adg_proposer_set_params": (i32, [vp, FP]),
+            "adg_proposer_get_params": (i32, [vp, FP]),
+            "adg_proposer_gradient": (i32, [vp, FP, U8P, FP, i64, FP, DP]),
+            "adg_proposer_train": (i32, [vp, FP, U8P, i64, i32, i32, ctypes.c_float, ctypes.c_uint64, DP]),
+            "adg_proposer_predict": (i32, [vp, FP, i64, FP]),
+            "adg_proposer_features": (i32, [vp, DP, FP, I32P]),
+            "adg_decision_log": (i32, [vp, i32]),
+            "adg_decision_log_size": (i64, [vp]),
+            "adg_decision_log_get": (i32, [vp, FP, U8P]),
+            "adg_decision_log_write": (i32, [vp, ctypes.c_char_p]),
             "adg_state_write": (i32, [vp, ctypes.c_char_p]),
             "adg_state_read": (i32, [vp, ctypes.c_char_p]),
             "adg_lagr_points": (i32, [vp, DP]),
@@ -143,6 +159,16 @@ class AdgError(RuntimeError):
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fp(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _u8p(a):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
 
 
 def _ip(a):
@@ -236,6 +262,69 @@ class Solution:
         """INTERP_LAGRANGE or INTERP_HERMITE (P3, needs m = 3) for eval_up, flux,
         fp_to_coe, rhs_lagr_*, stage and run_hyperbolic."""
         self._check(lib().adg_set_interpolation(self._h, family))
+
+    # ---- learned refinement proposer ----
+    def set_refine_mode(self, mode, threshold=0.5):
+        """REFINE_PREDICT (forward-Euler prediction, the reference), REFINE_LEARNED
+        (network proposer + threshold judge) or REFINE_CURRENT (current norms + judge)."""
+        self._check(lib().adg_set_refine_mode(self._h, mode, threshold))
+
+    def proposer_init(self, seed):
+        self._check(lib().adg_proposer_init(self._h, seed))
+
+    def proposer_params(self):
+        out = np.zeros(PROPOSER_PARAMS, np.float32)
+        self._check(lib().adg_proposer_get_params(self._h, _fp(out)))
+        return out
+
+    def set_proposer_params(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        assert p.size == PROPOSER_PARAMS
+        self._check(lib().adg_proposer_set_params(self._h, _fp(p)))
+
+    def proposer_gradient(self, feats, labels, weights=None):
+        x = np.ascontiguousarray(feats, np.float32).reshape(-1, PROPOSER_INPUTS)
+        y = np.ascontiguousarray(labels, np.uint8)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+        g = np.zeros(PROPOSER_PARAMS, np.float32)
+        loss = ctypes.c_double()
+        self._check(lib().adg_proposer_gradient(self._h, _fp(x), _u8p(y), None if w is None else _fp(w), len(y),
+                                                _fp(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def proposer_train(self, feats, labels, epochs, batch, lr, seed):
+        x = np.ascontiguousarray(feats, np.float32).reshape(-1, PROPOSER_INPUTS)
+        y = np.ascontiguousarray(labels, np.uint8)
+        loss = ctypes.c_double()
+        self._check(lib().adg_proposer_train(self._h, _fp(x), _u8p(y), len(y), epochs, batch, lr, seed,
+                                             ctypes.byref(loss)))
+        return loss.value
+
+    def proposer_predict(self, feats):
+        x = np.ascontiguousarray(feats, np.float32).reshape(-1, PROPOSER_INPUTS)
+        out = np.zeros(len(x), np.float32)
+        self._check(lib().adg_proposer_predict(self._h, _fp(x), len(x), _fp(out)))
+        return out
+
+    def proposer_features(self, eps):
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(eps, np.float64), (self.num_instances,)))
+        feats = np.zeros((self.num_elem, PROPOSER_INPUTS), np.float32)
+        leaf = np.zeros(self.num_elem, np.int32)
+        self._check(lib().adg_proposer_features(self._h, _dp(e), _fp(feats), _ip(leaf)))
+        return feats, leaf
+
+    def decision_log(self, enable):
+        self._check(lib().adg_decision_log(self._h, 1 if enable else 0))
+
+    def decision_log_get(self):
+        n = int(lib().adg_decision_log_size(self._h))
+        feats = np.zeros((n, PROPOSER_INPUTS), np.float32)
+        labels = np.zeros(n, np.uint8)
+        self._check(lib().adg_decision_log_get(self._h, _fp(feats), _u8p(labels)))
+        return feats, labels
+
+    def decision_log_write(self, path):
+        self._check(lib().adg_decision_log_write(self._h, os.fsencode(path)))
 
     def set_precision(self, precision):
         """PREC_FP64 or PREC_FP32_SPLIT for stage() and the fused form of run_hyperbolic."""

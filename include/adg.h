@@ -243,6 +243,8 @@ typedef struct {
   int64_t max_elem;       /* largest element count seen */
   double seconds;         /* wall time of the loop */
   double final_time;
+  int64_t redone;         /* instance steps the threshold judge sent back (refine modes 1, 2) */
+  int64_t proposed;       /* elements added by the proposer (refine modes 1, 2) */
 } adg_run_stats;
 
 /* RK3 on u_t + sum_d coef_d u_{x_d} = 0 with a fixed grid
@@ -290,6 +292,54 @@ int adg_set_instance_thresholds(adg_ctx* c, const double* eps, const double* eta
  * three on the live grid the first time a kernel meets a grid-size class and
  * keeps the fastest. */
 int adg_set_kernel_variant(adg_ctx* c, int variant);
+
+/* Learned refinement proposer (north-star learned part; no reference
+ * counterpart).  The reference refines by predicting u^{n+1} with one
+ * forward-Euler stage and giving every leaf whose predicted norm exceeds eps
+ * its children (example/02_hyperbolic_05_burgers_adapt.cpp:246-291,
+ * DGAdapt::refine, source/DGAdapt.cpp:371-404): ADG_REFINE_PREDICT, the
+ * default.  ADG_REFINE_LEARNED replaces the prediction by a multilayer
+ * perceptron on surplus norms of the current solution (ADG_PROPOSER_INPUTS
+ * features per element, two hidden layers of 64 ReLU units, logistic output;
+ * leaves with probability > threshold get their children).  ADG_REFINE_CURRENT
+ * proposes the leaves whose current norm exceeds eps (the method without the
+ * network).  In both, the threshold test stays as judge: after the RK step,
+ * leaves above eps that lack children are refined and their instances redo
+ * the step from u^n (adg_run_stats.redone counts them).
+ * Features of an element: [0] log10(norm / eps) / 4, [1 + 2d], [2 + 2d] the
+ * same for its same-level neighbours at j - 2 and j + 2 in dimension d
+ * (periodic; levels 0 and 1 are their own neighbours, a missing one has norm
+ * 0), [1 + 2 dim] sum of levels / (dim nmax), [2 + 2 dim] max level / nmax,
+ * zeros after; clamped to [-4, 4].  Parameters: FP32, row-major W1[64][12],
+ * b1[64], W2[64][64], b2[64], W3[64], b3.  Training (FP32, Adam, class-balanced
+ * logistic loss) and inference (FP16 weights and activations, FP32
+ * accumulation) run on the device. */
+#define ADG_PROPOSER_INPUTS 12
+#define ADG_PROPOSER_PARAMS 5057
+enum { ADG_REFINE_PREDICT = 0, ADG_REFINE_LEARNED = 1, ADG_REFINE_CURRENT = 2 };
+int adg_set_refine_mode(adg_ctx* c, int mode, double threshold);
+int adg_proposer_init(adg_ctx* c, uint64_t seed);
+int adg_proposer_set_params(adg_ctx* c, const float* params);
+int adg_proposer_get_params(adg_ctx* c, float* params);
+/* gradient of the weighted logistic loss (sum of weights normalised to 1) over
+ * n samples; weights may be null (all 1) */
+int adg_proposer_gradient(adg_ctx* c, const float* feats, const uint8_t* labels, const float* weights, int64_t n,
+                          float* grad, double* loss);
+/* minibatch Adam; loss: class-balanced mean loss of the last epoch */
+int adg_proposer_train(adg_ctx* c, const float* feats, const uint8_t* labels, int64_t n, int epochs, int batch,
+                       float lr, uint64_t seed, double* loss);
+int adg_proposer_predict(adg_ctx* c, const float* feats, int64_t n, float* prob);
+/* features [E][ADG_PROPOSER_INPUTS] and leaf flags of the current grid; eps per instance */
+int adg_proposer_features(adg_ctx* c, const double* eps, float* feats, int32_t* leaf);
+/* Decision log: while enabled, every predict-and-refine step appends the
+ * features of each leaf at the start of the step and whether the refinement
+ * after the prediction gave it children.  File layout (little-endian):
+ * char magic[8] = "ADGDLOG1"; int32 inputs, 0; int64 n; float32 feats[n][inputs];
+ * uint8 labels[n]. */
+int adg_decision_log(adg_ctx* c, int enable);
+int64_t adg_decision_log_size(adg_ctx* c);
+int adg_decision_log_get(adg_ctx* c, float* feats, uint8_t* labels);
+int adg_decision_log_write(adg_ctx* c, const char* path);
 
 /* Storage precision of adg_stage and of adg_run_hyperbolic's fused form
  * (north-star precision form; no reference counterpart, the reference is FP64

@@ -1,0 +1,226 @@
+"""Learned refinement proposer on the GPU: features, gradients and FP16
+inference against the CPU oracle (oracle/proposer_oracle.py), training,
+the decision log, the threshold judge of the proposer modes, and what the
+trained network gains over the method without it.
+
+Tolerances: features are FP32 logarithms (1e-5 absolute); the FP32 gradient
+is compared with the float64 oracle at 1e-4 relative (norm of the
+difference); FP16 inference probabilities at 2e-2 absolute."""
+import os
+
+import numpy as np
+import pytest
+
+import pyadg
+import proposer_oracle as po
+import workloads
+from common import ic_params, problem
+
+pytestmark = pytest.mark.gpu
+
+
+def burgers(nmax=6, level_init=4):
+    s = pyadg.Solution(2, 2, 3, nmax, True, level_init)
+    s.init_adaptive_intp_builtin(ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.08, (0.3, 0.6))]), 1e-4)
+    return s
+
+
+def test_features_match_the_oracle():
+    print("[case] proposer features", flush=True)
+    s = burgers()
+    eps = 1e-4
+    feats, leaf = s.proposer_features(eps)
+    lv, sp, _ = s.grid()
+    want, wleaf = po.features(lv, sp, s.get(pyadg.FIELD_UCOE), eps, 6)
+    assert np.array_equal(leaf, wleaf)
+    assert np.max(np.abs(feats - want)) < 1e-5
+    assert 0 < leaf.sum() < len(leaf)
+
+
+def test_batch_features_use_each_instance_threshold():
+    print("[case] proposer features, batch", flush=True)
+    ws = [workloads.c2_burgers(seed=s, nmax=5, level_init=3, t_final=0.01) for s in (2, 7)]
+    b = workloads.make_batch(ws)
+    eps = [1e-4, 3e-3]
+    feats, leaf = b.proposer_features(eps)
+    for i in range(2):
+        first, count = b.instance_range(i)
+        lv, sp, _ = b.instance_grid(i)
+        want, wleaf = po.features(lv, sp, b.instance_get(i), eps[i], 5)
+        assert np.array_equal(leaf[first:first + count], wleaf)
+        assert np.max(np.abs(feats[first:first + count] - want)) < 1e-5
+
+
+def _random_problem(seed, n=1500):
+    rng = np.random.default_rng(seed)
+    p = (rng.standard_normal(po.NPARAMS) * 0.3).astype(np.float32)
+    x = rng.uniform(-3, 3, (n, po.F)).astype(np.float32)
+    y = rng.integers(0, 2, n).astype(np.uint8)
+    w = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    return p, x, y, w
+
+
+def test_gradient_matches_the_oracle():
+    print("[case] proposer gradient", flush=True)
+    s = burgers(4, 2)
+    p, x, y, w = _random_problem(3)
+    s.set_proposer_params(p)
+    g, loss = s.proposer_gradient(x, y, w)
+    wl, wg = po.loss_and_grad(p.astype(np.float64), x.astype(np.float64), y, w.astype(np.float64))
+    assert abs(loss - wl) < 1e-5 * abs(wl)
+    assert np.linalg.norm(g - wg) < 1e-4 * np.linalg.norm(wg)
+    g1, _ = s.proposer_gradient(x, y)  # unit weights
+    _, wg1 = po.loss_and_grad(p.astype(np.float64), x.astype(np.float64), y)
+    assert np.linalg.norm(g1 - wg1) < 1e-4 * np.linalg.norm(wg1)
+
+
+def test_fp16_inference_matches_the_oracle():
+    print("[case] proposer inference", flush=True)
+    s = burgers(4, 2)
+    p, x, _, _ = _random_problem(5, 4000)
+    s.set_proposer_params(p)
+    prob = s.proposer_predict(x)
+    _, want, _, _ = po.forward(p.astype(np.float64), x.astype(np.float64))
+    assert np.max(np.abs(prob - want)) < 2e-2
+    assert np.mean(np.abs(prob - want)) < 2e-3
+
+
+def test_training_learns_a_rule_and_is_deterministic():
+    print("[case] proposer training", flush=True)
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-2, 2, (20000, po.F)).astype(np.float32)
+    y = ((x[:, 0] + 0.5 * x[:, 1] - 0.3 * x[:, 2]) > 0.8).astype(np.uint8)  # unbalanced classes
+    params = []
+    for _ in range(2):
+        s = burgers(4, 2)
+        s.proposer_init(11)
+        loss0 = s.proposer_gradient(x, y)[1]
+        loss = s.proposer_train(x, y, epochs=8, batch=1024, lr=3e-3, seed=4)
+        prob = s.proposer_predict(x)
+        acc_pos = np.mean(prob[y == 1] > 0.5)
+        acc_neg = np.mean(prob[y == 0] <= 0.5)
+        print(f"[proposer training] loss {loss0:.4f} -> {loss:.4f}, recall {acc_pos:.4f}, specificity {acc_neg:.4f}",
+              flush=True)
+        assert loss < 0.3 * loss0 and acc_pos > 0.95 and acc_neg > 0.95
+        params.append(s.proposer_params())
+    assert np.array_equal(params[0], params[1])
+
+
+def test_decision_log_records_the_predict_step(tmp_path):
+    print("[case] decision log", flush=True)
+    s = burgers()
+    eps = 1e-4
+    f0, leaf0 = s.proposer_features(eps)
+    s.decision_log(True)
+    prob = problem(flux_kind=0, alpha=[1.2, 1.2], wave_speed=[1.0, 1.0], cfl=0.2, t_final=1.0, adaptive=1, eps=eps,
+                   eta=1e-5, max_steps=3, form=1)
+    s.run_hyperbolic(prob)
+    feats, labels = s.decision_log_get()
+    assert len(labels) > leaf0.sum() and 0 < labels.sum() < len(labels)
+    # the first step's samples are the leaves of the initial grid, in element order
+    n0 = int(leaf0.sum())
+    assert np.array_equal(feats[:n0], f0[leaf0 == 1])
+    path = tmp_path / "log.adgdlog"
+    s.decision_log_write(str(path))
+    data = path.read_bytes()
+    assert data[:8] == b"ADGDLOG1"
+    nin, _ = np.frombuffer(data, "<i4", 2, 8)
+    n = int(np.frombuffer(data, "<i8", 1, 16)[0])
+    assert nin == po.F and n == len(labels) and len(data) == 24 + n * po.F * 4 + n
+    assert np.array_equal(np.frombuffer(data, "<f4", n * po.F, 24).reshape(n, po.F), feats)
+    assert np.array_equal(np.frombuffer(data, np.uint8, n, 24 + n * po.F * 4), labels)
+
+
+def _start_leaves(s, nmax):
+    lv, sp, _ = s.grid()
+    _, leaf = po.features(lv, sp, s.get(pyadg.FIELD_UCOE), 1.0, nmax)
+    return {(tuple(l), tuple(j)) for l, j, f in zip(lv.tolist(), sp.tolist(), leaf) if f}
+
+
+def _judge_holds(s, eps, nmax, start):
+    """after an accepted step, every leaf of the start grid whose norm exceeds eps has its children
+    (the rule of the predict-and-refine step the proposer replaces)"""
+    lv, sp, _ = s.grid()
+    u = s.get(pyadg.FIELD_UCOE)
+    _, leaf = po.features(lv, sp, u, eps, nmax)
+    norm = np.sqrt(np.sum(u.reshape(len(lv), -1) ** 2, axis=1))
+    for l, j, f, nrm in zip(lv.tolist(), sp.tolist(), leaf, norm):
+        if f and nrm > eps and (tuple(l), tuple(j)) in start:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("mode", [pyadg.REFINE_CURRENT, pyadg.REFINE_LEARNED])
+def test_judge_refines_every_start_leaf_above_eps(mode):
+    print(f"[case] proposer judge mode={mode}", flush=True)
+    eps = 1e-4
+    s = burgers()
+    if mode == pyadg.REFINE_LEARNED:
+        s.proposer_init(1)  # an untrained network: the judge must still hold
+    s.set_refine_mode(mode, 0.5)
+    for _ in range(3):
+        start = _start_leaves(s, 6)
+        # eta = 0: coarsening removes nothing, so the judge's post-condition is observable
+        st = s.run_hyperbolic(problem(flux_kind=0, alpha=[1.2, 1.2], wave_speed=[1.0, 1.0], cfl=0.2, t_final=1.0,
+                                      adaptive=1, eps=eps, eta=0.0, max_steps=1, form=1))
+        assert st.steps == 1
+        assert _judge_holds(s, eps, 6, start)
+
+
+def _burgers_exact(x, y, t):
+    """u_t + (u^2/2)_x + (u^2/2)_y = 0, u0 = sin(2 pi (x + y)): u = sin(2 pi (x + y - 2 u t)) by Newton
+    (monotone before the shock, t < 1 / (4 pi))"""
+    s = x[:, None] + y[None, :]
+    u = np.sin(2 * np.pi * s)
+    for _ in range(60):
+        g = u - np.sin(2 * np.pi * (s - 2 * u * t))
+        u = u - g / (1 + 4 * np.pi * t * np.cos(2 * np.pi * (s - 2 * u * t)))
+    return u
+
+
+def test_learned_proposer_gains_over_the_methods_without_it():
+    """Smooth 2D Burgers, u0 = sin(2 pi (x + y - shift)), k = 2, P3 Lagrange flux, NMAX 7, eps 1e-4, to
+    t = 0.05 (before the shock).  The network is trained on the decision logs of plain runs with shifts
+    0.0 .. 0.7 and evaluated on shifts 0.15 and 0.55.  Against proposing from the current norms (the
+    method without the network) it sends far fewer steps back to the threshold judge; against the
+    forward-Euler prediction (the reference) it saves the prediction stage, and its L2 error to the exact
+    solution is within 1.1x of the prediction's."""
+    nm, l0, eps, tf = 7, 5, 1e-4, 0.05
+
+    def make(shifts):
+        s = pyadg.Solution(2, 2, 3, nm, True, l0)
+        s.init_batch(len(shifts), True, l0)
+        s.init_adaptive_intp_builtin_batch([ic_params(0, shift=sh) for sh in shifts], eps)
+        return s
+
+    prob = problem(flux_kind=0, alpha=[1.2, 1.2], wave_speed=[1.0, 1.0], cfl=0.2, t_final=tf, adaptive=1, eps=eps,
+                   eta=eps / 10, form=1)
+    b = make([0.1 * i for i in range(8)])
+    b.decision_log(True)
+    b.run_hyperbolic(prob)
+    x, y = b.decision_log_get()
+    net = make([0.0])
+    net.proposer_init(7)
+    loss = net.proposer_train(x, y, epochs=20, batch=1024, lr=2e-3, seed=3)
+    params = net.proposer_params()
+    p = net.proposer_predict(x)
+    print(f"[gain] {len(y)} logged decisions, {int(y.sum())} refined; loss {loss:.4f}, recall at 0.01 "
+          f"{np.mean(p[y == 1] > 0.01):.5f}", flush=True)
+    xm = (np.arange(96) + 0.5) / 96
+    for sh in (0.15, 0.55):
+        ue = _burgers_exact(xm - sh, xm, tf)
+        res = {}
+        for mode, thr in ((pyadg.REFINE_PREDICT, 0.5), (pyadg.REFINE_CURRENT, 0.5), (pyadg.REFINE_LEARNED, 0.01)):
+            s = make([sh])
+            s.set_proposer_params(params)
+            s.set_refine_mode(mode, thr)
+            st = s.run_hyperbolic(prob)
+            err = float(np.sqrt(np.mean((s.eval_tensor(xm) - ue) ** 2)))
+            res[mode] = (st, err)
+            stages = (4 if mode == pyadg.REFINE_PREDICT else 3 + 3 * st.redone / st.steps)
+            print(f"[gain] shift {sh} mode {mode}: steps {st.steps}, redone {st.redone}, RK stages per accepted step "
+                  f"{stages:.2f}, max elements {st.max_elem}, L2 error {err:.3e}", flush=True)
+        plain, cur, learned = res[pyadg.REFINE_PREDICT], res[pyadg.REFINE_CURRENT], res[pyadg.REFINE_LEARNED]
+        assert cur[0].redone >= 8 and learned[0].redone * 4 <= cur[0].redone
+        assert 3 + 3 * learned[0].redone / learned[0].steps < 3.5
+        assert learned[1] <= 1.1 * plain[1]
