@@ -101,7 +101,7 @@ void log_labels(adg::Context& c, const std::vector<double>& eps, const std::vect
     for (int64_t e = c.off[i]; e < c.off[i + 1]; ++e) {
       if (!ll[e]) continue;
       c.log_feats.insert(c.log_feats.end(), &lf[e * adg::kPropIn], &lf[e * adg::kPropIn] + adg::kPropIn);
-      c.log_labels.push_back(std::pow(s[e], 1. / 2.) > eps[i] ? 1 : 0);
+      c.log_labels.push_back(adg::threshold_norm(s[e], eps[i]) > eps[i] ? 1 : 0);
     }
 }
 

@@ -1187,27 +1187,29 @@ void Context::upload_grid() {
   uslot.resize(n * D);
   uhash.resize(n);
   uinst.resize(n);
-  std::vector<int32_t> dsl(n), fb(n), fml(n);
-  for (int i = 0; i < B(); ++i) {
+  std::vector<int32_t> dsl(n), fb(n), fml(n), ls(n, 0);
+  parallel_for(B(), [&](int i) {
     const HostGrid& g = grids[i];
     std::copy(g.slot.begin(), g.slot.end(), uslot.begin() + off[i] * D);
     std::copy(g.hash.begin(), g.hash.end(), uhash.begin() + off[i]);
     std::fill(uinst.begin() + off[i], uinst.begin() + off[i + 1], i);
-  }
+    for (int64_t e = 0; e < g.E; ++e)
+      for (int d = 0; d < D; ++d) ls[off[i] + e] += level_of_slot(g.slot[e * D + d]);
+  });
   for (int d = 0; d < D; ++d) {
-    int64_t fib_off = 0;
-    for (int i = 0; i < B(); ++i) fib_off += grids[i].nfib[d];
+    std::vector<int64_t> fib0(B() + 1, 0);  // first fiber of every instance
+    for (int i = 0; i < B(); ++i) fib0[i + 1] = fib0[i] + grids[i].nfib[d];
+    const int64_t fib_off = fib0[B()];
     if ((fib_off << nmax) > (int64_t{1} << 31) - 1) throw std::runtime_error("fiber map exceeds int32 range");
-    fib_off = 0;
-    for (int i = 0; i < B(); ++i) {
+    parallel_for(B(), [&](int i) {
       const HostGrid& g = grids[i];
+      const int32_t shift = static_cast<int32_t>(fib0[i] << nmax);
       for (int64_t e = 0; e < g.E; ++e) {
         dsl[off[i] + e] = g.dslot[d][e];
-        fb[off[i] + e] = g.fbase[d][e] + static_cast<int32_t>(fib_off << nmax);
+        fb[off[i] + e] = g.fbase[d][e] + shift;
         fml[off[i] + e] = g.fmaxl[d][e];
       }
-      fib_off += g.nfib[d];
-    }
+    });
     upload(g_slot[d], dsl, stream);
     upload(g_fbase[d], fb, stream);
     upload(g_fmaxl[d], fml, stream);
@@ -1223,9 +1225,6 @@ void Context::upload_grid() {
                          g_fbase[d].p, g_slot[d].p, n);
   }
   upload(g_inst, uinst, stream);
-  std::vector<int32_t> ls(n, 0);
-  for (int64_t e = 0; e < n; ++e)
-    for (int d = 0; d < D; ++d) ls[e] += level_of_slot(uslot[e * D + d]);
   upload(g_lsum, ls, stream);
 }
 
@@ -1942,11 +1941,11 @@ int64_t Context::refine_instances(const std::vector<double>& eps) {
   if (static_cast<int>(eps.size()) != B()) throw std::runtime_error("refine: one threshold per instance expected");
   std::vector<double> s(E());
   indicator_sumsq(s.data());
-  for (double& v : s) v = std::pow(v, 1. / 2.);
   std::vector<std::vector<int32_t>> ns(B());
   std::vector<int64_t> added(B(), 0);
   parallel_for(B(), [&](int i) {
-    const std::vector<double> ni(s.begin() + off[i], s.begin() + off[i + 1]);
+    std::vector<double> ni(off[i + 1] - off[i]);
+    for (size_t e = 0; e < ni.size(); ++e) ni[e] = threshold_norm(s[off[i] + e], eps[i]);
     ns[i] = refine_slots(grids[i], ni, eps[i], added[i]);
   });
   int64_t tot = 0;
@@ -1961,11 +1960,11 @@ int64_t Context::coarsen_instances(const std::vector<double>& eta) {
   if (static_cast<int>(eta.size()) != B()) throw std::runtime_error("coarsen: one threshold per instance expected");
   std::vector<double> s(E());
   indicator_sumsq(s.data());
-  for (double& v : s) v = std::pow(v, 1. / 2.);
   std::vector<std::vector<int32_t>> ns(B());
   std::vector<int64_t> removed(B(), 0);
   parallel_for(B(), [&](int i) {
-    const std::vector<double> ni(s.begin() + off[i], s.begin() + off[i + 1]);
+    std::vector<double> ni(off[i + 1] - off[i]);
+    for (size_t e = 0; e < ni.size(); ++e) ni[e] = threshold_norm(s[off[i] + e], eta[i]);
     ns[i] = coarsen_slots(grids[i], ni, eta[i], removed[i]);
   });
   int64_t tot = 0;
@@ -2041,6 +2040,10 @@ void Context::init_adaptive_intp_batch(double (*f)(const double*, int, void*), v
   for (int d = 0; d < D; ++d) { tabs[d] = table(1); flx[d] = 0; }
   partial_sums(tabs, flx, m, k, 1.0, uintp.p, u.p);
   sync();
+  // the host point values only serve the incremental refinement above; keeping
+  // them would make every later grid change copy E*P values on the host
+  std::vector<double>().swap(up_host);
+  std::vector<char>().swap(new_add);
 }
 
 // DGAdapt::init_separable_scalar_sum (source/DGAdapt.cpp:120-147): project

@@ -14,6 +14,7 @@
 // DGAdapt::coarsen, source/DGAdapt.cpp:371-404,675-688,982-1016,1250-1385)
 // expressed as set operations on the table.
 #pragma once
+#include <cmath>
 #include <cstdint>
 #include <vector>
 
@@ -105,6 +106,18 @@ struct HostGrid {
 // indicator (pow(sum of squares, 0.5)).  Returns the new slot list; `added`
 // counts the new elements.
 std::vector<int32_t> refine_slots(const HostGrid& g, const std::vector<double>& norm, double eps, int64_t& added);
+
+// DGAdapt::indicator_norm takes pow(sum of squares, 0.5)
+// (source/DGAdapt.cpp:1018-1030).  glibc's pow and the correctly rounded sqrt
+// differ by at most one ulp, so sqrt decides every comparison with the
+// threshold except within a few ulps of it, where pow is evaluated: the
+// refine / coarsen decisions equal those of pow everywhere.
+inline double threshold_norm(double sumsq, double thr) {
+  const double r = std::sqrt(sumsq);
+  const double a = std::fabs(thr);
+  const double band = 4.0 * (std::nextafter(a, HUGE_VAL) - a);
+  return std::fabs(r - thr) <= band ? std::pow(sumsq, 1. / 2.) : r;
+}
 
 // Children-free elements whose indicator is below eta and whose maximum level
 // is positive are removed repeatedly until none qualifies.

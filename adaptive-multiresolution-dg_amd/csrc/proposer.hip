@@ -497,7 +497,7 @@ int64_t Context::proposer_step(const std::function<void(int, int, double)>& stag
       std::vector<double> s(E());
       indicator_sumsq(s.data());
       for (int i = 0; i < nb; ++i)
-        for (int64_t e = off[i]; e < off[i + 1]; ++e) score[e] = std::pow(s[e], 1. / 2.) > eps[i] ? 1. : 0.;
+        for (int64_t e = off[i]; e < off[i + 1]; ++e) score[e] = threshold_norm(s[e], eps[i]) > eps[i] ? 1. : 0.;
     }
     std::vector<std::vector<int32_t>> ns(nb);
     std::vector<int64_t> added(nb, 0);
@@ -525,7 +525,6 @@ int64_t Context::proposer_step(const std::function<void(int, int, double)>& stag
     HostTimer timer(*this, HOST_ADAPT);
     std::vector<double> s(E());
     indicator_sumsq(s.data());
-    for (double& v : s) v = std::pow(v, 1. / 2.);
     std::vector<std::vector<int32_t>> ns(nb);
     std::vector<int64_t> added(nb, 0);
     parallel_for(nb, [&](int i) {
@@ -533,7 +532,8 @@ int64_t Context::proposer_step(const std::function<void(int, int, double)>& stag
       const HostGrid& g = grids[i];
       std::vector<double> si(g.E, 0.);
       for (int64_t e = 0; e < g.E; ++e)
-        si[e] = (s[off[i] + e] > eps[i] && start_leaves[i].find(HostGrid::pack(&g.slot[e * D], D)) >= 0) ? 1. : 0.;
+        si[e] = (threshold_norm(s[off[i] + e], eps[i]) > eps[i] &&
+                 start_leaves[i].find(HostGrid::pack(&g.slot[e * D], D)) >= 0) ? 1. : 0.;
       ns[i] = refine_slots(g, si, 0.5, added[i]);
     });
     std::vector<char> changed(nb, 0);

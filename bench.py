@@ -104,6 +104,12 @@ def main():
                     help="contexts (HIP streams, host threads) the batch is split over in the chosen form")
     ap.add_argument("--batch", type=int, default=256,
                     help="independent seeded instances solved together in the chosen form (c2, c5)")
+    ap.add_argument("--refine", default="predict", choices=("predict", "learned"),
+                    help="refinement of the chosen form's adaptive steps: forward-Euler prediction (the reference) "
+                         "or the learned proposer with the threshold judge")
+    ap.add_argument("--threshold", type=float, default=0.01, help="proposer probability threshold")
+    ap.add_argument("--train-steps", type=int, default=40, help="time steps of each logged training run")
+    ap.add_argument("--train-samples", type=int, default=2000000, help="cap on the logged decisions trained on")
     ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"),
                     help="storage precision of the chosen form's fused stages (fp32: split FP32 tables, FP64 "
                          "accumulation, RK and refine/coarsen norms)")
@@ -141,7 +147,34 @@ def main():
         c.set_precision(prec)
         if c is not sol:
             c.save_state(0)
+    if batched:
         print(f"[bench] batch init done: {sum(b.num_elem for b in bsols)} elements", file=sys.stderr, flush=True)
+    learned = None
+    if args.refine == "learned" and w.problem is not None and w.problem.adaptive:
+        # decision logs of plain runs of 64 other seeds train the proposer (not timed)
+        t0 = time.perf_counter()
+        train = workloads.make_batch([workload(10001 + i) for i in range(64)])
+        train.decision_log(True)
+        tp = pyadg.Problem.from_buffer_copy(w.problem)
+        tp.form, tp.max_steps = 1, args.train_steps
+        train.run_hyperbolic(tp)
+        x, y = train.decision_log_get()
+        if len(y) > args.train_samples:  # a seeded subsample bounds the training time
+            keep = np.sort(np.random.default_rng(args.seed).choice(len(y), args.train_samples, replace=False))
+            x, y = x[keep], y[keep]
+        print(f"[bench] proposer training on {len(y)} logged decisions", file=sys.stderr, flush=True)
+        train.proposer_init(args.seed)
+        loss = train.proposer_train(x, y, epochs=20, batch=1024, lr=2e-3, seed=args.seed)
+        params = train.proposer_params()
+        del train
+        for c in chosen_sols:
+            c.set_proposer_params(params)
+            c.set_refine_mode(pyadg.REFINE_LEARNED, args.threshold)
+        learned = {"training_samples": int(len(y)), "refined_fraction": float(y.mean()), "final_loss": loss,
+                   "training_seconds": time.perf_counter() - t0, "threshold": args.threshold,
+                   "training_steps_per_instance": args.train_steps}
+        print(f"[bench] proposer trained on {len(y)} logged decisions in {learned['training_seconds']:.1f} s",
+              file=sys.stderr, flush=True)
 
     def measure(sols, form):
         """Time --steps full solves (after --warmup) of one form; returns figures and HIP-event profile.
@@ -162,8 +195,9 @@ def main():
         results = [[] for _ in sols]
 
         def worker(i, n):
-            for _ in range(n):
+            for j in range(n):
                 results[i].append(solve(i))
+                print(f"[bench] form {form}, context {i}: solve {j + 1}/{n} done", file=sys.stderr, flush=True)
 
         def run_all(n):
             th = [threading.Thread(target=worker, args=(i, n)) for i in range(len(sols))]
@@ -185,6 +219,7 @@ def main():
             s.sync()
         sec = time.perf_counter() - t0
         dof_stages = sum(st.dof_stages for r in results for st in r)
+        redone = sum(st.redone for r in results for st in r)
         time_steps = max(sum(st.steps for st in r) for r in results)
         max_elem = sum(max(st.max_elem for st in r) for r in results)
         prof = {k: {"ms": 0.0, "count": 0, "bytes": 0.0} for k in sols[0].profile_get()}
@@ -204,14 +239,14 @@ def main():
               f"({time_steps} time steps) in {sec:.3f} s, {dof_stages / sec:.4g} DoF-stage/s", file=sys.stderr,
               flush=True)
         return {"value": dof_stages / sec, "sec": sec, "dof_stages": dof_stages, "time_steps": time_steps,
-                "max_elem": max_elem, "prof": prof, "host": host, "u": u, "keys": keys,
+                "max_elem": max_elem, "prof": prof, "host": host, "u": u, "keys": keys, "redone": redone,
                 "sim_time": results[0][-1].final_time}
 
     plain = measure(sol, 0)
     fused = plain if w.linear else measure(sol, 1)
     if batched:
         chosen = measure(bsols, 1)
-    elif w.linear or prec == pyadg.PREC_FP64:
+    elif w.linear or (prec == pyadg.PREC_FP64 and learned is None):
         chosen = fused
     else:
         chosen = measure(chosen_sols, 1)
@@ -271,6 +306,9 @@ def main():
                        time_steps_per_solve=plain["time_steps"] // max(args.steps, 1),
                        time_step_cap=args.time_steps or None,
                        max_elements=int(chosen["max_elem"]), dof_stage_updates=int(chosen["dof_stages"]),
+                       refine=args.refine if learned else "predict", learned_proposer=learned,
+                       chosen_time_steps_per_solve=chosen["time_steps"] // max(args.steps, 1),
+                       judge_redone_instance_steps=int(chosen["redone"]),
                        sim_time=chosen["sim_time"]),
         "roofline": roofline, "cpu_baseline": cpu,
     }
