@@ -269,8 +269,18 @@ def main():
         avg_ms = pc["ms"] / pc["count"]
         avg_bytes = pc["bytes"] / pc["count"]
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+        # HBM bytes per launch: the traffic / algorithmic ratio measured by the FETCH_SIZE and
+        # WRITE_SIZE counter passes recorded under profiles/, applied to this run's launches
+        traffic, traffic_src = None, None
+        tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_k_rhs_final_c2.json")
+        if args.workload == "c2" and names[cat] == "k_rhs_final" and os.path.exists(tf):
+            with open(tf) as f:
+                t = json.load(f)
+            traffic = t["traffic_over_algorithmic"] * avg_bytes
+            traffic_src = "profiles/traffic_k_rhs_final_c2.json (ratio %.2f)" % t["traffic_over_algorithmic"]
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": names[cat],
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                    "kernel": names[cat],
                     "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes, "launches": pc["count"],
                     "kernel_ms_by_category": {k: round(v["ms"], 3) for k, v in prof.items()},
                     "host_ms_inclusive": {k: round(v, 3) for k, v in chosen["host"].items()}}
