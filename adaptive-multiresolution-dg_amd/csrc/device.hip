@@ -128,6 +128,21 @@ constexpr int kElemBlock = 512;
 
 inline unsigned nblocks(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// The gather kernels renumber their blocks so that every XCD works on one
+// contiguous range of elements (for a batch: a contiguous range of whole
+// instances), whose fiber neighbours then stay in that XCD's L2.  The
+// renumbering is a bijection of block ids and changes no arithmetic.  The
+// phase kernels keep the hardware order: their work items are task-major and
+// tasks differ in cost (ancestor vs descendant walks), so contiguous ranges
+// would load some XCDs far more than others.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned b = blockIdx.x, n = gridDim.x;
+  const unsigned x = b % kXcds, i = b / kXcds, per = n / kXcds, extra = n % kXcds;
+  return x * per + min(x, extra) + i;
+}
+
 
 // ---------------------------------------------------------------------------
 // One 1D sweep (FastMultiplyLU::transform_1D, source/FastMultiplyLU.cpp:436-512).
@@ -154,7 +169,7 @@ struct SweepArgs {
 
 template <int PF1, int PT1>
 __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t gid = static_cast<int64_t>(xcd_block()) * blockDim.x + threadIdx.x;
   if (gid >= a.E * a.R) return;
   const int64_t e = gid / a.R;
   int r = static_cast<int>(gid - e * a.R);
@@ -310,7 +325,7 @@ __device__ __forceinline__ void surplus_madd(float& coe, const SurplusArgs& a, i
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t gid = static_cast<int64_t>(xcd_block()) * blockDim.x + threadIdx.x;
   const int64_t per = a.E * a.P;
   if (gid >= per * a.ncomp) return;
   const int64_t c = gid / per;
@@ -680,7 +695,7 @@ __global__ __launch_bounds__(kFinalBlock) void k_up_final_pt(const TermDev* term
                                                            const double* vcoef, int kind, double s, int ncomp, int cap) {
   __shared__ double vals[kFinalBlock];
   const int t = threadIdx.x / per_block, lo = threadIdx.x - t * per_block;
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * per_block + lo;
+  const int64_t gid = static_cast<int64_t>(xcd_block()) * per_block + lo;
   const bool live = gid < E * P;
   int64_t e = 0;
   int q = 0;
@@ -714,7 +729,7 @@ __global__ __launch_bounds__(kFinalBlock) void k_rhs_final_pt(const TermDev* ter
                                                             const double* dtv, const int32_t* inst, int cap) {
   __shared__ double vals[kFinalBlock];
   const int t = threadIdx.x / per_block, lo = threadIdx.x - t * per_block;
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * per_block + lo;
+  const int64_t gid = static_cast<int64_t>(xcd_block()) * per_block + lo;
   const bool live = gid < E * K;
   int64_t e = 0;
   int q = 0;
@@ -872,7 +887,7 @@ __global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, i
   const int per_term = EB * nch, nslot = blockDim.x / per_term;
   const int slot = tid / per_term, rem = tid - slot * per_term;
   const int el = rem / nch, c = rem - (rem / nch) * nch;
-  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * EB;
+  const int64_t e0 = static_cast<int64_t>(xcd_block()) * EB;
   int so[kMaxDim], osz[kMaxDim];
   for (int d = 0; d < D; ++d) osz[d] = PT1;
   strides_dev(osz, D, so);
@@ -909,7 +924,7 @@ __global__ __launch_bounds__(kElemBlock) void k_rhs_final(const TermDev* terms, 
   const int per_term = EB * nch, nslot = blockDim.x / per_term;
   const int slot = tid / per_term, rem = tid - slot * per_term;
   const int el = rem / nch, c = rem - (rem / nch) * nch;
-  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * EB;
+  const int64_t e0 = static_cast<int64_t>(xcd_block()) * EB;
   int so[kMaxDim], osz[kMaxDim];
   for (int d = 0; d < D; ++d) osz[d] = PFK;
   strides_dev(osz, D, so);
