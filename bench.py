@@ -102,7 +102,7 @@ def main():
                     help="cap the time steps of every solve (0: run to the workload's final time)")
     ap.add_argument("--streams", type=int, default=2,
                     help="contexts (HIP streams, host threads) the batch is split over in the chosen form")
-    ap.add_argument("--batch", type=int, default=256,
+    ap.add_argument("--batch", type=int, default=1024,
                     help="independent seeded instances solved together in the chosen form (c2, c5)")
     ap.add_argument("--refine", default="predict", choices=("predict", "learned"),
                     help="refinement of the chosen form's adaptive steps: forward-Euler prediction (the reference) "
@@ -283,6 +283,8 @@ def main():
                     "kernel": names[cat],
                     "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes, "launches": pc["count"],
                     "kernel_ms_by_category": {k: round(v["ms"], 3) for k, v in prof.items()},
+                    "avg_algorithmic_bytes_by_category": {k: (v["bytes"] / v["count"] if v["count"] else 0.)
+                                                          for k, v in prof.items()},
                     "host_ms_inclusive": {k: round(v, 3) for k, v in chosen["host"].items()}}
 
     cpu = None
