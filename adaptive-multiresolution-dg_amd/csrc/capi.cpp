@@ -284,6 +284,21 @@ int adg_lagr_points(adg_ctx* h, double* x) {
   });
 }
 
+int adg_viscosity_elements(adg_ctx* h, double shock_kappa, int64_t* count, int32_t* flags) {
+  return guard(h, [&](adg::Context& c) {
+    const int64_t n = c.viscosity_elements(shock_kappa);
+    if (count) *count = n;
+    if (flags) {
+      std::fill(flags, flags + c.E(), 0);
+      for (int64_t e : c.visc_elems) flags[e] = 1;
+    }
+  });
+}
+
+int adg_viscosity_flux(adg_ctx* h, double coefficient, int overlap) {
+  return guard(h, [&](adg::Context& c) { c.viscosity_flux(coefficient, overlap != 0); });
+}
+
 int adg_set_refine_mode(adg_ctx* h, int mode, double threshold) {
   return guard(h, [&](adg::Context& c) {
     if (mode != ADG_REFINE_PREDICT && mode != ADG_REFINE_LEARNED && mode != ADG_REFINE_CURRENT)

@@ -227,6 +227,19 @@ class Context {
   DevBuf<double> u_next, fused_tmp;
   int64_t grid_version = 0;
 
+  // ---- artificial viscosity, matrix-free (viscosity_option 3 of the
+  // reference's Burgers driver) ----
+  // DGAdapt::update_viscosity_element: childless elements at the finest level
+  // whose norm exceeds 2^((1 - k) |l|_1) by more than kappa decades
+  int64_t viscosity_elements(double kappa);
+  std::vector<int64_t> visc_elems;  // union element indices
+  // LargViscosInterpolation::support_gradu_Lagr_fast on the Lagrange family:
+  // fucoe[d] = surplus transform of coef * du/dx_d at the points inside a
+  // viscosity element's support (counted with multiplicity when overlap)
+  void viscosity_flux(double coef, bool overlap);
+  void eval_der_up(int d0);  // FastLagrIntp::eval_der_up_Lagr
+  DevBuf<double> t_pt_d1, visc_w;
+
   // ---- learned refinement proposer (proposer.hip) ----
   // Features of every element for the proposer, from the current u:
   // [0] log10(norm / eps) / 4 of the element, [1 + 2d], [2 + 2d] the same for

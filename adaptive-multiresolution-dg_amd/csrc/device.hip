@@ -2061,6 +2061,121 @@ void Context::init_adaptive_intp_batch(double (*f)(const double*, int, void*), v
   std::vector<char>().swap(new_add);
 }
 
+// fp = coef * du/dx at the points with weight w > 0: w counts the viscosity
+// supports holding the point (overlap) or marks it (w = 1)
+__global__ __launch_bounds__(kBlock) void k_visc_points(const double* up, const double* w, double* fp, int64_t n,
+                                                         double coef, int overlap) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double wi = w[i];
+  fp[i] = wi == 0. ? 0. : (overlap ? wi * coef * up[i] : coef * up[i]);
+}
+
+void Context::eval_der_up(int d0) {
+  if (interp != 0) throw std::runtime_error("derivative point values: the Lagrange family is needed");
+  if (t_pt_d1.cap == 0)
+    upload(t_pt_d1, to_blocks(alpt_at_lagr_points(alpt, lagr, 1), 1 << nmax, k + 1, m + 1), stream);
+  if (E() > 0) hipLaunchKernelGGL(k_fill, dim3(nblocks(E() * P)), dim3(kBlock), 0, stream, up.p, E() * P, 0.);
+  SweepTable tabs[kMaxDim];
+  int flx[kMaxDim];
+  for (int d = 0; d < D; ++d) { tabs[d] = table(0); flx[d] = 0; }
+  tabs[d0] = {t_pt_d1.p, 1 << nmax};
+  partial_sums(tabs, flx, k, m, 1.0, u.p, up.p);
+}
+
+int64_t Context::viscosity_elements(double kappa) {
+  std::vector<double> s(E());
+  indicator_sumsq(s.data());
+  visc_elems.clear();
+  for (int i = 0; i < B(); ++i) {
+    const HostGrid& g = grids[i];
+    for (int64_t e = 0; e < g.E; ++e) {
+      int32_t sl[kMaxDim];
+      int lmax = 0, lsum = 0, have = 0;
+      for (int d = 0; d < D; ++d) {
+        const int l = level_of_slot(g.slot[e * D + d]);
+        lmax = std::max(lmax, l);
+        lsum += l;
+      }
+      if (lmax != nmax) continue;
+      for (int d = 0; d < D && have == 0; ++d) {
+        for (int q = 0; q < D; ++q) sl[q] = g.slot[e * D + q];
+        const int sd = sl[d], l = level_of_slot(sd);
+        if (l == 0) { sl[d] = 1; have += g.find(sl) >= 0; }
+        else if (l < nmax) {
+          sl[d] = 2 * sd; have += g.find(sl) >= 0;
+          sl[d] = 2 * sd + 1; have += g.find(sl) >= 0;
+        }
+      }
+      if (have != 0) continue;
+      const double num_l2_norm = std::pow(s[off[i] + e], 1. / 2.);
+      const double theory_bound = std::pow(2., (-k + 1.0) * lsum);
+      if (std::log10(num_l2_norm) - std::log10(theory_bound) > kappa) visc_elems.push_back(off[i] + e);
+    }
+  }
+  return static_cast<int64_t>(visc_elems.size());
+}
+
+void Context::viscosity_flux(double coef, bool overlap) {
+  LagrangeScope lagrange_scope(*this);
+  const int p1 = m + 1;
+  std::vector<double> w(static_cast<size_t>(E()) * P, 0.);
+  // supports of the viscosity elements, per instance
+  std::vector<std::vector<double>> box(B());
+  for (int64_t v : visc_elems)
+    for (int d = 0; d < D; ++d) {
+      double xl, xr;
+      cell_support(level(v, d), suppt_of_slot(uslot[v * D + d]), xl, xr);
+      box[uinst[v]].push_back(xl);
+      box[uinst[v]].push_back(xr);
+    }
+  parallel_for(B(), [&](int i) {
+    const std::vector<double>& bx = box[i];
+    const int nv = static_cast<int>(bx.size() / (2 * D));
+    if (nv == 0) return;
+    for (int64_t e = off[i]; e < off[i + 1]; ++e) {
+      // DGSolution::update_viscosity_intersect_element: supports meet in every dimension
+      double exl[kMaxDim], exr[kMaxDim];
+      for (int d = 0; d < D; ++d) cell_support(level(e, d), suppt_of_slot(uslot[e * D + d]), exl[d], exr[d]);
+      bool any = false;
+      for (int v = 0; v < nv && !any; ++v) {
+        bool meet = true;
+        for (int d = 0; d < D && meet; ++d) {
+          const double vl = bx[(v * D + d) * 2], vr = bx[(v * D + d) * 2 + 1];
+          meet = !((exl[d] >= vr) || (exr[d] <= vl));
+        }
+        any = meet;
+      }
+      if (!any) continue;
+      for (int q = 0; q < P; ++q) {
+        double pos[kMaxDim];
+        int rem = q;
+        for (int d = D - 1; d >= 0; --d) {
+          pos[d] = lagr.point(uslot[e * D + d] * p1 + rem % p1);
+          rem /= p1;
+        }
+        int num = 0;
+        for (int v = 0; v < nv; ++v) {
+          bool in = true;
+          for (int d = 0; d < D && in; ++d) in = pos[d] >= bx[(v * D + d) * 2] && pos[d] <= bx[(v * D + d) * 2 + 1];
+          if (in) { ++num; if (!overlap) break; }
+        }
+        w[e * P + q] = num;
+      }
+    }
+  });
+  upload(visc_w, w, stream);
+  const int64_t n = E() * P;
+  for (int d0 = 0; d0 < D; ++d0) {
+    eval_der_up(d0);
+    if (n > 0)
+      hipLaunchKernelGGL(k_visc_points, dim3(nblocks(n)), dim3(kBlock), 0, stream, up.p, visc_w.p, fp.p + d0 * n, n,
+                         coef, overlap ? 1 : 0);
+  }
+  same_flux = false;
+  surplus_transform(fp.p, fucoe.p, interA.p, interB.p, D);
+}
+
 // DGAdapt::init_separable_scalar_sum (source/DGAdapt.cpp:120-147): project
 // sum_i prod_d f_i(x_d) onto the grid, then refine every leaf whose
 // coefficient norm exceeds eps with separably initialised children

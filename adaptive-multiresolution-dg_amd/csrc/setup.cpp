@@ -433,7 +433,7 @@ OperatorTables::OperatorTables(const Basis1D& U, const Basis1D& V, bool with_flu
   }
 }
 
-std::vector<double> alpt_at_lagr_points(const Basis1D& alpt, const Basis1D& lagr) {
+std::vector<double> alpt_at_lagr_points(const Basis1D& alpt, const Basis1D& lagr, int deriv) {
   const int na = alpt.size(), nl = lagr.size();
   std::vector<double> t(static_cast<size_t>(na) * nl, 0.);
   for (int m1 = 0; m1 < nl; ++m1) {
@@ -443,7 +443,7 @@ std::vector<double> alpt_at_lagr_points(const Basis1D& alpt, const Basis1D& lagr
       alpt.decode(m2, n, j, p);
       double xl, xr;
       cell_support(n, j, xl, xr);
-      t[static_cast<size_t>(m2) * nl + m1] = (pos < xl || pos > xr) ? 0. : alpt.val(m2, pos, 0, 0);
+      t[static_cast<size_t>(m2) * nl + m1] = (pos < xl || pos > xr) ? 0. : alpt.val(m2, pos, deriv, 0);
     }
   }
   return t;

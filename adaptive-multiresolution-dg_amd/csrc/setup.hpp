@@ -79,7 +79,9 @@ struct OperatorTables {
 // T[alpt_order * n_lagr + lagr_order] = value of Alpert basis at the Lagrange
 // point (0 outside the closed support); the transpose that FastLagrIntp builds
 // from LagrInterpolation::Lag_pt_Alpt_1D (source/FastMultiplyLU.cpp:1339-1360).
-std::vector<double> alpt_at_lagr_points(const Basis1D& alpt, const Basis1D& lagr);
+// deriv = 1: the first derivative (LagrInterpolation::eval_Lag_pt_at_Alpt_1D_d1,
+// source/Interplation.cpp:104-187), the table of FastLagrIntp::eval_der_up_Lagr.
+std::vector<double> alpt_at_lagr_points(const Basis1D& alpt, const Basis1D& lagr, int deriv = 0);
 // The same for the Hermite degrees of freedom: value or first derivative of
 // the Alpert basis at the Hermite point (HermInterpolation::eval_Her_pt_at_Alpt_1D,
 // source/Interplation.cpp:1886-1976).

@@ -81,6 +81,8 @@ def lib():
             "adg_set_interpolation": (i32, [vp, i32]),
             "adg_set_precision": (i32, [vp, i32]),
             "adg_set_refine_mode": (i32, [vp, i32, dbl]),
+            "adg_viscosity_elements": (i32, [vp, dbl, P(i64), I32P]),
+            "adg_viscosity_flux": (i32, [vp, dbl, i32]),
             "adg_proposer_init": (i32, [vp, ctypes.c_uint64]),
             "adg_proposer_set_params": (i32, [vp, FP]),
             "adg_proposer_get_params": (i32, [vp, FP]),
@@ -262,6 +264,17 @@ class Solution:
         """INTERP_LAGRANGE or INTERP_HERMITE (P3, needs m = 3) for eval_up, flux,
         fp_to_coe, rhs_lagr_*, stage and run_hyperbolic."""
         self._check(lib().adg_set_interpolation(self._h, family))
+
+    # ---- artificial viscosity (matrix-free) ----
+    def viscosity_elements(self, kappa):
+        """count and per-element flags of DGAdapt::update_viscosity_element"""
+        n = ctypes.c_int64()
+        flags = np.zeros(self.num_elem, np.int32)
+        self._check(lib().adg_viscosity_elements(self._h, kappa, ctypes.byref(n), _ip(flags)))
+        return n.value, flags
+
+    def viscosity_flux(self, coef, overlap=False):
+        self._check(lib().adg_viscosity_flux(self._h, coef, 1 if overlap else 0))
 
     # ---- learned refinement proposer ----
     def set_refine_mode(self, mode, threshold=0.5):

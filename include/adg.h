@@ -293,6 +293,24 @@ int adg_set_instance_thresholds(adg_ctx* c, const double* eps, const double* eta
  * keeps the fastest. */
 int adg_set_kernel_variant(adg_ctx* c, int variant);
 
+/* Artificial viscosity in its matrix-free form (viscosity_option 3 of
+ * example/02_hyperbolic_05_burgers_adapt.cpp:440-458).
+ * adg_viscosity_elements: DGAdapt::update_viscosity_element
+ * (source/DGAdapt.cpp:727-759) for every instance: childless elements at the
+ * finest level whose norm exceeds the smooth-solution bound 2^((1 - k)|l|_1)
+ * by more than shock_kappa decades; count = DGSolution::num_artific_visc_elem,
+ * flags (may be null) 1 per viscosity element.
+ * adg_viscosity_flux: LargViscosInterpolation::support_gradu_Lagr_fast
+ * (source/Interplation.cpp:4176-4189,579-640): for each d, coefficient * du/dx_d
+ * at the Lagrange points inside a viscosity element's closed support (times the
+ * number of such supports when overlap != 0), on the elements whose supports
+ * meet a viscosity element's in every dimension, zero elsewhere; then the
+ * surplus transform into fucoe[d].  Always on the Lagrange family; the volume
+ * term (HyperbolicDiffFluxLagrRHS::rhs_vol_scalar) is adg_rhs_lagr_vol with the
+ * Lagrange family selected. */
+int adg_viscosity_elements(adg_ctx* c, double shock_kappa, int64_t* count, int32_t* flags);
+int adg_viscosity_flux(adg_ctx* c, double coefficient, int overlap);
+
 /* Learned refinement proposer (north-star learned part; no reference
  * counterpart).  The reference refines by predicting u^{n+1} with one
  * forward-Euler stage and giving every leaf whose predicted norm exceeds eps

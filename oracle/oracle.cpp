@@ -559,6 +559,75 @@ struct or_ctx {
     partial_sums(mats, cols, flx, k, m, 1.0, [](OElem& e) { return e.ucoe; }, [](OElem& e) -> std::vector<double>& { return e.up; });
   }
 
+  // ---- artificial viscosity, matrix-free option (viscosity_option 3 of
+  // example/02_hyperbolic_05_burgers_adapt.cpp:440-458) ----
+  std::vector<double> pt_d1;  // derivative of the Alpert basis at the Lagrange points
+  std::vector<OElem*> visc;   // DGSolution::viscosity_element
+
+  // DGAdapt::update_viscosity_element (source/DGAdapt.cpp:727-759): childless
+  // elements at the finest level whose norm exceeds the smooth-solution bound
+  // 2^((1 - k) |l|_1) by more than kappa decades
+  int64_t viscosity_elements(double kappa) {
+    visc.clear();
+    for (OElem* e : sorted_elems()) {
+      if (!e->chd.empty()) continue;
+      if (*std::max_element(e->level.begin(), e->level.end()) != nmax) continue;
+      const double num_l2_norm = norm2(e->ucoe);
+      const int level_l1_norm = std::accumulate(e->level.begin(), e->level.end(), 0);
+      const double theory_bound = std::pow(2., (-k + 1.0) * level_l1_norm);
+      if (std::log10(num_l2_norm) - std::log10(theory_bound) > kappa) visc.push_back(e);
+    }
+    return static_cast<int64_t>(visc.size());
+  }
+
+  // LargViscosInterpolation::support_gradu_Lagr_fast (source/Interplation.cpp:
+  // 4176-4189, 579-640): coefficient * du/dx_d at the Lagrange points that lie in
+  // (the closed support of) a viscosity element, on the elements whose supports
+  // meet a viscosity element's in every dimension (DGSolution::
+  // update_viscosity_intersect_element, source/DGSolution.cpp:1339-1347); zero
+  // elsewhere; then the surplus transform of every component.
+  void viscosity_flux(double coef, bool overlap) {
+    if (pt_d1.empty()) pt_d1 = adg::alpt_at_lagr_points(alpt, lagr, 1);
+    for (auto& it : dg)
+      for (int d = 0; d < D; ++d) std::fill(it.second.fp[d].begin(), it.second.fp[d].end(), 0.);
+    std::vector<OElem*> inter;
+    for (OElem* e : sorted_elems()) {
+      for (OElem* v : visc) {
+        bool meet = true;
+        for (int d = 0; d < D && meet; ++d) meet = intersect(e->xl[d], e->xr[d], v->xl[d], v->xr[d]);
+        if (meet) { inter.push_back(e); break; }
+      }
+    }
+    for (int d0 = 0; d0 < D; ++d0) {
+      for (auto& it : dg) std::fill(it.second.up.begin(), it.second.up.end(), 0.);
+      std::vector<const double*> mats(D, pt.data());
+      mats[d0] = pt_d1.data();
+      std::vector<int> cols(D, lagr.size()), flx(D, 0);
+      partial_sums(mats, cols, flx, k, m, 1.0, [](OElem& e) { return e.ucoe; },
+                   [](OElem& e) -> std::vector<double>& { return e.up; });
+      for (OElem* e : inter) {
+        std::vector<int> idx(D, 0);
+        std::vector<double> pos(D);
+        for (int q = 0; q < P; ++q) {
+          for (int d = 0; d < D; ++d) pos[d] = lagr.point(e->ord[d] * (m + 1) + idx[d]);
+          int num = 0;
+          for (OElem* v : visc) {
+            bool in = true;
+            for (int d = 0; d < D && in; ++d) in = pos[d] >= v->xl[d] && pos[d] <= v->xr[d];
+            if (in) { ++num; if (!overlap) break; }
+          }
+          if (!overlap) e->fp[d0][q] = num ? coef * e->up[q] : 0.;
+          else e->fp[d0][q] = num ? num * coef * e->up[q] : 0.;
+          for (int d = D - 1; d >= 0; --d) { if (++idx[d] < m + 1) break; idx[d] = 0; }
+        }
+      }
+    }
+    const int keep = interp;
+    interp = 0;
+    fp_to_coe();
+    interp = keep;
+  }
+
   void rhs_lagr(bool flux_term) {
     for (int d = 0; d < D; ++d) {
       std::vector<const double*> mats(D);
@@ -883,6 +952,17 @@ void or_sweep(or_ctx* c, int table, int flx, int lu, int dim, const int32_t* siz
 }
 
 void or_set_interpolation(or_ctx* c, int family) { c->set_interp(family); }
+int64_t or_viscosity_elements(or_ctx* c, double kappa, int32_t* flags) {
+  const int64_t n = c->viscosity_elements(kappa);
+  if (flags) {
+    auto v = c->sorted_elems();
+    std::unordered_map<OElem*, int> isv;
+    for (OElem* e : c->visc) isv[e] = 1;
+    for (size_t i = 0; i < v.size(); ++i) flags[i] = isv.count(v[i]) ? 1 : 0;
+  }
+  return n;
+}
+void or_viscosity_flux(or_ctx* c, double coef, int overlap) { c->viscosity_flux(coef, overlap != 0); }
 void or_eval_up(or_ctx* c) { c->eval_up(); }
 void or_set_level_sum_cap(or_ctx* c, int cap) { c->cap = cap; }
 void or_flux(or_ctx* c, int kind, const double* params) { c->flux(kind, params); }

@@ -36,6 +36,10 @@ void or_sweep(or_ctx* c, int table, int flx, int lu, int dim, const int32_t* siz
  * and 1, dim <= 2 for the flux) */
 void or_set_interpolation(or_ctx* c, int family);
 void or_eval_up(or_ctx* c);                                   /* FastLagrIntp::eval_up_Lagr */
+/* DGAdapt::update_viscosity_element; flags (may be null): 1 per viscosity element, hash order */
+int64_t or_viscosity_elements(or_ctx* c, double kappa, int32_t* flags);
+/* LargViscosInterpolation::support_gradu_Lagr_fast (Lagrange family): fucoe_intp[d] */
+void or_viscosity_flux(or_ctx* c, double coef, int overlap);
 /* _coarse_grid variants: sweeps and assembled operators skip elements whose
  * level sum exceeds cap (negative: no cap). */
 void or_set_level_sum_cap(or_ctx* c, int cap);

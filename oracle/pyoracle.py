@@ -37,6 +37,8 @@ def lib():
             "or_sweep": (None, [vp, i32, i32, i32, i32, P(ctypes.c_int32), P(ctypes.c_int32), dbl, P(dbl), P(dbl)]),
             "or_eval_up": (None, [vp]),
             "or_set_interpolation": (None, [vp, i32]),
+            "or_viscosity_elements": (i64, [vp, dbl, P(ctypes.c_int32)]),
+            "or_viscosity_flux": (None, [vp, dbl, i32]),
             "or_set_level_sum_cap": (None, [vp, ctypes.c_int]),
             "or_flux": (None, [vp, i32, P(dbl)]),
             "or_fp_to_coe": (None, [vp]),
@@ -144,6 +146,15 @@ class Oracle:
         out = np.zeros(self.num_elem * int(np.prod(so)))
         lib().or_sweep(self._h, table, int(flx), lu, dim, _ip(si), _ip(so), coef, _dp(x), _dp(out))
         return out
+
+    def viscosity_elements(self, kappa):
+        """count and per-element flags (hash order) of DGAdapt::update_viscosity_element"""
+        flags = np.zeros(self.num_elem, np.int32)
+        n = lib().or_viscosity_elements(self._h, kappa, flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        return int(n), flags
+
+    def viscosity_flux(self, coef, overlap=False):
+        lib().or_viscosity_flux(self._h, coef, 1 if overlap else 0)
 
     def set_interpolation(self, family):
         """0 Lagrange, 1 Hermite P3 (needs m = 3)."""
