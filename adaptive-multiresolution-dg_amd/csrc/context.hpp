@@ -232,7 +232,10 @@ class Context {
   // DGAdapt::update_viscosity_element: childless elements at the finest level
   // whose norm exceeds 2^((1 - k) |l|_1) by more than kappa decades
   int64_t viscosity_elements(double kappa);
-  std::vector<int64_t> visc_elems;  // union element indices
+  std::vector<int64_t> visc_elems;  // union element indices at the time of the update
+  // the same elements by instance and packed slot vector: they stay valid when
+  // refinement renumbers the elements (the reference holds Element pointers)
+  std::vector<std::vector<uint64_t>> visc_keys;
   // LargViscosInterpolation::support_gradu_Lagr_fast on the Lagrange family:
   // fucoe[d] = surplus transform of coef * du/dx_d at the points inside a
   // viscosity element's support (counted with multiplicity when overlap)

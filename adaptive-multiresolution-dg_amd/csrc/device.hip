@@ -2087,6 +2087,7 @@ int64_t Context::viscosity_elements(double kappa) {
   std::vector<double> s(E());
   indicator_sumsq(s.data());
   visc_elems.clear();
+  visc_keys.assign(B(), {});
   for (int i = 0; i < B(); ++i) {
     const HostGrid& g = grids[i];
     for (int64_t e = 0; e < g.E; ++e) {
@@ -2110,7 +2111,10 @@ int64_t Context::viscosity_elements(double kappa) {
       if (have != 0) continue;
       const double num_l2_norm = std::pow(s[off[i] + e], 1. / 2.);
       const double theory_bound = std::pow(2., (-k + 1.0) * lsum);
-      if (std::log10(num_l2_norm) - std::log10(theory_bound) > kappa) visc_elems.push_back(off[i] + e);
+      if (std::log10(num_l2_norm) - std::log10(theory_bound) > kappa) {
+        visc_elems.push_back(off[i] + e);
+        visc_keys[i].push_back(HostGrid::pack(&g.slot[e * D], D));
+      }
     }
   }
   return static_cast<int64_t>(visc_elems.size());
@@ -2122,12 +2126,17 @@ void Context::viscosity_flux(double coef, bool overlap) {
   std::vector<double> w(static_cast<size_t>(E()) * P, 0.);
   // supports of the viscosity elements, per instance
   std::vector<std::vector<double>> box(B());
-  for (int64_t v : visc_elems)
-    for (int d = 0; d < D; ++d) {
-      double xl, xr;
-      cell_support(level(v, d), suppt_of_slot(uslot[v * D + d]), xl, xr);
-      box[uinst[v]].push_back(xl);
-      box[uinst[v]].push_back(xr);
+  for (int i = 0; i < B() && i < static_cast<int>(visc_keys.size()); ++i)
+    for (uint64_t key : visc_keys[i]) {
+      const int32_t le = grids[i].index.find(key);
+      if (le < 0) continue;  // removed since the update
+      const int64_t v = off[i] + le;
+      for (int d = 0; d < D; ++d) {
+        double xl, xr;
+        cell_support(level(v, d), suppt_of_slot(uslot[v * D + d]), xl, xr);
+        box[i].push_back(xl);
+        box[i].push_back(xr);
+      }
     }
   parallel_for(B(), [&](int i) {
     const std::vector<double>& bx = box[i];

@@ -2,9 +2,10 @@
 // reference class API: the default path of
 // example/02_hyperbolic_05_burgers_adapt.cpp (separable initial value
 // sin(2 pi (x + y)), same-flux Hermite interpolation with is_intp {true, false},
-// forward-Euler prediction, refine, SSP RK3, coarsen; no artificial
-// viscosity), then the error against BurgersExact.  Runs on the GPU through
-// the facade.
+// forward-Euler prediction, refine, SSP RK3, coarsen), with its artificial
+// viscosity option 3 (shock detector, viscous flux interpolated on the Lagrange
+// family, explicit SSP RK3), then the error against BurgersExact.  Runs on the
+// GPU through the facade.
 #include "AllBasis.h"
 #include "AlptBasis.h"
 #include "BilinearForm.h"
@@ -41,13 +42,19 @@ int main(int argc, char* argv[]) {
   Interpolation::VEC_NUM = DGSolution::VEC_NUM;
 
   int NMAX = 3, N_init = 3, is_init_sparse = 1, host_flux = 0, max_steps = 0, num_gauss_pt_compute_error = 1;
-  double final_time = 0.03, cfl_hyper = 0.2, refine_eps = 1e-2, coarsen_eta = -1.;
+  double final_time = 0.03, cfl = 0.05, cfl_hyper = 0.2, refine_eps = 1e-2, coarsen_eta = -1.;
+  int viscosity_option = 0, support_overlap = 0;
+  double shock_kappa = 0.0, artificial_viscosity_nu0 = 1.;
   OptionsParser args(argc, argv);
   args.AddOption(&NMAX, "-NM", "--max-mesh-level", "Maximum mesh level");
   args.AddOption(&N_init, "-N0", "--initial-mesh-level", "Mesh level in initialization");
   args.AddOption(&is_init_sparse, "-s", "--sparse-grid-initial", "Use full grid (0) or sparse grid (1) in initialization");
   args.AddOption(&final_time, "-tf", "--final-time", "Final time; start time is 0.");
-  args.AddOption(&cfl_hyper, "-cfl", "--cfl-number", "CFL number");
+  args.AddOption(&cfl, "-cfl", "--cfl-number", "CFL number when there are viscosity elements");
+  args.AddOption(&viscosity_option, "-v", "--artificial-viscosity-option", "artificial viscosity option (0 or 3)");
+  args.AddOption(&artificial_viscosity_nu0, "-nu", "--artificial-viscosity-value", "artificial viscosity nu0");
+  args.AddOption(&shock_kappa, "-k", "--shock-kappa", "shock detector constant kappa");
+  args.AddOption(&support_overlap, "-sp", "--support-viscosity-overlap", "consider overlap of support (1) or not (0)");
   args.AddOption(&refine_eps, "-r", "--refine-epsilon", "refine parameter epsilon");
   args.AddOption(&coarsen_eta, "-c", "--coarsen-eta", "coarsen parameter eta (negative: refine_eps / 10)");
   args.AddOption(&num_gauss_pt_compute_error, "-g", "--num-gauss-pt-compute-error", "number of gauss points in computing error");
@@ -59,6 +66,10 @@ int main(int argc, char* argv[]) {
   if (!args.Good()) { args.PrintUsage(std::cout); return 1; }
   if (N_init > NMAX) { std::cout << "Mesh level in initialization should not be larger than Maximum mesh level" << std::endl; return 1; }
   if (coarsen_eta < 0) coarsen_eta = refine_eps / 10.;
+  if (viscosity_option != 0 && viscosity_option != 3) {
+    std::cout << "only artificial viscosity option 3 (matrix-free) is provided by the GPU library" << std::endl;
+    return 1;
+  }
 
   Hash hash;
   LagrBasis::set_interp_msh01();
@@ -68,6 +79,7 @@ int main(int argc, char* argv[]) {
   AllBasis<HermBasis> all_bas_herm(NMAX);
   OperatorMatrix1D<AlptBasis, AlptBasis> oper_matx_alpt(all_bas_alpt, all_bas_alpt, "period");
   OperatorMatrix1D<HermBasis, AlptBasis> oper_matx_herm(all_bas_herm, all_bas_alpt, "period");
+  OperatorMatrix1D<LagrBasis, AlptBasis> oper_matx_lagr(all_bas_lagr, all_bas_alpt, "period");
 
   DGAdapt dg_solu(is_init_sparse == 1, N_init, NMAX, all_bas_alpt, all_bas_lagr, all_bas_herm, hash, refine_eps,
                   coarsen_eta, true, true);
@@ -88,6 +100,12 @@ int main(int argc, char* argv[]) {
   HermInterpolation interp_herm(dg_solu);
   const std::vector<std::vector<bool>> is_intp_herm{std::vector<bool>{true, false}};
   FastHermIntp fast_herm_intp(dg_solu, interp_herm.Her_pt_Alpt_1D);
+  // artificial viscosity: nu * u_x interpolated on the Lagrange family
+  HyperbolicDiffFluxLagrRHS fast_rhs_lagr(dg_solu, oper_matx_lagr);
+  LargViscosInterpolation interp_lagr(dg_solu);
+  FastLagrIntp fast_lagr_intp(dg_solu, interp_lagr.Lag_pt_Alpt_1D, interp_lagr.Lag_pt_Alpt_1D_d1);
+  const std::vector<std::vector<bool>> is_intp_lagr{std::vector<bool>(DIM, true)};
+  int total_visc_elem = 0;
   auto interpolate = [&]() {
     if (host_flux) interp_herm.nonlinear_Herm_2D_fast(func_flux, func_flux_d1, func_flux_d2, is_intp_herm, fast_herm_intp);
     else interp_herm.nonlinear_Herm_2D_fast(ADG_FLUX_BURGERS, 0.0, is_intp_herm, fast_herm_intp);
@@ -105,10 +123,17 @@ int main(int argc, char* argv[]) {
   double curr_time = 0.;
   int num_time_step = 0;
   while (curr_time < final_time && (max_steps <= 0 || num_time_step < max_steps)) {
+    int num_visc_elem = 0;
+    if (viscosity_option != 0) {
+      dg_solu.update_viscosity_element(shock_kappa);
+      num_visc_elem = dg_solu.num_artific_visc_elem();
+      total_visc_elem += num_visc_elem;
+    }
     const std::vector<int> max_mesh = dg_solu.max_mesh_level_vec();
     double sum_c_dx = 0.;
     for (int d = 0; d < DIM; d++) sum_c_dx += std::abs(wave_speed[d]) * std::pow(2., max_mesh[d]);
-    double dt = cfl_hyper / sum_c_dx;
+    double dt = cfl / sum_c_dx;
+    if (num_visc_elem == 0) dt = cfl_hyper / sum_c_dx;
     dt = std::min(dt, final_time - curr_time);
     {
       dg_solu.copy_ucoe_to_predict();
@@ -132,11 +157,17 @@ int main(int argc, char* argv[]) {
     lax_friedrichs(linear);
     RK3SSP odeSolver(linear, dt);
     odeSolver.init();
+    const double h = 1. / std::pow(2., dg_solu.max_mesh_level());
     for (int stage = 0; stage < odeSolver.num_stage; ++stage) {
       interpolate();
       dg_solu.set_rhs_zero();
       fast_rhs_herm.rhs_vol_scalar();
       fast_rhs_herm.rhs_flx_intp_scalar();
+      if (num_visc_elem != 0) {
+        interp_lagr.support_gradu_Lagr_fast(dg_solu.viscosity_element, -artificial_viscosity_nu0 * h, is_intp_lagr,
+                                            fast_lagr_intp, support_overlap == 1);
+        fast_rhs_lagr.rhs_vol_scalar();
+      }
       odeSolver.set_rhs_zero();
       odeSolver.add_rhs_to_eigenvec();
       odeSolver.add_rhs_matrix(linear);
@@ -153,7 +184,8 @@ int main(int argc, char* argv[]) {
     for (size_t i = 0; i < it.second.size(); ++i) checksum += (it.first % 97 + 1) * (i + 1) * it.second[i];
   std::cout.precision(17);
   std::cout << "steps " << num_time_step << " time " << curr_time << " elements " << dg_solu.size_elem()
-            << " dof " << dg_solu.get_dof() << " checksum " << checksum << std::endl;
+            << " dof " << dg_solu.get_dof() << " checksum " << checksum << " viscosity elements " << total_visc_elem
+            << std::endl;
   if (!state_file.empty()) {
     IO inout(dg_solu);
     inout.output_state(state_file);

@@ -235,6 +235,19 @@ DGAdapt::DGAdapt(const bool sparse_, const int level_init_, const int NMAX_, All
 
 void DGAdapt::refine() { adg_facade_check(ctx_, adg_refine(ctx_, eps, nullptr), "DGAdapt::refine"); }
 void DGAdapt::coarsen() { adg_facade_check(ctx_, adg_coarsen(ctx_, eta, nullptr), "DGAdapt::coarsen"); }
+void DGAdapt::update_viscosity_element(const double shock_kappa) {
+  int64_t n = 0;
+  adg_facade_check(ctx_, adg_viscosity_elements(ctx_, shock_kappa, &n, nullptr), "update_viscosity_element");
+  viscosity_element.n = n;
+}
+
+void LargViscosInterpolation::support_gradu_Lagr_fast(const ViscosityElementSet&, const double coefficient,
+                                                      std::vector<std::vector<bool>>, FastLagrIntp&,
+                                                      const bool overlap) {
+  // the viscosity elements are those of the last DGAdapt::update_viscosity_element on this solution
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_viscosity_flux(c, coefficient, overlap ? 1 : 0), "support_gradu_Lagr_fast");
+}
 
 namespace {
 double call_separable(double x, int d, int ifunc, void* user) {

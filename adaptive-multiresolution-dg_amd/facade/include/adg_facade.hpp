@@ -110,6 +110,14 @@ class OperatorMatrix1D {
   const std::string boundary;
 };
 
+// The viscosity element set of DGSolution (include/DGSolution.h:222-225): held
+// by the library context; the facade keeps its size.
+struct ViscosityElementSet {
+  int64_t n = 0;
+  size_t size() const { return static_cast<size_t>(n); }
+  bool empty() const { return n == 0; }
+};
+
 class DGSolution {
  public:
   DGSolution(const bool sparse, const int level_init, const int NMAX, AllBasis<AlptBasis>& all_bas,
@@ -123,6 +131,9 @@ class DGSolution {
   static std::vector<int> ind_var_vec;
 
   int max_mesh_level() const;
+  // DGSolution::num_artific_visc_elem (source/DGSolution.cpp)
+  int num_artific_visc_elem() const { return static_cast<int>(viscosity_element.n); }
+  ViscosityElementSet viscosity_element;
   std::vector<int> max_mesh_level_vec() const;
   int nmax_level() const { return NMAX; }
   // DGSolution::get_error_no_separable_scalar (source/DGSolution.cpp:326-335): {L1, L2, Linf}
@@ -164,6 +175,8 @@ class DGAdapt : public DGSolution {
   static std::vector<int> indicator_var_adapt;
   void refine();
   void coarsen();
+  // DGAdapt::update_viscosity_element (source/DGAdapt.cpp:727-759)
+  void update_viscosity_element(const double shock_kappa);
   // DGAdapt::init_separable_scalar_sum (source/DGAdapt.cpp:120-147)
   void init_separable_scalar_sum(std::vector<std::function<double(double, int)>> sum_func);
   // DGAdapt::filter / filter_local (source/DGAdapt.cpp:461-555)
@@ -300,6 +313,17 @@ class LagrInterpolation {
                                          std::vector<std::vector<bool>> is_intp, FastLagrIntp& fastLagr,
                                          const int mesh_nmax);
   DGSolution* const dgsolution_ptr;
+};
+
+// LargViscosInterpolation (include/Interpolation.h:220-240): the viscous flux
+// nu * u_x of the artificial viscosity, interpolated on the Lagrange family.
+class LargViscosInterpolation : public LagrInterpolation {
+ public:
+  using LagrInterpolation::LagrInterpolation;
+  // LargViscosInterpolation::support_gradu_Lagr_fast (source/Interplation.cpp:4176-4189)
+  void support_gradu_Lagr_fast(const ViscosityElementSet& support_element, const double coefficient,
+                               std::vector<std::vector<bool>> is_intp, FastLagrIntp& fastLagr,
+                               const bool overlap = false);
 };
 
 class HyperbolicLagrRHS {

@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from state_file import unpack
@@ -79,3 +80,25 @@ def test_burgers_hermite_driver_converges_to_the_exact_solution():
         errs.append(e[1])
     print("[hermite burgers L2 errors]", errs, flush=True)
     assert errs[2] < errs[1] < errs[0] and errs[2] < errs[0] / 8, errs
+
+
+def test_burgers_hermite_driver_with_artificial_viscosity(tmp_path):
+    # past the shock time 1 / (4 pi): option 3 marks shock elements, adds the interpolated viscous flux and
+    # keeps the run bounded; host closures and the device flux id still give identical final states.  The
+    # parameters are those of the reference's own option-3 run line (explicit SSP RK3 needs cfl 0.01 with
+    # nu0 = 2), at NMAX 5
+    args = ["-NM", "5", "-N0", "2", "-s", "0", "-r", "1e-3", "-c", "1e-4", "-v", "3", "-sp", "1", "-cfl", "0.01",
+            "-nu", "2.0", "-tf", "0.15"]
+    sa, sb = str(tmp_path / "dev.adgstate"), str(tmp_path / "host.adgstate")
+    a = run("burgers_adapt_herm", *args, "-st", sa)
+    b = run("burgers_adapt_herm", *args, "-hf", "1", "-st", sb)
+    la = [l for l in a.splitlines() if l.startswith("steps")][0]
+    lb = [l for l in b.splitlines() if l.startswith("steps")][0]
+    print("[hermite burgers with viscosity]", la, flush=True)
+    assert la == lb
+    assert open(sa, "rb").read() == open(sb, "rb").read()
+    assert int(re.search(r"viscosity elements (\d+)", la).group(1)) > 0
+    assert abs(float(re.search(r"time ([0-9.eE+-]+)", la).group(1)) - 0.15) < 1e-12
+    _, insts = unpack(open(sa, "rb").read())
+    u = np.asarray(insts[0][2])
+    assert np.all(np.isfinite(u)) and np.max(np.abs(u)) < 10.
