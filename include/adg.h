@@ -299,7 +299,9 @@ int adg_set_kernel_variant(adg_ctx* c, int variant);
  * (source/DGAdapt.cpp:727-759) for every instance: childless elements at the
  * finest level whose norm exceeds the smooth-solution bound 2^((1 - k)|l|_1)
  * by more than shock_kappa decades; count = DGSolution::num_artific_visc_elem,
- * flags (may be null) 1 per viscosity element.
+ * flags (may be null) 1 per viscosity element.  The set is held by element
+ * key until the next call: a refine in between keeps it, elements a coarsen
+ * removed are skipped.
  * adg_viscosity_flux: LargViscosInterpolation::support_gradu_Lagr_fast
  * (source/Interplation.cpp:4176-4189,579-640): for each d, coefficient * du/dx_d
  * at the Lagrange points inside a viscosity element's closed support (times the

@@ -91,3 +91,21 @@ def test_batch_instances_use_their_own_viscosity_elements():
             a = fu[d][first * 16:(first + count) * 16]
             assert np.array_equal(a, o.get(pyoracle.FIELD_FUCOE + d)), f"instance {i} fucoe[{d}]"
     assert n == total
+
+
+def test_viscosity_elements_survive_refinement():
+    # the reference holds the viscosity elements by pointer, so the refine of the predict step keeps them;
+    # the library keeps them by key across the renumbering that refine makes
+    g, o = pair(circle_ic(), nmax=6, level_init=3)
+    n, flags = g.viscosity_elements(0.0)
+    assert n == o.viscosity_elements(0.0)[0] and n > 0
+    before = g.num_elem
+    g.refine(1e-5)
+    o.refine(1e-5)
+    print(f"[viscosity] refine: {before} -> {g.num_elem} elements, {n} viscosity elements", flush=True)
+    assert g.num_elem > before and same_grid(g, o)
+    g.viscosity_flux(-1.0 / 64, True)
+    o.viscosity_flux(-1.0 / 64, True)
+    for d in range(2):
+        a, b = g.get(pyadg.FIELD_FUCOE + d), o.get(pyoracle.FIELD_FUCOE + d)
+        assert np.array_equal(a, b), f"fucoe[{d}]: " + first_diff(a, b)
