@@ -364,6 +364,38 @@ double product_volume(const Basis1D& U, int ru, const Basis1D& V, int cv, int du
   return gl10(f, xl, xmid) + gl10(f, xmid, xr);
 }
 
+double product_volume_interv(const Basis1D& B, int ru, int rv, int rw, int du, int dv, int nq) {
+  int nu, ju, pu, nv, jv, pv, nw, jw, pw;
+  B.decode(ru, nu, ju, pu);
+  B.decode(rv, nv, jv, pv);
+  B.decode(rw, nw, jw, pw);
+  double uxl, uxr, vxl, vxr, wxl, wxr;
+  cell_support(nu, ju, uxl, uxr);
+  cell_support(nv, jv, vxl, vxr);
+  cell_support(nw, jw, wxl, wxr);
+  if ((vxl >= uxr) || (vxr <= uxl)) return 0.;
+  if ((vxl >= wxr) || (vxr <= wxl)) return 0.;
+  if ((uxl >= wxr) || (uxr <= wxl)) return 0.;
+  const int max_level = std::max(std::max(nu, nv), nw);
+  double xl = uxl, xr = uxr;
+  if (nv == max_level) { xl = vxl; xr = vxr; }
+  else if (nw == max_level) { xl = wxl; xr = wxr; }
+  const double xmid = (xl + xr) / 2.;
+  const std::vector<double>& x = kGLx.at(nq - 1);
+  const std::vector<double>& w = kGLw.at(nq - 1);
+  // Quad::GL_1D with nq points
+  auto gl = [&](double tl, double tr) {
+    double integral = 0;
+    for (int i = 0; i < nq; ++i) {
+      const double t = (x[i] - (-1.)) / (1. - (-1.)) * (tr - tl) + tl;
+      integral += w[i] * (B.val(ru, t, du) * B.val(rv, t, dv));
+    }
+    integral *= (tr - tl) / 2.;
+    return integral;
+  };
+  return gl(xl, xmid) + gl(xmid, xr);
+}
+
 double product_edge_dis_v_period(const Basis1D& U, int ru, const Basis1D& V, int cv, int slu, int slv) {
   int nv, jv, pv;
   V.decode(cv, nv, jv, pv);

@@ -65,6 +65,11 @@ template <class F> double gl10(F&& f, double tl, double tr);
 
 // Basis::product_volume(u, v, du, dv) with 10 GL points.
 double product_volume(const Basis1D& U, int ru, const Basis1D& V, int cv, int du, int dv);
+// Basis::product_volume_interv(v, w, du, dv, nq) (source/Basis.cpp:83-119):
+// integral of u^(du) v^(dv) over the support of w, with nq Gauss-Legendre
+// points on each half of the finest of the three supports; 0 unless the three
+// supports intersect pairwise.  U, V, W are basis orders of one family.
+double product_volume_interv(const Basis1D& B, int ru, int rv, int rw, int du, int dv, int nq);
 // Basis::product_edge_dis_v(u, v, sl_u, sl_v, 0, 0, "period").
 double product_edge_dis_v_period(const Basis1D& U, int ru, const Basis1D& V, int cv, int slu, int slv);
 

@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstdio>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -628,6 +629,223 @@ struct or_ctx {
     interp = keep;
   }
 
+  // ---- artificial viscosity, assembled (viscosity options 1 and 2 of
+  // example/02_hyperbolic_05_burgers_adapt.cpp:343-420) ----
+  // ArtificialViscosity::assemble_matrix (source/BilinearForm.cpp:1190-1349):
+  // for each viscosity element w and every pair (test, solution) of elements of
+  // its related set (option 1: ptr_vol_intp, the elements whose supports
+  // intersect w's in every dimension; option 2: find_all_parents, w and every
+  // ancestor reached through present parents), the entries
+  // coef * sum_dd prod_d (d == dd ? stiff_d : mass_d) of the 1D integrals over
+  // w's support (product_volume_interv with k + 1 and k points), kept when
+  // |entry| >= ROUND_OFF; the local matrices are added in viscosity-element
+  // order.  Rows and columns: element position in hash-key order times K plus
+  // the local basis index.
+  std::map<std::pair<int64_t, int64_t>, double> visc_A;  // (row, column)
+  void viscosity_operator(double coef, int option) {
+    auto v = sorted_elems();
+    std::unordered_map<const OElem*, int64_t> pos;
+    for (size_t i = 0; i < v.size(); ++i) pos[v[i]] = static_cast<int64_t>(i);
+    std::map<std::array<int, 3>, std::array<double, 2>> tab;  // visc_mass, visc_stiff
+    const int kp = k + 1;
+    visc_A.clear();
+    for (OElem* w : visc) {
+      std::vector<OElem*> rel;
+      if (option == 1) {
+        for (OElem* e : v) {
+          bool meet = true;
+          for (int d = 0; d < D && meet; ++d) meet = intersect(w->xl[d], w->xr[d], e->xl[d], e->xr[d]);
+          if (meet) rel.push_back(e);
+        }
+      } else {
+        std::set<OElem*> all{w}, cur{w};
+        while (true) {
+          std::set<OElem*> add;
+          for (OElem* e : cur)
+            for (auto& p : e->par) { all.insert(p.second); add.insert(p.second); }
+          if (add.empty()) break;
+          cur = add;
+        }
+        rel.assign(all.begin(), all.end());
+      }
+      std::map<std::pair<int64_t, int64_t>, double> local;
+      for (OElem* t : rel) {
+        for (OElem* sl : rel) {
+          std::vector<int> it(D, 0);
+          for (int ti = 0; ti < K; ++ti) {
+            std::vector<int> is(D, 0);
+            for (int si = 0; si < K; ++si) {
+              std::vector<double> mass(D), stiff(D);
+              for (int d = 0; d < D; ++d) {
+                const std::array<int, 3> key{t->ord[d] * kp + it[d], sl->ord[d] * kp + is[d], w->ord[d] * kp};
+                auto f = tab.find(key);
+                if (f == tab.end()) {
+                  const double ms = adg::product_volume_interv(alpt, key[0], key[1], key[2], 0, 0, k + 1);
+                  const double st = adg::product_volume_interv(alpt, key[0], key[1], key[2], 1, 1, k);
+                  f = tab.emplace(key, std::array<double, 2>{ms, st}).first;
+                }
+                mass[d] = f->second[0];
+                stiff[d] = f->second[1];
+              }
+              double mat_ij = 0.;
+              for (int dd = 0; dd < D; ++dd) {
+                double mat_ij_dim = coef;
+                for (int d = 0; d < D; ++d) mat_ij_dim *= (d == dd) ? stiff[d] : mass[d];
+                mat_ij += mat_ij_dim;
+              }
+              if (std::abs(mat_ij) >= ROUND_OFF) local[{pos[t] * K + ti, pos[sl] * K + si}] = mat_ij;
+              for (int d = D - 1; d >= 0; --d) { if (++is[d] < kp) break; is[d] = 0; }
+            }
+            for (int d = D - 1; d >= 0; --d) { if (++it[d] < kp) break; it[d] = 0; }
+          }
+        }
+      }
+      for (auto& e : local) {
+        auto f = visc_A.find(e.first);
+        if (f == visc_A.end()) visc_A[e.first] = e.second;
+        else f->second = f->second + e.second;
+      }
+    }
+  }
+
+  // Column-major sparse product y = A x from zero (Eigen's ColMajor
+  // sparse * dense loop: each row accumulates in increasing column order).
+  static std::vector<double> spmv(const std::vector<std::map<int64_t, double>>& cols, const std::vector<double>& x) {
+    std::vector<double> y(x.size(), 0.);
+    for (size_t j = 0; j < cols.size(); ++j)
+      for (auto& e : cols[j]) y[e.first] += e.second * x[j];
+    return y;
+  }
+  std::vector<std::map<int64_t, double>> visc_cols() const {
+    std::vector<std::map<int64_t, double>> cols(dg.size() * K);
+    for (auto& e : visc_A) cols[e.first.second][e.first.first] = e.second;
+    return cols;
+  }
+  std::vector<double> gather_ucoe() {
+    auto v = sorted_elems();
+    std::vector<double> u(v.size() * K);
+    for (size_t i = 0; i < v.size(); ++i) std::copy(v[i]->ucoe.begin(), v[i]->ucoe.end(), u.begin() + i * K);
+    return u;
+  }
+  void scatter_ucoe(const std::vector<double>& u) {
+    auto v = sorted_elems();
+    for (size_t i = 0; i < v.size(); ++i) std::copy(u.begin() + i * K, u.begin() + (i + 1) * K, v[i]->ucoe.begin());
+  }
+
+  // odeSolver.add_rhs_matrix(artific_visc_operator): rhs += (A u), the product
+  // evaluated first (Eigen evaluates a product that may alias into a temporary)
+  void rhs_viscosity_operator() {
+    const auto r = spmv(visc_cols(), gather_ucoe());
+    auto v = sorted_elems();
+    for (size_t i = 0; i < v.size(); ++i)
+      for (int q = 0; q < K; ++q) v[i]->rhs[q] += r[i * K + q];
+  }
+
+  // IMEX43 (source/ODESolver.cpp:872-1070) with the assembled viscosity
+  // operator as implicit part G and the default linear solver "cg".
+  static constexpr double imex_alpha = 0.24169426078821, imex_beta = 0.06042356519705, imex_eta = 0.12915286960590;
+  std::vector<std::map<int64_t, double>> imex_M, imex_A;
+  std::vector<double> imex_tn, imex_invdiag, imex_G[4], imex_F[3];
+  double imex_dt = 0.;
+  void imex43_init(double dt) {
+    imex_dt = dt;
+    imex_A = visc_cols();
+    const size_t n = imex_A.size();
+    // I - (dt * alpha) * A: the union of both patterns (init_cg_solver)
+    const double c = dt * imex_alpha;
+    imex_M.assign(n, {});
+    for (size_t j = 0; j < n; ++j) {
+      for (auto& e : imex_A[j]) imex_M[j][e.first] = (e.first == static_cast<int64_t>(j)) ? 1. - c * e.second : 0. - c * e.second;
+      if (imex_M[j].find(static_cast<int64_t>(j)) == imex_M[j].end()) imex_M[j][j] = 1.;
+    }
+    // DiagonalPreconditioner: inverse of a non-zero diagonal entry, else 1
+    imex_invdiag.assign(n, 1.);
+    for (size_t j = 0; j < n; ++j) {
+      auto f = imex_M[j].find(static_cast<int64_t>(j));
+      if (f != imex_M[j].end() && f->second != 0.) imex_invdiag[j] = 1. / f->second;
+    }
+    imex_tn = gather_ucoe();  // ODESolver::init
+  }
+
+  // Eigen's ConjugateGradient<SparseMatrix, Lower | Upper> with the diagonal
+  // preconditioner, tolerance 1e-12, at most 1000 iterations and zero initial
+  // guess (internal::conjugate_gradient of Eigen 3, a dependency the reference
+  // does not vendor; restated from its published algorithm, dot products summed
+  // in index order).  Returns the iteration count, -1 when the relative
+  // residual stays above the tolerance (the reference then exits).
+  int cg_solve(const std::vector<double>& b, std::vector<double>& x) {
+    const double tol = 1e-12;
+    const int max_iters = 1000;
+    const size_t n = b.size();
+    x.assign(n, 0.);
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& c) {
+      double s = 0.;
+      for (size_t i = 0; i < n; ++i) s += a[i] * c[i];
+      return s;
+    };
+    std::vector<double> residual = b;  // b - M * 0
+    const double rhs_norm2 = dot(b, b);
+    if (rhs_norm2 == 0) return 0;
+    const double threshold = std::max(tol * tol * rhs_norm2, std::numeric_limits<double>::min());
+    double residual_norm2 = dot(residual, residual);
+    if (residual_norm2 < threshold) return 0;
+    std::vector<double> p(n), z(n);
+    for (size_t i = 0; i < n; ++i) p[i] = imex_invdiag[i] * residual[i];
+    double abs_new = dot(residual, p);
+    int i = 0;
+    while (i < max_iters) {
+      const std::vector<double> tmp = spmv(imex_M, p);
+      const double alpha = abs_new / dot(p, tmp);
+      for (size_t q = 0; q < n; ++q) x[q] += alpha * p[q];
+      for (size_t q = 0; q < n; ++q) residual[q] -= alpha * tmp[q];
+      residual_norm2 = dot(residual, residual);
+      if (residual_norm2 < threshold) break;
+      for (size_t q = 0; q < n; ++q) z[q] = imex_invdiag[q] * residual[q];
+      const double abs_old = abs_new;
+      abs_new = dot(residual, z);
+      const double beta = abs_new / abs_old;
+      for (size_t q = 0; q < n; ++q) p[q] = z[q] + beta * p[q];
+      ++i;
+    }
+    return std::sqrt(residual_norm2 / rhs_norm2) <= tol ? i : -1;
+  }
+
+  // IMEX43::step_stage + final: stages 0 and 1 are implicit only, stages 2..4
+  // take the element rhs as the explicit part F of the current solution.
+  int imex43_stage(int stage) {
+    const size_t n = imex_tn.size();
+    const double dt = imex_dt, a = imex_alpha;
+    std::vector<double> b(n), x;
+    const std::vector<double>& tn = imex_tn;
+    if (stage >= 2) {
+      auto v = sorted_elems();
+      std::vector<double>& F = imex_F[stage - 2];
+      F.resize(n);
+      for (size_t i = 0; i < v.size(); ++i) std::copy(v[i]->rhs.begin(), v[i]->rhs.end(), F.begin() + i * K);
+    }
+    auto& G = imex_G;
+    auto& F = imex_F;
+    if (stage == 4) {
+      for (size_t i = 0; i < n; ++i)
+        x.push_back(tn[i] + dt * (1. / 6. * F[0][i] + 1. / 6. * F[1][i] + 2. / 3. * F[2][i]) +
+                    dt * (1. / 6. * G[1][i] + 1. / 6. * G[2][i] + 2. / 3. * G[3][i]));
+      scatter_ucoe(x);
+      return 0;
+    }
+    for (size_t i = 0; i < n; ++i) {
+      if (stage == 0) b[i] = tn[i];
+      else if (stage == 1) b[i] = tn[i] + dt * (-a) * G[0][i];
+      else if (stage == 2) b[i] = tn[i] + dt * F[0][i] + dt * (1 - a) * G[1][i];
+      else
+        b[i] = tn[i] + dt * 0.25 * F[0][i] + dt * 0.25 * F[1][i] + dt * imex_beta * G[0][i] + dt * imex_eta * G[1][i] +
+               dt * (0.5 - imex_beta - imex_eta - a) * G[2][i];
+    }
+    const int it = cg_solve(b, x);
+    G[stage] = spmv(imex_A, x);
+    scatter_ucoe(x);
+    return it;
+  }
+
   void rhs_lagr(bool flux_term) {
     for (int d = 0; d < D; ++d) {
       std::vector<const double*> mats(D);
@@ -963,6 +1181,20 @@ int64_t or_viscosity_elements(or_ctx* c, double kappa, int32_t* flags) {
   return n;
 }
 void or_viscosity_flux(or_ctx* c, double coef, int overlap) { c->viscosity_flux(coef, overlap != 0); }
+int64_t or_viscosity_operator(or_ctx* c, double coef, int option) {
+  c->viscosity_operator(coef, option);
+  return static_cast<int64_t>(c->visc_A.size());
+}
+void or_viscosity_operator_coo(or_ctx* c, int64_t* rows, int64_t* cols, double* vals) {
+  size_t i = 0;
+  for (auto& e : c->visc_A) { rows[i] = e.first.first; cols[i] = e.first.second; vals[i] = e.second; ++i; }
+}
+void or_rhs_viscosity_operator(or_ctx* c) { c->rhs_viscosity_operator(); }
+double or_product_volume_interv(or_ctx* c, int ru, int rv, int rw, int du, int dv, int nq) {
+  return adg::product_volume_interv(c->alpt, ru, rv, rw, du, dv, nq);
+}
+void or_imex43_init(or_ctx* c, double dt) { c->imex43_init(dt); }
+int or_imex43_stage(or_ctx* c, int stage) { return c->imex43_stage(stage); }
 void or_eval_up(or_ctx* c) { c->eval_up(); }
 void or_set_level_sum_cap(or_ctx* c, int cap) { c->cap = cap; }
 void or_flux(or_ctx* c, int kind, const double* params) { c->flux(kind, params); }

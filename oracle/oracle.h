@@ -40,6 +40,12 @@ void or_eval_up(or_ctx* c);                                   /* FastLagrIntp::e
 int64_t or_viscosity_elements(or_ctx* c, double kappa, int32_t* flags);
 /* LargViscosInterpolation::support_gradu_Lagr_fast (Lagrange family): fucoe_intp[d] */
 void or_viscosity_flux(or_ctx* c, double coef, int overlap);
+int64_t or_viscosity_operator(or_ctx* c, double coef, int option);
+void or_viscosity_operator_coo(or_ctx* c, int64_t* rows, int64_t* cols, double* vals);
+void or_rhs_viscosity_operator(or_ctx* c);
+double or_product_volume_interv(or_ctx* c, int ru, int rv, int rw, int du, int dv, int nq);
+void or_imex43_init(or_ctx* c, double dt);
+int or_imex43_stage(or_ctx* c, int stage);
 /* _coarse_grid variants: sweeps and assembled operators skip elements whose
  * level sum exceeds cap (negative: no cap). */
 void or_set_level_sum_cap(or_ctx* c, int cap);

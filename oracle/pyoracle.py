@@ -39,6 +39,12 @@ def lib():
             "or_set_interpolation": (None, [vp, i32]),
             "or_viscosity_elements": (i64, [vp, dbl, P(ctypes.c_int32)]),
             "or_viscosity_flux": (None, [vp, dbl, i32]),
+            "or_viscosity_operator": (i64, [vp, dbl, i32]),
+            "or_viscosity_operator_coo": (None, [vp, P(i64), P(i64), P(dbl)]),
+            "or_rhs_viscosity_operator": (None, [vp]),
+            "or_product_volume_interv": (dbl, [vp, i32, i32, i32, i32, i32, i32]),
+            "or_imex43_init": (None, [vp, dbl]),
+            "or_imex43_stage": (i32, [vp, i32]),
             "or_set_level_sum_cap": (None, [vp, ctypes.c_int]),
             "or_flux": (None, [vp, i32, P(dbl)]),
             "or_fp_to_coe": (None, [vp]),
@@ -155,6 +161,32 @@ class Oracle:
 
     def viscosity_flux(self, coef, overlap=False):
         lib().or_viscosity_flux(self._h, coef, 1 if overlap else 0)
+
+    def viscosity_operator(self, coef, option):
+        """ArtificialViscosity::assemble_matrix; returns the number of entries"""
+        self._visc_nnz = int(lib().or_viscosity_operator(self._h, coef, option))
+        return self._visc_nnz
+
+    def viscosity_operator_coo(self):
+        n = self._visc_nnz
+        rows, cols, vals = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n)
+        P = ctypes.POINTER
+        lib().or_viscosity_operator_coo(self._h, rows.ctypes.data_as(P(ctypes.c_int64)),
+                                        cols.ctypes.data_as(P(ctypes.c_int64)), _dp(vals))
+        return rows, cols, vals
+
+    def rhs_viscosity_operator(self):
+        lib().or_rhs_viscosity_operator(self._h)
+
+    def product_volume_interv(self, ru, rv, rw, du, dv, nq):
+        return float(lib().or_product_volume_interv(self._h, ru, rv, rw, du, dv, nq))
+
+    def imex43_init(self, dt):
+        lib().or_imex43_init(self._h, dt)
+
+    def imex43_stage(self, stage):
+        """IMEX43::step_stage + final; returns the CG iteration count (-1: no convergence)"""
+        return int(lib().or_imex43_stage(self._h, stage))
 
     def set_interpolation(self, family):
         """0 Lagrange, 1 Hermite P3 (needs m = 3)."""
