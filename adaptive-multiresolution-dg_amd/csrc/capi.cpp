@@ -299,6 +299,33 @@ int adg_viscosity_flux(adg_ctx* h, double coefficient, int overlap) {
   return guard(h, [&](adg::Context& c) { c.viscosity_flux(coefficient, overlap != 0); });
 }
 
+int adg_viscosity_operator(adg_ctx* h, double coefficient, int option, int64_t* blocks) {
+  return guard(h, [&](adg::Context& c) {
+    const int64_t n = c.visc_operator_assemble(coefficient, option);
+    if (blocks) *blocks = n;
+  });
+}
+
+int adg_rhs_viscosity_operator(adg_ctx* h) {
+  return guard(h, [&](adg::Context& c) { c.rhs_visc_operator(); });
+}
+
+int adg_imex43_init(adg_ctx* h, double dt) {
+  return guard(h, [&](adg::Context& c) { c.imex43_init(dt); });
+}
+
+int adg_imex43_stage(adg_ctx* h, int stage, int* cg_iterations) {
+  int it = 0;
+  const int rc = guard(h, [&](adg::Context& c) { it = c.imex43_stage(stage); });
+  if (rc != ADG_OK) return rc;
+  if (cg_iterations) *cg_iterations = it;
+  if (it < 0) {
+    h->err = "IMEX43: conjugate gradient did not reach its tolerance";
+    return ADG_E_NOCONVERGE;
+  }
+  return ADG_OK;
+}
+
 int adg_set_refine_mode(adg_ctx* h, int mode, double threshold) {
   return guard(h, [&](adg::Context& c) {
     if (mode != ADG_REFINE_PREDICT && mode != ADG_REFINE_LEARNED && mode != ADG_REFINE_CURRENT)

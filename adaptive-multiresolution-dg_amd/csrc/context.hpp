@@ -243,6 +243,28 @@ class Context {
   void eval_der_up(int d0);  // FastLagrIntp::eval_der_up_Lagr
   DevBuf<double> t_pt_d1, visc_w;
 
+  // ---- artificial viscosity as an assembled operator (options 1 and 2) and
+  // IMEX43 (imex.hip) ----
+  // ArtificialViscosity::assemble_matrix over the current viscosity elements;
+  // returns the number of K x K blocks
+  int64_t visc_operator_assemble(double coef, int option);
+  void visc_operator_check() const;
+  // y = A x (mode 0), y += (A x) (mode 1), y = (I - c A) x (mode 2); device pointers
+  void visc_operator_apply(const double* x, double* y, int mode, double c);
+  void rhs_visc_operator();  // rhs += (A u)
+  void imex43_init(double dt);
+  int imex43_stage(int stage);  // largest CG iteration count, -1 without convergence
+  int cg_solve(double c);
+  int64_t vop_blocks = 0, vop_version = -1, vop_nnz_entries = 0, imex_version = -1;
+  std::vector<int32_t> vop_key_tid;   // 1D integral key -> table entry
+  std::vector<double> vop_tab_host;   // per entry: mass, stiffness ((k+1)^2 each)
+  DevBuf<int32_t> vop_rowptr, vop_col, vop_recptr, vop_rec, cg_state, cg_inst_chunk;
+  DevBuf<double> vop_tab, vop_val, imex_tn, imex_G[4], imex_F[2], cg_b, cg_r, cg_p, cg_t, cg_inv, cg_part, cg_scal;
+  DevBuf<int64_t> cg_chunk;
+  int cg_nchunk = 0;
+  std::vector<int> cg_iters;  // per instance, last solve
+  double imex_dt = 0.;
+
   // ---- learned refinement proposer (proposer.hip) ----
   // Features of every element for the proposer, from the current u:
   // [0] log10(norm / eps) / 4 of the element, [1 + 2d], [2 + 2d] the same for

@@ -3,9 +3,10 @@
 // example/02_hyperbolic_05_burgers_adapt.cpp (separable initial value
 // sin(2 pi (x + y)), same-flux Hermite interpolation with is_intp {true, false},
 // forward-Euler prediction, refine, SSP RK3, coarsen), with its artificial
-// viscosity option 3 (shock detector, viscous flux interpolated on the Lagrange
-// family, explicit SSP RK3), then the error against BurgersExact.  Runs on the
-// GPU through the facade.
+// viscosity options: 1 and 2 (assembled viscosity operator, IMEX43 or explicit
+// SSP RK3) and 3 (viscous flux interpolated on the Lagrange family, explicit
+// SSP RK3), then the error against BurgersExact.  Runs on the GPU through the
+// facade.
 #include "AllBasis.h"
 #include "AlptBasis.h"
 #include "BilinearForm.h"
@@ -43,7 +44,7 @@ int main(int argc, char* argv[]) {
 
   int NMAX = 3, N_init = 3, is_init_sparse = 1, host_flux = 0, max_steps = 0, num_gauss_pt_compute_error = 1;
   double final_time = 0.03, cfl = 0.05, cfl_hyper = 0.2, refine_eps = 1e-2, coarsen_eta = -1.;
-  int viscosity_option = 0, support_overlap = 0;
+  int viscosity_option = 0, support_overlap = 0, is_imex_option = 1;
   double shock_kappa = 0.0, artificial_viscosity_nu0 = 1.;
   OptionsParser args(argc, argv);
   args.AddOption(&NMAX, "-NM", "--max-mesh-level", "Maximum mesh level");
@@ -54,6 +55,7 @@ int main(int argc, char* argv[]) {
   args.AddOption(&viscosity_option, "-v", "--artificial-viscosity-option", "artificial viscosity option (0 or 3)");
   args.AddOption(&artificial_viscosity_nu0, "-nu", "--artificial-viscosity-value", "artificial viscosity nu0");
   args.AddOption(&shock_kappa, "-k", "--shock-kappa", "shock detector constant kappa");
+  args.AddOption(&is_imex_option, "-imex", "--implicit-explicit", "Use IMEX (1) or explicit SSP RK (0)");
   args.AddOption(&support_overlap, "-sp", "--support-viscosity-overlap", "consider overlap of support (1) or not (0)");
   args.AddOption(&refine_eps, "-r", "--refine-epsilon", "refine parameter epsilon");
   args.AddOption(&coarsen_eta, "-c", "--coarsen-eta", "coarsen parameter eta (negative: refine_eps / 10)");
@@ -66,8 +68,9 @@ int main(int argc, char* argv[]) {
   if (!args.Good()) { args.PrintUsage(std::cout); return 1; }
   if (N_init > NMAX) { std::cout << "Mesh level in initialization should not be larger than Maximum mesh level" << std::endl; return 1; }
   if (coarsen_eta < 0) coarsen_eta = refine_eps / 10.;
-  if (viscosity_option != 0 && viscosity_option != 3) {
-    std::cout << "only artificial viscosity option 3 (matrix-free) is provided by the GPU library" << std::endl;
+  if (viscosity_option < 0 || viscosity_option > 3) { std::cout << "unknown artificial viscosity option" << std::endl; return 1; }
+  if (is_imex_option == 1 && viscosity_option == 3) {
+    std::cout << "In artificial viscosity option 3, we do not assemble matrix thus IMEX could not be applied" << std::endl;
     return 1;
   }
 
@@ -105,7 +108,7 @@ int main(int argc, char* argv[]) {
   LargViscosInterpolation interp_lagr(dg_solu);
   FastLagrIntp fast_lagr_intp(dg_solu, interp_lagr.Lag_pt_Alpt_1D, interp_lagr.Lag_pt_Alpt_1D_d1);
   const std::vector<std::vector<bool>> is_intp_lagr{std::vector<bool>(DIM, true)};
-  int total_visc_elem = 0;
+  int total_visc_elem = 0, total_cg_iterations = 0;
   auto interpolate = [&]() {
     if (host_flux) interp_herm.nonlinear_Herm_2D_fast(func_flux, func_flux_d1, func_flux_d2, is_intp_herm, fast_herm_intp);
     else interp_herm.nonlinear_Herm_2D_fast(ADG_FLUX_BURGERS, 0.0, is_intp_herm, fast_herm_intp);
@@ -155,15 +158,57 @@ int main(int argc, char* argv[]) {
     dg_solu.copy_predict_to_ucoe();
     HyperbolicAlpt linear(dg_solu, oper_matx_alpt);
     lax_friedrichs(linear);
+    const double h = 1. / std::pow(2., dg_solu.max_mesh_level());
+    if (num_visc_elem != 0 && (viscosity_option == 1 || viscosity_option == 2)) {
+      ArtificialViscosity artific_visc_operator(dg_solu, oper_matx_alpt, all_bas_alpt);
+      artific_visc_operator.assemble_matrix(-artificial_viscosity_nu0 * h, viscosity_option);
+      if (is_imex_option == 1) {
+        IMEX43 odeSolver(artific_visc_operator, dt);
+        odeSolver.init();
+        for (int stage = 0; stage < odeSolver.num_stage; ++stage) {
+          // the explicit part only for stages 2, 3, 4
+          if (stage >= 2) {
+            interpolate();
+            dg_solu.set_rhs_zero();
+            fast_rhs_herm.rhs_vol_scalar();
+            fast_rhs_herm.rhs_flx_intp_scalar();
+            odeSolver.set_rhs_zero();
+            odeSolver.add_rhs_to_eigenvec();
+            odeSolver.add_rhs_matrix(linear);
+          }
+          odeSolver.step_stage(stage);
+          odeSolver.final();
+          total_cg_iterations += odeSolver.last_cg_iterations;
+        }
+      } else {
+        RK3SSP odeSolver(linear, dt);
+        odeSolver.init();
+        for (int stage = 0; stage < odeSolver.num_stage; ++stage) {
+          interpolate();
+          dg_solu.set_rhs_zero();
+          fast_rhs_herm.rhs_vol_scalar();
+          fast_rhs_herm.rhs_flx_intp_scalar();
+          odeSolver.set_rhs_zero();
+          odeSolver.add_rhs_to_eigenvec();
+          odeSolver.add_rhs_matrix(linear);
+          odeSolver.add_rhs_matrix(artific_visc_operator);
+          odeSolver.step_stage(stage);
+          odeSolver.final();
+        }
+      }
+      dg_solu.coarsen();
+      curr_time += dt;
+      num_time_step++;
+      continue;
+    }
     RK3SSP odeSolver(linear, dt);
     odeSolver.init();
-    const double h = 1. / std::pow(2., dg_solu.max_mesh_level());
     for (int stage = 0; stage < odeSolver.num_stage; ++stage) {
       interpolate();
       dg_solu.set_rhs_zero();
       fast_rhs_herm.rhs_vol_scalar();
       fast_rhs_herm.rhs_flx_intp_scalar();
-      if (num_visc_elem != 0) {
+      if (num_visc_elem != 0 && viscosity_option == 3) {
         interp_lagr.support_gradu_Lagr_fast(dg_solu.viscosity_element, -artificial_viscosity_nu0 * h, is_intp_lagr,
                                             fast_lagr_intp, support_overlap == 1);
         fast_rhs_lagr.rhs_vol_scalar();
@@ -185,7 +230,7 @@ int main(int argc, char* argv[]) {
   std::cout.precision(17);
   std::cout << "steps " << num_time_step << " time " << curr_time << " elements " << dg_solu.size_elem()
             << " dof " << dg_solu.get_dof() << " checksum " << checksum << " viscosity elements " << total_visc_elem
-            << std::endl;
+            << " cg iterations " << total_cg_iterations << std::endl;
   if (!state_file.empty()) {
     IO inout(dg_solu);
     inout.output_state(state_file);

@@ -619,6 +619,57 @@ void HyperbolicAlpt::assemble_matrix_flx_scalar(const int dim, const int sign, c
   has_flx_ = true;
 }
 
+ArtificialViscosity::ArtificialViscosity(DGSolution& dg, const OperatorMatrix1D<AlptBasis, AlptBasis>&,
+                                         AllBasis<AlptBasis>&)
+    : BilinearForm(dg) {}
+
+void ArtificialViscosity::assemble_matrix(const double viscosity_coefficient, const int viscosity_option) {
+  // ArtificialViscosity::assemble_matrix (source/BilinearForm.cpp:1317-1323) over the viscosity
+  // elements of the last DGAdapt::update_viscosity_element on this solution
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_viscosity_operator(c, viscosity_coefficient, viscosity_option, nullptr),
+                   "ArtificialViscosity::assemble_matrix");
+  assembled_ = true;
+}
+
+void ArtificialViscosity::apply_add() const {
+  if (!assembled_) return;  // an empty matrix
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_rhs_viscosity_operator(c), "ArtificialViscosity (rhs += A u)");
+}
+
+IMEX43::IMEX43(BilinearForm& implicit, const double dt_, const std::string linear_solver)
+    : ODESolver(implicit), dt(dt_) {
+  if (!dynamic_cast<ArtificialViscosity*>(&implicit)) {
+    std::cout << "IMEX43: the GPU library provides an ArtificialViscosity implicit part only" << std::endl;
+    std::exit(1);
+  }
+  if (linear_solver != "cg") {
+    std::cout << "IMEX43: the GPU library provides the conjugate-gradient solver (\"cg\") only" << std::endl;
+    std::exit(1);
+  }
+}
+
+void IMEX43::init() {
+  adg_facade_check(dgsolution_ptr->ctx(), adg_imex43_init(dgsolution_ptr->ctx(), dt), "IMEX43::init");
+}
+
+void IMEX43::step_stage(const int stage) {
+  adg_ctx* c = dgsolution_ptr->ctx();
+  const int rc = adg_imex43_stage(c, stage, &last_cg_iterations);
+  if (rc == ADG_E_NOCONVERGE) {
+    std::cout << "fail in IMEX43::step_stage() in stage " << stage << std::endl;
+    std::exit(1);
+  }
+  adg_facade_check(c, rc, "IMEX43::step_stage");
+}
+
+void IMEX43::step_rk() {
+  std::cout << "IMEX43::step_rk needs an assembled explicit operator, which the GPU library does not keep; "
+               "call step_stage with the explicit right-hand side" << std::endl;
+  std::exit(1);
+}
+
 ODESolver::ODESolver(BilinearForm& b) : dgoperator_ptr(&b), dgsolution_ptr(b.dgsolution_ptr) {}
 ODESolver::ODESolver(DGSolution& dg) : dgoperator_ptr(nullptr), dgsolution_ptr(&dg) {}
 void ODESolver::init() { adg_facade_check(dgsolution_ptr->ctx(), adg_rk_init(dgsolution_ptr->ctx()), "ODESolver::init"); }

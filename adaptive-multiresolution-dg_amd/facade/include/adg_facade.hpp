@@ -355,7 +355,7 @@ class BilinearForm {
   explicit BilinearForm(DGSolution& dgsolution);
   virtual ~BilinearForm() {}
   // rhs += A u
-  void apply_add() const;
+  virtual void apply_add() const;
   DGSolution* const dgsolution_ptr;
 
  protected:
@@ -374,6 +374,20 @@ class HyperbolicAlpt : public BilinearForm {
   void assemble_matrix_scalar_coarse_grid(const std::vector<double>& eqnCoefficient, const int mesh_nmax);
   void assemble_matrix_flx_scalar_coarse_grid(const int dim, const int sign, const int mesh_nmax,
                                               const double coefficient = 1.);
+};
+
+// ArtificialViscosity (include/BilinearForm.h:376-408): the matrix of
+// viscosity_coefficient * grad u . grad v over the viscosity elements' related
+// elements, assembled and applied on the device.
+class ArtificialViscosity : public BilinearForm {
+ public:
+  ArtificialViscosity(DGSolution& dgsolution, const OperatorMatrix1D<AlptBasis, AlptBasis>& oper_matx_alpt,
+                      AllBasis<AlptBasis>& all_basis_alpt);
+  void assemble_matrix(const double viscosity_coefficient, const int viscosity_option = 1);
+  void apply_add() const override;
+
+ private:
+  bool assembled_ = false;
 };
 
 class ODESolver {
@@ -420,6 +434,22 @@ class RK2Midpoint : public ODESolver {
   void step_stage(const int stage) override;
   const int num_stage = 2;
   const double dt;
+};
+
+// IMEX43 (include/ODESolver.h:571-635) with an ArtificialViscosity as the
+// implicit part: the implicit solves (conjugate gradient, the reference's
+// default "cg") run on the device.  step_stage(stage) for stages 2..4 takes
+// the solution's rhs (volume, flux and add_rhs_matrix terms) as the explicit
+// part.
+class IMEX43 : public ODESolver {
+ public:
+  IMEX43(BilinearForm& bilinearForm_implicit, const double dt, const std::string linear_solver = "cg");
+  void init() override;
+  void step_rk() override;
+  void step_stage(const int stage) override;
+  const int num_stage = 5;
+  const double dt;
+  int last_cg_iterations = 0;
 };
 
 // Text outputs of the reference's IO class (source/IO.cpp): point values on

@@ -83,6 +83,10 @@ def lib():
             "adg_set_refine_mode": (i32, [vp, i32, dbl]),
             "adg_viscosity_elements": (i32, [vp, dbl, P(i64), I32P]),
             "adg_viscosity_flux": (i32, [vp, dbl, i32]),
+            "adg_viscosity_operator": (i32, [vp, dbl, i32, ctypes.POINTER(i64)]),
+            "adg_rhs_viscosity_operator": (i32, [vp]),
+            "adg_imex43_init": (i32, [vp, dbl]),
+            "adg_imex43_stage": (i32, [vp, i32, ctypes.POINTER(i32)]),
             "adg_proposer_init": (i32, [vp, ctypes.c_uint64]),
             "adg_proposer_set_params": (i32, [vp, FP]),
             "adg_proposer_get_params": (i32, [vp, FP]),
@@ -275,6 +279,24 @@ class Solution:
 
     def viscosity_flux(self, coef, overlap=False):
         self._check(lib().adg_viscosity_flux(self._h, coef, 1 if overlap else 0))
+
+    def viscosity_operator(self, coef, option):
+        """ArtificialViscosity::assemble_matrix; returns the number of coupled element blocks"""
+        n = ctypes.c_int64()
+        self._check(lib().adg_viscosity_operator(self._h, coef, option, ctypes.byref(n)))
+        return n.value
+
+    def rhs_viscosity_operator(self):
+        self._check(lib().adg_rhs_viscosity_operator(self._h))
+
+    def imex43_init(self, dt):
+        self._check(lib().adg_imex43_init(self._h, dt))
+
+    def imex43_stage(self, stage):
+        """IMEX43::step_stage + final; returns the largest CG iteration count"""
+        it = ctypes.c_int()
+        self._check(lib().adg_imex43_stage(self._h, stage, ctypes.byref(it)))
+        return it.value
 
     # ---- learned refinement proposer ----
     def set_refine_mode(self, mode, threshold=0.5):

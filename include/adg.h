@@ -35,7 +35,8 @@ enum {
   ADG_E_HIP = -2,      /* HIP runtime error */
   ADG_E_ANCESTOR = -3, /* surplus transform found a missing ancestor (grid has a hole) */
   ADG_E_NONFINITE = -4,/* non-finite value detected */
-  ADG_E_INTERNAL = -5
+  ADG_E_INTERNAL = -5,
+  ADG_E_NOCONVERGE = -6 /* an implicit solve ended above its tolerance */
 };
 
 /* Fields (coefficient arrays) of a context. */
@@ -312,6 +313,35 @@ int adg_set_kernel_variant(adg_ctx* c, int variant);
  * Lagrange family selected. */
 int adg_viscosity_elements(adg_ctx* c, double shock_kappa, int64_t* count, int32_t* flags);
 int adg_viscosity_flux(adg_ctx* c, double coefficient, int overlap);
+
+/* Artificial viscosity as an assembled operator A (viscosity options 1 and 2
+ * of example/02_hyperbolic_05_burgers_adapt.cpp:343-420) and the IMEX43 time
+ * stepper that treats it implicitly.
+ * adg_viscosity_operator: ArtificialViscosity::assemble_matrix(coefficient,
+ * option) (source/BilinearForm.cpp:1190-1349) over the viscosity elements of
+ * the last adg_viscosity_elements call, every instance.  option 1: the
+ * elements whose supports intersect the viscosity element's in every dimension
+ * (ptr_vol_intp); option 2: the viscosity element and its ancestors
+ * (find_all_parents).  blocks (may be null): number of coupled (row element,
+ * column element) pairs.  Valid until the grid changes.
+ * adg_rhs_viscosity_operator: ODESolver::add_rhs_matrix(artific_visc_operator)
+ * (include/ODESolver.h:30): rhs += (A u), the product formed first.
+ * adg_imex43_init: IMEX43(artific_visc_operator, dt) with the default "cg"
+ * solver, then init() (source/ODESolver.cpp:883-912, include/ODESolver.h:93):
+ * keeps u^n and the diagonal of I - dt alpha A.  One dt for every instance.
+ * adg_imex43_stage: IMEX43::step_stage(stage) + final()
+ * (source/ODESolver.cpp:994-1070).  Stages 0 and 1 are implicit only; stages
+ * 2..4 take the rhs field, computed by the caller on the current u (volume,
+ * flux and penalty terms), as the explicit part.  Leaves the stage value in
+ * u.  Each implicit solve is Eigen's ConjugateGradient (Jacobi
+ * preconditioner, tolerance 1e-12, at most 1000 iterations), per instance.
+ * cg_iterations (may be null): the largest iteration count over the
+ * instances; ADG_E_NOCONVERGE when a solve ends above its tolerance (the
+ * reference exits). */
+int adg_viscosity_operator(adg_ctx* c, double coefficient, int option, int64_t* blocks);
+int adg_rhs_viscosity_operator(adg_ctx* c);
+int adg_imex43_init(adg_ctx* c, double dt);
+int adg_imex43_stage(adg_ctx* c, int stage, int* cg_iterations);
 
 /* Learned refinement proposer (north-star learned part; no reference
  * counterpart).  The reference refines by predicting u^{n+1} with one

@@ -87,8 +87,8 @@ def test_burgers_hermite_driver_with_artificial_viscosity(tmp_path):
     # keeps the run bounded; host closures and the device flux id still give identical final states.  The
     # parameters are those of the reference's own option-3 run line (explicit SSP RK3 needs cfl 0.01 with
     # nu0 = 2), at NMAX 5
-    args = ["-NM", "5", "-N0", "2", "-s", "0", "-r", "1e-3", "-c", "1e-4", "-v", "3", "-sp", "1", "-cfl", "0.01",
-            "-nu", "2.0", "-tf", "0.15"]
+    args = ["-NM", "5", "-N0", "2", "-s", "0", "-r", "1e-3", "-c", "1e-4", "-v", "3", "-imex", "0", "-sp", "1",
+            "-cfl", "0.01", "-nu", "2.0", "-tf", "0.15"]
     sa, sb = str(tmp_path / "dev.adgstate"), str(tmp_path / "host.adgstate")
     a = run("burgers_adapt_herm", *args, "-st", sa)
     b = run("burgers_adapt_herm", *args, "-hf", "1", "-st", sb)
@@ -99,6 +99,29 @@ def test_burgers_hermite_driver_with_artificial_viscosity(tmp_path):
     assert open(sa, "rb").read() == open(sb, "rb").read()
     assert int(re.search(r"viscosity elements (\d+)", la).group(1)) > 0
     assert abs(float(re.search(r"time ([0-9.eE+-]+)", la).group(1)) - 0.15) < 1e-12
+    _, insts = unpack(open(sa, "rb").read())
+    u = np.asarray(insts[0][2])
+    assert np.all(np.isfinite(u)) and np.max(np.abs(u)) < 10.
+
+
+@pytest.mark.parametrize("option,imex", [(1, 1), (2, 1), (1, 0)])
+def test_burgers_hermite_driver_with_assembled_viscosity(tmp_path, option, imex):
+    # options 1 and 2 on the reference's own IMEX run line (-v 1 -imex 1 -cfl 0.2 -nu 2.0), NMAX 5, past the
+    # shock time: the assembled viscosity operator, implicit in IMEX43 (or explicit in SSP-RK3 at the smaller
+    # step of the option-3 line), keeps the run bounded; host closures and the device flux id agree bit for bit
+    cfl = "0.2" if imex else "0.01"
+    args = ["-NM", "5", "-N0", "2", "-s", "0", "-r", "1e-3", "-c", "1e-4", "-v", str(option), "-imex", str(imex),
+            "-cfl", cfl, "-nu", "2.0", "-tf", "0.15"]
+    sa, sb = str(tmp_path / "dev.adgstate"), str(tmp_path / "host.adgstate")
+    a = run("burgers_adapt_herm", *args, "-st", sa)
+    b = run("burgers_adapt_herm", *args, "-hf", "1", "-st", sb)
+    la = [l for l in a.splitlines() if l.startswith("steps")][0]
+    lb = [l for l in b.splitlines() if l.startswith("steps")][0]
+    print(f"[hermite burgers, viscosity option {option}, imex {imex}]", la, flush=True)
+    assert la == lb
+    assert open(sa, "rb").read() == open(sb, "rb").read()
+    assert int(re.search(r"viscosity elements (\d+)", la).group(1)) > 0
+    assert (int(re.search(r"cg iterations (\d+)", la).group(1)) > 0) == (imex == 1)
     _, insts = unpack(open(sa, "rb").read())
     u = np.asarray(insts[0][2])
     assert np.all(np.isfinite(u)) and np.max(np.abs(u)) < 10.
