@@ -870,7 +870,7 @@ int adg_set_instance_thresholds(adg_ctx* h, const double* eps, const double* eta
 
 int adg_set_kernel_variant(adg_ctx* h, int variant) {
   return guard(h, [&](adg::Context& c) {
-    if (variant < -1 || variant > 2) throw std::runtime_error("kernel variant must be -1, 0, 1 or 2");
+    if (variant < -1 || variant > 3) throw std::runtime_error("kernel variant must be -1, 0, 1, 2 or 3");
     c.kernel_variant = variant;
   });
 }

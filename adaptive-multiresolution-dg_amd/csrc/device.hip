@@ -783,14 +783,15 @@ __device__ __forceinline__ void strides_dev(const int* sz, int D, int* st) {
 }
 
 // acc[r][i] = sum over related sources of element e, in ascending slot order,
-// of sum_k in[src][bin[r] + k*sdd] * M(s*PF1+k, st*PF1+i), for r < nr.
-// Returns false when e lies above the coarse-grid cap (its outputs are zero).
-template <int PF1, int PT1, class T>
+// of sum_k in[src][bin[r] + k*sdd] * M(s*PF1+k, st*PF1+i), for r < nr <= RC.
+// RC (1 or kRC) sizes the register arrays.  Returns false when e lies above
+// the coarse-grid cap (its outputs are zero).
+template <int PF1, int PT1, class T, int RC>
 __device__ __forceinline__ bool term_block(const TermDev& t, int64_t e, const int* bin, int nr, int cap,
-                                           T (&acc)[kRC][PT1]) {
+                                           T (&acc)[RC][PT1]) {
   using Ops = SweepOps<T>;
 #pragma unroll
-  for (int r = 0; r < kRC; ++r)
+  for (int r = 0; r < RC; ++r)
 #pragma unroll
     for (int i = 0; i < PT1; ++i) acc[r][i] = T(0);
   const int st = t.slot[e];
@@ -814,7 +815,7 @@ __device__ __forceinline__ bool term_block(const TermDev& t, int64_t e, const in
       for (int i = 0; i < PT1; ++i) m[k][i] = Ops::mat(t, Mr + k * PT1 + i);
     const int64_t xs = static_cast<int64_t>(src) * t.Tin;
 #pragma unroll
-    for (int r = 0; r < kRC; ++r) {
+    for (int r = 0; r < RC; ++r) {
       if (r >= nr) break;
       T xv[PF1];
 #pragma unroll
@@ -830,24 +831,24 @@ __device__ __forceinline__ bool term_block(const TermDev& t, int64_t e, const in
 
 // Evaluates term t for (element e, chunk c) and stores coef * value at
 // out[bout + i*so_dd] (zero above the coarse-grid cap).
-template <int PF1, int PT1, class T, class O>
+template <int PF1, int PT1, class T, class O, int RC>
 __device__ __forceinline__ void term_chunk_store(const TermDev& t, int64_t e, int c, int rc, int D, const int* so,
                                                  int cap, O* out) {
   const int R = t.Tin / t.isz[t.dd];
   const int r0 = c * rc;
   const int nr = min(rc, R - r0);
-  int bin[kRC], bout[kRC];
+  int bin[RC], bout[RC];
 #pragma unroll
-  for (int r = 0; r < kRC; ++r) {
+  for (int r = 0; r < RC; ++r) {
     bin[r] = 0;
     bout[r] = 0;
     if (r < nr) rest_offsets(r0 + r, t.dd, D, t.isz, t.sin, so, bin[r], bout[r]);
   }
-  T acc[kRC][PT1];
-  const bool live = term_block<PF1, PT1, T>(t, e, bin, nr, cap, acc);
+  T acc[RC][PT1];
+  const bool live = term_block<PF1, PT1, T, RC>(t, e, bin, nr, cap, acc);
   const int sod = so[t.dd];
 #pragma unroll
-  for (int r = 0; r < kRC; ++r) {
+  for (int r = 0; r < RC; ++r) {
     if (r >= nr) break;
 #pragma unroll
     for (int i = 0; i < PT1; ++i) out[bout[r] + i * sod] = live ? O(acc[r][i] * T(t.coef)) : O(0);
@@ -857,7 +858,7 @@ __device__ __forceinline__ void term_chunk_store(const TermDev& t, int64_t e, in
 // First sweeps of the partial sums of one phase: thread per (task, element,
 // chunk of rc rest indices); variant v (1: rc = 1, 2: rc = kRC) selects the
 // work-item offsets.
-template <int PF1, int PT1, class T>
+template <int PF1, int PT1, class T, int RC>
 __global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntask, int64_t total, int D, int cap,
                                                   int v, int rc) {
   const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -871,14 +872,14 @@ __global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntas
   const int c = static_cast<int>(local - e * nch);
   int so[kMaxDim];
   strides_dev(tk.osz, D, so);
-  term_chunk_store<PF1, PT1, T, T>(tk.t, e, c, rc, D, so, cap, SweepOps<T>::out(tk) + e * tk.Tout);
+  term_chunk_store<PF1, PT1, T, T, RC>(tk.t, e, c, rc, D, so, cap, SweepOps<T>::out(tk) + e * tk.Tout);
 }
 
 // Last sweep of every eval_up partial sum, accumulated in ordering order into
 // the point value, followed by the point flux (LagrInterpolation::eval_fp_Lag).
 // A block holds EB elements; threads are (term, element, chunk), term-major,
 // and write their values to LDS; the point values then add the terms in order.
-template <int PF1, int PT1, class T>
+template <int PF1, int PT1, class T, int RC>
 __global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, int nterm, int EB, int nch, int rc,
                                                            int64_t E, int P, int D, T* up, T* fp,
                                                            const double* vcoef, int kind, double s, int ncomp, int cap) {
@@ -893,7 +894,7 @@ __global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, i
   strides_dev(osz, D, so);
   if (e0 + el < E)
     for (int t = slot; t < nterm; t += nslot)
-      term_chunk_store<PF1, PT1, T, double>(terms[t], e0 + el, c, rc, D, so, cap,
+      term_chunk_store<PF1, PT1, T, double, RC>(terms[t], e0 + el, c, rc, D, so, cap,
                                             vals + (static_cast<int64_t>(t) * EB + el) * P);
   __syncthreads();
   const int64_t n = E * P;
@@ -913,7 +914,7 @@ __global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, i
 // accumulated in the reference order, then the RK stage update written to
 // u_next (u itself is still read by other threads' penalty sweeps).  Same
 // block structure as k_up_final.
-template <int PFM, int PFK, class T>
+template <int PFM, int PFK, class T, int RC>
 __global__ __launch_bounds__(kElemBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens,
                                                             int npen, int EB, int nch, int rc, int64_t E, int K, int D,
                                                             double* rhs, const double* u, const double* tn,
@@ -931,8 +932,8 @@ __global__ __launch_bounds__(kElemBlock) void k_rhs_final(const TermDev* terms, 
   if (e0 + el < E)
     for (int t = slot; t < nterm + npen; t += nslot) {
       double* out = vals + (static_cast<int64_t>(t) * EB + el) * K;
-      if (t < nterm) term_chunk_store<PFM, PFK, T, double>(terms[t], e0 + el, c, rc, D, so, cap, out);
-      else term_chunk_store<PFK, PFK, double, double>(pens[t - nterm], e0 + el, c, rc, D, so, cap, out);
+      if (t < nterm) term_chunk_store<PFM, PFK, T, double, RC>(terms[t], e0 + el, c, rc, D, so, cap, out);
+      else term_chunk_store<PFK, PFK, double, double, RC>(pens[t - nterm], e0 + el, c, rc, D, so, cap, out);
     }
   __syncthreads();
   for (int o = tid; o < EB * K; o += blockDim.x) {
@@ -951,9 +952,12 @@ template <int PF1, int PT1, class T>
 void launch_phase_t(const TaskDev* tasks, int ntask, int64_t total, int D, int cap, int v, hipStream_t s) {
   if (v == 0)
     hipLaunchKernelGGL((k_phase_pt<PF1, T>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap);
+  else if (v == 1)
+    hipLaunchKernelGGL((k_phase<PF1, PT1, T, 1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D,
+                       cap, v, 1);
   else
-    hipLaunchKernelGGL((k_phase<PF1, PT1, T>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap,
-                       v, v == 1 ? 1 : kRC);
+    hipLaunchKernelGGL((k_phase<PF1, PT1, T, kRC>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D,
+                       cap, v, kRC);
 }
 
 // v: 0 thread per output, 1 thread per (element, rest index), 2 thread per
@@ -1814,7 +1818,10 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     const TaskDev* tasks = pl.up_tasks.p + off;
     const int nt = pl.up_phase_ntask[s];
     const std::array<int64_t, 3> tot = pl.up_phase_total[s];
-    tuned_launch(TUNE_UP_PHASE + tc, [&](int v) { launch_phase<T>(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
+    tuned_launch(TUNE_UP_PHASE + tc, [&](int v) {
+      if (v > 2) throw std::runtime_error("phase kernels: no mapping 3");
+      launch_phase<T>(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
+    });
     prof_end();
     off += pl.up_phase_ntask[s];
   }
@@ -1834,7 +1841,7 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
 #undef ADG_UP
         return;
       }
-      const int rc = v == 1 ? 1 : kRC;
+      const int rc = v == 1 ? 1 : (v == 3 ? 2 : kRC);
       const int R = P / (m + 1);
       const int nch = (R + rc - 1) / rc;
       const int EB = final_elements_per_block(pl.n_up_terms, nch, P);
@@ -1842,9 +1849,18 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
       const unsigned bs = static_cast<unsigned>(final_term_slots(pl.n_up_terms, nch, EB) * EB * nch);
       const size_t lds = sizeof(double) * pl.n_up_terms * EB * P;
 #define ADG_UF(K1, M1)                                                                                             \
-  if (k + 1 == K1 && m + 1 == M1)                                                                                  \
-    hipLaunchKernelGGL((k_up_final<K1, M1, T>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p, pl.n_up_terms, EB, nch, \
-                       rc, E(), P, D, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);
+  if (k + 1 == K1 && m + 1 == M1) {                                                                                \
+    if (rc == 1)                                                                                                   \
+      hipLaunchKernelGGL((k_up_final<K1, M1, T, 1>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p, pl.n_up_terms, \
+                         EB, nch, rc, E(), P, D, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);      \
+    else if (rc == 2)                                                                                              \
+      hipLaunchKernelGGL((k_up_final<K1, M1, T, 2>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p, pl.n_up_terms, \
+                         EB, nch, rc, E(), P, D, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);      \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_up_final<K1, M1, T, kRC>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p,              \
+                         pl.n_up_terms, EB, nch, rc, E(), P, D, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp,  \
+                         lsum_cap);                                                                                \
+  }
       ADG_UF(2, 2) ADG_UF(2, 3) ADG_UF(2, 4) ADG_UF(2, 5) ADG_UF(3, 2) ADG_UF(3, 3) ADG_UF(3, 4) ADG_UF(3, 5)
       ADG_UF(4, 2) ADG_UF(4, 3) ADG_UF(4, 4) ADG_UF(4, 5)
 #undef ADG_UF
@@ -1862,7 +1878,10 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     const TaskDev* tasks = pl.rhs_tasks.p + off;
     const int nt = pl.rhs_phase_ntask[s];
     const std::array<int64_t, 3> tot = pl.rhs_phase_total[s];
-    tuned_launch(TUNE_RHS_PHASE + tc, [&](int v) { launch_phase<T>(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
+    tuned_launch(TUNE_RHS_PHASE + tc, [&](int v) {
+      if (v > 2) throw std::runtime_error("phase kernels: no mapping 3");
+      launch_phase<T>(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
+    });
     prof_end();
     off += pl.rhs_phase_ntask[s];
   }
@@ -1886,7 +1905,7 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
 #undef ADG_RP
         return;
       }
-      const int rc = v == 1 ? 1 : kRC;
+      const int rc = v == 1 ? 1 : (v == 3 ? 2 : kRC);
       const int R = K / (k + 1);
       const int nch = (R + rc - 1) / rc;
       const int EB = final_elements_per_block(nt, nch, K);
@@ -1894,10 +1913,20 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
       const unsigned bs = static_cast<unsigned>(final_term_slots(nt, nch, EB) * EB * nch);
       const size_t lds = sizeof(double) * nt * EB * K;
 #define ADG_RF(M1, K1)                                                                                             \
-  if (m + 1 == M1 && k + 1 == K1)                                                                                  \
-    hipLaunchKernelGGL((k_rhs_final<M1, K1, T>), dim3(nb), dim3(bs), lds, stream, pl.rhs_terms.p, pl.n_rhs_terms,     \
-                       pl.pens.p, pl.n_pen, EB, nch, rc, E(), K, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt,  \
-                       dtv, g_inst.p, lsum_cap);
+  if (m + 1 == M1 && k + 1 == K1) {                                                                                \
+    if (rc == 1)                                                                                                   \
+      hipLaunchKernelGGL((k_rhs_final<M1, K1, T, 1>), dim3(nb), dim3(bs), lds, stream, pl.rhs_terms.p,              \
+                         pl.n_rhs_terms, pl.pens.p, pl.n_pen, EB, nch, rc, E(), K, D, rhs.p, u.p, tn.p, u_next.p,  \
+                         scheme, stage, dt, dtv, g_inst.p, lsum_cap);                                              \
+    else if (rc == 2)                                                                                              \
+      hipLaunchKernelGGL((k_rhs_final<M1, K1, T, 2>), dim3(nb), dim3(bs), lds, stream, pl.rhs_terms.p,              \
+                         pl.n_rhs_terms, pl.pens.p, pl.n_pen, EB, nch, rc, E(), K, D, rhs.p, u.p, tn.p, u_next.p,  \
+                         scheme, stage, dt, dtv, g_inst.p, lsum_cap);                                              \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_rhs_final<M1, K1, T, kRC>), dim3(nb), dim3(bs), lds, stream, pl.rhs_terms.p,            \
+                         pl.n_rhs_terms, pl.pens.p, pl.n_pen, EB, nch, rc, E(), K, D, rhs.p, u.p, tn.p, u_next.p,  \
+                         scheme, stage, dt, dtv, g_inst.p, lsum_cap);                                              \
+  }
       ADG_RF(2, 2) ADG_RF(2, 3) ADG_RF(2, 4) ADG_RF(3, 2) ADG_RF(3, 3) ADG_RF(3, 4) ADG_RF(4, 2) ADG_RF(4, 3)
       ADG_RF(4, 4) ADG_RF(5, 2) ADG_RF(5, 3) ADG_RF(5, 4)
 #undef ADG_RF
@@ -2362,7 +2391,7 @@ void Context::tuned_launch(int cls, const std::function<void(int)>& launch) {
   hip_check(hipEventCreate(&b), "hipEventCreate");
   float best = 0.f;
   int bv = 0;
-  for (int v = 0; v < 3; ++v) {
+  for (int v = 0; v < 4; ++v) {
     hip_check(hipEventRecord(a, stream), "hipEventRecord");
     if (!try_launch(v)) continue;
     hip_check(hipEventRecord(b, stream), "hipEventRecord");
