@@ -562,7 +562,7 @@ struct TaskDev {  // a term written to an output buffer
   int Tout;
   int osz[kMaxDim];
   int R;              // rest indices per element (Tin / input size in the swept dimension)
-  int64_t begin_v[3];  // first work item of this task in the phase, per thread mapping
+  int64_t begin_v[4];  // first work item of this task in the phase, per thread mapping
 };
 
 // Arithmetic of a sweep in the storage type T: FP64 (x * M added in order)
@@ -955,13 +955,17 @@ void launch_phase_t(const TaskDev* tasks, int ntask, int64_t total, int D, int c
   else if (v == 1)
     hipLaunchKernelGGL((k_phase<PF1, PT1, T, 1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D,
                        cap, v, 1);
+  else if (v == 3)
+    hipLaunchKernelGGL((k_phase<PF1, PT1, T, 2>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D,
+                       cap, v, 2);
   else
     hipLaunchKernelGGL((k_phase<PF1, PT1, T, kRC>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D,
                        cap, v, kRC);
 }
 
 // v: 0 thread per output, 1 thread per (element, rest index), 2 thread per
-// (element, kRC rest indices); total counts the work items of that mapping.
+// (element, kRC rest indices), 3 thread per (element, 2 rest indices); total
+// counts the work items of that mapping.
 template <class R>
 void launch_phase(int pf1, int pt1, const TaskDev* tasks, int ntask, int64_t total, int D, int cap, int v,
                   hipStream_t s) {
@@ -1128,7 +1132,7 @@ struct Context::FusedPlan {
   int flux_kind = -1;
   double alpha[kMaxDim] = {0, 0, 0, 0};
   std::vector<int> up_phase_ntask, rhs_phase_ntask;
-  std::vector<std::array<int64_t, 3>> up_phase_total, rhs_phase_total;  // work items per thread mapping
+  std::vector<std::array<int64_t, 4>> up_phase_total, rhs_phase_total;  // work items per thread mapping
   int n_up_terms = 0, n_rhs_terms = 0, n_pen = 0;
   // algorithmic bytes per launch (8 B x elements x (inputs + outputs))
   std::vector<double> up_phase_bytes, rhs_phase_bytes;
@@ -1700,19 +1704,20 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     return TermDev{};
   };
   auto finish_phases = [&](std::vector<std::vector<TaskDev>>& ph, DevBuf<TaskDev>& dev, std::vector<int>& ntask,
-                           std::vector<std::array<int64_t, 3>>& total, std::vector<double>& bytes) {
+                           std::vector<std::array<int64_t, 4>>& total, std::vector<double>& bytes) {
     std::vector<TaskDev> flat;
     ntask.clear();
     total.clear();
     bytes.clear();
     for (auto& tasks : ph) {
-      std::array<int64_t, 3> begin{0, 0, 0};
+      std::array<int64_t, 4> begin{0, 0, 0, 0};
       double b = 0;
       for (TaskDev& tk : tasks) {
-        for (int v = 0; v < 3; ++v) tk.begin_v[v] = begin[v];
+        for (int v = 0; v < 4; ++v) tk.begin_v[v] = begin[v];
         begin[0] += E() * tk.Tout;
         begin[1] += E() * tk.R;
         begin[2] += E() * ((tk.R + kRC - 1) / kRC);
+        begin[3] += E() * ((tk.R + 1) / 2);
         b += E() * (in_bytes(tk.t) * tk.t.Tin + w * tk.Tout);
       }
       bytes.push_back(b);
@@ -1817,11 +1822,8 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     prof_begin(PROF_SWEEP, pl.up_phase_bytes[s]);
     const TaskDev* tasks = pl.up_tasks.p + off;
     const int nt = pl.up_phase_ntask[s];
-    const std::array<int64_t, 3> tot = pl.up_phase_total[s];
-    tuned_launch(TUNE_UP_PHASE + tc, [&](int v) {
-      if (v > 2) throw std::runtime_error("phase kernels: no mapping 3");
-      launch_phase<T>(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
-    });
+    const std::array<int64_t, 4> tot = pl.up_phase_total[s];
+    tuned_launch(TUNE_UP_PHASE + tc, [&](int v) { launch_phase<T>(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
     prof_end();
     off += pl.up_phase_ntask[s];
   }
@@ -1877,11 +1879,8 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     prof_begin(PROF_SWEEP, pl.rhs_phase_bytes[s]);
     const TaskDev* tasks = pl.rhs_tasks.p + off;
     const int nt = pl.rhs_phase_ntask[s];
-    const std::array<int64_t, 3> tot = pl.rhs_phase_total[s];
-    tuned_launch(TUNE_RHS_PHASE + tc, [&](int v) {
-      if (v > 2) throw std::runtime_error("phase kernels: no mapping 3");
-      launch_phase<T>(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
-    });
+    const std::array<int64_t, 4> tot = pl.rhs_phase_total[s];
+    tuned_launch(TUNE_RHS_PHASE + tc, [&](int v) { launch_phase<T>(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
     prof_end();
     off += pl.rhs_phase_ntask[s];
   }

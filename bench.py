@@ -100,7 +100,7 @@ def main():
     ap.add_argument("--no-events", action="store_true", help="time without per-kernel HIP events")
     ap.add_argument("--time-steps", type=int, default=0,
                     help="cap the time steps of every solve (0: run to the workload's final time)")
-    ap.add_argument("--streams", type=int, default=2,
+    ap.add_argument("--streams", type=int, default=3,
                     help="contexts (HIP streams, host threads) the batch is split over in the chosen form")
     ap.add_argument("--batch", type=int, default=1024,
                     help="independent seeded instances solved together in the chosen form (c2, c5)")

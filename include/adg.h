@@ -289,8 +289,7 @@ int adg_set_instance_thresholds(adg_ctx* c, const double* eps, const double* eta
 /* Thread mapping of the fused stage kernels (no counterpart in the
  * reference): 0 one thread per output value, 1 one thread per (element, index
  * over the non-swept dimensions), 2 one thread per (element, four such
- * indices), 3 one thread per (element, two such indices) in the last kernels
- * of a stage (the first sweeps then use mapping 0).  All mappings give
+ * indices), 3 one thread per (element, two such indices).  All mappings give
  * identical results.  -1 (the default) times them on the live grid the first
  * time a kernel meets a grid-size class and keeps the fastest. */
 int adg_set_kernel_variant(adg_ctx* c, int variant);
