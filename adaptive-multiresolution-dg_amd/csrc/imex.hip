@@ -15,9 +15,9 @@
 // thread-per-row product whose rows accumulate in increasing column order, the
 // order of Eigen's column-major sparse product (bit-exact); the products inside
 // the implicit solves, whose result is fixed only to the solver's tolerance,
-// give each element row a workgroup that splits the row's entries over its
-// threads and adds the partial sums in a fixed tree (deterministic, and with
-// hundreds of entries per row far more parallel than a thread per row).
+// give each row a workgroup that splits the row's entries over its threads and
+// adds the partial sums in a fixed tree (deterministic, and with hundreds of
+// entries per row far more parallel than a thread per row).
 //
 // IMEX43 (source/ODESolver.cpp:872-1070) solves (I - dt alpha A) x = b four
 // times per step with Eigen's ConjugateGradient (Jacobi preconditioner,
@@ -110,39 +110,34 @@ __global__ __launch_bounds__(kVB) void k_vop_spmv(int64_t n, int K, const int32_
   else y[r] = (b0 == b1) ? x[r] : t;
 }
 
-// y = A x (mode 0) or y = (I - c A) x (mode 2): a workgroup per element row,
-// entries of each of the K rows split over the threads, partial sums added in
-// a fixed tree.
+// y = A x (mode 0) or y = (I - c A) x (mode 2): a workgroup per row, the
+// row's entries split over the threads, partial sums added in a fixed tree.
 constexpr int kWB = 256;
 __global__ __launch_bounds__(kWB) void k_vop_spmv_wide(int K, const int32_t* rowptr, const int32_t* col,
                                                        const double* val, const double* x, double* y, int mode,
                                                        double c) {
   __shared__ double red[kWB];
-  const int64_t e = blockIdx.x;
+  const int64_t r = blockIdx.x;
+  const int64_t e = r / K;
+  const int i = static_cast<int>(r - e * K);
   const int b0 = rowptr[e], b1 = rowptr[e + 1];
   const int nent = (b1 - b0) * K;
-  for (int i = 0; i < K; ++i) {
-    double t = 0.;
-    for (int q = threadIdx.x; q < nent; q += kWB) {
-      const int bq = q / K, j = q - bq * K;
-      const int b = b0 + bq;
-      const int64_t ce = col[b];
-      const double a = val[(static_cast<size_t>(b) * K + i) * K + j];
-      const double m = (mode == 2) ? ((ce == e && j == i) ? 1. - c * a : 0. - c * a) : a;
-      t += m * x[ce * K + j];
-    }
-    red[threadIdx.x] = t;
-    __syncthreads();
-    for (int w = kWB / 2; w > 0; w /= 2) {
-      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      const int64_t r = e * K + i;
-      y[r] = (mode == 2 && b0 == b1) ? x[r] : red[0];
-    }
+  double t = 0.;
+  for (int q = threadIdx.x; q < nent; q += kWB) {
+    const int bq = q / K, j = q - bq * K;
+    const int b = b0 + bq;
+    const int64_t ce = col[b];
+    const double a = val[(static_cast<size_t>(b) * K + i) * K + j];
+    const double m = (mode == 2) ? ((ce == e && j == i) ? 1. - c * a : 0. - c * a) : a;
+    t += m * x[ce * K + j];
+  }
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = kWB / 2; w > 0; w /= 2) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
+  if (threadIdx.x == 0) y[r] = (mode == 2 && b0 == b1) ? x[r] : red[0];
 }
 
 // Jacobi preconditioner of I - c A: 1 / diagonal when non-zero, else 1.
@@ -486,7 +481,7 @@ void Context::visc_operator_apply(const double* x, double* y, int mode, double c
     hipLaunchKernelGGL(k_vop_spmv, dim3(vblocks(n)), dim3(kVB), 0, stream, n, K, vop_rowptr.p, vop_col.p, vop_val.p,
                        x, y, mode, c);
   else
-    hipLaunchKernelGGL(k_vop_spmv_wide, dim3(static_cast<unsigned>(E())), dim3(kWB), 0, stream, K, vop_rowptr.p,
+    hipLaunchKernelGGL(k_vop_spmv_wide, dim3(static_cast<unsigned>(n)), dim3(kWB), 0, stream, K, vop_rowptr.p,
                        vop_col.p, vop_val.p, x, y, mode, c);
 }
 
