@@ -24,7 +24,7 @@
 // tolerance 1e-12, at most 1000 iterations, zero initial guess).  The solve runs
 // on the device for every instance at once: per-instance dot products
 // (deterministic two-level sums over fixed chunks), per-instance scalars and
-// convergence flags, one flag read-back per iteration.
+// convergence flags, read back every fourth iteration.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -542,13 +542,20 @@ int Context::cg_solve(double c) {
   dots(cg_r.p, nullptr, 1);
   hipLaunchKernelGGL(k_cg_begin, dim3((nB + 63) / 64), dim3(64), 0, stream, nB, cg_scal.p, cg_state.p);
   std::vector<int32_t> state(2 * nB);
-  for (int iter = 0; iter <= kCgMaxIters; ++iter) {
-    hip_check(hipMemcpyAsync(state.data(), cg_state.p, state.size() * sizeof(int32_t), hipMemcpyDeviceToHost, stream),
-              "CG state");
-    hip_check(hipStreamSynchronize(stream), "CG state");
-    bool any = false;
-    for (int i = 0; i < nB; ++i) any = any || state[2 * i];
-    if (!any) break;
+  // The flags are read every kCheck iterations: an instance that converged
+  // in between is skipped by every update kernel, so the extra launches
+  // change nothing.
+  constexpr int kCheck = 4;
+  for (int iter = 0; iter <= kCgMaxIters + kCheck; ++iter) {
+    if (iter % kCheck == 0) {
+      hip_check(hipMemcpyAsync(state.data(), cg_state.p, state.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               stream),
+                "CG state");
+      hip_check(hipStreamSynchronize(stream), "CG state");
+      bool any = false;
+      for (int i = 0; i < nB; ++i) any = any || state[2 * i];
+      if (!any) break;
+    }
     visc_operator_apply(cg_p.p, cg_t.p, 2, c);
     dots(cg_p.p, cg_t.p, 0);
     hipLaunchKernelGGL(k_cg_update, dim3(vblocks(n)), dim3(kVB), 0, stream, n, K, g_inst.p, cg_state.p, cg_scal.p,
@@ -560,6 +567,8 @@ int Context::cg_solve(double c) {
   }
   hip_check(hipGetLastError(), "CG");
   std::vector<double> scal(static_cast<size_t>(nB) * kS);
+  hip_check(hipMemcpyAsync(state.data(), cg_state.p, state.size() * sizeof(int32_t), hipMemcpyDeviceToHost, stream),
+            "CG state");
   hip_check(hipMemcpyAsync(scal.data(), cg_scal.p, scal.size() * sizeof(double), hipMemcpyDeviceToHost, stream),
             "CG scalars");
   hip_check(hipStreamSynchronize(stream), "CG scalars");
