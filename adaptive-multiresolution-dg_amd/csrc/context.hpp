@@ -217,8 +217,8 @@ class Context {
   std::map<const double*, std::unique_ptr<SplitTable>> split_tabs;
   const SplitTable& split_table(const double* dev, size_t n);
   // thread mapping of the fused kernels: 0 per output, 1 per (element, rest
-  // index), 2 per (element, 4 rest indices); -1 lets the autotuner choose
-  // per kernel class and grid-size class
+  // index), 2 per (element, 4 rest indices), 3 per (element, 2 rest indices);
+  // -1 lets the autotuner choose per kernel class and grid-size class
   enum { TUNE_UP_PHASE = 0, TUNE_UP_FINAL, TUNE_RHS_PHASE, TUNE_RHS_FINAL };
   int kernel_variant = -1;
   std::map<int, int> tuned;

@@ -2356,7 +2356,7 @@ void Context::eval_tensor(int nx, const double* x1d, double* out_host) {
   sync();
 }
 
-// Thread-mapping autotuner of the fused stage kernels.  The three mappings
+// Thread-mapping autotuner of the fused stage kernels.  The four mappings
 // compute identical values (same sums in the same order), so the first time
 // a kernel class meets a grid-size class, every mapping is launched once on
 // the live data with events around it and the fastest is kept; the outputs of
