@@ -343,8 +343,7 @@ int adg_rhs_viscosity_operator(adg_ctx* c);
 int adg_imex43_init(adg_ctx* c, double dt);
 int adg_imex43_stage(adg_ctx* c, int stage, int* cg_iterations);
 
-/* Learned refinement proposer (north-star learned part; no reference
- * counterpart).  The reference refines by predicting u^{n+1} with one
+/* Learned refinement proposer (no reference counterpart).  The reference refines by predicting u^{n+1} with one
  * forward-Euler stage and giving every leaf whose predicted norm exceeds eps
  * its children (example/02_hyperbolic_05_burgers_adapt.cpp:246-291,
  * DGAdapt::refine, source/DGAdapt.cpp:371-404): ADG_REFINE_PREDICT, the
@@ -391,9 +390,8 @@ int64_t adg_decision_log_size(adg_ctx* c);
 int adg_decision_log_get(adg_ctx* c, float* feats, uint8_t* labels);
 int adg_decision_log_write(adg_ctx* c, const char* path);
 
-/* Storage precision of adg_stage and of adg_run_hyperbolic's fused form
- * (north-star precision form; no reference counterpart, the reference is FP64
- * throughout).  ADG_PREC_FP32_SPLIT runs the fiber transforms, point values,
+/* Storage precision of adg_stage and of adg_run_hyperbolic's fused form (no
+ * reference counterpart: the reference is FP64 throughout).  ADG_PREC_FP32_SPLIT runs the fiber transforms, point values,
  * flux and surplus transform in FP32, every 1D table and surplus weight split
  * into two FP32 parts (M = hi + lo, both applied by fused multiply-adds); the
  * right-hand side accumulation, the LF penalty, the RK combination and the
