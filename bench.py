@@ -88,6 +88,66 @@ def cpu_baseline(w, max_seconds, threads):
                                "transform_1D loop, g++ -O3 -fopenmp" % (dof_stages, sec)}
 
 
+def imex_sample(args):
+    """IMEX43 with the assembled artificial viscosity (viscosity option 1 of
+    example/02_hyperbolic_05_burgers_adapt.cpp:343-420) on a fixed grid: the
+    rotating-wave initial value of c5 (seed 1) on the NMAX 6 grid its adaptive
+    initialisation builds, shock detector kappa 0, viscosity -nu0 h with
+    nu0 = 2, h = 2^-6, LF flux terms as the explicit part.  A time step is the
+    detector, the assembly and the five IMEX43 stages; the GPU and the CPU
+    oracle run the same steps from the same state.  Prints one JSON line."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    import runs
+
+    w = workloads.c5_rotating(seed=1, nmax=6, level_init=4)
+    prob = pyadg.Problem.from_buffer_copy(w.problem)
+    dt, coef = 0.05 / (2 * 0.25 * 2 ** 6), -2.0 / 2 ** 6
+
+    def step(s):
+        s.viscosity_elements(0.0)
+        s.viscosity_operator(coef, 1)
+        s.imex43_init(dt)
+        it = 0
+        for st in range(5):
+            if st >= 2:
+                runs.hyperbolic_rhs(s, prob)
+            it += max(s.imex43_stage(st), 0)
+        return it
+
+    g = workloads.make_solution(w)
+    step(g)  # warm-up
+    g.sync()
+    n_gpu, t0 = 20, time.perf_counter()
+    iters = sum(step(g) for _ in range(n_gpu))
+    g.sync()
+    gpu_sec = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    pyoracle.set_threads(threads)
+    o = pyoracle.Oracle(w.dim, w.k, w.m, w.nmax, w.sparse, w.level_init)
+    o._keep.append(w.ic)
+    o.init_adaptive_intp_c(pyadg.ic_eval_ptr(), ctypes.cast(ctypes.pointer(w.ic), ctypes.c_void_p), w.init_eps)
+    step(o)  # the same first step as the GPU's warm-up
+    n_cpu, cpu_sec = 0, 0.0
+    while n_cpu < n_gpu and cpu_sec < args.cpu_seconds:
+        t0 = time.perf_counter()
+        step(o)
+        cpu_sec += time.perf_counter() - t0
+        n_cpu += 1
+    dofs = g.num_elem * g.K
+    print(json.dumps({
+        "metric": "IMEX43 time steps/s with the assembled artificial viscosity (option 1), fixed grid",
+        "value": n_gpu / gpu_sec, "unit": "time steps/s", "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic", "config": {"workload": "c5 rotating-wave initial value (seed 1), NMAX 6, level_init 4",
+                                        "elements": g.num_elem, "dofs": dofs, "dt": dt, "viscosity": coef,
+                                        "gpu_steps": n_gpu, "cg_iterations_per_step": iters / n_gpu},
+        "ms_per_step": 1e3 * gpu_sec / n_gpu,
+        "cpu_baseline": {"value": n_cpu / cpu_sec, "unit": "time steps/s", "cores": threads, "kind": "port",
+                         "sample": "CPU oracle, %d IMEX43 steps of the same problem after the same first step "
+                                   "(assembly as std::map of entries, conjugate gradient in index order)" % n_cpu},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,11 +173,17 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"),
                     help="storage precision of the chosen form's fused stages (fp32: split FP32 tables, FP64 "
                          "accumulation, RK and refine/coarsen norms)")
+    ap.add_argument("--imex", action="store_true",
+                    help="instead of the main measurement: IMEX43 with the assembled artificial viscosity, GPU "
+                         "against the CPU oracle on one fixed grid")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.imex:
+        imex_sample(args)
+        return
     make = workloads.WORKLOADS[args.workload]
     batched = args.workload in ("c2", "c5") and args.batch > 1
 
