@@ -165,7 +165,12 @@ int adg_destroy(adg_ctx* h) {
 
 const char* adg_last_error(adg_ctx* h) { return h ? h->err.c_str() : "null context"; }
 
-int adg_sync(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.sync(); }); }
+int adg_sync(adg_ctx* h) {
+  return guard(h, [](adg::Context& c) {
+    c.sync();
+    c.check_error_flag();
+  });
+}
 
 int adg_grid_init(adg_ctx* h, int sparse, int level_init) {
   return guard(h, [&](adg::Context& c) {
@@ -465,7 +470,12 @@ int adg_flux_set(adg_ctx* h, int ncomp, const double* fp) {
     c.sync();
   });
 }
-int adg_fp_to_coe(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.fp_to_coe(); }); }
+int adg_fp_to_coe(adg_ctx* h) {
+  return guard(h, [](adg::Context& c) {
+    c.fp_to_coe();
+    c.check_error_flag();  // a missing ancestor (hole in the grid)
+  });
+}
 int adg_rhs_zero(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_zero(); }); }
 int adg_rhs_lagr_vol(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_lagr(false); }); }
 int adg_rhs_lagr_flx(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_lagr(true); }); }
@@ -676,6 +686,7 @@ int adg_run_linear(adg_ctx* h, const double* coef, double dt, int64_t nsteps, ad
       }
     }
     c.sync();
+    c.check_error_flag();
     const auto t1 = std::chrono::steady_clock::now();
     if (st) {
       st->steps = nsteps;
@@ -722,7 +733,8 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
         for (int d = 0; d < c.D; ++d) {
           int lmax = 0;
           for (int64_t e = c.off[i]; e < c.off[i + 1]; ++e) lmax = std::max(lmax, c.level(e, d));
-          sum_c_dx += std::abs(p->wave_speed[d]) * std::pow(2., lmax);
+          const double ws = c.inst_wave.empty() ? p->wave_speed[d] : c.inst_wave[static_cast<size_t>(i) * c.D + d];
+          sum_c_dx += std::abs(ws) * std::pow(2., lmax);
         }
         double dt = p->cfl / sum_c_dx;
         dt = std::min(dt, p->t_final - t[i]);
@@ -761,6 +773,9 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
       }
       dof_stages += 3 * c.E() * c.K;
       if (p->adaptive) c.coarsen_instances(eta);
+      // a non-finite stage value (or a hole found by the surplus transform)
+      // ends the run with an error instead of stepping on
+      c.check_error_flag();
       ++steps;
       bool finished = false;
       for (int i = 0; i < nb; ++i) {
@@ -868,9 +883,27 @@ int adg_set_instance_thresholds(adg_ctx* h, const double* eps, const double* eta
   });
 }
 
+int adg_set_instance_params(adg_ctx* h, const double* coef_scale, const double* alpha, const double* wave_speed) {
+  return guard(h, [&](adg::Context& c) {
+    const size_t n = static_cast<size_t>(c.B()) * c.D;
+    for (const double* a : {coef_scale, alpha, wave_speed})
+      if (a)
+        for (size_t i = 0; i < n; ++i)
+          if (!std::isfinite(a[i])) throw std::runtime_error("instance parameters must be finite");
+    if (coef_scale) c.inst_scale.assign(coef_scale, coef_scale + n);
+    else c.inst_scale.clear();
+    if (alpha) c.inst_alpha.assign(alpha, alpha + n);
+    else c.inst_alpha.clear();
+    if (wave_speed) c.inst_wave.assign(wave_speed, wave_speed + n);
+    else c.inst_wave.clear();
+    if (!c.inst_alpha.empty()) c.upload_inst_alpha();
+    c.vcoef_version = -1;  // the variable coefficient is sampled again
+  });
+}
+
 int adg_set_kernel_variant(adg_ctx* h, int variant) {
   return guard(h, [&](adg::Context& c) {
-    if (variant < -1 || variant > 3) throw std::runtime_error("kernel variant must be -1, 0, 1, 2 or 3");
+    if (variant < -1 || variant >= adg::Context::kVariants) throw std::runtime_error("kernel variant must be -1 or 0..6");
     c.kernel_variant = variant;
   });
 }

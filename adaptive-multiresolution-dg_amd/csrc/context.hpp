@@ -17,6 +17,8 @@
 
 namespace adg {
 
+struct TaskDev;
+
 struct HipError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
@@ -25,7 +27,9 @@ void hip_check(hipError_t e, const char* what);
 // runs fn(0..n-1) on the library's host thread pool (serially for small n)
 void parallel_for(int n, const std::function<void(int)>& fn);
 
-constexpr int ADG_FLUX_VARCOEF_ID = 2;  // ADG_FLUX_VARCOEF of include/adg.h
+constexpr int ADG_FLUX_VARCOEF_ID = 2;         // ADG_FLUX_VARCOEF of include/adg.h
+constexpr int ADG_FLUX_VARCOEF_SCALED_ID = 3;  // ADG_FLUX_VARCOEF_SCALED
+inline bool is_varcoef(int kind) { return kind == ADG_FLUX_VARCOEF_ID || kind == ADG_FLUX_VARCOEF_SCALED_ID; }
 
 // Growable device buffer.
 template <class T>
@@ -93,7 +97,15 @@ class Context {
   // per-instance adaptivity thresholds set by the caller (empty: the run's
   // scalar eps / eta apply to every instance) and per-instance step sizes
   std::vector<double> inst_eps, inst_eta, inst_dt;
-  DevBuf<double> d_inst_dt;
+  // per-instance problem parameters (B x D each; empty: the problem's own):
+  // scale of the variable coefficient (ADG_FLUX_VARCOEF_SCALED), LF
+  // constants and wave speeds of the time-step rule
+  std::vector<double> inst_scale, inst_alpha, inst_wave;
+  DevBuf<double> d_inst_dt, d_pen_coef, d_inst_scale;  // d_pen_coef[d * B + i] = -alpha_{i,d} / 2
+  DevBuf<double> vcoef_1d;  // 1 + s sin(2 pi x) at the points of a 1D grid
+  int64_t vcoef_version = -1;
+  // uploads d_pen_coef from inst_alpha (per-instance LF penalty constants)
+  void upload_inst_alpha();
   DevBuf<int32_t> g_inst;
   bool use_inst_dt = false;
   std::vector<double> up_host;    // point values of the adaptive interpolation
@@ -131,7 +143,27 @@ class Context {
 
   // grid arrays
   DevBuf<int32_t> g_slot[kMaxDim], g_fbase[kMaxDim], g_fmaxl[kMaxDim], g_fmap[kMaxDim];
-  DevBuf<uint32_t> g_fbits[kMaxDim];  // occupancy bit of every fiber-map entry
+  DevBuf<uint32_t> g_fbits[kMaxDim];  // occupancy bit of every fiber-map entry (+ one zero word)
+  int64_t fbits_words[kMaxDim] = {0, 0, 0, 0};
+  int maxfib = 1;  // largest fiber (elements) over all dimensions and instances
+  // fiber order of every dimension for the fiber-tile kernels (built lazily
+  // after a grid change): elements by (fiber, slot) and prefix popcounts of
+  // the occupancy words
+  DevBuf<int32_t> g_perm[kMaxDim], g_pre[kMaxDim], g_cnt;
+  DevBuf<uint8_t> scan_tmp;
+  int64_t order_version = -1;
+  void ensure_fiber_order();
+  // tiles of whole fibers, te = 64 << which elements per tile window
+  struct TileSetH {
+    int te = 0, max_tile = 0;
+    int64_t version = -1, ntile = 0;
+    DevBuf<int32_t> tbeg[kMaxDim], tend[kMaxDim];
+  };
+  TileSetH tsets[3];
+  TileSetH& tile_set(int which);
+  template <class T>
+  void tile_launch(int which, int pf1, int pt1, const struct TaskDev* tasks, int R, int ntask);
+  DevBuf<double> tile_tv, tile_pv;  // per-term values of the tile form's last sweeps
   DevBuf<int32_t> g_remap;
   DevBuf<int32_t> g_lsum;  // level sum of each element
   // Level-sum cap of the reference's _coarse_grid variants: sweeps and
@@ -155,14 +187,14 @@ class Context {
 
   // operators (stream ordered)
   void sweep(const double* in, const int* size_in, double* out, double* accum, const int* size_out, int dd,
-             SweepTable tab, bool flx, int lu, double coef, int pf, int pt);
+             SweepTable tab, bool flx, int lu, double coef, int pf, int pt, const double* coef_inst = nullptr);
   void partial_sums(const SweepTable* tabs, const int* flx, int pf, int pt, double coef, const double* src, double* dst);
   void eval_up();
   void flux(int kind, const double* params);
   void fp_to_coe();
   void rhs_zero();
   void rhs_lagr(bool flux_term);
-  void rhs_single_dim(SweepTable tab, int d, double coef);
+  void rhs_single_dim(SweepTable tab, int d, double coef, const double* coef_inst = nullptr);
   void rhs_penalty(const double* alpha);
   void rhs_linear(const double* coef);
   void rk_init();
@@ -217,8 +249,10 @@ class Context {
   std::map<const double*, std::unique_ptr<SplitTable>> split_tabs;
   const SplitTable& split_table(const double* dev, size_t n);
   // thread mapping of the fused kernels: 0 per output, 1 per (element, rest
-  // index), 2 per (element, 4 rest indices), 3 per (element, 2 rest indices);
-  // -1 lets the autotuner choose per kernel class and grid-size class
+  // index), 2 per (element, 4 rest indices), 3 per (element, 2 rest indices),
+  // 4..6 fiber tiles of 64, 128, 256 elements (LDS-staged); -1 lets the
+  // autotuner choose per kernel class and grid-size class
+  static constexpr int kVariants = 7;
   enum { TUNE_UP_PHASE = 0, TUNE_UP_FINAL, TUNE_RHS_PHASE, TUNE_RHS_FINAL };
   int kernel_variant = -1;
   std::map<int, int> tuned;

@@ -20,6 +20,7 @@
 
 #include "context.hpp"
 #include "fiber.hpp"
+#include "kern.hpp"
 
 namespace adg {
 
@@ -117,32 +118,8 @@ void parallel_for(int n, const std::function<void(int)>& fn) {
 namespace {
 
 constexpr int kBlock = 256;
-// Final fused kernels: a block is (terms x outputs); each wavefront evaluates
-// one term for consecutive outputs (same element, same fiber walk), the
-// terms of an output are then added in order from LDS.
-constexpr int kFinalBlock = 1024;
-// Per-element final kernels hold a block of accumulators per thread: at most
-// 512 threads per block leaves them 256 VGPRs.
-constexpr int kElemBlock = 512;
-
 
 inline unsigned nblocks(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
-
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// The gather kernels renumber their blocks so that every XCD works on one
-// contiguous range of elements (for a batch: a contiguous range of whole
-// instances), whose fiber neighbours then stay in that XCD's L2.  The
-// renumbering is a bijection of block ids and changes no arithmetic.  The
-// phase kernels keep the hardware order: their work items are task-major and
-// tasks differ in cost (ancestor vs descendant walks), so contiguous ranges
-// would load some XCDs far more than others.
-constexpr unsigned kXcds = 8;
-__device__ __forceinline__ unsigned xcd_block() {
-  const unsigned b = blockIdx.x, n = gridDim.x;
-  const unsigned x = b % kXcds, i = b / kXcds, per = n / kXcds, extra = n % kXcds;
-  return x * per + min(x, extra) + i;
-}
-
 
 // ---------------------------------------------------------------------------
 // One 1D sweep (FastMultiplyLU::transform_1D, source/FastMultiplyLU.cpp:436-512).
@@ -165,6 +142,8 @@ struct SweepArgs {
   int rsz[3], rsin[3], rsout[3];
   int flx, lu;
   double coef;
+  const double* coef_inst;  // per-instance coefficient (replaces coef when non-null)
+  const int32_t* inst;
 };
 
 template <int PF1, int PT1>
@@ -215,9 +194,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(SweepArgs a) {
       for (int i = 0; i < PT1; ++i) acc[i] += xv * Mr[kk * PT1 + i];
     }
   });
+  const double cf = a.coef_inst ? a.coef_inst[a.inst[e]] : a.coef;
 #pragma unroll
   for (int i = 0; i < PT1; ++i) {
-    const double v = acc[i] * a.coef;
+    const double v = acc[i] * cf;
     if (a.out) a.out[ob + i * a.sdd_out] = v;
     if (a.accum) a.accum[ob + i * a.sdd_out] += v;
   }
@@ -339,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
     const int pd = (q / a.stride_d) % a.p1;
     for (int ic = 0; ic < a.p1; ++ic) {
       const int anc = a.fmap[a.fbase[e] + a.anc_slot[s * a.p1 + ic]];
-      if (anc < 0) { a.err[0] = 1; continue; }
+      if (anc < 0) { atomicOr(a.err, ERR_ANCESTOR); continue; }
       const int mm = q + (a.anc_point[s * a.p1 + ic] - pd) * a.stride_d;
       surplus_madd(coe, a, (s * a.p1 + pd) * a.p1 + ic, in[static_cast<int64_t>(anc) * a.P + mm]);
     }
@@ -348,25 +328,16 @@ __global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
   else a.out32[c * a.comp_stride + e * a.P + q] = coe;
 }
 
-// One explicit RK stage update from u_n (t), the current stage value (uu) and
-// the right-hand side (rr): forward Euler (source/ODESolver.cpp:209-213),
-// RK2 midpoint (:247-259) and SSP-RK3 (:285-301).
-__device__ __forceinline__ double rk_combine(int scheme, int stage, double dt, double t, double uu, double rr) {
-  // dt = 0 freezes an instance of a batch (its u and u_n both hold the
-  // accepted solution while other instances redo their step)
-  if (dt == 0.) return stage == 0 ? t : uu;
-  if (scheme == 2) return stage == 0 ? t + 0.5 * dt * rr : t + dt * rr;
-  if (scheme == 0 || stage == 0) return t + dt * rr;
-  if (stage == 1) return 3. / 4. * t + 1. / 4. * (uu + dt * rr);
-  return 1. / 3. * t + 2. / 3. * (uu + dt * rr);
-}
-
+// One RK stage update (rk_combine); a non-finite result raises ERR_NONFINITE.
 __global__ __launch_bounds__(kBlock) void k_rk(double* u, const double* tn, const double* r, int64_t n, int scheme,
-                                                int stage, double dt, const double* dtv, const int32_t* inst, int K) {
+                                                int stage, double dt, const double* dtv, const int32_t* inst, int K,
+                                                int32_t* err) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (dtv) dt = dtv[inst[i / K]];
-  u[i] = rk_combine(scheme, stage, dt, tn[i], u[i], r[i]);
+  const double v = rk_combine(scheme, stage, dt, tn[i], u[i], r[i]);
+  u[i] = v;
+  if (!isfinite(v)) atomicOr(err, ERR_NONFINITE);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill(double* p, int64_t n, double v) {
@@ -446,6 +417,36 @@ __global__ __launch_bounds__(kBlock) void k_lagr_norm(LagrNormArgs a) {
     norm += val;
   }
   a.out[e] = norm;
+}
+
+// Variable coefficient at every interpolation point: out[(d*E + e)*P + q] =
+// scale[inst[e]*D + d] * a1[1D point index of coordinate (d+1) mod D].
+struct VcoefArgs {
+  double* out;
+  const double* a1;
+  const int32_t* slot[kMaxDim];
+  const int32_t* inst;
+  const double* scale;
+  int64_t E;
+  int D, P, p1;
+};
+
+__global__ __launch_bounds__(kBlock) void k_vcoef(VcoefArgs a) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid >= a.E * a.P) return;
+  const int64_t e = gid / a.P;
+  const int q = static_cast<int>(gid - e * a.P);
+  int idx[kMaxDim];
+  int rem = q;
+  for (int d = a.D - 1; d >= 0; --d) {
+    idx[d] = rem % a.p1;
+    rem /= a.p1;
+  }
+  for (int d = 0; d < a.D; ++d) {
+    const int dn = (d + 1) % a.D;
+    const double v = a.a1[a.slot[dn][e] * a.p1 + idx[dn]];
+    a.out[(d * a.E + e) * a.P + q] = a.scale ? a.scale[a.inst[e] * a.D + d] * v : v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -532,470 +533,6 @@ __global__ __launch_bounds__(kBlock) void k_remap(double* dst, const double* src
   dst[i] = (o >= 0) ? src[static_cast<int64_t>(o) * T + t] : 0.;
 }
 
-// ---------------------------------------------------------------------------
-// Fused stage phases.  A "term" is one 1D sweep evaluated at a single output
-// value; a phase evaluates many terms in one launch (thread per output).
-struct TermDev {
-  const double* in;
-  const double* M;
-  // FP32 precision mode: input in32 (or in, rounded on load, when in32 is
-  // null) and the table split as M = Mh + Ml
-  const float* in32;
-  const float* Mh;
-  const float* Ml;
-  const int32_t* slot;
-  const int32_t* fbase;
-  const int32_t* fmaxl;
-  const int32_t* fmap;
-  const uint32_t* fbits;
-  const int32_t* lsum;
-  int ns, pt1, dd, Tin, flx, lu;
-  int isz[kMaxDim];  // sizes of the input tensor
-  int sin[kMaxDim];  // strides of the input tensor
-  double coef;
-};
-
-struct TaskDev {  // a term written to an output buffer
-  TermDev t;
-  double* out;
-  float* out32;  // FP32 precision mode
-  int Tout;
-  int osz[kMaxDim];
-  int R;              // rest indices per element (Tin / input size in the swept dimension)
-  int64_t begin_v[4];  // first work item of this task in the phase, per thread mapping
-};
-
-// Arithmetic of a sweep in the storage type T: FP64 (x * M added in order)
-// or FP32 with the 1D table split as M = Mh + Ml in FP32, both parts applied
-// by fused multiply-adds, the small part first.
-template <class T>
-struct SweepOps;
-template <>
-struct SweepOps<double> {
-  struct M { double m; };
-  static __device__ __forceinline__ double in(const TermDev& t, int64_t o) { return t.in[o]; }
-  static __device__ __forceinline__ M mat(const TermDev& t, int64_t o) { return {t.M[o]}; }
-  static __device__ __forceinline__ M zero() { return {0.}; }
-  static __device__ __forceinline__ void madd(double& acc, double x, M m) { acc += x * m.m; }
-  static __device__ __forceinline__ double* out(const TaskDev& tk) { return tk.out; }
-};
-template <>
-struct SweepOps<float> {
-  struct M { float h, l; };
-  static __device__ __forceinline__ float in(const TermDev& t, int64_t o) {
-    return t.in32 ? t.in32[o] : static_cast<float>(t.in[o]);
-  }
-  static __device__ __forceinline__ M mat(const TermDev& t, int64_t o) { return {t.Mh[o], t.Ml[o]}; }
-  static __device__ __forceinline__ M zero() { return {0.f, 0.f}; }
-  static __device__ __forceinline__ void madd(float& acc, float x, M m) {
-    acc = fmaf(x, m.l, acc);
-    acc = fmaf(x, m.h, acc);
-  }
-  static __device__ __forceinline__ float* out(const TaskDev& tk) { return tk.out32; }
-};
-
-// Value of one sweep at output (e, idx): coef * sum over related sources, in
-// ascending slot order, of sum_k in[src][idx with dd <- k] * M[s*PF1+k][st*pt1+idx[dd]].
-// Loads of up to four sources are issued before their terms are added in order.
-template <int PF1, class T = double>
-__device__ __forceinline__ T term_value(const TermDev& t, int64_t e, const int* idx, int D, int cap) {
-  using Ops = SweepOps<T>;
-  int base = 0;
-  for (int d = 0; d < D; ++d)
-    if (d != t.dd) base += idx[d] * t.sin[d];
-  const int sdd = t.sin[t.dd];
-  const int st = t.slot[e];
-  const int fb = t.fbase[e];
-  const int64_t Mc = static_cast<int64_t>(st) * t.ns * PF1 * t.pt1 + idx[t.dd];
-  int maxl = t.fmaxl[e];
-  if (cap >= 0) {  // coarse grid, as in k_sweep
-    const int ls = t.lsum[e];
-    if (ls > cap) return T(0);
-    maxl = min(maxl, cap - ls + slot_level(st));
-  }
-  T acc = T(0);
-  // sources are gathered NB at a time so that their loads are in flight together
-  constexpr int NB = (PF1 <= 4) ? 4 : 2;
-  int pend[NB];
-  int np = 0;
-  auto flush = [&]() {
-    int src[NB];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) src[j] = (j < np) ? t.fmap[fb + pend[j]] : -1;
-    T xv[NB][PF1];
-    typename Ops::M mv[NB][PF1];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      const int64_t x = static_cast<int64_t>(src[j] < 0 ? 0 : src[j]) * t.Tin + base;
-      const int64_t Mr = Mc + (j < np ? pend[j] : 0) * PF1 * t.pt1;
-#pragma unroll
-      for (int kk = 0; kk < PF1; ++kk) {
-        xv[j][kk] = (src[j] >= 0) ? Ops::in(t, x + kk * sdd) : T(0);
-        mv[j][kk] = (src[j] >= 0) ? Ops::mat(t, Mr + kk * t.pt1) : Ops::zero();
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < NB; ++j)
-      if (src[j] >= 0) {
-#pragma unroll
-        for (int kk = 0; kk < PF1; ++kk) Ops::madd(acc, xv[j][kk], mv[j][kk]);
-      }
-    np = 0;
-  };
-  for_each_present(st, maxl, t.flx != 0, t.lu, t.fbits, fb, [&](int s) {
-    pend[np++] = s;
-    if (np == NB) flush();
-  });
-  if (np > 0) flush();
-  return acc * T(t.coef);
-}
-
-__device__ __forceinline__ void unflatten(int o, const int* sz, int D, int* idx) {
-  for (int d = D - 1; d >= 0; --d) {
-    idx[d] = o % sz[d];
-    o /= sz[d];
-  }
-}
-
-// Per-output form of k_phase: thread per (task, element, output).
-template <int PF1, class T>
-__global__ __launch_bounds__(kBlock) void k_phase_pt(const TaskDev* tasks, int ntask, int64_t total, int D, int cap) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= total) return;
-  int j = 0;
-  while (j + 1 < ntask && gid >= tasks[j + 1].begin_v[0]) ++j;
-  const TaskDev& tk = tasks[j];
-  const int64_t local = gid - tk.begin_v[0];
-  const int64_t e = local / tk.Tout;
-  const int o = static_cast<int>(local - e * tk.Tout);
-  int idx[kMaxDim];
-  unflatten(o, tk.osz, D, idx);
-  SweepOps<T>::out(tk)[local] = term_value<PF1, T>(tk.t, e, idx, D, cap);
-}
-
-// Point flux of the fused eval_up kernels (kind -1: none, the Hermite flux runs after).
-template <class T>
-__device__ __forceinline__ void flux_point(int kind, T u, double s, const double* vcoef, T* fp, int64_t gid, int64_t n,
-                                           int ncomp) {
-  if (kind == 0) {
-    fp[gid] = u * u * T(0.5);
-  } else if (kind == 1) {
-    fp[gid] = sin(u) * T(s);
-    fp[n + gid] = cos(u) * T(s);
-  } else if (kind == 2) {
-    for (int c = 0; c < ncomp; ++c) fp[c * n + gid] = T(vcoef[c * n + gid]) * u;
-  }
-}
-
-// Per-output form of k_up_final: threads are (point, term) pairs, one term per
-// wavefront, and the point's first thread adds the terms in order.
-template <int PF1, class T>
-__global__ __launch_bounds__(kFinalBlock) void k_up_final_pt(const TermDev* terms, int nterm, int per_block, int64_t E,
-                                                           int P, int p1, int D, T* up, T* fp,
-                                                           const double* vcoef, int kind, double s, int ncomp, int cap) {
-  __shared__ double vals[kFinalBlock];
-  const int t = threadIdx.x / per_block, lo = threadIdx.x - t * per_block;
-  const int64_t gid = static_cast<int64_t>(xcd_block()) * per_block + lo;
-  const bool live = gid < E * P;
-  int64_t e = 0;
-  int q = 0;
-  if (live) {
-    e = gid / P;
-    q = static_cast<int>(gid - e * P);
-  }
-  double v = 0.;
-  if (live && t < nterm) {
-    int sz[kMaxDim], idx[kMaxDim];
-    for (int d = 0; d < D; ++d) sz[d] = p1;
-    unflatten(q, sz, D, idx);
-    v = term_value<PF1, T>(terms[t], e, idx, D, cap);
-  }
-  vals[threadIdx.x] = v;
-  __syncthreads();
-  if (!live || t != 0) return;
-  double u = 0.;
-  for (int j = 0; j < nterm; ++j) u += vals[j * per_block + lo];
-  const T uu = T(u);
-  up[gid] = uu;
-  flux_point<T>(kind, uu, s, vcoef, fp, gid, E * P, ncomp);
-}
-
-// Per-output form of k_rhs_final: threads are (DoF, term) pairs.
-template <int PFM, int PFK, class T>
-__global__ __launch_bounds__(kFinalBlock) void k_rhs_final_pt(const TermDev* terms, int nterm, const TermDev* pens,
-                                                            int npen, int per_block, int64_t E, int K, int k1, int D,
-                                                            double* rhs, const double* u, const double* tn,
-                                                            double* u_next, int scheme, int stage, double dt,
-                                                            const double* dtv, const int32_t* inst, int cap) {
-  __shared__ double vals[kFinalBlock];
-  const int t = threadIdx.x / per_block, lo = threadIdx.x - t * per_block;
-  const int64_t gid = static_cast<int64_t>(xcd_block()) * per_block + lo;
-  const bool live = gid < E * K;
-  int64_t e = 0;
-  int q = 0;
-  if (live) {
-    e = gid / K;
-    q = static_cast<int>(gid - e * K);
-  }
-  double v = 0.;
-  if (live && t < nterm + npen) {
-    int sz[kMaxDim], idx[kMaxDim];
-    for (int d = 0; d < D; ++d) sz[d] = k1;
-    unflatten(q, sz, D, idx);
-    v = (t < nterm) ? double(term_value<PFM, T>(terms[t], e, idx, D, cap))
-                    : term_value<PFK, double>(pens[t - nterm], e, idx, D, cap);
-  }
-  vals[threadIdx.x] = v;
-  __syncthreads();
-  if (!live || t != 0) return;
-  double r = 0.;
-  for (int j = 0; j < nterm + npen; ++j) r += vals[j * per_block + lo];
-  rhs[gid] = r;
-  if (dtv) dt = dtv[inst[e]];
-  u_next[gid] = rk_combine(scheme, stage, dt, tn[gid], u[gid], r);
-}
-
-
-// A thread evaluates one term for up to kRC "rest" indices of one element
-// (the indices of the dimensions other than the swept one) and every output
-// index of the swept dimension: one walk over the fiber serves all of them.
-constexpr int kRC = 4;
-
-// Input and output offsets of rest index r (row-major over the dimensions
-// other than dd) in tensors with input sizes isz / strides si and output
-// strides so.
-__device__ __forceinline__ void rest_offsets(int r, int dd, int D, const int* isz, const int* si, const int* so,
-                                             int& bin, int& bout) {
-  bin = 0;
-  bout = 0;
-  for (int d = D - 1; d >= 0; --d) {
-    if (d == dd) continue;
-    const int q = r % isz[d];
-    r /= isz[d];
-    bin += q * si[d];
-    bout += q * so[d];
-  }
-}
-
-__device__ __forceinline__ void strides_dev(const int* sz, int D, int* st) {
-  st[D - 1] = 1;
-  for (int d = D - 2; d >= 0; --d) st[d] = st[d + 1] * sz[d + 1];
-}
-
-// acc[r][i] = sum over related sources of element e, in ascending slot order,
-// of sum_k in[src][bin[r] + k*sdd] * M(s*PF1+k, st*PF1+i), for r < nr <= RC.
-// RC (1 or kRC) sizes the register arrays.  Returns false when e lies above
-// the coarse-grid cap (its outputs are zero).
-template <int PF1, int PT1, class T, int RC>
-__device__ __forceinline__ bool term_block(const TermDev& t, int64_t e, const int* bin, int nr, int cap,
-                                           T (&acc)[RC][PT1]) {
-  using Ops = SweepOps<T>;
-#pragma unroll
-  for (int r = 0; r < RC; ++r)
-#pragma unroll
-    for (int i = 0; i < PT1; ++i) acc[r][i] = T(0);
-  const int st = t.slot[e];
-  const int fb = t.fbase[e];
-  int maxl = t.fmaxl[e];
-  if (cap >= 0) {  // coarse grid, as in k_sweep
-    const int ls = t.lsum[e];
-    if (ls > cap) return false;
-    maxl = min(maxl, cap - ls + slot_level(st));
-  }
-  const int sdd = t.sin[t.dd];
-  const int64_t Mst = static_cast<int64_t>(st) * t.ns * PF1 * PT1;
-  for_each_present(st, maxl, t.flx != 0, t.lu, t.fbits, fb, [&](int s) {
-    const int src = t.fmap[fb + s];
-    if (src < 0) return;
-    const int64_t Mr = Mst + s * PF1 * PT1;
-    typename Ops::M m[PF1][PT1];
-#pragma unroll
-    for (int k = 0; k < PF1; ++k)
-#pragma unroll
-      for (int i = 0; i < PT1; ++i) m[k][i] = Ops::mat(t, Mr + k * PT1 + i);
-    const int64_t xs = static_cast<int64_t>(src) * t.Tin;
-#pragma unroll
-    for (int r = 0; r < RC; ++r) {
-      if (r >= nr) break;
-      T xv[PF1];
-#pragma unroll
-      for (int k = 0; k < PF1; ++k) xv[k] = Ops::in(t, xs + bin[r] + k * sdd);
-#pragma unroll
-      for (int i = 0; i < PT1; ++i)
-#pragma unroll
-        for (int k = 0; k < PF1; ++k) Ops::madd(acc[r][i], xv[k], m[k][i]);
-    }
-  });
-  return true;
-}
-
-// Evaluates term t for (element e, chunk c) and stores coef * value at
-// out[bout + i*so_dd] (zero above the coarse-grid cap).
-template <int PF1, int PT1, class T, class O, int RC>
-__device__ __forceinline__ void term_chunk_store(const TermDev& t, int64_t e, int c, int rc, int D, const int* so,
-                                                 int cap, O* out) {
-  const int R = t.Tin / t.isz[t.dd];
-  const int r0 = c * rc;
-  const int nr = min(rc, R - r0);
-  int bin[RC], bout[RC];
-#pragma unroll
-  for (int r = 0; r < RC; ++r) {
-    bin[r] = 0;
-    bout[r] = 0;
-    if (r < nr) rest_offsets(r0 + r, t.dd, D, t.isz, t.sin, so, bin[r], bout[r]);
-  }
-  T acc[RC][PT1];
-  const bool live = term_block<PF1, PT1, T, RC>(t, e, bin, nr, cap, acc);
-  const int sod = so[t.dd];
-#pragma unroll
-  for (int r = 0; r < RC; ++r) {
-    if (r >= nr) break;
-#pragma unroll
-    for (int i = 0; i < PT1; ++i) out[bout[r] + i * sod] = live ? O(acc[r][i] * T(t.coef)) : O(0);
-  }
-}
-
-// First sweeps of the partial sums of one phase: thread per (task, element,
-// chunk of rc rest indices); variant v (1: rc = 1, 2: rc = kRC) selects the
-// work-item offsets.
-template <int PF1, int PT1, class T, int RC>
-__global__ __launch_bounds__(kBlock) void k_phase(const TaskDev* tasks, int ntask, int64_t total, int D, int cap,
-                                                  int v, int rc) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (gid >= total) return;
-  int j = 0;
-  while (j + 1 < ntask && gid >= tasks[j + 1].begin_v[v]) ++j;
-  const TaskDev& tk = tasks[j];
-  const int nch = (tk.R + rc - 1) / rc;
-  const int64_t local = gid - tk.begin_v[v];
-  const int64_t e = local / nch;
-  const int c = static_cast<int>(local - e * nch);
-  int so[kMaxDim];
-  strides_dev(tk.osz, D, so);
-  term_chunk_store<PF1, PT1, T, T, RC>(tk.t, e, c, rc, D, so, cap, SweepOps<T>::out(tk) + e * tk.Tout);
-}
-
-// Last sweep of every eval_up partial sum, accumulated in ordering order into
-// the point value, followed by the point flux (LagrInterpolation::eval_fp_Lag).
-// A block holds EB elements; threads are (term, element, chunk), term-major,
-// and write their values to LDS; the point values then add the terms in order.
-template <int PF1, int PT1, class T, int RC>
-__global__ __launch_bounds__(kElemBlock) void k_up_final(const TermDev* terms, int nterm, int EB, int nch, int rc,
-                                                           int64_t E, int P, int D, T* up, T* fp,
-                                                           const double* vcoef, int kind, double s, int ncomp, int cap) {
-  extern __shared__ double vals[];  // [nterm][EB][P]
-  const int tid = threadIdx.x;
-  const int per_term = EB * nch, nslot = blockDim.x / per_term;
-  const int slot = tid / per_term, rem = tid - slot * per_term;
-  const int el = rem / nch, c = rem - (rem / nch) * nch;
-  const int64_t e0 = static_cast<int64_t>(xcd_block()) * EB;
-  int so[kMaxDim], osz[kMaxDim];
-  for (int d = 0; d < D; ++d) osz[d] = PT1;
-  strides_dev(osz, D, so);
-  if (e0 + el < E)
-    for (int t = slot; t < nterm; t += nslot)
-      term_chunk_store<PF1, PT1, T, double, RC>(terms[t], e0 + el, c, rc, D, so, cap,
-                                            vals + (static_cast<int64_t>(t) * EB + el) * P);
-  __syncthreads();
-  const int64_t n = E * P;
-  for (int o = tid; o < EB * P; o += blockDim.x) {
-    const int ol = o / P, q = o - ol * P;
-    if (e0 + ol >= E) break;
-    double u = 0.;
-    for (int j = 0; j < nterm; ++j) u += vals[(static_cast<int64_t>(j) * EB + ol) * P + q];
-    const int64_t gid = (e0 + ol) * P + q;
-    const T uu = T(u);
-    up[gid] = uu;
-    flux_point<T>(kind, uu, s, vcoef, fp, gid, n, ncomp);
-  }
-}
-
-// Last sweep of every right-hand-side partial sum and the penalty sweeps,
-// accumulated in the reference order, then the RK stage update written to
-// u_next (u itself is still read by other threads' penalty sweeps).  Same
-// block structure as k_up_final.
-template <int PFM, int PFK, class T, int RC>
-__global__ __launch_bounds__(kElemBlock) void k_rhs_final(const TermDev* terms, int nterm, const TermDev* pens,
-                                                            int npen, int EB, int nch, int rc, int64_t E, int K, int D,
-                                                            double* rhs, const double* u, const double* tn,
-                                                            double* u_next, int scheme, int stage, double dt,
-                                                            const double* dtv, const int32_t* inst, int cap) {
-  extern __shared__ double vals[];  // [nterm + npen][EB][K]
-  const int tid = threadIdx.x;
-  const int per_term = EB * nch, nslot = blockDim.x / per_term;
-  const int slot = tid / per_term, rem = tid - slot * per_term;
-  const int el = rem / nch, c = rem - (rem / nch) * nch;
-  const int64_t e0 = static_cast<int64_t>(xcd_block()) * EB;
-  int so[kMaxDim], osz[kMaxDim];
-  for (int d = 0; d < D; ++d) osz[d] = PFK;
-  strides_dev(osz, D, so);
-  if (e0 + el < E)
-    for (int t = slot; t < nterm + npen; t += nslot) {
-      double* out = vals + (static_cast<int64_t>(t) * EB + el) * K;
-      if (t < nterm) term_chunk_store<PFM, PFK, T, double, RC>(terms[t], e0 + el, c, rc, D, so, cap, out);
-      else term_chunk_store<PFK, PFK, double, double, RC>(pens[t - nterm], e0 + el, c, rc, D, so, cap, out);
-    }
-  __syncthreads();
-  for (int o = tid; o < EB * K; o += blockDim.x) {
-    const int ol = o / K, q = o - ol * K;
-    if (e0 + ol >= E) break;
-    double r = 0.;
-    for (int j = 0; j < nterm + npen; ++j) r += vals[(static_cast<int64_t>(j) * EB + ol) * K + q];
-    const int64_t gid = (e0 + ol) * K + q;
-    rhs[gid] = r;
-    const double dte = dtv ? dtv[inst[e0 + ol]] : dt;
-    u_next[gid] = rk_combine(scheme, stage, dte, tn[gid], u[gid], r);
-  }
-}
-
-template <int PF1, int PT1, class T>
-void launch_phase_t(const TaskDev* tasks, int ntask, int64_t total, int D, int cap, int v, hipStream_t s) {
-  if (v == 0)
-    hipLaunchKernelGGL((k_phase_pt<PF1, T>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D, cap);
-  else if (v == 1)
-    hipLaunchKernelGGL((k_phase<PF1, PT1, T, 1>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D,
-                       cap, v, 1);
-  else if (v == 3)
-    hipLaunchKernelGGL((k_phase<PF1, PT1, T, 2>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D,
-                       cap, v, 2);
-  else
-    hipLaunchKernelGGL((k_phase<PF1, PT1, T, kRC>), dim3(nblocks(total)), dim3(kBlock), 0, s, tasks, ntask, total, D,
-                       cap, v, kRC);
-}
-
-// v: 0 thread per output, 1 thread per (element, rest index), 2 thread per
-// (element, kRC rest indices), 3 thread per (element, 2 rest indices); total
-// counts the work items of that mapping.
-template <class R>
-void launch_phase(int pf1, int pt1, const TaskDev* tasks, int ntask, int64_t total, int D, int cap, int v,
-                  hipStream_t s) {
-#define ADG_PH(F, T) if (pf1 == F && pt1 == T) { launch_phase_t<F, T, R>(tasks, ntask, total, D, cap, v, s); return; }
-  ADG_PH(2, 2) ADG_PH(2, 3) ADG_PH(2, 4) ADG_PH(2, 5) ADG_PH(3, 2) ADG_PH(3, 3) ADG_PH(3, 4) ADG_PH(3, 5)
-  ADG_PH(4, 2) ADG_PH(4, 3) ADG_PH(4, 4) ADG_PH(4, 5) ADG_PH(5, 2) ADG_PH(5, 3) ADG_PH(5, 4)
-#undef ADG_PH
-  throw std::runtime_error("phase: unsupported basis sizes");
-}
-
-// Outputs per block of a per-output final kernel with nterm terms: a power
-// of two, at most one wavefront, with nterm * outputs <= kFinalBlock.
-inline int final_outputs_per_block(int nterm) {
-  int o = 64;
-  while (o > 1 && o * nterm > kFinalBlock) o /= 2;
-  return o;
-}
-
-// Elements per block of a per-element final kernel with nt terms, nch chunks
-// per element and `outs` outputs per element (at most kElemBlock threads and
-// 64 KB of LDS), and the number of term slots: threads are (slot, element,
-// chunk) and a slot takes the terms slot, slot + nslot, ...
-inline int final_elements_per_block(int nt, int nch, int outs) {
-  if (nch > kElemBlock || nt * outs * 8 > 65536) throw std::runtime_error("final kernel: block too large");
-  int eb = std::max(1, std::min(kElemBlock / (nt * nch), 65536 / (nt * outs * 8)));
-  if (nch == 1 && eb >= 64) eb -= eb % 64;  // whole wavefronts per term
-  return eb;
-}
-
-inline int final_term_slots(int nt, int nch, int eb) { return std::max(1, std::min(nt, kElemBlock / (eb * nch))); }
 
 template <int PF1, int PT1>
 void launch_sweep_t(const SweepArgs& a, hipStream_t s) {
@@ -1127,18 +664,26 @@ struct Context::FusedPlan {
   int interp = -1;
   int precision = -1;
   const float* fucoe32 = nullptr;
-  const double *u = nullptr, *tn = nullptr, *fucoe = nullptr, *tmp = nullptr;
+  const double *u = nullptr, *tn = nullptr, *fucoe = nullptr, *tmp = nullptr, *pen_coef = nullptr;
   double* unext = nullptr;
+  double* tv = nullptr;
+  const int32_t *perm0 = nullptr, *pre0 = nullptr;
   int flux_kind = -1;
   double alpha[kMaxDim] = {0, 0, 0, 0};
   std::vector<int> up_phase_ntask, rhs_phase_ntask;
-  std::vector<std::array<int64_t, 4>> up_phase_total, rhs_phase_total;  // work items per thread mapping
+  std::vector<std::array<int64_t, 4>> up_phase_total, rhs_phase_total;  // work items per per-element mapping
   int n_up_terms = 0, n_rhs_terms = 0, n_pen = 0;
   // algorithmic bytes per launch (8 B x elements x (inputs + outputs))
   std::vector<double> up_phase_bytes, rhs_phase_bytes;
   double up_final_bytes = 0, rhs_final_bytes = 0;
   DevBuf<TaskDev> up_tasks, rhs_tasks;
   DevBuf<TermDev> up_terms, rhs_terms, pens;
+  // fiber-tile form of the last sweeps: tasks writing one value buffer per
+  // term (tile_tv: eval_up, then right-hand-side terms; tile_pv: penalties)
+  DevBuf<TaskDev> up_final_tasks, rhs_final_tasks, pen_tasks;
+  // rest indices of the tasks of each launch (tile LDS sizing)
+  std::vector<int> up_phase_R, rhs_phase_R;
+  int up_final_R = 0, rhs_final_R = 0, pen_R = 0;
 };
 
 Context::~Context() {
@@ -1241,14 +786,20 @@ void Context::upload_grid() {
     g_fmap[d].ensure(static_cast<size_t>(nmap));
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(nmap)), dim3(kBlock), 0, stream, g_fmap[d].p, nmap, -1);
     const int64_t nwords = (nmap + 31) / 32;
-    g_fbits[d].ensure(static_cast<size_t>(nwords));
-    hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(nwords)), dim3(kBlock), 0, stream, g_fbits[d].p, nwords, 0u);
+    // one zero word past the end: the position of the cell after the last
+    // fiber is read from it
+    g_fbits[d].ensure(static_cast<size_t>(nwords + 1));
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(nwords + 1)), dim3(kBlock), 0, stream, g_fbits[d].p, nwords + 1, 0u);
+    fbits_words[d] = nwords;
     if (n > 0)
       hipLaunchKernelGGL(k_fmap_scatter, dim3(nblocks(n)), dim3(kBlock), 0, stream, g_fmap[d].p, g_fbits[d].p,
                          g_fbase[d].p, g_slot[d].p, n);
   }
   upload(g_inst, uinst, stream);
   upload(g_lsum, ls, stream);
+  maxfib = 1;
+  for (const HostGrid& g : grids)
+    for (int dd = 0; dd < D; ++dd) maxfib = std::max(maxfib, g.maxfib[dd]);
 }
 
 void Context::ensure_fields() {
@@ -1339,7 +890,7 @@ void Context::set_grids(const std::vector<std::vector<int32_t>>& slots, bool kee
                          E() * K, 0.);
   }
   // variable coefficients depend on the point set
-  if (flux_kind == ADG_FLUX_VARCOEF_ID) compute_vcoef();
+  if (is_varcoef(flux_kind)) compute_vcoef();
 }
 
 void Context::upload_inst_dt() {
@@ -1348,7 +899,7 @@ void Context::upload_inst_dt() {
 }
 
 void Context::sweep(const double* in, const int* sin, double* out, double* accum, const int* sout, int dd,
-                    SweepTable tab, bool flx, int lu, double coef, int pf, int pt) {
+                    SweepTable tab, bool flx, int lu, double coef, int pf, int pt, const double* coef_inst) {
   if (E() == 0) return;
   SweepArgs a{};
   a.in = in; a.out = out; a.accum = accum; a.M = tab.dev; a.ns = tab.ns;
@@ -1376,6 +927,8 @@ void Context::sweep(const double* in, const int* sin, double* out, double* accum
   a.flx = flx ? 1 : 0;
   a.lu = lu;
   a.coef = coef;
+  a.coef_inst = coef_inst;
+  a.inst = g_inst.p;
   if (sin[dd] != pf + 1 || sout[dd] != pt + 1) throw std::runtime_error("sweep: size mismatch in swept dimension");
   prof_begin(PROF_SWEEP, 8.0 * static_cast<double>(a.E) * (a.Tin + a.Tout));
   launch_sweep(a, pf + 1, pt + 1, stream);
@@ -1414,23 +967,30 @@ void Context::eval_up() {
 }
 
 void Context::compute_vcoef() {
-  // a_d(x) = 1 + s*sin(2 pi x_{(d+1) mod D}) at every Lagrange point; the
-  // coefficient is sampled on the host with the same libm as the oracle.
-  std::vector<double> h(static_cast<size_t>(E()) * P * D);
-  for (int64_t e = 0; e < E(); ++e) {
-    for (int q = 0; q < P; ++q) {
-      int idx[kMaxDim];
-      int rem = q;
-      for (int d = D - 1; d >= 0; --d) { idx[d] = rem % (m + 1); rem /= (m + 1); }
-      for (int d = 0; d < D; ++d) {
-        const int dn = (d + 1) % D;
-        const double x = lagr.point(uslot[e * D + dn] * (m + 1) + idx[dn]);
-        h[(static_cast<size_t>(d) * E() + e) * P + q] = 1. + flux_param * std::sin(2. * PI * x);
-      }
+  // a_d(x) = c_d (1 + s*sin(2 pi x_{(d+1) mod D})) at every Lagrange point.
+  // The factor 1 + s*sin(..) depends on one coordinate only, so it is
+  // tabulated on the host for the 2^NMAX (m+1) points of a 1D grid (with the
+  // oracle's libm) and gathered on the device; c_d is the instance's scale
+  // (ADG_FLUX_VARCOEF_SCALED), absent for ADG_FLUX_VARCOEF.
+  const int p1 = m + 1;
+  const int npt = (1 << nmax) * p1;
+  std::vector<double> a1(npt);
+  for (int c = 0; c < npt; ++c) a1[c] = 1. + flux_param * std::sin(2. * PI * lagr.point(c));
+  upload(vcoef_1d, a1, stream);
+  vcoef_version = grid_version;
+  const bool scaled = flux_kind == ADG_FLUX_VARCOEF_SCALED_ID;
+  if (scaled) {
+    if (static_cast<int>(inst_scale.size()) != B() * D) {
+      inst_scale.assign(static_cast<size_t>(B()) * D, 1.);
     }
+    upload(d_inst_scale, inst_scale, stream);
   }
-  upload(vcoef, h, stream);
   ensure_fields();
+  VcoefArgs a{};
+  a.out = vcoef.p; a.a1 = vcoef_1d.p; a.inst = g_inst.p; a.scale = scaled ? d_inst_scale.p : nullptr;
+  for (int dd = 0; dd < D; ++dd) a.slot[dd] = g_slot[dd].p;
+  a.E = E(); a.D = D; a.P = P; a.p1 = p1;
+  if (E() > 0) hipLaunchKernelGGL(k_vcoef, dim3(nblocks(E() * P)), dim3(kBlock), 0, stream, a);
 }
 
 namespace {
@@ -1457,7 +1017,7 @@ void Context::flux(int kind, const double* params) {
     prof_end();
     return;
   }
-  if (kind == ADG_FLUX_VARCOEF_ID && (flux_kind != kind || flux_param != s)) {
+  if (is_varcoef(kind) && (flux_kind != kind || flux_param != s || vcoef_version != grid_version)) {
     flux_kind = kind;
     flux_param = s;
     compute_vcoef();
@@ -1469,7 +1029,7 @@ void Context::flux(int kind, const double* params) {
   const int64_t n = E() * P;
   if (n == 0) return;
   const int nout = (kind == 0) ? 1 : D;
-  prof_begin(PROF_FLUX, 8.0 * static_cast<double>(n) * (1 + nout + (kind == 2 ? D : 0)));
+  prof_begin(PROF_FLUX, 8.0 * static_cast<double>(n) * (1 + nout + (is_varcoef(kind) ? D : 0)));
   hipLaunchKernelGGL(k_flux, dim3(nblocks(n)), dim3(kBlock), 0, stream, up.p, fp.p, vcoef.p, n, kind, s, D);
   prof_end();
 }
@@ -1561,14 +1121,24 @@ void Context::rhs_lagr(bool flux_term) {
   }
 }
 
-void Context::rhs_single_dim(SweepTable tab, int d, double coef) {
+void Context::rhs_single_dim(SweepTable tab, int d, double coef, const double* coef_inst) {
   int sz[kMaxDim];
   for (int i = 0; i < D; ++i) sz[i] = k + 1;
-  sweep(u.p, sz, nullptr, rhs.p, sz, d, tab, true, LU_FULL, coef, k, k);
+  sweep(u.p, sz, nullptr, rhs.p, sz, d, tab, true, LU_FULL, coef, k, k, coef_inst);
 }
 
 void Context::rhs_penalty(const double* alpha) {
-  for (int d = 0; d < D; ++d) rhs_single_dim(table(8), d, -alpha[d] / 2.);
+  // per-instance constants replace alpha when the caller set them
+  const bool per = !inst_alpha.empty();
+  for (int d = 0; d < D; ++d)
+    rhs_single_dim(table(8), d, -alpha[d] / 2., per ? d_pen_coef.p + static_cast<size_t>(d) * B() : nullptr);
+}
+
+void Context::upload_inst_alpha() {
+  std::vector<double> pc(static_cast<size_t>(B()) * D);
+  for (int i = 0; i < B(); ++i)
+    for (int d = 0; d < D; ++d) pc[static_cast<size_t>(d) * B() + i] = -inst_alpha[static_cast<size_t>(i) * D + d] / 2.;
+  upload(d_pen_coef, pc, stream);
 }
 
 void Context::rhs_linear(const double* coef) {
@@ -1588,7 +1158,7 @@ void Context::rk_stage(int scheme, int stage, double dt) {
   if (E() > 0) {
     prof_begin(PROF_RK, 8.0 * static_cast<double>(E()) * K * 4);
     hipLaunchKernelGGL(k_rk, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, tn.p, rhs.p, E() * K, scheme,
-                       stage, dt, use_inst_dt ? d_inst_dt.p : nullptr, g_inst.p, K);
+                       stage, dt, use_inst_dt ? d_inst_dt.p : nullptr, g_inst.p, K, err_flag.p);
     prof_end();
   }
 }
@@ -1617,16 +1187,22 @@ int prod_of(const int* sz, int D) {
 }  // namespace
 
 Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
+  // The terms hold the fiber-order arrays of the tile kernels: build them
+  // first (they are reallocated only when the grid changes, which also
+  // invalidates the plan).
+  ensure_fiber_order();
   // one cached plan per buffer of the u / u_next pair (the stage swaps them)
   int slot = -1;
   for (int i = 0; i < 2; ++i)
     if (plans[i] && plans[i]->u == u.p) slot = i;
+  const double* pen_coef = inst_alpha.empty() ? nullptr : d_pen_coef.p;
   if (slot >= 0) {
     FusedPlan& pl = *plans[slot];
     const bool same_alpha = std::equal(pl.alpha, pl.alpha + D, p.alpha);
     if (pl.version == grid_version && pl.tn == tn.p && pl.fucoe == fucoe.p && pl.unext == u_next.p &&
         pl.tmp == fused_tmp.p && pl.flux_kind == p.flux_kind && pl.interp == interp && same_alpha &&
-        pl.precision == precision && pl.fucoe32 == fucoe32.p)
+        pl.precision == precision && pl.fucoe32 == fucoe32.p && pl.tv == tile_tv.p && pl.pen_coef == pen_coef &&
+        pl.perm0 == g_perm[0].p && pl.pre0 == g_pre[0].p)
       return pl;
   } else {
     slot = !plans[0] ? 0 : (!plans[1] ? 1 : 0);
@@ -1640,15 +1216,23 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
   int T = 1;
   for (int d = 0; d < D; ++d) T *= mx;
   const int nchains = nord + 2 * D * nord;
-  const size_t per = static_cast<size_t>(std::max<int64_t>(E(), 1)) * T;
+  const size_t Ee = static_cast<size_t>(std::max<int64_t>(E(), 1));
+  const size_t per = Ee * T;
   fused_tmp.ensure(per * nchains * std::max(D - 1, 1));
-  u_next.ensure(static_cast<size_t>(std::max<int64_t>(E(), 1)) * K);
+  u_next.ensure(Ee * K);
+  // per-term value buffers of the fiber-tile last sweeps
+  tile_tv.ensure(std::max(Ee * P * nord, Ee * K * 2 * D * nord));
+  tile_pv.ensure(Ee * K * D);
   if (!plans[slot]) plans[slot].reset(new FusedPlan());
   FusedPlan& pl = *plans[slot];
   pl.version = grid_version;
   pl.interp = interp;
   pl.precision = precision;
   pl.fucoe32 = fucoe32.p;
+  pl.pen_coef = pen_coef;
+  pl.tv = tile_tv.p;
+  pl.perm0 = g_perm[0].p;
+  pl.pre0 = g_pre[0].p;
   const bool f32 = precision == 1;
   // algorithmic bytes of a term input: FP64 coefficients, else the storage word
   const double w = f32 ? 4.0 : 8.0;
@@ -1669,11 +1253,25 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     t.slot = g_slot[dd].p; t.fbase = g_fbase[dd].p; t.fmaxl = g_fmaxl[dd].p; t.fmap = g_fmap[dd].p;
     t.fbits = g_fbits[dd].p;
     t.lsum = g_lsum.p;
+    t.perm = g_perm[dd].p;
+    t.pre = g_pre[dd].p;
+    t.inst = g_inst.p;
     t.Tin = prod_of(sin_sz, D);
     std::copy(sin_sz, sin_sz + D, t.isz);
     strides_of(sin_sz, D, t.sin);
     t.flx = flx ? 1 : 0; t.lu = lu; t.coef = coef;
     return t;
+  };
+  // a term written to a buffer with output sizes osz
+  auto make_task = [&](const TermDev& t, const int* osz, double* out) {
+    TaskDev tk{};
+    tk.t = t;
+    tk.out = out;
+    tk.out32 = reinterpret_cast<float*>(out);  // FP32 storage uses the first half of the buffer
+    tk.Tout = prod_of(osz, D);
+    std::copy(osz, osz + D, tk.osz);
+    tk.R = t.Tin / t.isz[t.dd];
+    return tk;
   };
   // one chain of partial sums: returns its last term; appends first-step tasks per phase
   auto chain = [&](int cid, const double* src, const float* src32, int pf, int pt, const SweepTable* tabs,
@@ -1690,13 +1288,7 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
       TermDev t = make_term(cur, cur32, sin_sz, dd, tabs[dd], pt + 1, flx[dd] != 0, lus[o][st], st == 0 ? coef : 1.0,
                             f32);
       if (st == D - 1) return t;
-      TaskDev tk{};
-      tk.t = t;
-      tk.out = tmp(cid, st);
-      tk.out32 = reinterpret_cast<float*>(tk.out);  // FP32 storage uses the first half of the buffer
-      tk.Tout = prod_of(sz, D);
-      std::copy(sz, sz + D, tk.osz);
-      tk.R = t.Tin / sin_sz[dd];
+      TaskDev tk = make_task(t, sz, tmp(cid, st));
       phase_tasks[st].push_back(tk);
       cur = tk.out;
       cur32 = tk.out32;
@@ -1704,8 +1296,10 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     return TermDev{};
   };
   auto finish_phases = [&](std::vector<std::vector<TaskDev>>& ph, DevBuf<TaskDev>& dev, std::vector<int>& ntask,
-                           std::vector<std::array<int64_t, 4>>& total, std::vector<double>& bytes) {
+                           std::vector<std::array<int64_t, 4>>& total, std::vector<double>& bytes,
+                           std::vector<int>& rest) {
     std::vector<TaskDev> flat;
+    rest.clear();
     ntask.clear();
     total.clear();
     bytes.clear();
@@ -1721,6 +1315,9 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
         b += E() * (in_bytes(tk.t) * tk.t.Tin + w * tk.Tout);
       }
       bytes.push_back(b);
+      int R = 0;
+      for (const TaskDev& tk : tasks) R = std::max(R, tk.R);
+      rest.push_back(R);
       ntask.push_back(static_cast<int>(tasks.size()));
       total.push_back(begin);
       flat.insert(flat.end(), tasks.begin(), tasks.end());
@@ -1731,15 +1328,23 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
   {
     std::vector<std::vector<TaskDev>> ph(std::max(D - 1, 0));
     std::vector<TermDev> terms;
+    std::vector<TaskDev> finals;
     SweepTable tabs[kMaxDim];
     int flx[kMaxDim];
     for (int d = 0; d < D; ++d) { tabs[d] = table(0); flx[d] = 0; }
-    for (int o = 0; o < nord; ++o) terms.push_back(chain(o, u.p, nullptr, k, m, tabs, flx, o, 1.0, ph));
-    finish_phases(ph, pl.up_tasks, pl.up_phase_ntask, pl.up_phase_total, pl.up_phase_bytes);
+    int psz[kMaxDim];
+    for (int d = 0; d < D; ++d) psz[d] = m + 1;
+    for (int o = 0; o < nord; ++o) {
+      terms.push_back(chain(o, u.p, nullptr, k, m, tabs, flx, o, 1.0, ph));
+      finals.push_back(make_task(terms.back(), psz, tile_tv.p + static_cast<size_t>(o) * Ee * P));
+    }
+    finish_phases(ph, pl.up_tasks, pl.up_phase_ntask, pl.up_phase_total, pl.up_phase_bytes, pl.up_phase_R);
+    pl.up_final_R = finals[0].R;
     upload(pl.up_terms, terms, stream);
+    upload(pl.up_final_tasks, finals, stream);
     pl.n_up_terms = static_cast<int>(terms.size());
     const int nc = (p.flux_kind == 0) ? 1 : D;
-    double b = w * E() * P * (1 + nc) + (p.flux_kind == ADG_FLUX_VARCOEF_ID ? 8.0 * E() * P * nc : 0.);
+    double b = w * E() * P * (1 + nc) + (is_varcoef(p.flux_kind) ? 8.0 * E() * P * nc : 0.);
     for (const TermDev& t : terms) b += in_bytes(t) * E() * t.Tin;
     pl.up_final_bytes = b;
   }
@@ -1747,6 +1352,9 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
   {
     std::vector<std::vector<TaskDev>> ph(std::max(D - 1, 0));
     std::vector<TermDev> terms;
+    std::vector<TaskDev> finals;
+    int ksz[kMaxDim];
+    for (int d = 0; d < D; ++d) ksz[d] = k + 1;
     int cid = nord;
     for (int ft = 0; ft < 2; ++ft) {
       for (int d = 0; d < D; ++d) {
@@ -1759,31 +1367,89 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
         const int64_t coff = static_cast<int64_t>(same_flux ? 0 : d) * E() * P;
         const double* src = fucoe.p + coff;
         const float* src32 = f32 ? fucoe32.p + coff : nullptr;
-        for (int o = 0; o < nord; ++o)
+        for (int o = 0; o < nord; ++o) {
           terms.push_back(chain(cid++, src, src32, m, k, tabs, flx, o, ft ? 0.5 : 1.0, ph));
+          finals.push_back(make_task(terms.back(), ksz, tile_tv.p + (terms.size() - 1) * Ee * K));
+        }
       }
     }
-    finish_phases(ph, pl.rhs_tasks, pl.rhs_phase_ntask, pl.rhs_phase_total, pl.rhs_phase_bytes);
+    finish_phases(ph, pl.rhs_tasks, pl.rhs_phase_ntask, pl.rhs_phase_total, pl.rhs_phase_bytes, pl.rhs_phase_R);
+    pl.rhs_final_R = finals[0].R;
     upload(pl.rhs_terms, terms, stream);
+    upload(pl.rhs_final_tasks, finals, stream);
     pl.n_rhs_terms = static_cast<int>(terms.size());
     double b = 8.0 * E() * K * (1 + 3 + D);  // rhs out, u, tn and u_next, penalty inputs
     for (const TermDev& t : terms) b += in_bytes(t) * E() * t.Tin;
     pl.rhs_final_bytes = b;
     std::vector<TermDev> pens;
-    int ksz[kMaxDim];
-    for (int d = 0; d < D; ++d) ksz[d] = k + 1;
-    // the LF penalty stays FP64 in every precision
-    for (int d = 0; d < D; ++d)
+    std::vector<TaskDev> pen_tasks;
+    // the LF penalty stays FP64 in every precision; per-instance constants
+    // (-alpha_d / 2 of each instance) when the caller set them
+    for (int d = 0; d < D; ++d) {
       pens.push_back(make_term(u.p, nullptr, ksz, d, table(8), k + 1, true, LU_FULL, -p.alpha[d] / 2., false));
+      if (pen_coef) pens.back().coef_inst = pen_coef + static_cast<size_t>(d) * B();
+      pen_tasks.push_back(make_task(pens.back(), ksz, tile_pv.p + static_cast<size_t>(d) * Ee * K));
+    }
     upload(pl.pens, pens, stream);
+    upload(pl.pen_tasks, pen_tasks, stream);
+    pl.pen_R = pen_tasks[0].R;
     pl.n_pen = D;
   }
   return pl;
 }
 
+// Fiber order of every dimension (perm, prefix popcounts) for the current grid.
+void Context::ensure_fiber_order() {
+  if (order_version == grid_version) return;
+  HostTimer timer(*this, HOST_UPLOAD);
+  for (int d = 0; d < D; ++d) {
+    const int64_t nw = fbits_words[d];
+    g_cnt.ensure(static_cast<size_t>(nw + 1));
+    g_pre[d].ensure(static_cast<size_t>(nw + 1));
+    g_perm[d].ensure(static_cast<size_t>(std::max<int64_t>(E(), 1)));
+    const size_t sb = tile_scan_bytes(nw);
+    scan_tmp.ensure(std::max<size_t>(sb, 1));
+    tile_prefix(g_fbits[d].p, nw, g_cnt.p, g_pre[d].p, scan_tmp.p, scan_tmp.cap, stream);
+  }
+  order_version = grid_version;
+}
+
+// Tiles of whole fibers for the tile-size variant (te = 64 << which).
+Context::TileSetH& Context::tile_set(int which) {
+  ensure_fiber_order();
+  TileSetH& ts = tsets[which];
+  if (ts.version == grid_version) return ts;
+  HostTimer timer(*this, HOST_UPLOAD);
+  ts.te = 64 << which;
+  ts.ntile = std::max<int64_t>((E() + ts.te - 1) / ts.te, 1);
+  ts.max_tile = ts.te - 1 + maxfib;
+  for (int d = 0; d < D; ++d) {
+    ts.tbeg[d].ensure(static_cast<size_t>(ts.ntile));
+    ts.tend[d].ensure(static_cast<size_t>(ts.ntile));
+    tile_order(g_fbits[d].p, g_pre[d].p, g_fbase[d].p, g_slot[d].p, E(), 1 << nmax, ts.te, ts.ntile, g_perm[d].p,
+               ts.tbeg[d].p, ts.tend[d].p, stream);
+  }
+  ts.version = grid_version;
+  return ts;
+}
+
+// One tile launch of ntask tasks with pf1 x pt1 blocks; throws (before any
+// launch) when a tile of this size does not fit in LDS.
+template <class T>
+void Context::tile_launch(int which, int pf1, int pt1, const TaskDev* tasks, int R, int ntask) {
+  if (order_version != grid_version) throw std::logic_error("tile launch: fiber order of an older grid");
+  TileSetH& ts = tile_set(which);
+  if (tile_rest_chunk(pf1, R, ts.max_tile, sizeof(T)) < 1) throw std::runtime_error("tile: does not fit in LDS");
+  const int32_t* tb[kMaxDim];
+  const int32_t* te[kMaxDim];
+  for (int d = 0; d < D; ++d) { tb[d] = ts.tbeg[d].p; te[d] = ts.tend[d].p; }
+  tile_sweep<T>(pf1, pt1, tasks, ntask, ts.ntile, ts.max_tile, tb, te, D, lsum_cap, stream);
+}
+
 void Context::stage_fused(const StageProblem& p, int scheme, int stage, double dt) {
   if (E() == 0) return;
-  if (p.flux_kind == ADG_FLUX_VARCOEF_ID && (flux_kind != p.flux_kind || flux_param != p.flux_param)) {
+  if (is_varcoef(p.flux_kind) &&
+      (flux_kind != p.flux_kind || flux_param != p.flux_param || vcoef_version != grid_version)) {
     flux_kind = p.flux_kind;
     flux_param = p.flux_param;
     compute_vcoef();
@@ -1803,7 +1469,10 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   else stage_body<double>(p, pl, scheme, stage, dt);
 }
 
-// One fused stage with intermediates stored in T (double, or float with split tables).
+// One fused stage with intermediates stored in T (double, or float with split
+// tables).  Every kernel class runs one of the mappings: 0..3 per-element
+// gathers (fused_elem.hip), 4..6 fiber tiles of 64, 128 or 256 elements
+// (fused_tile.hip); all give identical values.
 template <class T>
 void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int stage, double dt) {
   // Hermite: the final eval_up kernel writes point values only (kind -1); the
@@ -1816,6 +1485,8 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
   else { upp = up.p; fpp = fp.p; }
   const int tc = f32 ? 4 : 0;  // autotuner classes per precision
   const int ncomp = same_flux ? 1 : D;
+  const double* dtv = use_inst_dt ? d_inst_dt.p : nullptr;
+  T* tvT = reinterpret_cast<T*>(tile_tv.p);
   // point values and flux
   int64_t off = 0;
   for (size_t s = 0; s < pl.up_phase_ntask.size(); ++s) {
@@ -1823,49 +1494,23 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     const TaskDev* tasks = pl.up_tasks.p + off;
     const int nt = pl.up_phase_ntask[s];
     const std::array<int64_t, 4> tot = pl.up_phase_total[s];
-    tuned_launch(TUNE_UP_PHASE + tc, [&](int v) { launch_phase<T>(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
+    tuned_launch(TUNE_UP_PHASE + tc, [&](int v) {
+      if (v >= 4) tile_launch<T>(v - 4, k + 1, m + 1, tasks, pl.up_phase_R[s], nt);
+      else elem_phase<T>(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
+    });
     prof_end();
     off += pl.up_phase_ntask[s];
   }
   {
     prof_begin(PROF_FLUX, pl.up_final_bytes);
     tuned_launch(TUNE_UP_FINAL + tc, [&](int v) {
-      if (v == 0) {
-        const int64_t n = E() * P;
-        const int per_block = final_outputs_per_block(pl.n_up_terms);
-        const unsigned nb = static_cast<unsigned>((n + per_block - 1) / per_block);
-        const unsigned bs = static_cast<unsigned>(per_block * pl.n_up_terms);
-#define ADG_UP(K1)                                                                                                 \
-  if (k + 1 == K1)                                                                                                 \
-    hipLaunchKernelGGL((k_up_final_pt<K1, T>), dim3(nb), dim3(bs), 0, stream, pl.up_terms.p, pl.n_up_terms, per_block, \
-                       E(), P, m + 1, D, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);
-        ADG_UP(2) ADG_UP(3) ADG_UP(4)
-#undef ADG_UP
-        return;
+      if (v >= 4) {
+        tile_launch<T>(v - 4, k + 1, m + 1, pl.up_final_tasks.p, pl.up_final_R, pl.n_up_terms);
+        up_combine<T>(tvT, pl.n_up_terms, E() * P, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, stream);
+      } else {
+        elem_up_final<T>(v, k + 1, m + 1, pl.up_terms.p, pl.n_up_terms, E(), P, D, upp, fpp, vcoef.p, up_kind,
+                         p.flux_param, ncomp, lsum_cap, stream);
       }
-      const int rc = v == 1 ? 1 : (v == 3 ? 2 : kRC);
-      const int R = P / (m + 1);
-      const int nch = (R + rc - 1) / rc;
-      const int EB = final_elements_per_block(pl.n_up_terms, nch, P);
-      const unsigned nb = static_cast<unsigned>((E() + EB - 1) / EB);
-      const unsigned bs = static_cast<unsigned>(final_term_slots(pl.n_up_terms, nch, EB) * EB * nch);
-      const size_t lds = sizeof(double) * pl.n_up_terms * EB * P;
-#define ADG_UF(K1, M1)                                                                                             \
-  if (k + 1 == K1 && m + 1 == M1) {                                                                                \
-    if (rc == 1)                                                                                                   \
-      hipLaunchKernelGGL((k_up_final<K1, M1, T, 1>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p, pl.n_up_terms, \
-                         EB, nch, rc, E(), P, D, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);      \
-    else if (rc == 2)                                                                                              \
-      hipLaunchKernelGGL((k_up_final<K1, M1, T, 2>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p, pl.n_up_terms, \
-                         EB, nch, rc, E(), P, D, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, lsum_cap);      \
-    else                                                                                                           \
-      hipLaunchKernelGGL((k_up_final<K1, M1, T, kRC>), dim3(nb), dim3(bs), lds, stream, pl.up_terms.p,              \
-                         pl.n_up_terms, EB, nch, rc, E(), P, D, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp,  \
-                         lsum_cap);                                                                                \
-  }
-      ADG_UF(2, 2) ADG_UF(2, 3) ADG_UF(2, 4) ADG_UF(2, 5) ADG_UF(3, 2) ADG_UF(3, 3) ADG_UF(3, 4) ADG_UF(3, 5)
-      ADG_UF(4, 2) ADG_UF(4, 3) ADG_UF(4, 4) ADG_UF(4, 5)
-#undef ADG_UF
     });
     if (interp == 1)
       hipLaunchKernelGGL(k_flux_herm<T>, dim3(nblocks(E() * P)), dim3(kBlock), 0, stream, upp, fpp, E(), D,
@@ -1880,55 +1525,25 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     const TaskDev* tasks = pl.rhs_tasks.p + off;
     const int nt = pl.rhs_phase_ntask[s];
     const std::array<int64_t, 4> tot = pl.rhs_phase_total[s];
-    tuned_launch(TUNE_RHS_PHASE + tc, [&](int v) { launch_phase<T>(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream); });
+    tuned_launch(TUNE_RHS_PHASE + tc, [&](int v) {
+      if (v >= 4) tile_launch<T>(v - 4, m + 1, k + 1, tasks, pl.rhs_phase_R[s], nt);
+      else elem_phase<T>(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
+    });
     prof_end();
     off += pl.rhs_phase_ntask[s];
   }
   {
     prof_begin(PROF_RK, pl.rhs_final_bytes);
-    const int nt = pl.n_rhs_terms + pl.n_pen;
-    const double* dtv = use_inst_dt ? d_inst_dt.p : nullptr;
     tuned_launch(TUNE_RHS_FINAL + tc, [&](int v) {
-      if (v == 0) {
-        const int64_t n = E() * K;
-        const int per_block = final_outputs_per_block(nt);
-        const unsigned nb = static_cast<unsigned>((n + per_block - 1) / per_block);
-        const unsigned bs = static_cast<unsigned>(per_block * nt);
-#define ADG_RP(M1, K1)                                                                                             \
-  if (m + 1 == M1 && k + 1 == K1)                                                                                  \
-    hipLaunchKernelGGL((k_rhs_final_pt<M1, K1, T>), dim3(nb), dim3(bs), 0, stream, pl.rhs_terms.p, pl.n_rhs_terms,    \
-                       pl.pens.p, pl.n_pen, per_block, E(), K, k + 1, D, rhs.p, u.p, tn.p, u_next.p, scheme, stage, \
-                       dt, dtv, g_inst.p, lsum_cap);
-        ADG_RP(2, 2) ADG_RP(2, 3) ADG_RP(2, 4) ADG_RP(3, 2) ADG_RP(3, 3) ADG_RP(3, 4) ADG_RP(4, 2) ADG_RP(4, 3)
-        ADG_RP(4, 4) ADG_RP(5, 2) ADG_RP(5, 3) ADG_RP(5, 4)
-#undef ADG_RP
-        return;
+      if (v >= 4) {
+        tile_launch<T>(v - 4, m + 1, k + 1, pl.rhs_final_tasks.p, pl.rhs_final_R, pl.n_rhs_terms);
+        tile_launch<double>(v - 4, k + 1, k + 1, pl.pen_tasks.p, pl.pen_R, pl.n_pen);
+        rhs_combine<T>(tvT, tile_pv.p, pl.n_rhs_terms, pl.n_pen, E(), K, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt,
+                       dtv, g_inst.p, err_flag.p, stream);
+      } else {
+        elem_rhs_final<T>(v, m + 1, k + 1, pl.rhs_terms.p, pl.n_rhs_terms, pl.pens.p, pl.n_pen, E(), K, D, rhs.p, u.p,
+                          tn.p, u_next.p, scheme, stage, dt, dtv, g_inst.p, lsum_cap, err_flag.p, stream);
       }
-      const int rc = v == 1 ? 1 : (v == 3 ? 2 : kRC);
-      const int R = K / (k + 1);
-      const int nch = (R + rc - 1) / rc;
-      const int EB = final_elements_per_block(nt, nch, K);
-      const unsigned nb = static_cast<unsigned>((E() + EB - 1) / EB);
-      const unsigned bs = static_cast<unsigned>(final_term_slots(nt, nch, EB) * EB * nch);
-      const size_t lds = sizeof(double) * nt * EB * K;
-#define ADG_RF(M1, K1)                                                                                             \
-  if (m + 1 == M1 && k + 1 == K1) {                                                                                \
-    if (rc == 1)                                                                                                   \
-      hipLaunchKernelGGL((k_rhs_final<M1, K1, T, 1>), dim3(nb), dim3(bs), lds, stream, pl.rhs_terms.p,              \
-                         pl.n_rhs_terms, pl.pens.p, pl.n_pen, EB, nch, rc, E(), K, D, rhs.p, u.p, tn.p, u_next.p,  \
-                         scheme, stage, dt, dtv, g_inst.p, lsum_cap);                                              \
-    else if (rc == 2)                                                                                              \
-      hipLaunchKernelGGL((k_rhs_final<M1, K1, T, 2>), dim3(nb), dim3(bs), lds, stream, pl.rhs_terms.p,              \
-                         pl.n_rhs_terms, pl.pens.p, pl.n_pen, EB, nch, rc, E(), K, D, rhs.p, u.p, tn.p, u_next.p,  \
-                         scheme, stage, dt, dtv, g_inst.p, lsum_cap);                                              \
-    else                                                                                                           \
-      hipLaunchKernelGGL((k_rhs_final<M1, K1, T, kRC>), dim3(nb), dim3(bs), lds, stream, pl.rhs_terms.p,            \
-                         pl.n_rhs_terms, pl.pens.p, pl.n_pen, EB, nch, rc, E(), K, D, rhs.p, u.p, tn.p, u_next.p,  \
-                         scheme, stage, dt, dtv, g_inst.p, lsum_cap);                                              \
-  }
-      ADG_RF(2, 2) ADG_RF(2, 3) ADG_RF(2, 4) ADG_RF(3, 2) ADG_RF(3, 3) ADG_RF(3, 4) ADG_RF(4, 2) ADG_RF(4, 3)
-      ADG_RF(4, 4) ADG_RF(5, 2) ADG_RF(5, 3) ADG_RF(5, 4)
-#undef ADG_RF
     });
     prof_end();
   }
@@ -1964,7 +1579,8 @@ void Context::check_error_flag() {
     int32_t z = 0;
     hip_check(hipMemcpyAsync(err_flag.p, &z, sizeof(int32_t), hipMemcpyHostToDevice, stream), "upload");
     sync();
-    throw std::runtime_error("surplus transform: missing ancestor element (grid has a hole)");
+    if (h & ERR_ANCESTOR) throw std::runtime_error("surplus transform: missing ancestor element (grid has a hole)");
+    throw std::runtime_error("RK stage: non-finite value in the solution");
   }
 }
 
@@ -2069,7 +1685,9 @@ void Context::init_adaptive_intp_batch(double (*f)(const double*, int, void*), v
     for (int i = 0; i < B(); ++i) {
       const std::vector<double> ni(s.begin() + off[i], s.begin() + off[i + 1]);
       int64_t ai = 0;
-      ns[i] = refine_slots(grids[i], ni, eps, ai);
+      // per-instance thresholds, when the caller set them, also refine the initialisation
+      const double ei = static_cast<int>(inst_eps.size()) == B() ? inst_eps[i] : eps;
+      ns[i] = refine_slots(grids[i], ni, ei, ai);
       added += ai;
     }
     if (added == 0) break;
@@ -2390,7 +2008,7 @@ void Context::tuned_launch(int cls, const std::function<void(int)>& launch) {
   hip_check(hipEventCreate(&b), "hipEventCreate");
   float best = 0.f;
   int bv = 0;
-  for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < kVariants; ++v) {
     hip_check(hipEventRecord(a, stream), "hipEventRecord");
     if (!try_launch(v)) continue;
     hip_check(hipEventRecord(b, stream), "hipEventRecord");

@@ -1,0 +1,357 @@
+// Fiber-tile kernels of the fused RK stage.
+//
+// A sweep along dimension d couples only the elements of one d-fiber (the
+// elements that share their cells in every other dimension).  The elements of
+// every dimension are kept in fiber order (perm: sorted by fiber, then by
+// slot), and consecutive whole fibers are grouped into tiles.  A workgroup
+// takes one (term, tile): it stages the term's input values of the tile's
+// elements in LDS with coalesced loads of whole element rows, then every
+// thread evaluates one (target element, rest index) -- the PT1 outputs of the
+// swept dimension -- by walking the target's related source cells in
+// ascending slot order and reading their values from LDS.  The position of a
+// present cell in fiber order is the rank of its occupancy bit
+// (pre[word] + popcount of the lower bits), so no fiber-map gather is needed.
+// The table blocks of a (target, source) pair are read from global memory
+// (L2): threads of one target (consecutive rest indices) share them.
+//
+// Every output sums its sources in ascending slot order, k inner, exactly as
+// the per-element kernels and the CPU oracle do, so the values are
+// bit-identical.  The last sweeps of the partial sums write each term's values
+// to its own buffer; up_combine / rhs_combine then add the terms of an output
+// in the reference order (the terms of one output come from tiles of
+// different dimensions).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <stdexcept>
+
+#include "kern.hpp"
+
+namespace adg {
+namespace {
+
+constexpr int kTileThreads = 256;
+constexpr int kBlock = 256;
+// LDS of a tile workgroup: metadata plus the staged inputs of a chunk of rest
+// indices (the chunk shrinks for large tiles)
+constexpr int kTileLds = 48 * 1024;
+constexpr int kMaxRest = 128;  // rest indices of one element: Tin / size_dd <= 5^3
+
+inline unsigned nblocks(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
+
+__device__ __forceinline__ int bit_rank(const uint32_t* bits, const int32_t* pre, int x) {
+  const uint32_t w = bits[x >> 5];
+  return pre[x >> 5] + __popc(w & ((1u << (x & 31)) - 1u));
+}
+
+// Present source slots of a target, ascending, with their positions in fiber
+// order: f(slot, position).  Short runs (ancestors, flux neighbours) test
+// their bits one by one; longer runs (descendants) walk the occupancy words,
+// whose set bits have consecutive positions.
+template <class F>
+__device__ __forceinline__ void for_each_present_pos(int st, int maxl, bool flx, int lu, const uint32_t* bits,
+                                                     const int32_t* pre, int fb, F&& f) {
+  for_each_related_ranges(st, maxl, flx, lu, [&](int lo, int hi) {
+    if (hi - lo <= 2) {
+      for (int s = lo; s < hi; ++s) {
+        const int x = fb + s;
+        const uint32_t w = bits[x >> 5];
+        const uint32_t b = 1u << (x & 31);
+        if (w & b) f(s, pre[x >> 5] + __popc(w & (b - 1u)));
+      }
+      return;
+    }
+    const int b0 = fb + lo, b1 = fb + hi;  // bit range [b0, b1)
+    for (int w = b0 >> 5; w <= (b1 - 1) >> 5; ++w) {
+      const uint32_t full = bits[w];
+      uint32_t m = full;
+      const int wb = w << 5;
+      if (wb < b0) m &= ~0u << (b0 - wb);
+      if (wb + 32 > b1) m &= ~0u >> (wb + 32 - b1);
+      if (!m) continue;
+      int pos = pre[w] + __popc(full & ((1u << __builtin_ctz(m)) - 1u));
+      while (m) {
+        const int b = __builtin_ctz(m);
+        m &= m - 1;
+        f(wb + b - fb, pos++);
+      }
+    }
+  });
+}
+
+__device__ __forceinline__ void rest_offsets_t(int r, int dd, int D, const int* isz, const int* si, const int* so,
+                                               int& bin, int& bout) {
+  bin = 0;
+  bout = 0;
+  for (int d = D - 1; d >= 0; --d) {
+    if (d == dd) continue;
+    const int q = r % isz[d];
+    r /= isz[d];
+    bin += q * si[d];
+    bout += q * so[d];
+  }
+}
+
+struct TileSet {
+  const int32_t* tbeg[kMaxDimK];
+  const int32_t* tend[kMaxDimK];
+};
+
+// Metadata bytes of a tile workgroup with at most max_tile elements.
+__host__ __device__ inline int tile_meta_bytes(int max_tile) { return ((max_tile * 3 * 4 + 15) / 16) * 16; }
+
+template <int PF1, int PT1, class T>
+__global__ __launch_bounds__(kTileThreads) void k_tile(const TaskDev* tasks, int64_t ntile, TileSet ts, int D,
+                                                       int cap, int max_tile, int lds_bytes) {
+  using Ops = SweepOps<T>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_bin[kMaxRest], s_bout[kMaxRest];
+  const int64_t j = blockIdx.x / ntile;
+  const int64_t tile = blockIdx.x - j * ntile;
+  const TaskDev& tk = tasks[j];
+  const TermDev& t = tk.t;
+  const int dd = t.dd;
+  const int p0 = ts.tbeg[dd][tile], p1 = ts.tend[dd][tile];
+  if (p1 <= p0) return;  // no fiber starts in this tile's window
+  const int n = p1 - p0;
+  int32_t* s_elem = reinterpret_cast<int32_t*>(smem);
+  int32_t* s_fb = s_elem + max_tile;
+  int32_t* s_sm = s_fb + max_tile;  // slot | (walk depth + 1) << 16; depth 0: above the level-sum cap
+  T* X = reinterpret_cast<T*>(smem + tile_meta_bytes(max_tile));
+  const int tid = threadIdx.x;
+  for (int i = tid; i < n; i += blockDim.x) {
+    const int e = t.perm[p0 + i];
+    const int st = t.slot[e];
+    int maxl = t.fmaxl[e];
+    if (cap >= 0) {  // coarse grid (FastMultiplyLU::transform_1D_coarse_grid)
+      const int ls = t.lsum[e];
+      maxl = ls > cap ? -1 : min(maxl, cap - ls + slot_level(st));
+    }
+    s_elem[i] = e;
+    s_fb[i] = t.fbase[e];
+    s_sm[i] = st | ((maxl + 1) << 16);
+  }
+  int so[kMaxDimK];
+  so[D - 1] = 1;
+  for (int d = D - 2; d >= 0; --d) so[d] = so[d + 1] * tk.osz[d + 1];
+  const int sod = so[dd];
+  const int sdd = t.sin[dd];
+  const int R = t.Tin / t.isz[dd];
+  const int rc = min(R, (lds_bytes - tile_meta_bytes(max_tile)) / (max_tile * PF1 * static_cast<int>(sizeof(T))));
+  const bool flx = t.flx != 0;
+  const int64_t Mrow = static_cast<int64_t>(t.ns) * PF1 * PT1;
+  T* out = Ops::out(tk);
+  for (int r0 = 0; r0 < R; r0 += rc) {
+    const int nr = min(rc, R - r0);
+    __syncthreads();  // metadata written / previous chunk's reads done
+    if (tid < nr) {
+      int bi, bo;
+      rest_offsets_t(r0 + tid, dd, D, t.isz, t.sin, so, bi, bo);
+      s_bin[tid] = bi;
+      s_bout[tid] = bo;
+    }
+    __syncthreads();
+    const int per = nr * PF1;
+    for (int i = tid; i < n * per; i += blockDim.x) {
+      const int p = i / per, rem = i - p * per;
+      const int r = rem / PF1, k = rem - r * PF1;
+      X[i] = Ops::in(t, static_cast<int64_t>(s_elem[p]) * t.Tin + s_bin[r] + k * sdd);
+    }
+    __syncthreads();
+    for (int it = tid; it < n * nr; it += blockDim.x) {
+      const int p = it / nr, r = it - p * nr;
+      const int sm = s_sm[p];
+      const int st = sm & 0xffff, maxl = (sm >> 16) - 1;
+      const int e = s_elem[p];
+      T acc[PT1];
+#pragma unroll
+      for (int i = 0; i < PT1; ++i) acc[i] = T(0);
+      if (maxl >= 0) {
+        const int64_t Mst = static_cast<int64_t>(st) * Mrow;
+        for_each_present_pos(st, maxl, flx, t.lu, t.fbits, t.pre, s_fb[p], [&](int s, int pos) {
+          const int li = pos - p0;
+          if (static_cast<unsigned>(li) >= static_cast<unsigned>(n)) return;
+          const T* x = X + (li * nr + r) * PF1;
+          T xv[PF1];
+#pragma unroll
+          for (int k = 0; k < PF1; ++k) xv[k] = x[k];
+          const int64_t Mr = Mst + static_cast<int64_t>(s) * PF1 * PT1;
+          typename Ops::M m[PF1][PT1];
+#pragma unroll
+          for (int k = 0; k < PF1; ++k)
+#pragma unroll
+            for (int i = 0; i < PT1; ++i) m[k][i] = Ops::mat(t, Mr + k * PT1 + i);
+#pragma unroll
+          for (int i = 0; i < PT1; ++i)
+#pragma unroll
+            for (int k = 0; k < PF1; ++k) Ops::madd(acc[i], xv[k], m[k][i]);
+        });
+      }
+      const T cf = T(term_coef(t, e));
+      T* o = out + static_cast<int64_t>(e) * tk.Tout + s_bout[r];
+#pragma unroll
+      for (int i = 0; i < PT1; ++i) o[i * sod] = maxl >= 0 ? acc[i] * cf : T(0);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_word_popc(const uint32_t* bits, int64_t nwords, int32_t* cnt) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < nwords) cnt[i] = __popc(bits[i]);
+  else if (i == nwords) cnt[i] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_i32t(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// perm[rank(fbase + slot)] = e; the first element of every fiber extends the
+// range of the tile its position falls in.
+__global__ __launch_bounds__(kBlock) void k_fiber_order(const uint32_t* bits, const int32_t* pre,
+                                                        const int32_t* fbase, const int32_t* slot, int64_t E, int ns,
+                                                        int te, int32_t* perm, int32_t* tbeg, int32_t* tend) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int fb = fbase[e];
+  const int pos = bit_rank(bits, pre, fb + slot[e]);
+  perm[pos] = static_cast<int32_t>(e);
+  if (pos == bit_rank(bits, pre, fb)) {
+    const int t = pos / te;
+    atomicMin(&tbeg[t], pos);
+    atomicMax(&tend[t], bit_rank(bits, pre, fb + ns));
+  }
+}
+
+// Term j's values start at tv + j * ts (ts: n values of a double slot, i.e.
+// 2n for FP32 storage).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_up_combine(const T* tv, int nterm, int64_t n, int64_t ts, T* up, T* fp,
+                                                       const double* vcoef, int kind, double fs, int ncomp) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double u = 0.;
+  for (int j = 0; j < nterm; ++j) u += static_cast<double>(tv[j * ts + i]);
+  const T uu = T(u);
+  up[i] = uu;
+  flux_point<T>(kind, uu, fs, vcoef, fp, i, n, ncomp);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_rhs_combine(const T* tv, const double* pv, int nterm, int npen, int64_t E,
+                                                        int K, double* rhs, const double* u, const double* tn,
+                                                        double* u_next, int scheme, int stage, double dt,
+                                                        const double* dtv, const int32_t* inst, int32_t* err) {
+  const int64_t n = E * K;
+  const int64_t ts = n * static_cast<int64_t>(sizeof(double) / sizeof(T));
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r = 0.;
+  for (int j = 0; j < nterm; ++j) r += static_cast<double>(tv[j * ts + i]);
+  for (int j = 0; j < npen; ++j) r += pv[j * n + i];
+  rhs[i] = r;
+  const double dte = dtv ? dtv[inst[i / K]] : dt;
+  const double un = rk_combine(scheme, stage, dte, tn[i], u[i], r);
+  u_next[i] = un;
+  if (!isfinite(un)) atomicOr(err, ERR_NONFINITE);
+}
+
+template <int PF1, int PT1, class T>
+void tile_sweep_t(const TaskDev* tasks, int ntask, int64_t ntile, const TileSet& ts, int max_tile, int D, int cap,
+                  int lds, hipStream_t s) {
+  const int64_t nb = ntile * ntask;
+  if (nb > INT_MAX) throw std::runtime_error("tile sweep: grid too large");
+  hipLaunchKernelGGL((k_tile<PF1, PT1, T>), dim3(static_cast<unsigned>(nb)), dim3(kTileThreads), lds, s, tasks, ntile,
+                     ts, D, cap, max_tile, lds);
+}
+
+}  // namespace
+
+size_t tile_scan_bytes(int64_t nwords) {
+  size_t bytes = 0;
+  hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, static_cast<const int32_t*>(nullptr),
+                                   static_cast<int32_t*>(nullptr), static_cast<int>(nwords + 1));
+  return bytes;
+}
+
+void tile_prefix(const uint32_t* bits, int64_t nwords, int32_t* cnt, int32_t* pre, void* scan_tmp,
+                 size_t scan_tmp_bytes, hipStream_t s) {
+  if (nwords + 1 > INT_MAX) throw std::runtime_error("tile build: fiber map too large");
+  hipLaunchKernelGGL(k_word_popc, dim3(nblocks(nwords + 1)), dim3(kBlock), 0, s, bits, nwords, cnt);
+  size_t bytes = scan_tmp_bytes;
+  if (hipcub::DeviceScan::ExclusiveSum(scan_tmp, bytes, cnt, pre, static_cast<int>(nwords + 1), s) != hipSuccess)
+    throw std::runtime_error("tile build: prefix scan failed");
+}
+
+void tile_order(const uint32_t* bits, const int32_t* pre, const int32_t* fbase, const int32_t* slot, int64_t E, int ns,
+                int te, int64_t ntile, int32_t* perm, int32_t* tbeg, int32_t* tend, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_i32t, dim3(nblocks(ntile)), dim3(kBlock), 0, s, tbeg, ntile, INT_MAX);
+  hipLaunchKernelGGL(k_fill_i32t, dim3(nblocks(ntile)), dim3(kBlock), 0, s, tend, ntile, -1);
+  if (E > 0)
+    hipLaunchKernelGGL(k_fiber_order, dim3(nblocks(E)), dim3(kBlock), 0, s, bits, pre, fbase, slot, E, ns, te, perm,
+                       tbeg, tend);
+}
+
+int tile_rest_chunk(int pf1, int R, int max_tile, int tbytes) {
+  const int avail = kTileLds - tile_meta_bytes(max_tile);
+  const int per = max_tile * pf1 * tbytes;
+  if (R > kMaxRest || avail < per) return 0;
+  return std::min(R, avail / per);
+}
+
+template <class T>
+void tile_sweep(int pf1, int pt1, const TaskDev* tasks, int ntask, int64_t ntile, int max_tile, const int32_t* const* tbeg,
+                const int32_t* const* tend, int D, int cap, hipStream_t s) {
+  TileSet ts{};
+  for (int d = 0; d < D; ++d) {
+    ts.tbeg[d] = tbeg[d];
+    ts.tend[d] = tend[d];
+  }
+  const int lds = kTileLds;
+#define ADG_TS(F, P)                                                         \
+  if (pf1 == F && pt1 == P) {                                                \
+    tile_sweep_t<F, P, T>(tasks, ntask, ntile, ts, max_tile, D, cap, lds, s); \
+    return;                                                                  \
+  }
+  ADG_TS(2, 2) ADG_TS(2, 3) ADG_TS(2, 4) ADG_TS(2, 5) ADG_TS(3, 2) ADG_TS(3, 3) ADG_TS(3, 4) ADG_TS(3, 5)
+  ADG_TS(4, 2) ADG_TS(4, 3) ADG_TS(4, 4) ADG_TS(4, 5) ADG_TS(5, 2) ADG_TS(5, 3) ADG_TS(5, 4)
+#undef ADG_TS
+  throw std::runtime_error("tile sweep: unsupported basis sizes");
+}
+
+template <class T>
+void up_combine(const T* tv, int nterm, int64_t n, T* up, T* fp, const double* vcoef, int kind, double fs, int ncomp,
+                hipStream_t s) {
+  const int64_t ts = n * static_cast<int64_t>(sizeof(double) / sizeof(T));
+  if (n > 0)
+    hipLaunchKernelGGL(k_up_combine<T>, dim3(nblocks(n)), dim3(kBlock), 0, s, tv, nterm, n, ts, up, fp, vcoef, kind,
+                       fs, ncomp);
+}
+
+template <class T>
+void rhs_combine(const T* tv, const double* pv, int nterm, int npen, int64_t E, int K, double* rhs, const double* u,
+                 const double* tn, double* u_next, int scheme, int stage, double dt, const double* dtv,
+                 const int32_t* inst, int32_t* err, hipStream_t s) {
+  if (E > 0)
+    hipLaunchKernelGGL(k_rhs_combine<T>, dim3(nblocks(E * K)), dim3(kBlock), 0, s, tv, pv, nterm, npen, E, K, rhs, u,
+                       tn, u_next, scheme, stage, dt, dtv, inst, err);
+}
+
+template void tile_sweep<double>(int, int, const TaskDev*, int, int64_t, int, const int32_t* const*,
+                                 const int32_t* const*, int, int, hipStream_t);
+template void tile_sweep<float>(int, int, const TaskDev*, int, int64_t, int, const int32_t* const*,
+                                const int32_t* const*, int, int, hipStream_t);
+template void up_combine<double>(const double*, int, int64_t, double*, double*, const double*, int, double, int,
+                                 hipStream_t);
+template void up_combine<float>(const float*, int, int64_t, float*, float*, const double*, int, double, int,
+                                hipStream_t);
+template void rhs_combine<double>(const double*, const double*, int, int, int64_t, int, double*, const double*,
+                                  const double*, double*, int, int, double, const double*, const int32_t*, int32_t*,
+                                  hipStream_t);
+template void rhs_combine<float>(const float*, const double*, int, int, int64_t, int, double*, const double*,
+                                 const double*, double*, int, int, double, const double*, const int32_t*, int32_t*,
+                                 hipStream_t);
+
+}  // namespace adg

@@ -62,6 +62,7 @@ void HostGrid::build(int D_, int nmax_, const std::vector<int32_t>& slots, const
     std::iota(by.begin(), by.end(), 0);
     std::sort(by.begin(), by.end(), [&](int64_t a, int64_t b) { return fk[a] < fk[b]; });
     int32_t nf = 0;
+    maxfib[dd] = 0;
     for (int64_t i = 0; i < n;) {
       int64_t j = i;
       int maxl = 0;
@@ -71,6 +72,7 @@ void HostGrid::build(int D_, int nmax_, const std::vector<int32_t>& slots, const
         fmaxl[dd][by[q]] = maxl;
       }
       ++nf;
+      maxfib[dd] = std::max<int32_t>(maxfib[dd], static_cast<int32_t>(j - i));
       i = j;
     }
     nfib[dd] = nf;

@@ -83,6 +83,7 @@ struct HostGrid {
   std::vector<int32_t> fbase[kMaxDim];      // [E]
   std::vector<int32_t> fmaxl[kMaxDim];      // [E]
   int32_t nfib[kMaxDim] = {0, 0, 0, 0};
+  int32_t maxfib[kMaxDim] = {0, 0, 0, 0};  // elements of the largest fiber
 
   static uint64_t pack(const int32_t* s, int D) {
     uint64_t k = 0;

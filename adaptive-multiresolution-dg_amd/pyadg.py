@@ -17,7 +17,7 @@ FIELD_UCOE, FIELD_RHS, FIELD_UP, FIELD_TN, FIELD_UINTP, FIELD_FUCOE = 0, 1, 2, 3
 TAB_ALPT_AT_LAGR, TAB_LA_UV, TAB_LA_UVX, TAB_LA_FLX = 0, 1, 2, 3
 TAB_AA_UV, TAB_AA_UVX, TAB_AA_ULFT, TAB_AA_URGT, TAB_AA_UJP = 4, 5, 6, 7, 8
 LU_FULL, LU_L, LU_U = 0, 1, 2
-FLUX_BURGERS, FLUX_SINCOS, FLUX_VARCOEF = 0, 1, 2
+FLUX_BURGERS, FLUX_SINCOS, FLUX_VARCOEF, FLUX_VARCOEF_SCALED = 0, 1, 2, 3
 INTERP_LAGRANGE, INTERP_HERMITE = 0, 1
 REFINE_PREDICT, REFINE_LEARNED, REFINE_CURRENT = 0, 1, 2
 PROPOSER_INPUTS, PROPOSER_PARAMS = 12, 5057
@@ -138,6 +138,7 @@ def lib():
             "adg_num_instances": (i32, [vp]),
             "adg_instance_range": (i32, [vp, i32, P(i64), P(i64)]),
             "adg_set_instance_thresholds": (i32, [vp, DP, DP]),
+            "adg_set_instance_params": (i32, [vp, DP, DP, DP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -535,6 +536,21 @@ class Solution:
         if e.size != self.num_instances or t.size != self.num_instances:
             raise ValueError("one threshold per instance expected")
         self._check(lib().adg_set_instance_thresholds(self._h, _dp(e), _dp(t)))
+
+    def set_instance_params(self, coef_scale=None, alpha=None, wave_speed=None):
+        """Per-instance (ninst x dim) variable-coefficient scales, LF constants
+        and time-step wave speeds; None keeps the problem's own."""
+        arrs = []
+        for a in (coef_scale, alpha, wave_speed):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.ascontiguousarray(a, np.float64)
+            if a.size != self.num_instances * self.dim:
+                raise ValueError("ninst x dim values expected")
+            arrs.append(a)
+        self._keep_params = arrs
+        self._check(lib().adg_set_instance_params(self._h, *[_dp(a) if a is not None else None for a in arrs]))
 
     def instance_grid(self, i):
         """(level, suppt, hash) of instance i."""

@@ -45,6 +45,24 @@ class Workload:
     dt: float = 0.0                             # linear runs: fixed step
     nsteps: int = 0
     problem: pyadg.Problem = None
+    # per-instance parameters of a batch (dim values each, or None): scale of
+    # the variable coefficient, LF constants, time-step wave speeds
+    coef_scale: list = None
+    alpha: list = None
+    wave_speed: list = None
+
+    def oracle_problem(self):
+        """The problem as the oracle's run loop reads it, with this instance's
+        own parameters in place of the batch's shared ones."""
+        from types import SimpleNamespace
+        p = self.problem
+        d = self.dim
+        ns = SimpleNamespace(**{f: getattr(p, f) for f, _ in p._fields_})
+        ns.alpha = list(self.alpha) if self.alpha is not None else [p.alpha[i] for i in range(d)]
+        ns.wave_speed = list(self.wave_speed) if self.wave_speed is not None else [p.wave_speed[i] for i in range(d)]
+        if self.coef_scale is not None:
+            ns.flux_params = (p.flux_param,) + tuple(self.coef_scale)
+        return ns
 
     def config(self):
         return {"workload": self.name, "description": self.description, "dim": self.dim, "k": self.k,
@@ -141,24 +159,72 @@ def c5_rotating(seed=1, nmax=9, level_init=6, t_final=1.0, eps=1e-3):
                     2, 2, 3, nmax, True, level_init, ic, eps, problem=prob)
 
 
-WORKLOADS = {"c1": c1_linear, "c2": c2_burgers, "c3": c3_varcoef_4d, "c5": c5_rotating}
+def c4_advection_3d(seed=1, nmax=7, level_init=3, t_final=0.25, amp=0.5):
+    """One of the 256 independent solves of the batching configuration: 3D
+    u_t + sum_d a_d(x) u_{x_d} = 0 on [0,1]^3 periodic with
+    a_d(x) = c_d (1 + amp sin(2 pi x_{(d+1) mod 3})), seeded speeds
+    c ~ U[0.5,2]^3, u0 a Gaussian bump exp(-|x - c0|^2 / w^2) (periodic
+    distance) with centre U[0.2,0.8]^3 and width U[0.05,0.15], adaptive with
+    eps ~ U[1e-4,1e-3], eta = eps/10, k=2, Lagrange P3 interpolation of a u
+    (LagrInterpolation::var_coeff_u_Lagr_fast), LF constants and time-step
+    speeds max|a_d| = 1.5 c_d, cfl 0.2 (the rule of
+    example/02_hyperbolic_02_var_coeff_coarse_grid.cpp:242-250)."""
+    g = SplitMix64(seed)
+    speed = [g.uniform(0.5, 2.0) for _ in range(3)]
+    center = [g.uniform(0.2, 0.8) for _ in range(3)]
+    width = g.uniform(0.05, 0.15)
+    eps = g.uniform(1e-4, 1e-3)
+    ic = _ic(4, bumps=[(1.0, width, center)])
+    amax = [c * (1.0 + amp) for c in speed]
+    prob = _problem(flux_kind=pyadg.FLUX_VARCOEF_SCALED, flux_param=amp, alpha=amax, wave_speed=amax, cfl=0.2,
+                    t_final=t_final, adaptive=1, eps=eps, eta=eps / 10, max_steps=0, t_start=0.0)
+    return Workload("c4_advection_3d_k2_m3_nmax%d" % nmax,
+                    "3D variable-coefficient advection (seed %d), speeds %s, bump width %.4f, adaptive eps=%.3g, "
+                    "NMAX=%d, sparse init N0=%d, t=%g" % (seed, ["%.3f" % c for c in speed], width, eps, nmax,
+                                                          level_init, t_final),
+                    3, 2, 3, nmax, True, level_init, ic, eps, problem=prob, coef_scale=speed, alpha=amax,
+                    wave_speed=amax)
+
+
+WORKLOADS = {"c1": c1_linear, "c2": c2_burgers, "c3": c3_varcoef_4d, "c4": c4_advection_3d, "c5": c5_rotating}
+
+
+def _instance_params(s, ws):
+    """Per-instance parameters of the workloads that carry them."""
+    if all(w.coef_scale is None and w.alpha is None and w.wave_speed is None for w in ws):
+        return
+    def arr(name):
+        vals = [getattr(w, name) for w in ws]
+        if all(v is None for v in vals):
+            return None
+        return [x for w, v in zip(ws, vals) for x in (v if v is not None else
+                                                          [getattr(w.problem, name)[d] for d in range(w.dim)])]
+    s.set_instance_params(arr("coef_scale"), arr("alpha"), arr("wave_speed"))
 
 
 def make_solution(w):
     s = pyadg.Solution(w.dim, w.k, w.m, w.nmax, w.sparse, w.level_init)
     s.init_adaptive_intp_builtin(w.ic, w.init_eps)
+    _instance_params(s, [w])
     return s
 
 
 def make_batch(workloads):
     """One context holding every workload of the list as an instance.  All must
-    share (dim, k, m, nmax, sparse, level_init, init_eps)."""
+    share (dim, k, m, nmax, sparse, level_init); different initialisation
+    thresholds become per-instance thresholds (also used by the run)."""
     w0 = workloads[0]
     for w in workloads[1:]:
-        if (w.dim, w.k, w.m, w.nmax, w.sparse, w.level_init, w.init_eps) != (
-                w0.dim, w0.k, w0.m, w0.nmax, w0.sparse, w0.level_init, w0.init_eps):
+        if (w.dim, w.k, w.m, w.nmax, w.sparse, w.level_init) != (
+                w0.dim, w0.k, w0.m, w0.nmax, w0.sparse, w0.level_init):
             raise ValueError("instances of one batch must share the discretisation")
     s = pyadg.Solution(w0.dim, w0.k, w0.m, w0.nmax, w0.sparse, w0.level_init)
     s.init_batch(len(workloads), w0.sparse, w0.level_init)
+    per_eps = len({w.init_eps for w in workloads}) > 1 or len({w.problem.eps for w in workloads if w.problem}) > 1
+    if per_eps:
+        if any(w.problem is None or w.init_eps != w.problem.eps for w in workloads):
+            raise ValueError("per-instance thresholds: the initialisation threshold must be the run's eps")
+        s.set_instance_thresholds([w.problem.eps for w in workloads], [w.problem.eta for w in workloads])
     s.init_adaptive_intp_builtin_batch([w.ic for w in workloads], w0.init_eps)
+    _instance_params(s, workloads)
     return s

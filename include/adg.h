@@ -66,7 +66,11 @@ enum { ADG_LU_FULL = 0, ADG_LU_L = 1, ADG_LU_U = 2 };
 enum {
   ADG_FLUX_BURGERS = 0,  /* f_d(u) = u*u*0.5 for every d (FluxFunction::burgers_flux_scalar) */
   ADG_FLUX_SINCOS = 1,   /* f_0 = sin(u)*s, f_1 = cos(u)*s (FluxFunction::sin_flux/cos_flux) */
-  ADG_FLUX_VARCOEF = 2   /* f_d = a_d(x)*u, a_d(x) = 1 + s*sin(2*pi*x_{(d+1) mod dim}) */
+  ADG_FLUX_VARCOEF = 2,  /* f_d = a_d(x)*u, a_d(x) = 1 + s*sin(2*pi*x_{(d+1) mod dim}) */
+  /* f_d = c_d a_d(x) u with the instance's scale c_d (adg_set_instance_params; 1 when unset):
+   * LagrInterpolation::var_coeff_u_Lagr_fast (source/Interplation.cpp:4202-4213) with
+   * coe_func(x, d) = c_d (1 + s sin(2 pi x_{(d+1) mod dim})) */
+  ADG_FLUX_VARCOEF_SCALED = 3
 };
 
 /* RK schemes (ODESolver.h): forward Euler, RK2 midpoint and SSP-RK3. */
@@ -80,6 +84,11 @@ enum { ADG_RK_EULER = 0, ADG_RK2 = 2, ADG_RK3 = 3 };
 int adg_create(int dim, int k, int m, int nmax, adg_ctx** out);
 int adg_destroy(adg_ctx* c);
 const char* adg_last_error(adg_ctx* c);
+/* Waits for the context's stream.  Kernels raise a device error flag instead
+ * of faulting: a non-finite RK stage value (ADG_E_NONFINITE, the reference
+ * drivers' blow-up check, example/02_hyperbolic_02_var_coeff_coarse_grid.cpp:389-396)
+ * or a missing ancestor in the surplus transform (ADG_E_ANCESTOR).  adg_sync,
+ * adg_fp_to_coe and the run drivers (every time step) report and clear it. */
 int adg_sync(adg_ctx* c);
 
 /* Grid: DGSolution/DGAdapt constructor (source/DGSolution.cpp:10-57,
@@ -278,7 +287,8 @@ int adg_run_hyperbolic(adg_ctx* c, const adg_problem* p, adg_run_stats* st);
  * time step and thresholds until each reaches t_final (instances that finish
  * early leave the batch and are put back at the end). */
 int adg_grid_init_batch(adg_ctx* c, int ninst, int sparse, int level_init);
-/* users[i] is passed to f for the points of instance i. */
+/* users[i] is passed to f for the points of instance i.  Instance i refines
+ * with its own threshold when adg_set_instance_thresholds set one (else eps). */
 int adg_init_adaptive_intp_batch(adg_ctx* c, adg_point_fn f, void* const* users, double eps);
 int adg_num_instances(adg_ctx* c);
 /* Elements [first, first + count) of the union belong to instance inst. */
@@ -286,12 +296,23 @@ int adg_instance_range(adg_ctx* c, int inst, int64_t* first, int64_t* count);
 /* Per-instance refine / coarsen thresholds (arrays of adg_num_instances values). */
 int adg_set_instance_thresholds(adg_ctx* c, const double* eps, const double* eta);
 
+/* Per-instance problem parameters of a batch (no reference counterpart: a
+ * reference driver solves one problem).  Each array holds ninst x dim values,
+ * instance-major, or is NULL for the adg_problem's own: coef_scale the c_d of
+ * ADG_FLUX_VARCOEF_SCALED, alpha the Lax-Friedrichs constants of the penalty
+ * (HyperbolicAlptRHS::rhs_flx_penalty_scalar, source/FastMultiplyLU.cpp:1269-1277),
+ * wave_speed the speeds of the time-step rule of adg_run_hyperbolic. */
+int adg_set_instance_params(adg_ctx* c, const double* coef_scale, const double* alpha, const double* wave_speed);
+
 /* Thread mapping of the fused stage kernels (no counterpart in the
  * reference): 0 one thread per output value, 1 one thread per (element, index
  * over the non-swept dimensions), 2 one thread per (element, four such
- * indices), 3 one thread per (element, two such indices).  All mappings give
- * identical results.  -1 (the default) times them on the live grid the first
- * time a kernel meets a grid-size class and keeps the fastest. */
+ * indices), 3 one thread per (element, two such indices), all gathering
+ * through the fiber map; 4, 5, 6 fiber tiles of 64, 128 or 256 elements: a
+ * workgroup stages the inputs of whole fibers in LDS and walks the related
+ * cells there.  All mappings give identical results.  -1 (the default) times
+ * them on the live grid the first time a kernel meets a grid-size class and
+ * keeps the fastest. */
 int adg_set_kernel_variant(adg_ctx* c, int variant);
 
 /* Artificial viscosity in its matrix-free form (viscosity_option 3 of

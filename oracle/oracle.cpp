@@ -929,9 +929,13 @@ struct or_ctx {
           if (kind == 0) v = u * u * 0.5;
           else if (kind == 1) v = (d == 0 ? std::sin(u) : std::cos(u)) * p[0];
           else {
+            // kind 2: a_d = 1 + s sin(2 pi x_{d+1}); kind 3: the same times the
+            // scale c_d = p[1 + d] (LagrInterpolation::var_coeff_u_Lagr_fast,
+            // source/Interplation.cpp:4202-4213, with that coe_func)
             const int dn = (d + 1) % D;
             const double x = lagr.point(e.ord[dn] * (m + 1) + idx[dn]);
-            const double a = 1. + p[0] * std::sin(2. * adg::PI * x);
+            const double a1 = 1. + p[0] * std::sin(2. * adg::PI * x);
+            const double a = kind == 3 ? p[1 + d] * a1 : a1;
             v = a * u;
           }
           e.fp[d][q] = v;

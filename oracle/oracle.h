@@ -50,7 +50,8 @@ int or_imex43_stage(or_ctx* c, int stage);
  * level sum exceeds cap (negative: no cap). */
 void or_set_level_sum_cap(or_ctx* c, int cap);
 /* flux at points: kind 0 burgers u^2/2 (all dims), 1 sin/cos rotating wave scaled by p[0],
- * 2 var-coeff a_d(x)*u with a_d = 1 + p[0]*sin(2*pi*x_{(d+1)%dim}) */
+ * 2 var-coeff a_d(x)*u with a_d = 1 + p[0]*sin(2*pi*x_{(d+1)%dim}),
+ * 3 var-coeff c_d a_d(x) u with c_d = p[1 + d] */
 void or_flux(or_ctx* c, int kind, const double* params);
 void or_fp_to_coe(or_ctx* c);                                 /* eval_fp_to_coe_D_Lag */
 void or_rhs_zero(or_ctx* c);

@@ -8,7 +8,8 @@ coarsen) and of example/02_hyperbolic_01_scalar_const_coefficient.cpp:164-179.
 
 def hyperbolic_rhs(o, prob):
     o.eval_up()
-    o.flux(prob.flux_kind, (prob.flux_param,))
+    # flux parameters: (s,) or, for the scaled variable coefficient, (s, c_0, .., c_{dim-1})
+    o.flux(prob.flux_kind, getattr(prob, "flux_params", (prob.flux_param,)))
     o.fp_to_coe()
     o.rhs_zero()
     o.rhs_lagr_vol()
