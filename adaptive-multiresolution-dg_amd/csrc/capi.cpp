@@ -143,6 +143,7 @@ int adg_create(int dim, int k, int m, int nmax, adg_ctx** out) {
   adg_ctx* h = new adg_ctx();
   try {
     h->c = new adg::Context(dim, k, m, nmax);
+    adg::hip_check(hipGetDevice(&h->c->device), "hipGetDevice");
   } catch (const adg::HipError& e) {
     h->err = e.what();
     *out = h;
@@ -244,34 +245,53 @@ int adg_field_get(adg_ctx* h, int field, double* host) {
   });
 }
 
+namespace {
+// Argument checks and tensor sizes of a host-array sweep; returns (pf, pt).
+void sweep_shape(const adg::Context& c, int table, int lu, int dim, const int32_t* size_in, const int32_t* size_out,
+                 int* sin, int* sout, int64_t& nin, int64_t& nout, int& pf, int& pt) {
+  if (dim < 0 || dim >= c.D) throw std::runtime_error("dimension out of range");
+  if (lu < 0 || lu > 2) throw std::runtime_error("lu must be 0, 1 or 2");
+  if (table < 0 || table > 8) throw std::runtime_error("unknown table id");
+  pf = (table == 0 || table >= 4) ? c.k : c.m;
+  pt = (table == 0) ? c.m : c.k;
+  nin = 1;
+  nout = 1;
+  const int mx = std::max(c.k, c.m) + 1;
+  for (int d = 0; d < c.D; ++d) {
+    sin[d] = size_in[d];
+    sout[d] = size_out[d];
+    if (sin[d] < 1 || sin[d] > mx || sout[d] < 1 || sout[d] > mx) throw std::runtime_error("tensor size out of range");
+    if (d != dim && sin[d] != sout[d]) throw std::runtime_error("sizes may differ only in the swept dimension");
+    nin *= sin[d];
+    nout *= sout[d];
+  }
+  if (sin[dim] != pf + 1 || sout[dim] != pt + 1) throw std::runtime_error("sweep: size mismatch in swept dimension");
+}
+
+// in -> out through device staging buffers din (E*nin) and dout (E*nout)
+void sweep_staged(adg::Context& c, int table, int flx, int lu, int dim, const int* sin, const int* sout, int64_t nin,
+                  int64_t nout, int pf, int pt, double coef, const double* in, double* out, double* din,
+                  double* dout) {
+  adg::hip_check(hipMemcpyAsync(din, in, c.E() * nin * sizeof(double), hipMemcpyHostToDevice, c.stream), "upload");
+  c.sweep(din, sin, dout, nullptr, sout, dim, c.table(table), flx != 0, lu, coef, pf, pt);
+  adg::hip_check(hipMemcpyAsync(out, dout, c.E() * nout * sizeof(double), hipMemcpyDeviceToHost, c.stream),
+                 "download");
+  c.sync();
+}
+}  // namespace
+
 int adg_sweep(adg_ctx* h, int table, int flx, int lu, int dim, const int32_t* size_in, const int32_t* size_out,
               double coef, const double* in, double* out) {
   return guard(h, [&](adg::Context& c) {
-    if (dim < 0 || dim >= c.D) throw std::runtime_error("dimension out of range");
-    if (lu < 0 || lu > 2) throw std::runtime_error("lu must be 0, 1 or 2");
-    const int pf = (table == 0 || table >= 4) ? c.k : c.m;
-    const int pt = (table == 0) ? c.m : c.k;
-    int sin[adg::kMaxDim], sout[adg::kMaxDim];
-    int64_t nin = 1, nout = 1;
-    const int mx = std::max(c.k, c.m) + 1;
-    for (int d = 0; d < c.D; ++d) {
-      sin[d] = size_in[d];
-      sout[d] = size_out[d];
-      if (sin[d] < 1 || sin[d] > mx || sout[d] < 1 || sout[d] > mx) throw std::runtime_error("tensor size out of range");
-      if (d != dim && sin[d] != sout[d]) throw std::runtime_error("sizes may differ only in the swept dimension");
-      nin *= sin[d];
-      nout *= sout[d];
-    }
+    int sin[adg::kMaxDim], sout[adg::kMaxDim], pf, pt;
+    int64_t nin, nout;
+    sweep_shape(c, table, lu, dim, size_in, size_out, sin, sout, nin, nout, pf, pt);
     c.tmpA.ensure(std::max<int64_t>(c.E() * nin, 1));
     c.tmpB.ensure(std::max<int64_t>(c.E() * nout, 1));
-    adg::hip_check(hipMemcpyAsync(c.tmpA.p, in, c.E() * nin * sizeof(double), hipMemcpyHostToDevice, c.stream),
-                   "upload");
-    c.sweep(c.tmpA.p, sin, c.tmpB.p, nullptr, sout, dim, c.table(table), flx != 0, lu, coef, pf, pt);
-    adg::hip_check(hipMemcpyAsync(out, c.tmpB.p, c.E() * nout * sizeof(double), hipMemcpyDeviceToHost, c.stream),
-                   "download");
-    c.sync();
+    sweep_staged(c, table, flx, lu, dim, sin, sout, nin, nout, pf, pt, coef, in, out, c.tmpA.p, c.tmpB.p);
   });
 }
+
 
 int adg_lagr_points(adg_ctx* h, double* x) {
   return guard(h, [&](adg::Context& c) {
@@ -547,6 +567,13 @@ double adg_ic_eval(const double* x, int dim, void* user) {
     }
   }
   return 0.;
+}
+
+int adg_init_from_points(adg_ctx* h, const double* values) {
+  return guard(h, [&](adg::Context& c) {
+    if (!values) throw std::runtime_error("null values");
+    c.init_from_points(values);
+  });
 }
 
 int adg_init_separable(adg_ctx* h, adg_sep_fn f, void* user, int nfunc, double eps) {
@@ -903,7 +930,7 @@ int adg_set_instance_params(adg_ctx* h, const double* coef_scale, const double* 
 
 int adg_set_kernel_variant(adg_ctx* h, int variant) {
   return guard(h, [&](adg::Context& c) {
-    if (variant < -1 || variant >= adg::Context::kVariants) throw std::runtime_error("kernel variant must be -1 or 0..6");
+    if (variant < -1 || variant >= adg::Context::kVariants) throw std::runtime_error("kernel variant must be -1 or 0..9");
     c.kernel_variant = variant;
   });
 }

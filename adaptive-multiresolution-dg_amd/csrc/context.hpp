@@ -115,6 +115,7 @@ class Context {
   double flux_param = 0.;
 
   hipStream_t stream = nullptr;
+  int device = 0;  // the GPU current at creation (the context's stream lives there)
 
   // tables
   DevBuf<double> t_pt, t_la_uv, t_la_uvx, t_la_flx, t_aa[5], t_lin[kMaxDim], t_lmass, t_wt;
@@ -162,7 +163,7 @@ class Context {
   TileSetH tsets[3];
   TileSetH& tile_set(int which);
   template <class T>
-  void tile_launch(int which, int pf1, int pt1, const struct TaskDev* tasks, int R, int ntask);
+  void tile_launch(int variant, int pf1, int pt1, const struct TaskDev* tasks, int R, int ntask);
   DevBuf<double> tile_tv, tile_pv;  // per-term values of the tile form's last sweeps
   DevBuf<int32_t> g_remap;
   DevBuf<int32_t> g_lsum;  // level sum of each element
@@ -210,6 +211,7 @@ class Context {
   void init_adaptive_intp(double (*f)(const double*, int, void*), void* user, double eps);
   void init_adaptive_intp_batch(double (*f)(const double*, int, void*), void* const* users, double eps);
   void init_separable(double (*f)(double, int, int, void*), void* user, int nfunc, double eps);
+  void init_from_points(const double* values);
   // u at every point of the tensor grid x1d^D (first dimension slowest)
   void eval_tensor(int nx, const double* x1d, double* out_host);
   DevBuf<uint64_t> eval_hkey;
@@ -250,9 +252,10 @@ class Context {
   const SplitTable& split_table(const double* dev, size_t n);
   // thread mapping of the fused kernels: 0 per output, 1 per (element, rest
   // index), 2 per (element, 4 rest indices), 3 per (element, 2 rest indices),
-  // 4..6 fiber tiles of 64, 128, 256 elements (LDS-staged); -1 lets the
-  // autotuner choose per kernel class and grid-size class
-  static constexpr int kVariants = 7;
+  // 4..6 fiber tiles of 64, 128, 256 elements (LDS-staged), a thread per
+  // (target, rest index); 7..9 the same tiles, a thread per target; -1 lets
+  // the autotuner choose per kernel class and grid-size class
+  static constexpr int kVariants = 10;
   enum { TUNE_UP_PHASE = 0, TUNE_UP_FINAL, TUNE_RHS_PHASE, TUNE_RHS_FINAL };
   int kernel_variant = -1;
   std::map<int, int> tuned;

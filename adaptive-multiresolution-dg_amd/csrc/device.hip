@@ -21,6 +21,7 @@
 #include "context.hpp"
 #include "fiber.hpp"
 #include "kern.hpp"
+#include "tile.hpp"
 
 namespace adg {
 
@@ -1436,14 +1437,17 @@ Context::TileSetH& Context::tile_set(int which) {
 // One tile launch of ntask tasks with pf1 x pt1 blocks; throws (before any
 // launch) when a tile of this size does not fit in LDS.
 template <class T>
-void Context::tile_launch(int which, int pf1, int pt1, const TaskDev* tasks, int R, int ntask) {
+void Context::tile_launch(int v, int pf1, int pt1, const TaskDev* tasks, int R, int ntask) {
   if (order_version != grid_version) throw std::logic_error("tile launch: fiber order of an older grid");
+  // variants 4..6: a thread per (target, rest index); 7..9: a thread per target
+  const int which = (v - 4) % 3;
+  const bool allr = v >= 7;
   TileSetH& ts = tile_set(which);
   if (tile_rest_chunk(pf1, R, ts.max_tile, sizeof(T)) < 1) throw std::runtime_error("tile: does not fit in LDS");
   const int32_t* tb[kMaxDim];
   const int32_t* te[kMaxDim];
   for (int d = 0; d < D; ++d) { tb[d] = ts.tbeg[d].p; te[d] = ts.tend[d].p; }
-  tile_sweep<T>(pf1, pt1, tasks, ntask, ts.ntile, ts.max_tile, tb, te, D, lsum_cap, stream);
+  tile_sweep<T>(pf1, pt1, tasks, ntask, ts.ntile, ts.max_tile, tb, te, D, lsum_cap, allr, stream);
 }
 
 void Context::stage_fused(const StageProblem& p, int scheme, int stage, double dt) {
@@ -1495,7 +1499,7 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     const int nt = pl.up_phase_ntask[s];
     const std::array<int64_t, 4> tot = pl.up_phase_total[s];
     tuned_launch(TUNE_UP_PHASE + tc, [&](int v) {
-      if (v >= 4) tile_launch<T>(v - 4, k + 1, m + 1, tasks, pl.up_phase_R[s], nt);
+      if (v >= 4) tile_launch<T>(v, k + 1, m + 1, tasks, pl.up_phase_R[s], nt);
       else elem_phase<T>(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
     });
     prof_end();
@@ -1505,7 +1509,7 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     prof_begin(PROF_FLUX, pl.up_final_bytes);
     tuned_launch(TUNE_UP_FINAL + tc, [&](int v) {
       if (v >= 4) {
-        tile_launch<T>(v - 4, k + 1, m + 1, pl.up_final_tasks.p, pl.up_final_R, pl.n_up_terms);
+        tile_launch<T>(v, k + 1, m + 1, pl.up_final_tasks.p, pl.up_final_R, pl.n_up_terms);
         up_combine<T>(tvT, pl.n_up_terms, E() * P, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, stream);
       } else {
         elem_up_final<T>(v, k + 1, m + 1, pl.up_terms.p, pl.n_up_terms, E(), P, D, upp, fpp, vcoef.p, up_kind,
@@ -1526,7 +1530,7 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     const int nt = pl.rhs_phase_ntask[s];
     const std::array<int64_t, 4> tot = pl.rhs_phase_total[s];
     tuned_launch(TUNE_RHS_PHASE + tc, [&](int v) {
-      if (v >= 4) tile_launch<T>(v - 4, m + 1, k + 1, tasks, pl.rhs_phase_R[s], nt);
+      if (v >= 4) tile_launch<T>(v, m + 1, k + 1, tasks, pl.rhs_phase_R[s], nt);
       else elem_phase<T>(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
     });
     prof_end();
@@ -1536,8 +1540,8 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     prof_begin(PROF_RK, pl.rhs_final_bytes);
     tuned_launch(TUNE_RHS_FINAL + tc, [&](int v) {
       if (v >= 4) {
-        tile_launch<T>(v - 4, m + 1, k + 1, pl.rhs_final_tasks.p, pl.rhs_final_R, pl.n_rhs_terms);
-        tile_launch<double>(v - 4, k + 1, k + 1, pl.pen_tasks.p, pl.pen_R, pl.n_pen);
+        tile_launch<T>(v, m + 1, k + 1, pl.rhs_final_tasks.p, pl.rhs_final_R, pl.n_rhs_terms);
+        tile_launch<double>(v, k + 1, k + 1, pl.pen_tasks.p, pl.pen_R, pl.n_pen);
         rhs_combine<T>(tvT, tile_pv.p, pl.n_rhs_terms, pl.n_pen, E(), K, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt,
                        dtv, g_inst.p, err_flag.p, stream);
       } else {
@@ -1705,6 +1709,24 @@ void Context::init_adaptive_intp_batch(double (*f)(const double*, int, void*), v
   // them would make every later grid change copy E*P values on the host
   std::vector<double>().swap(up_host);
   std::vector<char>().swap(new_add);
+}
+
+// Values at the interpolation points (degrees of freedom) of the current
+// family -> Alpert coefficients: hierarchical surplus transform, then the
+// interpolation-to-Alpert partial sums (FastLagrInit::eval_ucoe_Alpt_Lagr,
+// source/FastMultiplyLU.cpp:1617-1620; for the Hermite family
+// HermInterpolation::init_from_Herm_to_alpt, source/Interplation.cpp:5019-5030).
+void Context::init_from_points(const double* values) {
+  std::vector<double> h(values, values + static_cast<size_t>(E()) * P);
+  upload(up, h, stream);
+  ensure_fields();
+  surplus_transform(up.p, uintp.p, interA.p, interB.p, 1);
+  if (E() > 0) hipLaunchKernelGGL(k_fill, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, E() * K, 0.);
+  SweepTable tabs[kMaxDim];
+  int flx[kMaxDim];
+  for (int d = 0; d < D; ++d) { tabs[d] = table(1); flx[d] = 0; }
+  partial_sums(tabs, flx, m, k, 1.0, uintp.p, u.p);
+  check_error_flag();
 }
 
 // fp = coef * du/dx at the points with weight w > 0: w counts the viscosity

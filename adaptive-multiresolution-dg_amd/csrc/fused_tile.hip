@@ -28,6 +28,7 @@
 #include <stdexcept>
 
 #include "kern.hpp"
+#include "tile.hpp"
 
 namespace adg {
 namespace {
@@ -102,7 +103,20 @@ struct TileSet {
 // Metadata bytes of a tile workgroup with at most max_tile elements.
 __host__ __device__ inline int tile_meta_bytes(int max_tile) { return ((max_tile * 3 * 4 + 15) / 16) * 16; }
 
-template <int PF1, int PT1, class T>
+// The related sources of target position p (slot st, walk depth maxl) with
+// their positions, ascending: f(source slot, local position).
+template <class F>
+__device__ __forceinline__ void tile_sources(const TermDev& t, int st, int maxl, int fb, int p0, int n, F&& f) {
+  for_each_present_pos(st, maxl, t.flx != 0, t.lu, t.fbits, t.pre, fb, [&](int s, int pos) {
+    const int li = pos - p0;
+    if (static_cast<unsigned>(li) < static_cast<unsigned>(n)) f(s, li);
+  });
+}
+
+// A thread evaluates one target for RG consecutive rest indices of the
+// staged chunk (RG = 1: one rest index per thread), so one walk over the
+// fiber and one read of each table block serve RG rest indices.
+template <int PF1, int PT1, class T, int RG>
 __global__ __launch_bounds__(kTileThreads) void k_tile(const TaskDev* tasks, int64_t ntile, TileSet ts, int D,
                                                        int cap, int max_tile, int lds_bytes) {
   using Ops = SweepOps<T>;
@@ -120,8 +134,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(const TaskDev* tasks, int
   int32_t* s_fb = s_elem + max_tile;
   int32_t* s_sm = s_fb + max_tile;  // slot | (walk depth + 1) << 16; depth 0: above the level-sum cap
   T* X = reinterpret_cast<T*>(smem + tile_meta_bytes(max_tile));
-  const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += blockDim.x) {
+  const int tid = threadIdx.x, nth = blockDim.x;
+  for (int i = tid; i < n; i += nth) {
     const int e = t.perm[p0 + i];
     const int st = t.slot[e];
     int maxl = t.fmaxl[e];
@@ -139,8 +153,8 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(const TaskDev* tasks, int
   const int sod = so[dd];
   const int sdd = t.sin[dd];
   const int R = t.Tin / t.isz[dd];
-  const int rc = min(R, (lds_bytes - tile_meta_bytes(max_tile)) / (max_tile * PF1 * static_cast<int>(sizeof(T))));
-  const bool flx = t.flx != 0;
+  int rc = min(R, (lds_bytes - tile_meta_bytes(max_tile)) / (max_tile * PF1 * static_cast<int>(sizeof(T))));
+  rc = min(rc, kMaxRest / PF1);  // a staged row has at most kMaxRest <= blockDim values
   const int64_t Mrow = static_cast<int64_t>(t.ns) * PF1 * PT1;
   T* out = Ops::out(tk);
   for (int r0 = 0; r0 < R; r0 += rc) {
@@ -153,46 +167,65 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(const TaskDev* tasks, int
       s_bout[tid] = bo;
     }
     __syncthreads();
-    const int per = nr * PF1;
-    for (int i = tid; i < n * per; i += blockDim.x) {
-      const int p = i / per, rem = i - p * per;
-      const int r = rem / PF1, k = rem - r * PF1;
-      X[i] = Ops::in(t, static_cast<int64_t>(s_elem[p]) * t.Tin + s_bin[r] + k * sdd);
+    {
+      // staging: thread (q, c) copies value c of the rows q, q + step, ...
+      // (per = nr * PF1 <= kMaxRest values of a row, step = rows per pass)
+      const int per = nr * PF1;
+      const int step = nth / per, q0 = tid / per, c = tid - q0 * per;
+      if (q0 < step) {
+        const int r = c / PF1, k = c - r * PF1;
+        const int off = s_bin[r] + k * sdd;
+        for (int p = q0; p < n; p += step)
+          X[p * per + c] = Ops::in(t, static_cast<int64_t>(s_elem[p]) * t.Tin + off);
+      }
     }
     __syncthreads();
-    for (int it = tid; it < n * nr; it += blockDim.x) {
-      const int p = it / nr, r = it - p * nr;
-      const int sm = s_sm[p];
-      const int st = sm & 0xffff, maxl = (sm >> 16) - 1;
-      const int e = s_elem[p];
-      T acc[PT1];
+    // thread (q, g): rest indices g*RG .. g*RG + RG - 1 of the targets q, q + step, ...
+    const int ng = (nr + RG - 1) / RG;
+    const int step = nth / ng, q0 = tid / ng, g = tid - q0 * ng;
+    const int rb = g * RG, nrg = min(RG, nr - rb);
+    if (q0 < step) {
+      for (int p = q0; p < n; p += step) {
+        const int sm = s_sm[p];
+        const int st = sm & 0xffff, maxl = (sm >> 16) - 1;
+        const int e = s_elem[p];
+        T acc[RG][PT1];
 #pragma unroll
-      for (int i = 0; i < PT1; ++i) acc[i] = T(0);
-      if (maxl >= 0) {
-        const int64_t Mst = static_cast<int64_t>(st) * Mrow;
-        for_each_present_pos(st, maxl, flx, t.lu, t.fbits, t.pre, s_fb[p], [&](int s, int pos) {
-          const int li = pos - p0;
-          if (static_cast<unsigned>(li) >= static_cast<unsigned>(n)) return;
-          const T* x = X + (li * nr + r) * PF1;
-          T xv[PF1];
+        for (int r = 0; r < RG; ++r)
 #pragma unroll
-          for (int k = 0; k < PF1; ++k) xv[k] = x[k];
-          const int64_t Mr = Mst + static_cast<int64_t>(s) * PF1 * PT1;
-          typename Ops::M m[PF1][PT1];
+          for (int i = 0; i < PT1; ++i) acc[r][i] = T(0);
+        if (maxl >= 0) {
+          const int64_t Mst = static_cast<int64_t>(st) * Mrow;
+          tile_sources(t, st, maxl, s_fb[p], p0, n, [&](int s, int li) {
+            const int64_t Mr = Mst + static_cast<int64_t>(s) * PF1 * PT1;
+            typename Ops::M m[PF1][PT1];
 #pragma unroll
-          for (int k = 0; k < PF1; ++k)
+            for (int k = 0; k < PF1; ++k)
 #pragma unroll
-            for (int i = 0; i < PT1; ++i) m[k][i] = Ops::mat(t, Mr + k * PT1 + i);
+              for (int i = 0; i < PT1; ++i) m[k][i] = Ops::mat(t, Mr + k * PT1 + i);
+            const T* x = X + (li * nr + rb) * PF1;
 #pragma unroll
-          for (int i = 0; i < PT1; ++i)
+            for (int r = 0; r < RG; ++r) {
+              if (r >= nrg) break;
+              T xv[PF1];
 #pragma unroll
-            for (int k = 0; k < PF1; ++k) Ops::madd(acc[i], xv[k], m[k][i]);
-        });
+              for (int k = 0; k < PF1; ++k) xv[k] = x[r * PF1 + k];
+#pragma unroll
+              for (int i = 0; i < PT1; ++i)
+#pragma unroll
+                for (int k = 0; k < PF1; ++k) Ops::madd(acc[r][i], xv[k], m[k][i]);
+            }
+          });
+        }
+        const T cf = T(term_coef(t, e));
+        T* o = out + static_cast<int64_t>(e) * tk.Tout;
+#pragma unroll
+        for (int r = 0; r < RG; ++r) {
+          if (r >= nrg) break;
+#pragma unroll
+          for (int i = 0; i < PT1; ++i) o[s_bout[rb + r] + i * sod] = maxl >= 0 ? acc[r][i] * cf : T(0);
+        }
       }
-      const T cf = T(term_coef(t, e));
-      T* o = out + static_cast<int64_t>(e) * tk.Tout + s_bout[r];
-#pragma unroll
-      for (int i = 0; i < PT1; ++i) o[i * sod] = maxl >= 0 ? acc[i] * cf : T(0);
     }
   }
 }
@@ -260,11 +293,15 @@ __global__ __launch_bounds__(kBlock) void k_rhs_combine(const T* tv, const doubl
 
 template <int PF1, int PT1, class T>
 void tile_sweep_t(const TaskDev* tasks, int ntask, int64_t ntile, const TileSet& ts, int max_tile, int D, int cap,
-                  int lds, hipStream_t s) {
+                  int lds, bool allr, hipStream_t s) {
   const int64_t nb = ntile * ntask;
   if (nb > INT_MAX) throw std::runtime_error("tile sweep: grid too large");
-  hipLaunchKernelGGL((k_tile<PF1, PT1, T>), dim3(static_cast<unsigned>(nb)), dim3(kTileThreads), lds, s, tasks, ntile,
-                     ts, D, cap, max_tile, lds);
+  if (allr)
+    hipLaunchKernelGGL((k_tile<PF1, PT1, T, 4>), dim3(static_cast<unsigned>(nb)), dim3(kTileThreads), lds, s, tasks,
+                       ntile, ts, D, cap, max_tile, lds);
+  else
+    hipLaunchKernelGGL((k_tile<PF1, PT1, T, 1>), dim3(static_cast<unsigned>(nb)), dim3(kTileThreads), lds, s, tasks,
+                       ntile, ts, D, cap, max_tile, lds);
 }
 
 }  // namespace
@@ -297,13 +334,14 @@ void tile_order(const uint32_t* bits, const int32_t* pre, const int32_t* fbase, 
 int tile_rest_chunk(int pf1, int R, int max_tile, int tbytes) {
   const int avail = kTileLds - tile_meta_bytes(max_tile);
   const int per = max_tile * pf1 * tbytes;
-  if (R > kMaxRest || avail < per) return 0;
-  return std::min(R, avail / per);
+  // a staged row holds at most kMaxRest values of the chunk, one per thread
+  if (R > kMaxRest || avail < per || pf1 > kMaxRest) return 0;
+  return std::min(std::min(R, avail / per), kMaxRest / pf1);
 }
 
 template <class T>
 void tile_sweep(int pf1, int pt1, const TaskDev* tasks, int ntask, int64_t ntile, int max_tile, const int32_t* const* tbeg,
-                const int32_t* const* tend, int D, int cap, hipStream_t s) {
+                const int32_t* const* tend, int D, int cap, bool allr, hipStream_t s) {
   TileSet ts{};
   for (int d = 0; d < D; ++d) {
     ts.tbeg[d] = tbeg[d];
@@ -312,7 +350,7 @@ void tile_sweep(int pf1, int pt1, const TaskDev* tasks, int ntask, int64_t ntile
   const int lds = kTileLds;
 #define ADG_TS(F, P)                                                         \
   if (pf1 == F && pt1 == P) {                                                \
-    tile_sweep_t<F, P, T>(tasks, ntask, ntile, ts, max_tile, D, cap, lds, s); \
+    tile_sweep_t<F, P, T>(tasks, ntask, ntile, ts, max_tile, D, cap, lds, allr, s); \
     return;                                                                  \
   }
   ADG_TS(2, 2) ADG_TS(2, 3) ADG_TS(2, 4) ADG_TS(2, 5) ADG_TS(3, 2) ADG_TS(3, 3) ADG_TS(3, 4) ADG_TS(3, 5)
@@ -340,9 +378,9 @@ void rhs_combine(const T* tv, const double* pv, int nterm, int npen, int64_t E, 
 }
 
 template void tile_sweep<double>(int, int, const TaskDev*, int, int64_t, int, const int32_t* const*,
-                                 const int32_t* const*, int, int, hipStream_t);
+                                 const int32_t* const*, int, int, bool, hipStream_t);
 template void tile_sweep<float>(int, int, const TaskDev*, int, int64_t, int, const int32_t* const*,
-                                const int32_t* const*, int, int, hipStream_t);
+                                const int32_t* const*, int, int, bool, hipStream_t);
 template void up_combine<double>(const double*, int, int64_t, double*, double*, const double*, int, double, int,
                                  hipStream_t);
 template void up_combine<float>(const float*, int, int64_t, float*, float*, const double*, int, double, int,

@@ -220,6 +220,34 @@ void IO::write_error(const int NMAX, const std::vector<double>& err, const std::
   output << std::endl;
 }
 
+void IO::output_num_exa(const std::string& file_name, std::function<double(std::vector<double>)> exact_solution) const {
+  const int dim = DGSolution::DIM;
+  if (dim != 1 && dim != 2) {
+    std::cout << "dimension out of range in IO::output_num_exa()" << std::endl;
+    std::exit(1);
+  }
+  const std::vector<double> u = dgsolution_ptr->val_tensor(x_1D);
+  std::ofstream output(file_name);
+  const size_t n = x_1D.size();
+  for (size_t i = 0; i < u.size(); ++i) {
+    std::vector<double> x(dim);
+    if (dim == 1) x[0] = x_1D[i];
+    else { x[0] = x_1D[i / n]; x[1] = x_1D[i % n]; }
+    for (int d = 0; d < dim; ++d) output << x[d] << "   ";
+    output << u[i] << "   " << exact_solution(x) << std::endl;
+  }
+}
+
+void IO::output_shock_support(const std::string& file_name) const {
+  const ViscosityElementSet& v = dgsolution_ptr->viscosity_element;
+  const int D = DGSolution::DIM;
+  std::ofstream output(file_name);
+  for (size_t e = 0; e < v.supports.size() / (2 * D); ++e) {
+    for (int d = 0; d < D; ++d) output << v.supports[(e * D + d) * 2] << " " << v.supports[(e * D + d) * 2 + 1] << " ";
+    output << std::endl;
+  }
+}
+
 void IO::output_state(const std::string& file_name) const {
   adg_facade_check(dgsolution_ptr->ctx(), adg_state_write(dgsolution_ptr->ctx(), file_name.c_str()), "output_state");
 }
@@ -237,8 +265,27 @@ void DGAdapt::refine() { adg_facade_check(ctx_, adg_refine(ctx_, eps, nullptr), 
 void DGAdapt::coarsen() { adg_facade_check(ctx_, adg_coarsen(ctx_, eta, nullptr), "DGAdapt::coarsen"); }
 void DGAdapt::update_viscosity_element(const double shock_kappa) {
   int64_t n = 0;
-  adg_facade_check(ctx_, adg_viscosity_elements(ctx_, shock_kappa, &n, nullptr), "update_viscosity_element");
+  std::vector<int32_t> flags(static_cast<size_t>(std::max(size_elem(), 1)));
+  adg_facade_check(ctx_, adg_viscosity_elements(ctx_, shock_kappa, &n, flags.data()), "update_viscosity_element");
   viscosity_element.n = n;
+  // supports of the flagged elements (Element::xl / xr, source/Element.cpp:114-120)
+  std::vector<int> level, suppt;
+  elements(level, suppt);
+  viscosity_element.supports.clear();
+  const int D = DIM;
+  for (size_t e = 0; e < level.size() / D; ++e) {
+    if (!flags[e]) continue;
+    for (int d = 0; d < D; ++d) {
+      const int l = level[e * D + d], j = suppt[e * D + d];
+      double xl = 0., xr = 1.;
+      if (l > 1) {
+        xl = std::pow(2., -l + 1.) * (j - 1.) / 2.;
+        xr = std::pow(2., -l + 1.) * (j + 1.) / 2.;
+      }
+      viscosity_element.supports.push_back(xl);
+      viscosity_element.supports.push_back(xr);
+    }
+  }
 }
 
 void LargViscosInterpolation::support_gradu_Lagr_fast(const ViscosityElementSet&, const double coefficient,
@@ -416,6 +463,39 @@ void HermInterpolation::nonlinear_Herm_2D_fast(int flux_id, double flux_param, s
   fastHerm.eval_up_Herm();
   adg_facade_check(c, adg_flux(c, flux_id, &flux_param), "nonlinear_Herm_2D_fast");
   adg_facade_check(c, adg_fp_to_coe(c), "nonlinear_Herm_2D_fast");
+}
+
+void HermInterpolation::nonlinear_Herm_2D_PMAX5_scalar_fast(
+    std::function<double(std::vector<double>, int, int)>, std::function<double(std::vector<double>, int, int, int)>,
+    std::function<double(std::vector<double>, int, int, int, int)>,
+    std::function<double(std::vector<double>, int, int, int, int, int)>,
+    std::function<double(std::vector<double>, int, int, int, int, int, int)>, std::vector<std::vector<bool>>,
+    FastHermIntp&) {
+  std::cout << "HermInterpolation: P5 Hermite interpolation is not provided by the GPU library (use HermBasis::PMAX = 3)"
+            << std::endl;
+  std::exit(1);
+}
+
+void HermInterpolation::init_from_Herm_to_alpt(std::function<double(std::vector<double>, int, std::vector<int>)> func,
+                                               const OperatorMatrix1D<HermBasis, AlptBasis>&) {
+  adg_ctx* c = dgsolution_ptr->ctx();
+  adg_facade_check(c, adg_set_interpolation(c, ADG_INTERP_HERMITE), "init_from_Herm_to_alpt");
+  const int D = 2, P = 16;
+  const int64_t E = adg_num_elem(c);
+  std::vector<double> x(static_cast<size_t>(E) * P * D), v(static_cast<size_t>(E) * P);
+  adg_facade_check(c, adg_lagr_points(c, x.data()), "init_from_Herm_to_alpt");
+  // degree of freedom q = (a, b): point a % 2, derivative order a / 2 in x; b likewise in y
+  std::vector<double> pt(D);
+  std::vector<int> der(D);
+  for (int64_t e = 0; e < E; ++e)
+    for (int q = 0; q < P; ++q) {
+      const int a = q / 4, b = q % 4;
+      der[0] = a / 2;
+      der[1] = b / 2;
+      for (int d = 0; d < D; ++d) pt[d] = x[(e * P + q) * D + d];
+      v[e * P + q] = func(pt, 0, der);
+    }
+  adg_facade_check(c, adg_init_from_points(c, v.data()), "init_from_Herm_to_alpt");
 }
 
 HyperbolicHermRHS::HyperbolicHermRHS(DGSolution& dg, const OperatorMatrix1D<HermBasis, AlptBasis>&)

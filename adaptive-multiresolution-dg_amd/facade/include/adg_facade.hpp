@@ -114,6 +114,8 @@ class OperatorMatrix1D {
 // by the library context; the facade keeps its size.
 struct ViscosityElementSet {
   int64_t n = 0;
+  // supports of the elements at the last update: xl[0], xr[0], xl[1], ... per element
+  std::vector<double> supports;
   size_t size() const { return static_cast<size_t>(n); }
   bool empty() const { return n == 0; }
 };
@@ -249,6 +251,23 @@ class HermInterpolation {
   // Device flux by id (ADG_FLUX_BURGERS / ADG_FLUX_SINCOS): no host round trip.
   void nonlinear_Herm_2D_fast(int flux_id, double flux_param, std::vector<std::vector<bool>> is_intp,
                               FastHermIntp& fastHerm);
+  // HermInterpolation::nonlinear_Herm_2D_PMAX5_scalar_fast (source/Interplation.cpp:5001-5017):
+  // the P5 Hermite family is not provided by the GPU library; the reference
+  // drivers call it only when HermBasis::PMAX == 5, which the constructor
+  // already refuses.  Prints the reason and exits, as the reference does on
+  // unsupported settings.
+  void nonlinear_Herm_2D_PMAX5_scalar_fast(std::function<double(std::vector<double>, int, int)> func,
+                                           std::function<double(std::vector<double>, int, int, int)> func_d1,
+                                           std::function<double(std::vector<double>, int, int, int, int)> func_d2,
+                                           std::function<double(std::vector<double>, int, int, int, int, int)> func_d3,
+                                           std::function<double(std::vector<double>, int, int, int, int, int, int)> func_d4,
+                                           std::vector<std::vector<bool>> is_intp, FastHermIntp& fastHerm);
+  // HermInterpolation::init_from_Herm_to_alpt (source/Interplation.cpp:5019-5030):
+  // func(x, i, derivative orders) at every Hermite degree of freedom (value and
+  // slopes at the points of each element), the Hermite surplus transform and the
+  // Hermite-to-Alpert partial sums (adg_init_from_points).
+  void init_from_Herm_to_alpt(std::function<double(std::vector<double>, int, std::vector<int>)> func,
+                              const OperatorMatrix1D<HermBasis, AlptBasis>& matrix);
   DGSolution* const dgsolution_ptr;
 };
 
@@ -464,6 +483,12 @@ class IO {
   void write_error_time(const double tn, const std::vector<double>& err, const int init,
                         const std::string file_name) const;
   void write_error(const int NMAX, const std::vector<double>& err, const std::string file_name) const;
+  // IO::output_num_exa (source/IO.cpp:135-151, output_num_exa_1D/2D :230-245,369-388):
+  // x (y) u_num u_exa on the cell centres of the finest level
+  void output_num_exa(const std::string& file_name, std::function<double(std::vector<double>)> exact_solution) const;
+  // IO::output_shock_support (source/IO.cpp:518-532): xl xr per dimension of
+  // every viscosity element of the last DGAdapt::update_viscosity_element
+  void output_shock_support(const std::string& file_name) const;
   // Binary state files (element set + Alpert coefficients; adg_state_write /
   // adg_state_read).  No reference counterpart: checkpoints and golden fixtures.
   void output_state(const std::string& file_name) const;

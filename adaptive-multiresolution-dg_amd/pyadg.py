@@ -139,6 +139,7 @@ def lib():
             "adg_instance_range": (i32, [vp, i32, P(i64), P(i64)]),
             "adg_set_instance_thresholds": (i32, [vp, DP, DP]),
             "adg_set_instance_params": (i32, [vp, DP, DP, DP]),
+            "adg_init_from_points": (i32, [vp, DP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -447,6 +448,12 @@ class Solution:
         cfun = POINT_FN(cb)
         self._keep.append(cfun)
         self._check(lib().adg_init_adaptive_intp(self._h, cfun, None, eps))
+
+    def init_from_points(self, values):
+        """Values at the interpolation points of the current family -> Alpert coefficients."""
+        v = np.ascontiguousarray(values, np.float64)
+        assert v.size == self.num_elem * self.P
+        self._check(lib().adg_init_from_points(self._h, _dp(v)))
 
     def init_adaptive_intp_builtin(self, params, eps):
         """Initialise with the library's adg_ic_eval and an ICParams."""

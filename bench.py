@@ -1,19 +1,22 @@
 #!/usr/bin/env python3
 """Throughput of the adaptive sparse-grid DG time step on one MI355X.
 
-A "step" is one full solve of the workload from its initial state (restored
-from a device-resident snapshot): for the adaptive configurations every time
-step is predict (forward Euler RHS), refine, three SSP-RK3 stages, coarsen.
-The metric counts, for every RK stage, the active degrees of freedom it
-updates (DoF-stage updates); the predict stage is done but not counted.
-`value` = DoF-stage updates / wall seconds of exactly --steps timed solves,
-after --warmup untimed solves, with all data resident on the GPU.
+Default workload: the batching configuration, 256 independent 3D
+variable-coefficient advection solves (seeds 1..256: speeds, bump, eps per
+instance), NMAX 7, k=2, Lagrange P3, t=0.25.  A "step" is one full solve of
+the workload from its initial state (restored from a device-resident
+snapshot): every time step is predict (forward Euler RHS), refine, three
+SSP-RK3 stages, coarsen.  The metric counts, for every RK stage, the active
+degrees of freedom it updates (DoF-stage updates); the predict stage is done
+but not counted.  `value` = DoF-stage updates / wall seconds of exactly
+--steps timed solves, after --warmup untimed solves, with all data resident on
+the GPU.
 
-Prints one JSON line.  The roofline object describes the dominant kernel (the
-1D fiber sweep), timed with HIP events on the library's stream during the timed
-region; the cpu_baseline object times the CPU oracle (a restatement of the
-reference's algorithm with its data structures and OpenMP loop) on the first
-steps of the same workload.
+Prints one JSON line.  The roofline object describes the dominant kernel of
+the chosen form, timed with HIP events on the library's streams during the
+timed region; the cpu_baseline object times the CPU oracle (a restatement of
+the reference's algorithm with its data structures) on the same instances,
+one instance per host thread, for a bounded wall time.
 """
 import argparse
 import ctypes
@@ -35,57 +38,82 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "DoF-stage updates/s on 1 GPU: plain FP64 vs chosen form at equal L2 error"
 
 
-def cpu_baseline(w, max_seconds, threads):
-    """Times the CPU oracle on the same workload: whole solves from t=0 (a new
-    oracle per solve, its setup untimed) until max_seconds of stepping have
-    accumulated; for the linear workload, time steps until max_seconds."""
+def _oracle_for(w):
+    import pyoracle
+    o = pyoracle.Oracle(w.dim, w.k, w.m, w.nmax, w.sparse, w.level_init)
+    o._keep.append(w.ic)
+    o.init_adaptive_intp_c(pyadg.ic_eval_ptr(), ctypes.cast(ctypes.pointer(w.ic), ctypes.c_void_p), w.init_eps)
+    return o
+
+
+def cpu_baseline(ws, max_seconds, threads):
+    """Times the CPU oracle on the workload's own instances: `threads` host
+    threads, each solving instances (one OpenMP thread per oracle) from t=0 one
+    time step per call, until max_seconds of wall time have passed (the
+    oracle's setup is outside the timed windows).  For the linear workload:
+    time steps of the one instance on `threads` OpenMP threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     import runs
 
-    pyoracle.set_threads(threads)
-
-    def fresh():
-        o = pyoracle.Oracle(w.dim, w.k, w.m, w.nmax, w.sparse, w.level_init)
-        o._keep.append(w.ic)
-        o.init_adaptive_intp_c(pyadg.ic_eval_ptr(), ctypes.cast(ctypes.pointer(w.ic), ctypes.c_void_p), w.init_eps)
-        return o
-
-    sec, dof_stages, steps, solves = 0.0, 0, 0, 0
+    w = ws[0]
     if w.linear:
-        o = fresh()
+        pyoracle.set_threads(threads)
+        o = _oracle_for(w)
+        sec, dof_stages, steps = 0.0, 0, 0
         while sec < max_seconds and steps < w.nsteps:
             n = min(20, w.nsteps - steps)
             t0 = time.perf_counter()
             dof_stages += runs.run_linear(o, w.coef, w.dt, n)
             sec += time.perf_counter() - t0
             steps += n
-        sample = "CPU oracle, first %d of %d time steps of the solve" % (steps, w.nsteps)
-    else:
-        # whole solves from t = 0, advanced one time step per call so that the
-        # sample stops after max_seconds even inside a long solve
-        partial = False
-        while sec < max_seconds and solves < 5:
-            o = fresh()
-            prob = pyadg.Problem.from_buffer_copy(w.problem)
+        return {"value": dof_stages / sec, "unit": "DoF-stage updates/s", "cores": threads, "kind": "port",
+                "sample": "CPU oracle, first %d of %d time steps of the solve on %d OpenMP threads: %d DoF-stage "
+                          "updates in %.2f s" % (steps, w.nsteps, threads, dof_stages, sec)}
+    lock = threading.Lock()
+    state = {"next": 0, "dof": 0, "steps": 0, "solves": 0, "partial": 0}
+    deadline = [None]
+
+    def worker():
+        pyoracle.set_threads(1)
+        while True:
+            with lock:
+                i = state["next"]
+                state["next"] += 1
+            if i >= len(ws) or time.perf_counter() >= deadline[0]:
+                return
+            wi = ws[i]
+            o = _oracle_for(wi)
+            prob = wi.oracle_problem()
             prob.t_start, prob.max_steps = 0.0, 1
+            done = True
             while prob.t_start < prob.t_final:
-                t0 = time.perf_counter()
-                n, ds, t = runs.run_hyperbolic(o, prob)
-                sec += time.perf_counter() - t0
-                dof_stages += ds
-                steps += n
-                prob.t_start = t
-                if sec >= max_seconds and prob.t_start < prob.t_final:
-                    partial = True
+                if time.perf_counter() >= deadline[0]:
+                    done = False
                     break
-            solves += 1
+                n, ds, t = runs.run_hyperbolic(o, prob)
+                prob.t_start = t
+                with lock:
+                    state["dof"] += ds
+                    state["steps"] += n
+            with lock:
+                state["solves" if done else "partial"] += 1
             o.close()
-        sample = "CPU oracle, %d solve(s) of the workload from t=0 (%d time steps%s)" % (
-            solves, steps, ", the last one stopped at the time limit" if partial else "")
-    return {"value": dof_stages / sec, "unit": "DoF-stage updates/s", "cores": threads, "kind": "port",
-            "sample": sample + ": %d DoF-stage updates in %.2f s; reference data structures and OpenMP "
-                               "transform_1D loop, g++ -O3 -fopenmp" % (dof_stages, sec)}
+
+    th = [threading.Thread(target=worker) for _ in range(threads)]
+    t0 = time.perf_counter()
+    deadline[0] = t0 + max_seconds
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    sec = time.perf_counter() - t0
+    return {"value": state["dof"] / sec, "unit": "DoF-stage updates/s", "cores": threads, "kind": "port",
+            "sample": "CPU oracle on the workload's instances (%d threads, one instance and one OpenMP thread each, "
+                      "reference data structures, g++ -O3 -fopenmp): %d whole solves and %d stopped at the %.0f s "
+                      "wall limit, %d time steps, %d DoF-stage updates in %.2f s wall (oracle setup included in "
+                      "the wall time)" % (threads, state["solves"], state["partial"], max_seconds, state["steps"],
+                                          state["dof"], sec)}
 
 
 def imex_sample(args):
@@ -153,17 +181,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=sorted(workloads.WORKLOADS))
+    ap.add_argument("--workload", default="c4", choices=sorted(workloads.WORKLOADS))
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="stepping time of the CPU oracle sample")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="wall time of the CPU oracle sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-events", action="store_true", help="time without per-kernel HIP events")
     ap.add_argument("--time-steps", type=int, default=0,
                     help="cap the time steps of every solve (0: run to the workload's final time)")
     ap.add_argument("--streams", type=int, default=3,
                     help="contexts (HIP streams, host threads) the batch is split over in the chosen form")
-    ap.add_argument("--batch", type=int, default=1024,
-                    help="independent seeded instances solved together in the chosen form (c2, c5)")
+    ap.add_argument("--batch", type=int, default=None,
+                    help="independent seeded instances solved together in the chosen form (c4: 256, the "
+                         "configuration's own batch; c2, c5: 1024 seeds of the configuration)")
     ap.add_argument("--refine", default="predict", choices=("predict", "learned"),
                     help="refinement of the chosen form's adaptive steps: forward-Euler prediction (the reference) "
                          "or the learned proposer with the threshold judge")
@@ -185,10 +214,13 @@ def main():
         imex_sample(args)
         return
     make = workloads.WORKLOADS[args.workload]
-    batched = args.workload in ("c2", "c5") and args.batch > 1
+    seeded = args.workload in ("c2", "c4", "c5")
+    if args.batch is None:
+        args.batch = 256 if args.workload == "c4" else (1024 if seeded else 1)
+    batched = seeded and args.batch > 1
 
     def workload(seed):
-        return make(seed=seed) if args.workload in ("c2", "c5") else make()
+        return make(seed=seed) if seeded else make()
 
     w = workload(args.seed)
     print(f"[bench] workload {w.name}: init", file=sys.stderr, flush=True)
@@ -357,8 +389,9 @@ def main():
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         print(f"[bench] cpu baseline on {threads} threads", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(w, args.cpu_seconds, threads)
-        one = cpu_baseline(w, args.cpu_seconds / 4, 1)
+        cws = ws if batched else [w]
+        cpu = cpu_baseline(cws, args.cpu_seconds, threads)
+        one = cpu_baseline(cws, args.cpu_seconds / 4, 1)
         cpu["one_thread_value"] = one["value"]
 
     pname = ("FP32 (split FP32 tables; FP64 right-hand-side accumulation, RK and norms)"

@@ -206,6 +206,14 @@ typedef struct {
 } adg_ic_params;
 double adg_ic_eval(const double* x, int dim, void* user);
 
+/* Values at every element's interpolation points (degrees of freedom) of the
+ * current family, E x (m+1)^dim in hash order (for Hermite: values and slopes,
+ * as adg_lagr_points orders them) -> Alpert coefficients: the hierarchical
+ * surplus transform, then the interpolation-to-Alpert partial sums
+ * (FastLagrInit::eval_ucoe_Alpt_Lagr, source/FastMultiplyLU.cpp:1617-1620;
+ * HermInterpolation::init_from_Herm_to_alpt, source/Interplation.cpp:5019-5030). */
+int adg_init_from_points(adg_ctx* c, const double* values);
+
 /* Separable initial function u0(x) = sum_i prod_d f_i(x_d, d): f returns the
  * factor of term ifunc in dimension d at coordinate x. */
 typedef double (*adg_sep_fn)(double x, int d, int ifunc, void* user);
@@ -310,7 +318,9 @@ int adg_set_instance_params(adg_ctx* c, const double* coef_scale, const double* 
  * indices), 3 one thread per (element, two such indices), all gathering
  * through the fiber map; 4, 5, 6 fiber tiles of 64, 128 or 256 elements: a
  * workgroup stages the inputs of whole fibers in LDS and walks the related
- * cells there.  All mappings give identical results.  -1 (the default) times
+ * cells there, one thread per (element, index over the non-swept dimensions);
+ * 7, 8, 9 the same tiles, one thread per element for all those indices.  All
+ * mappings give identical results.  -1 (the default) times
  * them on the live grid the first time a kernel meets a grid-size class and
  * keeps the fastest. */
 int adg_set_kernel_variant(adg_ctx* c, int variant);

@@ -13,7 +13,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FACADE = os.path.join(ROOT, "adaptive-multiresolution-dg_amd", "facade")
 REF_EXAMPLES = "/root/reference/example"
 REF_DRIVERS = ["02_hyperbolic_01_scalar_const_coefficient", "02_hyperbolic_01_scalar_const_coefficient_coarse_grid",
-               "02_hyperbolic_02_var_coeff", "02_hyperbolic_02_var_coeff_coarse_grid"]
+               "02_hyperbolic_02_var_coeff", "02_hyperbolic_02_var_coeff_coarse_grid",
+               "02_hyperbolic_05_burgers_adapt", "02_hyperbolic_07_kpp"]
 
 
 def test_facade_and_drivers_build():

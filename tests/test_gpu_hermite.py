@@ -89,7 +89,7 @@ def test_hermite_sincos_flux_tolerance(form):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, -1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, -1])
 def test_hermite_fused_stage_matches_per_sweep_path(kind, variant):
     p = burgers_ic(2) if kind == 0 else sincos_ic()
     prob = problem(flux_kind=kind, flux_param=1.0, alpha=[1.2, 1.2], wave_speed=[1.0, 1.0], cfl=0.2, t_final=1.0)

@@ -180,7 +180,7 @@ def test_sincos_flux_tolerance():
 
 @pytest.mark.parametrize("dim,k,m,nmax,level_init,kind", [(2, 2, 3, 6, 4, 0), (2, 2, 3, 6, 4, 1), (3, 1, 2, 4, 3, 2),
                                                           (4, 1, 2, 4, 4, 2), (2, 3, 4, 5, 4, 2)])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, -1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, -1])
 def test_fused_stage_matches_per_sweep_path(dim, k, m, nmax, level_init, kind, variant):
     # the fused stage must reproduce the per-sweep kernels bit for bit (same sums, same order)
     if kind == 0:

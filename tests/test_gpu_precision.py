@@ -19,7 +19,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, -1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, -1])
 @pytest.mark.parametrize("case", ["burgers", "sincos", "varcoef3d", "hermite"])
 def test_fp32_stage_close_to_fp64(case, variant):
     print(f"[case] fp32 stage {case} variant={variant}", flush=True)

@@ -1,0 +1,132 @@
+"""Pins of oracle functions against closed forms and independent computations
+(CPU).  The host setup code these check (csrc/setup.cpp) is shared by the
+oracle and the device library, so GPU-against-oracle tests cannot catch a
+mistake in it; these can.
+
+- Hash::hash_key (source/Hash.cpp:55-114): key = ind1 + ind2 + ind3 with
+  ind1 the mixed-radix support index (radix 2^l_t), ind2 = 2^S sum_{t>=1}
+  [C(t + S_t, t) - C(t + S_{t-1}, t)] (S_t the prefix sums of the levels,
+  S the level sum) and ind3 = sum_{s<S} C(d-1+s, d-1) 2^s, evaluated here with
+  exact integer binomials for every element with level sum <= 8, d = 2..4,
+  against the oracle's and the library's implementations (double binomials
+  with int truncation, exact below row 23).
+- DGAdaptIntp::indicator_norm_intp_Lag weights (source/DGAdaptIntp.cpp:236-272):
+  the diagonal of the Lagrange mass matrix, u_v(i, i) = int L_i^2, against
+  12-point Gauss-Legendre quadrature on 16 sub-intervals of the support.
+- HyperbolicLagrRHS::rhs_flx_intp_scalar (source/FastMultiplyLU.cpp:1161-1189):
+  the L/U partial sums of the flux term against the dense tensor-product
+  operator over all element pairs (0.5 (ulft_vjp + urgt_vjp) in the flux
+  dimension, u_v in the other).
+"""
+import ctypes
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def hash_closed_form(lv, sp):
+    d = len(lv)
+    ind1 = 0
+    for l, j in zip(lv, sp):
+        ind1 = ind1 * 2 ** l + (j - 1) // 2
+    S = lv[0]
+    ind2 = 0
+    for t in range(1, d):
+        ind2 -= math.comb(t + S, t)
+        S += lv[t]
+        ind2 += math.comb(t + S, t)
+    ind2 *= 2 ** S
+    ind3 = sum(math.comb(d - 1 + s, d - 1) * 2 ** s for s in range(S))
+    return ind1 + ind2 + ind3
+
+
+def _library_hash():
+    lib = ctypes.CDLL(os.path.join(ROOT, "adaptive-multiresolution-dg_amd", "libadg_hip.so"))
+    f = lib.adg_hash_key
+    f.restype = ctypes.c_int32
+    f.argtypes = [ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int]
+    return f
+
+
+@pytest.mark.parametrize("dim", [2, 3, 4])
+def test_hash_key_values_closed_form(dim):
+    lib_hash = _library_hash()
+    n = 0
+    for lv in itertools.product(range(9), repeat=dim):
+        if sum(lv) > 8:
+            continue
+        ranges = [[1] if l == 0 else range(1, 2 ** l, 2) for l in lv]
+        for sp in itertools.product(*ranges):
+            want = hash_closed_form(lv, sp)
+            assert pyoracle.hash_key(list(lv), list(sp)) == want, (lv, sp)
+            a = (ctypes.c_int32 * dim)(*lv)
+            b = (ctypes.c_int32 * dim)(*sp)
+            assert lib_hash(a, b, dim) == want, (lv, sp)
+            n += 1
+    assert n == {2: 1280, 3: 4096, 4: 10496}[dim]  # sparse-grid element counts, N = 8
+
+
+def gl_fine(f, a, b, nsub=16, npt=12):
+    x, w = np.polynomial.legendre.leggauss(npt)
+    tot = 0.0
+    edges = np.linspace(a, b, nsub + 1)
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        t = 0.5 * (hi - lo) * x + 0.5 * (hi + lo)
+        tot += 0.5 * (hi - lo) * np.sum(w * np.array([f(tt) for tt in t]))
+    return tot
+
+
+@pytest.mark.parametrize("m", [1, 2, 3, 4])
+def test_lagrange_mass_diagonal_independent_quadrature(m):
+    nmax = 4
+    o = pyoracle.Oracle(1, 2, m, nmax, False, 0)
+    diag = o.lagr_mass()
+    p1 = m + 1
+    for order in range(len(diag)):
+        s = order // p1
+        level = 0 if s == 0 else int(s).bit_length()
+        if level <= 1:
+            a, b = 0.0, 1.0
+        else:
+            n = 1 << (level - 1)
+            a, b = (s - n) / n, (s - n + 1) / n
+        want = gl_fine(lambda x: o.basis_val(1, order, x) ** 2, a, b)
+        assert abs(diag[order] - want) <= 1e-13 * max(1.0, abs(want)), (order, diag[order], want)
+    # a planted mistake (the weight of the level-0 function p=0 halved) would fail
+    assert abs(0.5 * diag[0] - gl_fine(lambda x: o.basis_val(1, 0, x) ** 2, 0.0, 1.0)) > 1e-3
+
+
+def slots(o):
+    lev, sup, _ = o.grid()
+    return np.where(lev == 0, 0, (1 << np.maximum(lev - 1, 0)) + (sup - 1) // 2)
+
+
+@pytest.mark.parametrize("sparse", [True, False])
+def test_rhs_flx_equals_dense_tensor_operator(sparse):
+    k, m, n = 2, 3, 3
+    o = pyoracle.Oracle(2, k, m, n, sparse, n)
+    uv, flx = o.table(1), o.table(3)
+    rng = np.random.default_rng(7)
+    f0 = rng.standard_normal((o.num_elem, m + 1, m + 1))
+    f1 = rng.standard_normal((o.num_elem, m + 1, m + 1))
+    o.set(10, f0.ravel())
+    o.set(11, f1.ravel())
+    o.rhs_zero()
+    o.rhs_lagr_flx()
+    got = o.get(pyoracle.FIELD_RHS).reshape(o.num_elem, k + 1, k + 1)
+    s = slots(o)
+    want = np.zeros_like(got)
+    for t in range(o.num_elem):
+        for e in range(o.num_elem):
+            def blk(M, d):
+                return M[s[e, d] * (m + 1):(s[e, d] + 1) * (m + 1), s[t, d] * (k + 1):(s[t, d] + 1) * (k + 1)]
+            want[t] += 0.5 * (blk(flx, 0).T @ f0[e] @ blk(uv, 1))
+            want[t] += 0.5 * (blk(uv, 0).T @ f1[e] @ blk(flx, 1))
+    assert np.max(np.abs(got - want)) < 1e-12 * max(1.0, np.max(np.abs(want)))
