@@ -144,6 +144,7 @@ int adg_create(int dim, int k, int m, int nmax, adg_ctx** out) {
   try {
     h->c = new adg::Context(dim, k, m, nmax);
     adg::hip_check(hipGetDevice(&h->c->device), "hipGetDevice");
+    adg::noalloc_setup(h->c->device);  // device address of the scratch array, once per GPU
   } catch (const adg::HipError& e) {
     h->err = e.what();
     *out = h;
@@ -292,6 +293,21 @@ int adg_sweep(adg_ctx* h, int table, int flx, int lu, int dim, const int32_t* si
   });
 }
 
+int adg_sweep_noalloc(adg_ctx* h, int table, int flx, int lu, int dim, const int32_t* size_in,
+                      const int32_t* size_out, double coef, const double* in, double* out) {
+  return guard(h, [&](adg::Context& c) {
+    int sin[adg::kMaxDim], sout[adg::kMaxDim], pf, pt;
+    int64_t nin, nout;
+    sweep_shape(c, table, lu, dim, size_in, size_out, sin, sout, nin, nout, pf, pt);
+    const int64_t need = c.E() * (nin + nout);
+    if (need > static_cast<int64_t>(adg::kNoallocScratchDoubles))
+      throw std::runtime_error("adg_sweep_noalloc: E x (input + output) = " + std::to_string(need) +
+                               " values exceeds the scratch capacity of " +
+                               std::to_string(adg::kNoallocScratchDoubles) + " (use adg_sweep)");
+    double* s = adg::noalloc_scratch(c.device);  // the array's device address on this GPU
+    sweep_staged(c, table, flx, lu, dim, sin, sout, nin, nout, pf, pt, coef, in, out, s, s + c.E() * nin);
+  });
+}
 
 int adg_lagr_points(adg_ctx* h, double* x) {
   return guard(h, [&](adg::Context& c) {

@@ -27,6 +27,13 @@ void hip_check(hipError_t e, const char* what);
 // runs fn(0..n-1) on the library's host thread pool (serially for small n)
 void parallel_for(int n, const std::function<void(int)>& fn);
 
+// Allocation-free entry points (noalloc.hip): capacity of the per-GPU
+// scratch array and its device address on a device (set up by
+// noalloc_setup with that device current, once per device).
+constexpr size_t kNoallocScratchDoubles = size_t{1} << 22;
+void noalloc_setup(int device);
+double* noalloc_scratch(int device);
+
 constexpr int ADG_FLUX_VARCOEF_ID = 2;         // ADG_FLUX_VARCOEF of include/adg.h
 constexpr int ADG_FLUX_VARCOEF_SCALED_ID = 3;  // ADG_FLUX_VARCOEF_SCALED
 inline bool is_varcoef(int kind) { return kind == ADG_FLUX_VARCOEF_ID || kind == ADG_FLUX_VARCOEF_SCALED_ID; }

@@ -76,6 +76,7 @@ def lib():
             "adg_field_set": (i32, [vp, i32, DP]),
             "adg_field_get": (i32, [vp, i32, DP]),
             "adg_sweep": (i32, [vp, i32, i32, i32, i32, I32P, I32P, dbl, DP, DP]),
+            "adg_sweep_noalloc": (i32, [vp, i32, i32, i32, i32, I32P, I32P, dbl, DP, DP]),
             "adg_eval_up": (i32, [vp]),
             "adg_set_level_sum_cap": (i32, [vp, i32]),
             "adg_set_interpolation": (i32, [vp, i32]),
@@ -258,6 +259,16 @@ class Solution:
         x = np.ascontiguousarray(x, dtype=np.float64)
         out = np.zeros(self.num_elem * int(np.prod(so)))
         self._check(lib().adg_sweep(self._h, table, int(flx), lu, dim, _ip(si), _ip(so), coef, _dp(x), _dp(out)))
+        return out
+
+    def sweep_noalloc(self, table, flx, lu, dim, size_in, size_out, coef, x):
+        """adg_sweep through the allocation-free entry point (per-GPU scratch staging)."""
+        si = np.asarray(size_in, np.int32)
+        so = np.asarray(size_out, np.int32)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.num_elem * int(np.prod(so)))
+        self._check(lib().adg_sweep_noalloc(self._h, table, int(flx), lu, dim, _ip(si), _ip(so), coef, _dp(x),
+                                            _dp(out)))
         return out
 
     def lagr_points(self):

@@ -113,6 +113,19 @@ int adg_field_get(adg_ctx* c, int field, double* host);
 int adg_sweep(adg_ctx* c, int table, int flx, int lu, int dim, const int32_t* size_in,
               const int32_t* size_out, double coef, const double* in, double* out);
 
+/* Allocation-free variant of adg_sweep.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * checks and results as adg_sweep; the device copies of in and out are staged
+ * in the library's per-GPU scratch array (2^22 doubles, held by every GPU the
+ * library runs on until the process ends) instead of library-allocated
+ * buffers, and the call makes no hipMalloc / hipFree.  E x (input + output
+ * values per element) above 2^22 is refused (ADG_E_ARG).  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (other host
+ * threads, other contexts) race. */
+int adg_sweep_noalloc(adg_ctx* c, int table, int flx, int lu, int dim, const int32_t* size_in,
+                      const int32_t* size_out, double coef, const double* in, double* out);
+
 /* Coordinates of every element's interpolation points of the current family
  * (LagrBasis::intep_pt, source/LagrBasis.cpp:19-28; HermBasis::intep_pt,
  * source/HermBasis.cpp:9-30): x[(e*P + q)*dim + d], points in row-major
