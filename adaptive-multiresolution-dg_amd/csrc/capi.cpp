@@ -52,6 +52,8 @@ double* field_ptr(adg::Context& c, int field) {
   }
   if (field >= ADG_FIELD_FUCOE && field < ADG_FIELD_FUCOE + c.D)
     return c.fucoe.p + static_cast<int64_t>(c.same_flux ? 0 : field - ADG_FIELD_FUCOE) * c.E() * c.P;
+  if (field >= ADG_FIELD_FP && field < ADG_FIELD_FP + c.D)
+    return c.fp.p + static_cast<int64_t>(c.same_flux ? 0 : field - ADG_FIELD_FP) * c.E() * c.P;
   throw std::runtime_error("unknown field");
 }
 
@@ -804,15 +806,23 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
           std::vector<int32_t> ll;
           if (c.log_decisions) log_features(c, eps, lf, ll);
           c.copy_to_predict();
-          c.rk_init();
-          stage(ADG_RK_EULER, 0);
+          if (p->form == 1) {
+            c.rk_step(sp, ADG_RK_EULER, 1, dt1);
+          } else {
+            c.rk_init();
+            stage(ADG_RK_EULER, 0);
+          }
           if (c.log_decisions) log_labels(c, eps, lf, ll);
           c.refine_instances(eps);
           c.copy_predict_to_ucoe();
           max_elem = std::max(max_elem, c.E());
         }
-        c.rk_init();
-        for (int s = 0; s < 3; ++s) stage(ADG_RK3, s);
+        if (p->form == 1) {
+          c.rk_step(sp, ADG_RK3, 3, dt1);
+        } else {
+          c.rk_init();
+          for (int s = 0; s < 3; ++s) stage(ADG_RK3, s);
+        }
       }
       dof_stages += 3 * c.E() * c.K;
       if (p->adaptive) c.coarsen_instances(eta);
@@ -948,6 +958,48 @@ int adg_set_kernel_variant(adg_ctx* h, int variant) {
   return guard(h, [&](adg::Context& c) {
     if (variant < -1 || variant >= adg::Context::kVariants) throw std::runtime_error("kernel variant must be -1 or 0..9");
     c.kernel_variant = variant;
+  });
+}
+
+int adg_set_stage_launch(adg_ctx* h, int mode) {
+  return guard(h, [&](adg::Context& c) {
+    if (mode != 0 && mode != 1) throw std::runtime_error("stage launch mode must be 0 or 1");
+    // The persistent cooperative stage is switched off: in some runs its
+    // launch did not finish on the device, with inputs on which other runs
+    // finished, and the cause is not found.
+    if (mode == 1) throw std::runtime_error("persistent stage launch is switched off (did not always finish)");
+    c.stage_launch = mode;
+  });
+}
+
+int adg_stage_launch_count(adg_ctx* h, int64_t* launches, int64_t* stages) {
+  return guard(h, [&](adg::Context& c) {
+    if (launches) *launches = c.stage_launches;
+    if (stages) *stages = c.stage_count;
+  });
+}
+
+int adg_set_graph_step(adg_ctx* h, int on) {
+  return guard(h, [&](adg::Context& c) {
+    if (on != 0 && on != 1) throw std::runtime_error("graph step must be 0 or 1");
+    if (!on) c.drop_step_graphs();
+    c.graph_step = on == 1;
+  });
+}
+
+int adg_graph_step_count(adg_ctx* h, int64_t* captures, int64_t* replays) {
+  return guard(h, [&](adg::Context& c) {
+    if (captures) *captures = c.graph_captures;
+    if (replays) *replays = c.graph_replays;
+  });
+}
+
+int adg_rk_step(adg_ctx* h, const adg_problem* p, int scheme, double dt) {
+  return guard(h, [&](adg::Context& c) {
+    if (!p) throw std::runtime_error("null problem");
+    const int nstage = scheme == ADG_RK_EULER ? 1 : scheme == ADG_RK2 ? 2 : scheme == ADG_RK3 ? 3 : 0;
+    if (nstage == 0) throw std::runtime_error("invalid RK scheme");
+    c.rk_step(stage_problem(p, c.D), scheme, nstage, dt);
   });
 }
 

@@ -246,6 +246,45 @@ class Context {
   FusedPlan& fused_plan(const StageProblem& p);
   template <class T>
   void stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int stage, double dt);
+  // How a fused stage is launched: 0 one launch per phase (thread mappings
+  // chosen by the autotuner), 1 one persistent cooperative launch per stage
+  // (fiber tiles of 64 elements, four rest indices per thread; Lagrange family)
+  int stage_launch = 0;
+  template <class T>
+  void stage_coop(const StageProblem& p, FusedPlan& pl, int scheme, int stage, double dt);
+  int64_t stage_launches = 0, stage_count = 0;  // kernel launches and fused stages (counters)
+
+  // One explicit RK step of fused stages (u^n <- u, then `nstage` stages).
+  // With graph_step on, the step is recorded into a HIP graph the second
+  // time it runs on the same grid (the first run builds the plans and tunes
+  // the kernels) and replayed after that.  A graph holds the device pointers
+  // and launch shapes of its grid, so any change of the grid, of a buffer,
+  // of the problem or of the launch settings records a new one; four graphs
+  // are kept (a step of each kind from each buffer of the u / u_next pair).
+  void rk_step(const StageProblem& p, int scheme, int nstage, double dt);
+  void rk_step_eager(const StageProblem& p, int scheme, int nstage, double dt);
+  struct StepGraph {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec = nullptr;
+    int64_t launches = 0;  // kernel launches of the recorded step
+  };
+  std::vector<uintptr_t> step_key(const StageProblem& p, int scheme, int nstage, double dt) const;
+  void drop_step_graphs();
+  // Lazily built state (plans, fiber orders, tile sets, split tables, point
+  // coefficients, kernel tuning) uploads host data or times launches, which a
+  // graph cannot hold: while a step is recorded, such a build throws
+  // NeedsBuild before it changes anything, and the step runs without a graph.
+  struct NeedsBuild : std::logic_error {
+    explicit NeedsBuild(const char* what) : std::logic_error(std::string("needs a build while recording: ") + what) {}
+  };
+  bool recording = false;
+  void not_recording(const char* what) const;
+  bool graph_step = false;
+  static constexpr size_t kStepGraphs = 4;
+  StepGraph step_graphs[kStepGraphs];
+  size_t next_graph = 0;
+  std::vector<std::vector<uintptr_t>> step_seen;  // states of the last steps run without a graph
+  int64_t graph_captures = 0, graph_replays = 0;
 
   // Storage precision of the fused stage: 0 FP64; 1 FP32 fiber transforms,
   // point values, flux and surplus transform, with every 1D table and surplus

@@ -21,7 +21,9 @@
 #include "context.hpp"
 #include "fiber.hpp"
 #include "kern.hpp"
+#include "surplus_dev.hpp"
 #include "tile.hpp"
+#include "coop.hpp"
 
 namespace adg {
 
@@ -269,64 +271,9 @@ __global__ __launch_bounds__(kBlock) void k_flux_herm(const T* up, T* fp, int64_
   }
 }
 
-// One dimension of the hierarchical-surplus transform
-// (LagrInterpolation::eval_fp_to_coe_D_Lag, source/Interplation.cpp:1279-1400).
-struct SurplusArgs {
-  const double* in;
-  double* out;
-  const float* in32;  // FP32 storage (precision mode): in32 / out32 and the split weights
-  float* out32;
-  const float* wh;
-  const float* wl;
-  const int32_t* slot;
-  const int32_t* fbase;
-  const int32_t* fmap;
-  const int32_t* anc_slot;
-  const int32_t* anc_point;
-  const double* wt;
-  int32_t* err;
-  int64_t E, comp_stride;
-  int P, p1, stride_d, ncomp;
-};
-
-template <class T>
-__device__ __forceinline__ const T* surplus_in(const SurplusArgs& a);
-template <>
-__device__ __forceinline__ const double* surplus_in<double>(const SurplusArgs& a) { return a.in; }
-template <>
-__device__ __forceinline__ const float* surplus_in<float>(const SurplusArgs& a) { return a.in32; }
-
-// FP64: coe += wt * anc.  FP32: the weight is split, wt = wh + wl, and both
-// parts are applied with fused multiply-adds, the small part first.
-__device__ __forceinline__ void surplus_madd(double& coe, const SurplusArgs& a, int w, double x) { coe += a.wt[w] * x; }
-__device__ __forceinline__ void surplus_madd(float& coe, const SurplusArgs& a, int w, float x) {
-  coe = fmaf(a.wl[w], x, coe);
-  coe = fmaf(a.wh[w], x, coe);
-}
-
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_surplus(SurplusArgs a) {
-  const int64_t gid = static_cast<int64_t>(xcd_block()) * blockDim.x + threadIdx.x;
-  const int64_t per = a.E * a.P;
-  if (gid >= per * a.ncomp) return;
-  const int64_t c = gid / per;
-  const int64_t rem = gid - c * per;
-  const int64_t e = rem / a.P;
-  const int q = static_cast<int>(rem - e * a.P);
-  const T* in = surplus_in<T>(a) + c * a.comp_stride;
-  T coe = in[e * a.P + q];
-  const int s = a.slot[e];
-  if (s != 0) {
-    const int pd = (q / a.stride_d) % a.p1;
-    for (int ic = 0; ic < a.p1; ++ic) {
-      const int anc = a.fmap[a.fbase[e] + a.anc_slot[s * a.p1 + ic]];
-      if (anc < 0) { atomicOr(a.err, ERR_ANCESTOR); continue; }
-      const int mm = q + (a.anc_point[s * a.p1 + ic] - pd) * a.stride_d;
-      surplus_madd(coe, a, (s * a.p1 + pd) * a.p1 + ic, in[static_cast<int64_t>(anc) * a.P + mm]);
-    }
-  }
-  if constexpr (sizeof(T) == 8) a.out[c * a.comp_stride + e * a.P + q] = coe;
-  else a.out32[c * a.comp_stride + e * a.P + q] = coe;
+  surplus_point<T>(a, static_cast<int64_t>(xcd_block()) * blockDim.x + threadIdx.x);
 }
 
 // One RK stage update (rk_combine); a non-finite result raises ERR_NONFINITE.
@@ -689,6 +636,8 @@ struct Context::FusedPlan {
 
 Context::~Context() {
   if (stream) (void)hipStreamSynchronize(stream);
+  for (StepGraph& g : step_graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
   for (ProfRec& r : prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (hipEvent_t e : prof_pool) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
@@ -973,6 +922,7 @@ void Context::compute_vcoef() {
   // tabulated on the host for the 2^NMAX (m+1) points of a 1D grid (with the
   // oracle's libm) and gathered on the device; c_d is the instance's scale
   // (ADG_FLUX_VARCOEF_SCALED), absent for ADG_FLUX_VARCOEF.
+  not_recording("point coefficients");
   const int p1 = m + 1;
   const int npt = (1 << nmax) * p1;
   std::vector<double> a1(npt);
@@ -1090,6 +1040,7 @@ __global__ __launch_bounds__(kBlock) void k_split(const double* M, float* hi, fl
 const Context::SplitTable& Context::split_table(const double* dev, size_t n) {
   auto it = split_tabs.find(dev);
   if (it != split_tabs.end()) return *it->second;
+  not_recording("split table");
   std::unique_ptr<SplitTable> t(new SplitTable());
   t->hi.ensure(n);
   t->lo.ensure(n);
@@ -1206,8 +1157,10 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
         pl.perm0 == g_perm[0].p && pl.pre0 == g_pre[0].p)
       return pl;
   } else {
-    slot = !plans[0] ? 0 : (!plans[1] ? 1 : 0);
+    // keep the plan of the buffer the next stage starts from (u_next)
+    slot = (plans[0] && plans[0]->u == u_next.p) ? 1 : 0;
   }
+  not_recording("stage plan");
   HostTimer timer(*this, HOST_PLAN);
   // temporaries: (D-1) steps of 2^(D-1) eval_up chains and 2*D*2^(D-1) rhs chains
   std::vector<std::vector<int>> lus;
@@ -1402,6 +1355,7 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
 // Fiber order of every dimension (perm, prefix popcounts) for the current grid.
 void Context::ensure_fiber_order() {
   if (order_version == grid_version) return;
+  not_recording("fiber order");
   HostTimer timer(*this, HOST_UPLOAD);
   for (int d = 0; d < D; ++d) {
     const int64_t nw = fbits_words[d];
@@ -1420,6 +1374,7 @@ Context::TileSetH& Context::tile_set(int which) {
   ensure_fiber_order();
   TileSetH& ts = tsets[which];
   if (ts.version == grid_version) return ts;
+  not_recording("tile set");
   HostTimer timer(*this, HOST_UPLOAD);
   ts.te = 64 << which;
   ts.ntile = std::max<int64_t>((E() + ts.te - 1) / ts.te, 1);
@@ -1465,10 +1420,12 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
   if (interp == 1) check_herm_flux(p.flux_kind, D);
   if (precision == 1) {
     const size_t e = static_cast<size_t>(std::max<int64_t>(E(), 1));
+    if (up32.cap < e * P || fp32.cap < e * P * D) not_recording("FP32 buffers");
     up32.ensure(e * P);
     for (DevBuf<float>* b : {&fp32, &fucoe32, &interA32, &interB32}) b->ensure(e * P * D);
   }
   FusedPlan& pl = fused_plan(p);
+  ++stage_count;
   if (precision == 1) stage_body<float>(p, pl, scheme, stage, dt);
   else stage_body<double>(p, pl, scheme, stage, dt);
 }
@@ -1477,8 +1434,89 @@ void Context::stage_fused(const StageProblem& p, int scheme, int stage, double d
 // tables).  Every kernel class runs one of the mappings: 0..3 per-element
 // gathers (fused_elem.hip), 4..6 fiber tiles of 64, 128 or 256 elements
 // (fused_tile.hip); all give identical values.
+// The same stage as one persistent cooperative launch (fused_coop.hip).
+template <class T>
+void Context::stage_coop(const StageProblem& p, FusedPlan& pl, int scheme, int stage, double dt) {
+  constexpr bool f32 = sizeof(T) == 4;
+  if (interp != 0) throw std::runtime_error("persistent stage: the Lagrange family is needed");
+  if (order_version != grid_version) throw std::logic_error("persistent stage: fiber order of an older grid");
+  TileSetH& ts = tile_set(0);
+  auto fits = [&](int pf1, int R, int tb) {
+    if (tile_rest_chunk(pf1, R, ts.max_tile, tb) < 1) throw std::runtime_error("persistent stage: tile does not fit");
+  };
+  for (int R : pl.up_phase_R) fits(k + 1, R, sizeof(T));
+  for (int R : pl.rhs_phase_R) fits(m + 1, R, sizeof(T));
+  fits(k + 1, pl.up_final_R, sizeof(T));
+  fits(m + 1, pl.rhs_final_R, sizeof(T));
+  fits(k + 1, pl.pen_R, sizeof(double));
+  CoopStage S{};
+  const int ncomp = same_flux ? 1 : D;
+  auto add = [&](int kind, int dim, const TaskDev* t1, int64_t n1, const TaskDev* t2, int64_t n2) {
+    if (S.nphase >= kCoopMaxPhases) throw std::logic_error("persistent stage: too many phases");
+    S.ph[S.nphase++] = CoopPhase{kind, dim, t1, t2, n1, n2};
+  };
+  int64_t off = 0;
+  for (size_t s = 0; s < pl.up_phase_ntask.size(); ++s) {
+    add(COOP_TILE_UP, 0, pl.up_tasks.p + off, pl.up_phase_ntask[s] * ts.ntile, nullptr, 0);
+    off += pl.up_phase_ntask[s];
+  }
+  add(COOP_TILE_UP, 0, pl.up_final_tasks.p, pl.n_up_terms * ts.ntile, nullptr, 0);
+  add(COOP_UP_COMBINE, 0, nullptr, E() * P, nullptr, 0);
+  // surplus transform, dimension by dimension (Context::surplus_transform)
+  const int p1 = m + 1;
+  const SplitTable* w = f32 ? &split_table(t_wt.p, static_cast<size_t>(1 << nmax) * p1 * p1) : nullptr;
+  int stride = P;
+  for (int d = 0; d < D; ++d) {
+    stride /= p1;
+    SurplusArgs& a = S.sur[d];
+    if constexpr (f32) {
+      a.in32 = d == 0 ? fp32.p : (d % 2 == 1 ? interA32.p : interB32.p);
+      a.out32 = d == D - 1 ? fucoe32.p : (d % 2 == 0 ? interA32.p : interB32.p);
+      a.wh = w->hi.p;
+      a.wl = w->lo.p;
+    } else {
+      a.in = d == 0 ? fp.p : (d % 2 == 1 ? interA.p : interB.p);
+      a.out = d == D - 1 ? fucoe.p : (d % 2 == 0 ? interA.p : interB.p);
+    }
+    a.slot = g_slot[d].p; a.fbase = g_fbase[d].p; a.fmap = g_fmap[d].p;
+    a.anc_slot = t_anc_slot.p; a.anc_point = t_anc_point.p; a.wt = t_wt.p; a.err = err_flag.p;
+    a.E = E(); a.comp_stride = E() * P; a.P = P; a.p1 = p1; a.stride_d = stride; a.ncomp = ncomp;
+    add(COOP_SURPLUS, d, nullptr, E() * P * ncomp, nullptr, 0);
+  }
+  off = 0;
+  for (size_t s = 0; s < pl.rhs_phase_ntask.size(); ++s) {
+    add(COOP_TILE_RHS, 0, pl.rhs_tasks.p + off, pl.rhs_phase_ntask[s] * ts.ntile, nullptr, 0);
+    off += pl.rhs_phase_ntask[s];
+  }
+  add(COOP_TILE_RHS, 0, pl.rhs_final_tasks.p, pl.n_rhs_terms * ts.ntile, pl.pen_tasks.p, pl.n_pen * ts.ntile);
+  add(COOP_RHS_COMBINE, 0, nullptr, E() * K, nullptr, 0);
+  for (int d = 0; d < D; ++d) {
+    S.ts.tbeg[d] = ts.tbeg[d].p;
+    S.ts.tend[d] = ts.tend[d].p;
+  }
+  S.ntile = ts.ntile; S.max_tile = ts.max_tile; S.lds_bytes = kTileLds; S.D = D; S.cap = lsum_cap;
+  S.tv = tile_tv.p;
+  if constexpr (f32) { S.up = up32.p; S.fp = fp32.p; }
+  else { S.up = up.p; S.fp = fp.p; }
+  S.vcoef = vcoef.p; S.up_kind = p.flux_kind; S.ncomp = ncomp; S.n_up_terms = pl.n_up_terms; S.fs = p.flux_param;
+  S.nP = E() * P;
+  S.pv = tile_pv.p; S.n_rhs_terms = pl.n_rhs_terms; S.npen = pl.n_pen; S.K = K; S.nK = E() * K;
+  S.rhs = rhs.p; S.u = u.p; S.tn = tn.p; S.unext = u_next.p; S.scheme = scheme; S.stage = stage; S.dt = dt;
+  S.dtv = use_inst_dt ? d_inst_dt.p : nullptr; S.inst = g_inst.p; S.err = err_flag.p;
+  prof_begin(PROF_RK, pl.up_final_bytes + pl.rhs_final_bytes);
+  coop_stage<T>(k + 1, m + 1, S, device, stream);
+  prof_end();
+  ++stage_launches;
+  std::swap(u.p, u_next.p);
+  std::swap(u.cap, u_next.cap);
+}
+
 template <class T>
 void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int stage, double dt) {
+  if (stage_launch == 1) {
+    stage_coop<T>(p, pl, scheme, stage, dt);
+    return;
+  }
   // Hermite: the final eval_up kernel writes point values only (kind -1); the
   // point flux needs the element's other degrees of freedom and runs after it
   const int up_kind = interp == 1 ? -1 : p.flux_kind;
@@ -1501,6 +1539,7 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     tuned_launch(TUNE_UP_PHASE + tc, [&](int v) {
       if (v >= 4) tile_launch<T>(v, k + 1, m + 1, tasks, pl.up_phase_R[s], nt);
       else elem_phase<T>(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
+      ++stage_launches;
     });
     prof_end();
     off += pl.up_phase_ntask[s];
@@ -1511,18 +1550,23 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
       if (v >= 4) {
         tile_launch<T>(v, k + 1, m + 1, pl.up_final_tasks.p, pl.up_final_R, pl.n_up_terms);
         up_combine<T>(tvT, pl.n_up_terms, E() * P, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, stream);
+        stage_launches += 2;
       } else {
         elem_up_final<T>(v, k + 1, m + 1, pl.up_terms.p, pl.n_up_terms, E(), P, D, upp, fpp, vcoef.p, up_kind,
                          p.flux_param, ncomp, lsum_cap, stream);
+        ++stage_launches;
       }
     });
-    if (interp == 1)
+    if (interp == 1) {
       hipLaunchKernelGGL(k_flux_herm<T>, dim3(nblocks(E() * P)), dim3(kBlock), 0, stream, upp, fpp, E(), D,
                          p.flux_kind, T(p.flux_param), ncomp);
+      ++stage_launches;
+    }
     prof_end();
   }
   if constexpr (f32) surplus_transform32(fp32.p, fucoe32.p, interA32.p, interB32.p, ncomp);
   else surplus_transform(fp.p, fucoe.p, interA.p, interB.p, ncomp);
+  stage_launches += D;
   off = 0;
   for (size_t s = 0; s < pl.rhs_phase_ntask.size(); ++s) {
     prof_begin(PROF_SWEEP, pl.rhs_phase_bytes[s]);
@@ -1532,6 +1576,7 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     tuned_launch(TUNE_RHS_PHASE + tc, [&](int v) {
       if (v >= 4) tile_launch<T>(v, m + 1, k + 1, tasks, pl.rhs_phase_R[s], nt);
       else elem_phase<T>(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
+      ++stage_launches;
     });
     prof_end();
     off += pl.rhs_phase_ntask[s];
@@ -1544,15 +1589,154 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
         tile_launch<double>(v, k + 1, k + 1, pl.pen_tasks.p, pl.pen_R, pl.n_pen);
         rhs_combine<T>(tvT, tile_pv.p, pl.n_rhs_terms, pl.n_pen, E(), K, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt,
                        dtv, g_inst.p, err_flag.p, stream);
+        stage_launches += 3;
       } else {
         elem_rhs_final<T>(v, m + 1, k + 1, pl.rhs_terms.p, pl.n_rhs_terms, pl.pens.p, pl.n_pen, E(), K, D, rhs.p, u.p,
                           tn.p, u_next.p, scheme, stage, dt, dtv, g_inst.p, lsum_cap, err_flag.p, stream);
+        ++stage_launches;
       }
     });
     prof_end();
   }
   std::swap(u.p, u_next.p);
   std::swap(u.cap, u_next.cap);
+}
+
+void Context::not_recording(const char* what) const {
+  if (recording) throw NeedsBuild(what);
+}
+
+void Context::rk_step_eager(const StageProblem& p, int scheme, int nstage, double dt) {
+  rk_init();
+  for (int s = 0; s < nstage; ++s) stage_fused(p, scheme, s, dt);
+}
+
+// Everything a recorded step depends on besides the contents of device
+// memory: versions of the grid and of the tables built from it, the device
+// buffers (their addresses change when they grow), the problem, the launch
+// settings and, without per-instance steps, dt.
+std::vector<uintptr_t> Context::step_key(const StageProblem& p, int scheme, int nstage, double dt) const {
+  auto bits = [](double v) {
+    uint64_t b;
+    std::memcpy(&b, &v, sizeof b);
+    return static_cast<uintptr_t>(b);
+  };
+  auto ptr = [](const void* q) { return reinterpret_cast<uintptr_t>(q); };
+  std::vector<uintptr_t> key = {
+      static_cast<uintptr_t>(grid_version), static_cast<uintptr_t>(vcoef_version),
+      static_cast<uintptr_t>(order_version), ptr(u.p), ptr(u_next.p), ptr(tn.p), ptr(rhs.p), ptr(up.p), ptr(fp.p),
+      ptr(fucoe.p), ptr(interA.p), ptr(interB.p), ptr(fused_tmp.p), ptr(tile_tv.p), ptr(tile_pv.p), ptr(vcoef.p),
+      ptr(err_flag.p), ptr(g_inst.p), ptr(d_pen_coef.p), ptr(up32.p), ptr(fp32.p), ptr(fucoe32.p), ptr(interA32.p),
+      ptr(interB32.p), ptr(plans[0].get()), ptr(plans[1].get()), static_cast<uintptr_t>(precision),
+      static_cast<uintptr_t>(interp), static_cast<uintptr_t>(stage_launch), static_cast<uintptr_t>(kernel_variant + 1),
+      static_cast<uintptr_t>(lsum_cap), static_cast<uintptr_t>(E()), tuned.size(), static_cast<uintptr_t>(scheme),
+      static_cast<uintptr_t>(nstage), static_cast<uintptr_t>(p.flux_kind), bits(p.flux_param),
+      static_cast<uintptr_t>(use_inst_dt), use_inst_dt ? ptr(d_inst_dt.p) : bits(dt), inst_alpha.size()};
+  for (int d = 0; d < D; ++d) key.push_back(bits(p.alpha[d]));
+  return key;
+}
+
+void Context::drop_step_graphs() {
+  bool any = false;
+  for (StepGraph& g : step_graphs) any = any || g.exec;
+  if (any) sync();  // no recorded step is still running when it is destroyed
+  for (StepGraph& g : step_graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    g.key.clear();
+  }
+}
+
+void Context::rk_step(const StageProblem& p, int scheme, int nstage, double dt) {
+  if (!graph_step || prof_on || E() == 0 || nstage < 1) {
+    rk_step_eager(p, scheme, nstage, dt);
+    return;
+  }
+  const std::vector<uintptr_t> key = step_key(p, scheme, nstage, dt);
+  // the key without the u / u_next pair, which every odd step swaps
+  auto norm = [](std::vector<uintptr_t> k) {
+    k[3] = k[4] = 0;
+    return k;
+  };
+  auto swaps = [&]() {
+    for (int s = 0; s < nstage; ++s) {
+      std::swap(u.p, u_next.p);
+      std::swap(u.cap, u_next.cap);
+    }
+  };
+  for (StepGraph& g : step_graphs) {
+    if (g.exec && g.key == key) {
+      hip_check(hipGraphLaunch(g.exec, stream), "hipGraphLaunch");
+      ++graph_replays;
+      stage_count += nstage;
+      stage_launches += g.launches;
+      swaps();
+      return;
+    }
+  }
+  // A step is recorded only when the same step already ran without a graph
+  // on the same state: that run builds the plans, fiber orders and point-flux
+  // tables and tunes the kernels, which upload host data and time launches
+  // (nothing a graph can hold).
+  const std::vector<uintptr_t> nk = norm(key);
+  if (std::find(step_seen.begin(), step_seen.end(), nk) == step_seen.end()) {
+    rk_step_eager(p, scheme, nstage, dt);
+    if (step_seen.size() >= kStepGraphs) step_seen.erase(step_seen.begin());
+    step_seen.push_back(norm(step_key(p, scheme, nstage, dt)));
+    return;
+  }
+  StepGraph& g = step_graphs[next_graph];
+  next_graph = (next_graph + 1) % kStepGraphs;
+  if (g.exec) {
+    sync();
+    (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+  }
+  g.key.clear();
+  const int64_t launches0 = stage_launches, stages0 = stage_count;
+  double* const u0 = u.p;
+  double* const un0 = u_next.p;
+  const size_t ucap0 = u.cap, uncap0 = u_next.cap;
+  hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  hipGraph_t graph = nullptr;
+  recording = true;
+  try {
+    rk_step_eager(p, scheme, nstage, dt);
+  } catch (const std::exception& ex) {
+    // drop the recording and undo the stages' host-side swaps and counters
+    // (nothing of the step ran); when something had to be built first
+    // (NeedsBuild: it threw before changing anything), run the step
+    // without a graph
+    recording = false;
+    (void)hipStreamEndCapture(stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    u.p = u0; u_next.p = un0; u.cap = ucap0; u_next.cap = uncap0;
+    stage_launches = launches0;
+    stage_count = stages0;
+    if (!dynamic_cast<const NeedsBuild*>(&ex)) throw;
+    rk_step_eager(p, scheme, nstage, dt);
+    return;
+  }
+  recording = false;
+  hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+  size_t nodes = 0;
+  hipError_t e = hipGraphGetNodes(graph, nullptr, &nodes);
+  // a recorded persistent step is one copy and one launch per stage: a
+  // stage the runtime did not record must not be replayed
+  if (e == hipSuccess && stage_launch == 1 && nodes != static_cast<size_t>(1 + stage_launches - launches0))
+    e = hipErrorStreamCaptureUnsupported;
+  if (e == hipSuccess) e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    g.exec = nullptr;
+    throw std::runtime_error(std::string("step graph: recording failed: ") + hipGetErrorString(e));
+  }
+  g.launches = stage_launches - launches0;
+  g.key = key;
+  ++graph_captures;
+  // recording ran nothing on the device: run the step now (its host-side
+  // buffer swaps and counters were done while recording)
+  hip_check(hipGraphLaunch(g.exec, stream), "hipGraphLaunch");
 }
 
 void Context::save_state(int slot) {
@@ -1584,6 +1768,7 @@ void Context::check_error_flag() {
     hip_check(hipMemcpyAsync(err_flag.p, &z, sizeof(int32_t), hipMemcpyHostToDevice, stream), "upload");
     sync();
     if (h & ERR_ANCESTOR) throw std::runtime_error("surplus transform: missing ancestor element (grid has a hole)");
+    if (h & ERR_COOP) throw std::runtime_error("persistent stage: kernel was not launched cooperatively");
     throw std::runtime_error("RK stage: non-finite value in the solution");
   }
 }
@@ -2025,6 +2210,7 @@ void Context::tuned_launch(int cls, const std::function<void(int)>& launch) {
     launch(it->second);
     return;
   }
+  not_recording("kernel tuning");
   hipEvent_t a, b;
   hip_check(hipEventCreate(&a), "hipEventCreate");
   hip_check(hipEventCreate(&b), "hipEventCreate");

@@ -117,21 +117,34 @@ __device__ __forceinline__ double rk_combine(int scheme, int stage, double dt, d
 }
 
 // Point flux of the fused eval_up kernels (kind -1: none, the Hermite flux runs after).
+// The values of kinds 0 and 1 are formed first and stored at fixed addresses
+// (component 0 at fp[gid], component 1 at fp[n + gid]): when the branches
+// stored themselves, the compiler merged the kind-0 store with the second
+// kind-1 store into one store whose address came out as fp + n for both in
+// the persistent stage kernel.
 template <class T>
 __device__ __forceinline__ void flux_point(int kind, T u, double s, const double* vcoef, T* fp, int64_t gid, int64_t n,
                                            int ncomp) {
-  if (kind == 0) {
-    fp[gid] = u * u * T(0.5);
-  } else if (kind == 1) {
-    fp[gid] = sin(u) * T(s);
-    fp[n + gid] = cos(u) * T(s);
-  } else if (kind >= 2) {  // variable coefficient (scaled or not)
+  if (kind >= 2) {  // variable coefficient (scaled or not)
     for (int c = 0; c < ncomp; ++c) fp[c * n + gid] = T(vcoef[c * n + gid]) * u;
+    return;
   }
+  if (kind < 0) return;
+  T f0, f1 = T(0);
+  if (kind == 0) {
+    f0 = u * u * T(0.5);
+  } else {
+    f0 = sin(u) * T(s);
+    f1 = cos(u) * T(s);
+  }
+  fp[gid] = f0;
+  if (kind == 1) fp[n + gid] = f1;
 }
 
 // Error flags of the device (bit values of the context's flag word).
-enum : int32_t { ERR_ANCESTOR = 1, ERR_NONFINITE = 2 };
+// ERR_COOP: a persistent stage kernel found itself launched without the
+// cooperative guarantee (it exits at once, leaving the stage undone).
+enum : int32_t { ERR_ANCESTOR = 1, ERR_NONFINITE = 2, ERR_COOP = 4 };
 
 // ---- per-element gather kernels (fused_elem.hip): thread mappings 0..3 ----
 // v: 0 thread per output, 1 thread per (element, rest index), 2 thread per

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libadg_hip.so")
 _LIB = None
 
-FIELD_UCOE, FIELD_RHS, FIELD_UP, FIELD_TN, FIELD_UINTP, FIELD_FUCOE = 0, 1, 2, 3, 4, 10
+FIELD_UCOE, FIELD_RHS, FIELD_UP, FIELD_TN, FIELD_UINTP, FIELD_FUCOE, FIELD_FP = 0, 1, 2, 3, 4, 10, 20
 TAB_ALPT_AT_LAGR, TAB_LA_UV, TAB_LA_UVX, TAB_LA_FLX = 0, 1, 2, 3
 TAB_AA_UV, TAB_AA_UVX, TAB_AA_ULFT, TAB_AA_URGT, TAB_AA_UJP = 4, 5, 6, 7, 8
 LU_FULL, LU_L, LU_U = 0, 1, 2
@@ -141,6 +141,11 @@ def lib():
             "adg_set_instance_thresholds": (i32, [vp, DP, DP]),
             "adg_set_instance_params": (i32, [vp, DP, DP, DP]),
             "adg_init_from_points": (i32, [vp, DP]),
+            "adg_set_stage_launch": (i32, [vp, i32]),
+            "adg_stage_launch_count": (i32, [vp, P(i64), P(i64)]),
+            "adg_set_graph_step": (i32, [vp, i32]),
+            "adg_graph_step_count": (i32, [vp, P(i64), P(i64)]),
+            "adg_rk_step": (i32, [vp, P(Problem), i32, dbl]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -595,8 +600,32 @@ class Solution:
         """Thread mapping of the fused kernels (-1: autotuned)."""
         self._check(lib().adg_set_kernel_variant(self._h, variant))
 
+    def set_stage_launch(self, mode):
+        """0: one launch per phase of a fused stage; 1: one persistent cooperative launch per stage."""
+        self._check(lib().adg_set_stage_launch(self._h, mode))
+
+    def stage_launch_count(self):
+        """(kernel launches, stages) of the fused form so far."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(lib().adg_stage_launch_count(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def stage(self, problem, scheme, stage, dt):
         self._check(lib().adg_stage(self._h, ctypes.byref(problem), scheme, stage, dt))
+
+    def set_graph_step(self, on):
+        """Record whole fused RK steps into HIP graphs and replay them (1) or not (0)."""
+        self._check(lib().adg_set_graph_step(self._h, int(on)))
+
+    def graph_step_count(self):
+        """(steps recorded into graphs, steps replayed from graphs) so far."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(lib().adg_graph_step_count(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def rk_step(self, problem, scheme, dt):
+        """One whole fused RK step (u^n <- u, then every stage of the scheme)."""
+        self._check(lib().adg_rk_step(self._h, ctypes.byref(problem), scheme, dt))
 
     def save_state(self, slot=0):
         self._check(lib().adg_state_save(self._h, slot))

@@ -46,7 +46,8 @@ enum {
   ADG_FIELD_UP = 2,     /* Element::up_intp, values at Lagrange points, (m+1)^dim */
   ADG_FIELD_TN = 3,     /* RK stage base u^n (ExplicitRK::ucoe_tn) */
   ADG_FIELD_UINTP = 4,  /* Element::ucoe_intp (Lagrange surplus of the initial function) */
-  ADG_FIELD_FUCOE = 10  /* +d: Element::fucoe_intp[0][d], Lagrange surplus of flux d */
+  ADG_FIELD_FUCOE = 10, /* +d: Element::fucoe_intp[0][d], Lagrange surplus of flux d */
+  ADG_FIELD_FP = 20     /* +d: Element::fp_intp[0][d], flux d at the interpolation points (FP64 form) */
 };
 
 /* 1D operator tables (row = "from" basis, column = "to" basis). */
@@ -442,6 +443,38 @@ int adg_decision_log_write(adg_ctx* c, const char* path);
  * refine / coarsen norms stay FP64.  The per-sweep entry points are FP64. */
 enum { ADG_PREC_FP64 = 0, ADG_PREC_FP32_SPLIT = 1 };
 int adg_set_precision(adg_ctx* c, int precision);
+
+/* Launch form of adg_stage and of adg_run_hyperbolic's fused form (no
+ * reference counterpart).  0 (default): one launch per phase of the stage
+ * (3*dim + up to 3 launches), each with the thread mapping the autotuner or
+ * adg_set_kernel_variant picks.  1: one persistent cooperative launch per
+ * stage: every workgroup loops over the work items of a phase -- (term, tile
+ * of whole fibers) for the sweeps, points for the point flux, the surplus
+ * transform and the RK update -- and the grid waits at a barrier between
+ * phases; the tiles are those of kernel variant 7.  Lagrange family only (the
+ * Hermite family keeps mode 0).  Mode 1 is switched off and returns an error:
+ * in some runs its launch did not finish on the device (cause not found). */
+int adg_set_stage_launch(adg_ctx* c, int mode);
+/* Kernel launches and stages of the fused form since the context was created. */
+int adg_stage_launch_count(adg_ctx* c, int64_t* launches, int64_t* stages);
+
+/* Step graphs (no reference counterpart; the reference advances a step by
+ * calling each stage in turn, source/ODESolver.cpp:285-301).  With on = 1,
+ * adg_rk_step and the fused form of adg_run_hyperbolic record a whole step
+ * (u^n <- u and every stage) into a HIP graph the second time the same step
+ * runs on the same state -- grid, device buffers, problem, launch settings
+ * and dt (or the per-instance steps' buffer) -- and replay it from then on.
+ * Any change of that state, a grid change by refine or coarsen first of all,
+ * makes the next step run without a graph and the one after record a new
+ * graph.  Up to four recorded steps are kept.  Results are identical with and
+ * without graphs; with profiling on, steps run without graphs.  on = 0 drops
+ * the recorded steps. */
+int adg_set_graph_step(adg_ctx* c, int on);
+/* Steps recorded into graphs and steps replayed from them since creation. */
+int adg_graph_step_count(adg_ctx* c, int64_t* captures, int64_t* replays);
+/* One whole fused RK step of `scheme` (u^n <- u, then its stages; the same as
+ * adg_rk_init followed by adg_stage for every stage) with step size dt. */
+int adg_rk_step(adg_ctx* c, const adg_problem* p, int scheme, double dt);
 
 /* One fused RK stage of the same operator: point values, flux, surplus
  * transform, right-hand side (volume, flux and LF penalty terms) and the stage
