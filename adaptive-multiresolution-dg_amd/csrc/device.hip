@@ -1227,27 +1227,73 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     tk.R = t.Tin / t.isz[t.dd];
     return tk;
   };
-  // one chain of partial sums: returns its last term; appends first-step tasks per phase
-  auto chain = [&](int cid, const double* src, const float* src32, int pf, int pt, const SweepTable* tabs,
-                   const int* flx, int o, double coef, std::vector<std::vector<TaskDev>>& phase_tasks) {
-    int sz[kMaxDim];
-    for (int d = 0; d < D; ++d) sz[d] = pf + 1;
-    const double* cur = src;
-    const float* cur32 = src32;
-    for (int st = 0; st < D; ++st) {
-      const int dd = dims[o][st];
-      int sin_sz[kMaxDim];
-      std::copy(sz, sz + D, sin_sz);
-      sz[dd] = pt + 1;
-      TermDev t = make_term(cur, cur32, sin_sz, dd, tabs[dd], pt + 1, flx[dd] != 0, lus[o][st], st == 0 ? coef : 1.0,
-                            f32);
-      if (st == D - 1) return t;
-      TaskDev tk = make_task(t, sz, tmp(cid, st));
-      phase_tasks[st].push_back(tk);
-      cur = tk.out;
-      cur32 = tk.out32;
+  // Chains of partial sums (one per transform order and term), built as a
+  // prefix tree: chains whose first sweeps have the same input, dimension,
+  // table, relation and L/U mode share those sweeps' tasks and buffers.  A
+  // chain's coefficient (1 or 1/2) is applied at the first sweep that only
+  // chains with that coefficient pass through (the reference applies it in
+  // the first sweep).  Scaling by a power of two commutes exactly with every
+  // later product and sum (normal range), so every term value is
+  // bit-identical to the unshared chain's.  Returns the last term of each
+  // chain; appends the tasks of the shared sweeps per phase.
+  struct ChainSpec {
+    const double* src;
+    const float* src32;
+    SweepTable tabs[kMaxDim];
+    int flx[kMaxDim];
+    int o;
+    double coef;
+  };
+  auto build_chains = [&](const std::vector<ChainSpec>& specs, int pf, int pt, int slot0,
+                          std::vector<std::vector<TaskDev>>& phase_tasks) {
+    auto prefix_key = [&](const ChainSpec& c, int st) {
+      std::vector<int64_t> key = {reinterpret_cast<int64_t>(c.src)};
+      for (int j = 0; j <= st; ++j) {
+        const int dd = dims[c.o][j];
+        key.insert(key.end(), {dd, reinterpret_cast<int64_t>(c.tabs[dd].dev), c.flx[dd], lus[c.o][j]});
+      }
+      return key;
+    };
+    std::map<std::vector<int64_t>, std::vector<double>> coefs;  // coefficients of the chains through a node
+    for (const ChainSpec& c : specs)
+      for (int st = 0; st + 1 < D; ++st) {
+        std::vector<double>& v = coefs[prefix_key(c, st)];
+        if (std::find(v.begin(), v.end(), c.coef) == v.end()) v.push_back(c.coef);
+      }
+    std::map<std::vector<int64_t>, TaskDev> nodes;
+    std::vector<int> next_slot(std::max(D - 1, 1), slot0);
+    std::vector<TermDev> terms;
+    for (const ChainSpec& c : specs) {
+      int sz[kMaxDim];
+      for (int d = 0; d < D; ++d) sz[d] = pf + 1;
+      const double* cur = c.src;
+      const float* cur32 = c.src32;
+      bool applied = false;
+      for (int st = 0; st < D; ++st) {
+        const int dd = dims[c.o][st];
+        int sin_sz[kMaxDim];
+        std::copy(sz, sz + D, sin_sz);
+        sz[dd] = pt + 1;
+        if (st == D - 1) {
+          terms.push_back(make_term(cur, cur32, sin_sz, dd, c.tabs[dd], pt + 1, c.flx[dd] != 0, lus[c.o][st],
+                                    applied ? 1.0 : c.coef, f32));
+          break;
+        }
+        const std::vector<int64_t> key = prefix_key(c, st);
+        const bool here = !applied && coefs[key].size() == 1;
+        auto it = nodes.find(key);
+        if (it == nodes.end()) {
+          const TermDev t = make_term(cur, cur32, sin_sz, dd, c.tabs[dd], pt + 1, c.flx[dd] != 0, lus[c.o][st],
+                                      here ? c.coef : 1.0, f32);
+          it = nodes.emplace(key, make_task(t, sz, tmp(next_slot[st]++, st))).first;
+          phase_tasks[st].push_back(it->second);
+        }
+        applied = applied || here;
+        cur = it->second.out;
+        cur32 = it->second.out32;
+      }
     }
-    return TermDev{};
+    return terms;
   };
   auto finish_phases = [&](std::vector<std::vector<TaskDev>>& ph, DevBuf<TaskDev>& dev, std::vector<int>& ntask,
                            std::vector<std::array<int64_t, 4>>& total, std::vector<double>& bytes,
@@ -1288,10 +1334,16 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     for (int d = 0; d < D; ++d) { tabs[d] = table(0); flx[d] = 0; }
     int psz[kMaxDim];
     for (int d = 0; d < D; ++d) psz[d] = m + 1;
+    std::vector<ChainSpec> specs;
     for (int o = 0; o < nord; ++o) {
-      terms.push_back(chain(o, u.p, nullptr, k, m, tabs, flx, o, 1.0, ph));
-      finals.push_back(make_task(terms.back(), psz, tile_tv.p + static_cast<size_t>(o) * Ee * P));
+      ChainSpec c{u.p, nullptr, {}, {}, o, 1.0};
+      std::copy(tabs, tabs + D, c.tabs);
+      std::copy(flx, flx + D, c.flx);
+      specs.push_back(c);
     }
+    terms = build_chains(specs, k, m, 0, ph);
+    for (int o = 0; o < nord; ++o)
+      finals.push_back(make_task(terms[o], psz, tile_tv.p + static_cast<size_t>(o) * Ee * P));
     finish_phases(ph, pl.up_tasks, pl.up_phase_ntask, pl.up_phase_total, pl.up_phase_bytes, pl.up_phase_R);
     pl.up_final_R = finals[0].R;
     upload(pl.up_terms, terms, stream);
@@ -1309,24 +1361,26 @@ Context::FusedPlan& Context::fused_plan(const StageProblem& p) {
     std::vector<TaskDev> finals;
     int ksz[kMaxDim];
     for (int d = 0; d < D; ++d) ksz[d] = k + 1;
-    int cid = nord;
+    std::vector<ChainSpec> specs;
     for (int ft = 0; ft < 2; ++ft) {
       for (int d = 0; d < D; ++d) {
-        SweepTable tabs[kMaxDim];
-        int flx[kMaxDim];
+        ChainSpec c{};
         for (int dd = 0; dd < D; ++dd) {
-          if (dd == d) { tabs[dd] = ft ? table(3) : table(2); flx[dd] = ft; }
-          else { tabs[dd] = table(1); flx[dd] = 0; }
+          if (dd == d) { c.tabs[dd] = ft ? table(3) : table(2); c.flx[dd] = ft; }
+          else { c.tabs[dd] = table(1); c.flx[dd] = 0; }
         }
         const int64_t coff = static_cast<int64_t>(same_flux ? 0 : d) * E() * P;
-        const double* src = fucoe.p + coff;
-        const float* src32 = f32 ? fucoe32.p + coff : nullptr;
+        c.src = fucoe.p + coff;
+        c.src32 = f32 ? fucoe32.p + coff : nullptr;
+        c.coef = ft ? 0.5 : 1.0;
         for (int o = 0; o < nord; ++o) {
-          terms.push_back(chain(cid++, src, src32, m, k, tabs, flx, o, ft ? 0.5 : 1.0, ph));
-          finals.push_back(make_task(terms.back(), ksz, tile_tv.p + (terms.size() - 1) * Ee * K));
+          c.o = o;
+          specs.push_back(c);
         }
       }
     }
+    terms = build_chains(specs, m, k, nord, ph);
+    for (size_t i = 0; i < terms.size(); ++i) finals.push_back(make_task(terms[i], ksz, tile_tv.p + i * Ee * K));
     finish_phases(ph, pl.rhs_tasks, pl.rhs_phase_ntask, pl.rhs_phase_total, pl.rhs_phase_bytes, pl.rhs_phase_R);
     pl.rhs_final_R = finals[0].R;
     upload(pl.rhs_terms, terms, stream);

@@ -202,6 +202,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"),
                     help="storage precision of the chosen form's fused stages (fp32: split FP32 tables, FP64 "
                          "accumulation, RK and refine/coarsen norms)")
+    ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
+                    help="chosen form: record whole RK steps into HIP graphs and replay them (adg_set_graph_step)")
     ap.add_argument("--imex", action="store_true",
                     help="instead of the main measurement: IMEX43 with the assembled artificial viscosity, GPU "
                          "against the CPU oracle on one fixed grid")
@@ -243,6 +245,7 @@ def main():
     chosen_sols = bsols if batched else ([] if w.linear else [workloads.make_solution(w)])
     for c in chosen_sols:
         c.set_precision(prec)
+        c.set_graph_step(args.graph)
         if c is not sol:
             c.save_state(0)
     if batched:
@@ -311,11 +314,17 @@ def main():
             s.sync()
         for r in results:
             r.clear()
+        counts0 = [(s.stage_launch_count(), s.graph_step_count()) for s in sols]
         t0 = time.perf_counter()
         run_all(args.steps)
         for s in sols:
             s.sync()
         sec = time.perf_counter() - t0
+        counts1 = [(s.stage_launch_count(), s.graph_step_count()) for s in sols]
+        launches = sum(b[0][0] - a[0][0] for a, b in zip(counts0, counts1))
+        stages = sum(b[0][1] - a[0][1] for a, b in zip(counts0, counts1))
+        graphs = (sum(b[1][0] - a[1][0] for a, b in zip(counts0, counts1)),
+                  sum(b[1][1] - a[1][1] for a, b in zip(counts0, counts1)))
         dof_stages = sum(st.dof_stages for r in results for st in r)
         redone = sum(st.redone for r in results for st in r)
         time_steps = max(sum(st.steps for st in r) for r in results)
@@ -338,7 +347,8 @@ def main():
               flush=True)
         return {"value": dof_stages / sec, "sec": sec, "dof_stages": dof_stages, "time_steps": time_steps,
                 "max_elem": max_elem, "prof": prof, "host": host, "u": u, "keys": keys, "redone": redone,
-                "sim_time": results[0][-1].final_time}
+                "sim_time": results[0][-1].final_time, "launches_per_stage": launches / stages if stages else None,
+                "graph_steps_recorded": graphs[0], "graph_steps_replayed": graphs[1]}
 
     plain = measure(sol, 0)
     fused = plain if w.linear else measure(sol, 1)
@@ -402,10 +412,12 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == pyadg.PREC_FP64 or w.linear else "f32",
         "data": "synthetic",
         "config": dict(w.config(), step="one full solve from the saved initial state",
-                       form=("chosen: fused %s stages (3*dim launches per RK stage), %d seeded instances "
-                             "(seeds %d..%d) in %d context(s) solved concurrently"
-                             % (pname, args.batch, args.seed, args.seed + args.batch - 1, len(bsols)))
-                       if batched else "chosen: fused %s stages (3*dim launches per RK stage)" % pname,
+                       form=("chosen: fused %s stages, %d seeded instances (seeds %d..%d) in %d context(s) solved "
+                             "concurrently" % (pname, args.batch, args.seed, args.seed + args.batch - 1, len(bsols)))
+                       if batched else "chosen: fused %s stages" % pname,
+                       launches_per_stage=chosen["launches_per_stage"], graph_step=args.graph,
+                       graph_steps_recorded=chosen["graph_steps_recorded"],
+                       graph_steps_replayed=chosen["graph_steps_replayed"],
                        precision=args.precision if not w.linear else "fp64",
                        contexts=len(bsols) if batched else 1,
                        plain_form="plain FP64, one kernel per 1D sweep, one instance",
