@@ -979,6 +979,20 @@ int adg_stage_launch_count(adg_ctx* h, int64_t* launches, int64_t* stages) {
   });
 }
 
+int adg_kernel_tuning(adg_ctx* h, int32_t* pairs, int cap, int* count) {
+  return guard(h, [&](adg::Context& c) {
+    int n = 0;
+    for (const auto& kv : c.tuned) {
+      if (pairs && n < cap) {
+        pairs[2 * n] = kv.first;
+        pairs[2 * n + 1] = kv.second;
+      }
+      ++n;
+    }
+    if (count) *count = n;
+  });
+}
+
 int adg_set_graph_step(adg_ctx* h, int on) {
   return guard(h, [&](adg::Context& c) {
     if (on != 0 && on != 1) throw std::runtime_error("graph step must be 0 or 1");

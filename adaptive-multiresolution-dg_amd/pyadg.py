@@ -144,6 +144,7 @@ def lib():
             "adg_set_stage_launch": (i32, [vp, i32]),
             "adg_stage_launch_count": (i32, [vp, P(i64), P(i64)]),
             "adg_set_graph_step": (i32, [vp, i32]),
+            "adg_kernel_tuning": (i32, [vp, I32P, i32, P(i32)]),
             "adg_graph_step_count": (i32, [vp, P(i64), P(i64)]),
             "adg_rk_step": (i32, [vp, P(Problem), i32, dbl]),
         }
@@ -612,6 +613,15 @@ class Solution:
 
     def stage(self, problem, scheme, stage, dt):
         self._check(lib().adg_stage(self._h, ctypes.byref(problem), scheme, stage, dt))
+
+    def kernel_tuning(self):
+        """{(kernel class, grid-size class): variant} chosen by the autotuner so far."""
+        n = ctypes.c_int()
+        self._check(lib().adg_kernel_tuning(self._h, None, 0, ctypes.byref(n)))
+        buf = np.zeros(2 * max(n.value, 1), dtype=np.int32)
+        self._check(lib().adg_kernel_tuning(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n.value,
+                                            ctypes.byref(n)))
+        return {(int(buf[2 * i]) // 64, int(buf[2 * i]) % 64): int(buf[2 * i + 1]) for i in range(n.value)}
 
     def set_graph_step(self, on):
         """Record whole fused RK steps into HIP graphs and replay them (1) or not (0)."""

@@ -476,6 +476,13 @@ int adg_graph_step_count(adg_ctx* c, int64_t* captures, int64_t* replays);
  * adg_rk_init followed by adg_stage for every stage) with step size dt. */
 int adg_rk_step(adg_ctx* c, const adg_problem* p, int scheme, double dt);
 
+/* The autotuner's decisions so far (no reference counterpart): up to cap
+ * (key, variant) pairs, key = kernel class * 64 + grid-size class (log2 of the
+ * element count); classes 0..3 are the FP64 eval-up phases, eval-up final,
+ * right-hand-side phases and right-hand-side final, 4..7 the same in FP32.
+ * *count receives the number of decisions. */
+int adg_kernel_tuning(adg_ctx* c, int32_t* pairs, int cap, int* count);
+
 /* One fused RK stage of the same operator: point values, flux, surplus
  * transform, right-hand side (volume, flux and LF penalty terms) and the stage
  * update in 3*dim launches; bit-identical to adg_eval_up + adg_flux +
