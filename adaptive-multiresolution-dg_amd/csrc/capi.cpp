@@ -979,6 +979,15 @@ int adg_stage_launch_count(adg_ctx* h, int64_t* launches, int64_t* stages) {
   });
 }
 
+int adg_set_class_variants(adg_ctx* h, const int* v, int n) {
+  return guard(h, [&](adg::Context& c) {
+    if (n < 0 || n > 8 || (n > 0 && !v)) throw std::runtime_error("class variants: 0..8 values expected");
+    for (int i = 0; i < n; ++i)
+      if (v[i] < -1 || v[i] >= adg::Context::kVariants) throw std::runtime_error("kernel variant must be -1..9");
+    for (int i = 0; i < 8; ++i) c.class_variant[i] = i < n ? v[i] : -1;
+  });
+}
+
 int adg_kernel_tuning(adg_ctx* h, int32_t* pairs, int cap, int* count) {
   return guard(h, [&](adg::Context& c) {
     int n = 0;

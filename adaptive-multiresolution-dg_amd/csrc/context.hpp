@@ -170,7 +170,7 @@ class Context {
   TileSetH tsets[3];
   TileSetH& tile_set(int which);
   template <class T>
-  void tile_launch(int variant, int pf1, int pt1, const struct TaskDev* tasks, int R, int ntask);
+  void tile_launch(int variant, int pf1, int pt1, const struct TaskDev* tasks, int R, int ntask, bool last);
   DevBuf<double> tile_tv, tile_pv;  // per-term values of the tile form's last sweeps
   DevBuf<int32_t> g_remap;
   DevBuf<int32_t> g_lsum;  // level sum of each element
@@ -304,6 +304,8 @@ class Context {
   static constexpr int kVariants = 10;
   enum { TUNE_UP_PHASE = 0, TUNE_UP_FINAL, TUNE_RHS_PHASE, TUNE_RHS_FINAL };
   int kernel_variant = -1;
+  // per kernel class (0..3 FP64, 4..7 FP32): a fixed mapping, -1 the autotuner
+  int class_variant[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
   std::map<int, int> tuned;
   void tuned_launch(int cls, const std::function<void(int)>& launch);
   std::unique_ptr<FusedPlan> plans[2];
@@ -398,7 +400,7 @@ class Context {
 
   // Kernel timing with HIP events on this context's stream.  Categories:
   // 0 sweep, 1 surplus transform, 2 flux, 3 RK stage, 4 other.
-  enum { PROF_SWEEP = 0, PROF_SURPLUS, PROF_FLUX, PROF_RK, PROF_OTHER, PROF_NCAT };
+  enum { PROF_SWEEP = 0, PROF_SURPLUS, PROF_FLUX, PROF_RK, PROF_OTHER, PROF_RHS_PHASE, PROF_NCAT };
   bool prof_on = false;
   struct ProfRec { hipEvent_t a, b; int cat; double bytes; };
   std::vector<ProfRec> prof_pending;

@@ -1446,7 +1446,7 @@ Context::TileSetH& Context::tile_set(int which) {
 // One tile launch of ntask tasks with pf1 x pt1 blocks; throws (before any
 // launch) when a tile of this size does not fit in LDS.
 template <class T>
-void Context::tile_launch(int v, int pf1, int pt1, const TaskDev* tasks, int R, int ntask) {
+void Context::tile_launch(int v, int pf1, int pt1, const TaskDev* tasks, int R, int ntask, bool last) {
   if (order_version != grid_version) throw std::logic_error("tile launch: fiber order of an older grid");
   // variants 4..6: a thread per (target, rest index); 7..9: a thread per target
   const int which = (v - 4) % 3;
@@ -1456,7 +1456,7 @@ void Context::tile_launch(int v, int pf1, int pt1, const TaskDev* tasks, int R, 
   const int32_t* tb[kMaxDim];
   const int32_t* te[kMaxDim];
   for (int d = 0; d < D; ++d) { tb[d] = ts.tbeg[d].p; te[d] = ts.tend[d].p; }
-  tile_sweep<T>(pf1, pt1, tasks, ntask, ts.ntile, ts.max_tile, tb, te, D, lsum_cap, allr, stream);
+  tile_sweep<T>(pf1, pt1, tasks, ntask, ts.ntile, ts.max_tile, tb, te, D, lsum_cap, allr, last, stream);
 }
 
 void Context::stage_fused(const StageProblem& p, int scheme, int stage, double dt) {
@@ -1591,7 +1591,7 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     const int nt = pl.up_phase_ntask[s];
     const std::array<int64_t, 4> tot = pl.up_phase_total[s];
     tuned_launch(TUNE_UP_PHASE + tc, [&](int v) {
-      if (v >= 4) tile_launch<T>(v, k + 1, m + 1, tasks, pl.up_phase_R[s], nt);
+      if (v >= 4) tile_launch<T>(v, k + 1, m + 1, tasks, pl.up_phase_R[s], nt, false);
       else elem_phase<T>(k + 1, m + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
       ++stage_launches;
     });
@@ -1602,7 +1602,7 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     prof_begin(PROF_FLUX, pl.up_final_bytes);
     tuned_launch(TUNE_UP_FINAL + tc, [&](int v) {
       if (v >= 4) {
-        tile_launch<T>(v, k + 1, m + 1, pl.up_final_tasks.p, pl.up_final_R, pl.n_up_terms);
+        tile_launch<T>(v, k + 1, m + 1, pl.up_final_tasks.p, pl.up_final_R, pl.n_up_terms, true);
         up_combine<T>(tvT, pl.n_up_terms, E() * P, upp, fpp, vcoef.p, up_kind, p.flux_param, ncomp, stream);
         stage_launches += 2;
       } else {
@@ -1623,12 +1623,12 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
   stage_launches += D;
   off = 0;
   for (size_t s = 0; s < pl.rhs_phase_ntask.size(); ++s) {
-    prof_begin(PROF_SWEEP, pl.rhs_phase_bytes[s]);
+    prof_begin(PROF_RHS_PHASE, pl.rhs_phase_bytes[s]);
     const TaskDev* tasks = pl.rhs_tasks.p + off;
     const int nt = pl.rhs_phase_ntask[s];
     const std::array<int64_t, 4> tot = pl.rhs_phase_total[s];
     tuned_launch(TUNE_RHS_PHASE + tc, [&](int v) {
-      if (v >= 4) tile_launch<T>(v, m + 1, k + 1, tasks, pl.rhs_phase_R[s], nt);
+      if (v >= 4) tile_launch<T>(v, m + 1, k + 1, tasks, pl.rhs_phase_R[s], nt, false);
       else elem_phase<T>(m + 1, k + 1, tasks, nt, tot[v], D, lsum_cap, v, stream);
       ++stage_launches;
     });
@@ -1639,8 +1639,8 @@ void Context::stage_body(const StageProblem& p, FusedPlan& pl, int scheme, int s
     prof_begin(PROF_RK, pl.rhs_final_bytes);
     tuned_launch(TUNE_RHS_FINAL + tc, [&](int v) {
       if (v >= 4) {
-        tile_launch<T>(v, m + 1, k + 1, pl.rhs_final_tasks.p, pl.rhs_final_R, pl.n_rhs_terms);
-        tile_launch<double>(v, k + 1, k + 1, pl.pen_tasks.p, pl.pen_R, pl.n_pen);
+        tile_launch<T>(v, m + 1, k + 1, pl.rhs_final_tasks.p, pl.rhs_final_R, pl.n_rhs_terms, true);
+        tile_launch<double>(v, k + 1, k + 1, pl.pen_tasks.p, pl.pen_R, pl.n_pen, true);
         rhs_combine<T>(tvT, tile_pv.p, pl.n_rhs_terms, pl.n_pen, E(), K, rhs.p, u.p, tn.p, u_next.p, scheme, stage, dt,
                        dtv, g_inst.p, err_flag.p, stream);
         stage_launches += 3;
@@ -2259,8 +2259,9 @@ void Context::tuned_launch(int cls, const std::function<void(int)>& launch) {
       return false;
     }
   };
-  if (kernel_variant >= 0) {
-    if (!try_launch(kernel_variant)) launch(0);
+  const int fixed = kernel_variant >= 0 ? kernel_variant : class_variant[cls];
+  if (fixed >= 0) {
+    if (!try_launch(fixed)) launch(0);
     return;
   }
   int bucket = 0;

@@ -18,7 +18,10 @@ constexpr int kBlock = 256;
 
 inline unsigned nblocks(int64_t n) { return static_cast<unsigned>((n + kBlock - 1) / kBlock); }
 
-template <int PF1, int PT1, class T, int RG>
+// LAST: the launch of the last sweeps (term values) rather than a phase of
+// first sweeps; the code is the same, the separate instance tells the two
+// apart in kernel traces.
+template <int PF1, int PT1, class T, int RG, bool LAST>
 __global__ __launch_bounds__(kTileThreads) void k_tile(const TaskDev* tasks, int64_t ntile, TileSet ts, int D,
                                                        int cap, int max_tile, int lds_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -71,17 +74,17 @@ __global__ __launch_bounds__(kBlock) void k_rhs_combine(const T* tv, const doubl
     rhs_combine_point<T>(tv, pv, nterm, npen, E * K, K, rhs, u, tn, u_next, scheme, stage, dt, dtv, inst, err, i);
 }
 
-template <int PF1, int PT1, class T>
+template <int PF1, int PT1, class T, bool LAST>
 void tile_sweep_t(const TaskDev* tasks, int ntask, int64_t ntile, const TileSet& ts, int max_tile, int D, int cap,
                   int lds, bool allr, hipStream_t s) {
   const int64_t nb = ntile * ntask;
   if (nb > INT_MAX) throw std::runtime_error("tile sweep: grid too large");
   if (allr)
-    hipLaunchKernelGGL((k_tile<PF1, PT1, T, 4>), dim3(static_cast<unsigned>(nb)), dim3(kTileThreads), lds, s, tasks,
-                       ntile, ts, D, cap, max_tile, lds);
+    hipLaunchKernelGGL((k_tile<PF1, PT1, T, 4, LAST>), dim3(static_cast<unsigned>(nb)), dim3(kTileThreads), lds, s,
+                       tasks, ntile, ts, D, cap, max_tile, lds);
   else
-    hipLaunchKernelGGL((k_tile<PF1, PT1, T, 1>), dim3(static_cast<unsigned>(nb)), dim3(kTileThreads), lds, s, tasks,
-                       ntile, ts, D, cap, max_tile, lds);
+    hipLaunchKernelGGL((k_tile<PF1, PT1, T, 1, LAST>), dim3(static_cast<unsigned>(nb)), dim3(kTileThreads), lds, s,
+                       tasks, ntile, ts, D, cap, max_tile, lds);
 }
 
 }  // namespace
@@ -121,17 +124,18 @@ int tile_rest_chunk(int pf1, int R, int max_tile, int tbytes) {
 
 template <class T>
 void tile_sweep(int pf1, int pt1, const TaskDev* tasks, int ntask, int64_t ntile, int max_tile, const int32_t* const* tbeg,
-                const int32_t* const* tend, int D, int cap, bool allr, hipStream_t s) {
+                const int32_t* const* tend, int D, int cap, bool allr, bool last, hipStream_t s) {
   TileSet ts{};
   for (int d = 0; d < D; ++d) {
     ts.tbeg[d] = tbeg[d];
     ts.tend[d] = tend[d];
   }
   const int lds = kTileLds;
-#define ADG_TS(F, P)                                                         \
-  if (pf1 == F && pt1 == P) {                                                \
-    tile_sweep_t<F, P, T>(tasks, ntask, ntile, ts, max_tile, D, cap, lds, allr, s); \
-    return;                                                                  \
+#define ADG_TS(F, P)                                                                \
+  if (pf1 == F && pt1 == P) {                                                       \
+    if (last) tile_sweep_t<F, P, T, true>(tasks, ntask, ntile, ts, max_tile, D, cap, lds, allr, s);  \
+    else tile_sweep_t<F, P, T, false>(tasks, ntask, ntile, ts, max_tile, D, cap, lds, allr, s);      \
+    return;                                                                         \
   }
   ADG_TS(2, 2) ADG_TS(2, 3) ADG_TS(2, 4) ADG_TS(2, 5) ADG_TS(3, 2) ADG_TS(3, 3) ADG_TS(3, 4) ADG_TS(3, 5)
   ADG_TS(4, 2) ADG_TS(4, 3) ADG_TS(4, 4) ADG_TS(4, 5) ADG_TS(5, 2) ADG_TS(5, 3) ADG_TS(5, 4)
@@ -158,9 +162,9 @@ void rhs_combine(const T* tv, const double* pv, int nterm, int npen, int64_t E, 
 }
 
 template void tile_sweep<double>(int, int, const TaskDev*, int, int64_t, int, const int32_t* const*,
-                                 const int32_t* const*, int, int, bool, hipStream_t);
+                                 const int32_t* const*, int, int, bool, bool, hipStream_t);
 template void tile_sweep<float>(int, int, const TaskDev*, int, int64_t, int, const int32_t* const*,
-                                const int32_t* const*, int, int, bool, hipStream_t);
+                                const int32_t* const*, int, int, bool, bool, hipStream_t);
 template void up_combine<double>(const double*, int, int64_t, double*, double*, const double*, int, double, int,
                                  hipStream_t);
 template void up_combine<float>(const float*, int, int64_t, float*, float*, const double*, int, double, int,

@@ -19,10 +19,12 @@ void tile_order(const uint32_t* bits, const int32_t* pre, const int32_t* fbase, 
 // (ntile tiles per task); max_tile bounds the elements of a tile; tbeg / tend
 // per dimension.
 // allr: one thread per target accumulates four rest indices of a chunk (else
-// one thread per (target, rest index)).
+// one thread per (target, rest index)); last: the tasks are last sweeps
+// (launched as their own kernel instance, so traces tell them apart).
 template <class T>
 void tile_sweep(int pf1, int pt1, const TaskDev* tasks, int ntask, int64_t ntile, int max_tile,
-                const int32_t* const* tbeg, const int32_t* const* tend, int D, int cap, bool allr, hipStream_t s);
+                const int32_t* const* tbeg, const int32_t* const* tend, int D, int cap, bool allr, bool last,
+                hipStream_t s);
 // Rest indices staged per chunk by tile_sweep for a term with R rest indices
 // (0: the tile does not fit in LDS)
 int tile_rest_chunk(int pf1, int R, int max_tile, int tbytes);

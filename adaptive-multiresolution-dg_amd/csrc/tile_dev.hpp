@@ -89,6 +89,29 @@ __device__ __forceinline__ void rest_offsets_t(int r, int dd, int D, const int* 
   }
 }
 
+// Loads through the global address space: the descriptors hold generic
+// pointers, whose flat loads would also count against the LDS wait counter.
+template <class T>
+__device__ __forceinline__ T gload(const T* p, int64_t o) {
+  return ((const __attribute__((address_space(1))) T*)p)[o];
+}
+template <class T>
+struct TileLoad;
+template <>
+struct TileLoad<double> {
+  static __device__ __forceinline__ double in(const TermDev& t, int64_t o) { return gload(t.in, o); }
+  static __device__ __forceinline__ SweepOps<double>::M mat(const TermDev& t, int64_t o) { return {gload(t.M, o)}; }
+};
+template <>
+struct TileLoad<float> {
+  static __device__ __forceinline__ float in(const TermDev& t, int64_t o) {
+    return t.in32 ? gload(t.in32, o) : static_cast<float>(gload(t.in, o));
+  }
+  static __device__ __forceinline__ SweepOps<float>::M mat(const TermDev& t, int64_t o) {
+    return {gload(t.Mh, o), gload(t.Ml, o)};
+  }
+};
+
 struct TileSet {
   const int32_t* tbeg[kMaxDimK];
   const int32_t* tend[kMaxDimK];
@@ -149,7 +172,9 @@ __device__ __forceinline__ void tile_item(const TaskDev* tasks, int64_t ntile, c
   const int sod = so[dd];
   const int sdd = t.sin[dd];
   const int R = t.Tin / t.isz[dd];
-  int rc = min(R, (lds_bytes - tile_meta_bytes(max_tile)) / (max_tile * PF1 * static_cast<int>(sizeof(T))));
+  // rest indices per chunk, from this tile's own element count (n <= max_tile,
+  // so at least the chunk the host checked for a tile of max_tile elements)
+  int rc = min(R, (lds_bytes - tile_meta_bytes(max_tile)) / (n * PF1 * static_cast<int>(sizeof(T))));
   rc = min(rc, kMaxRest / PF1);  // a staged row has at most kMaxRest <= blockDim values
   const int64_t Mrow = static_cast<int64_t>(t.ns) * PF1 * PT1;
   T* out = Ops::out(tk);
@@ -172,7 +197,7 @@ __device__ __forceinline__ void tile_item(const TaskDev* tasks, int64_t ntile, c
         const int r = c / PF1, k = c - r * PF1;
         const int off = s_bin[r] + k * sdd;
         for (int p = q0; p < n; p += step)
-          X[p * per + c] = Ops::in(t, static_cast<int64_t>(s_elem[p]) * t.Tin + off);
+          X[p * per + c] = TileLoad<T>::in(t, static_cast<int64_t>(s_elem[p]) * t.Tin + off);
       }
     }
     __syncthreads();
@@ -198,7 +223,7 @@ __device__ __forceinline__ void tile_item(const TaskDev* tasks, int64_t ntile, c
 #pragma unroll
             for (int k = 0; k < PF1; ++k)
 #pragma unroll
-              for (int i = 0; i < PT1; ++i) m[k][i] = Ops::mat(t, Mr + k * PT1 + i);
+              for (int i = 0; i < PT1; ++i) m[k][i] = TileLoad<T>::mat(t, Mr + k * PT1 + i);
             const T* x = X + (li * nr + rb) * PF1;
 #pragma unroll
             for (int r = 0; r < RG; ++r) {

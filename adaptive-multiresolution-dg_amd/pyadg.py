@@ -145,6 +145,7 @@ def lib():
             "adg_stage_launch_count": (i32, [vp, P(i64), P(i64)]),
             "adg_set_graph_step": (i32, [vp, i32]),
             "adg_kernel_tuning": (i32, [vp, I32P, i32, P(i32)]),
+            "adg_set_class_variants": (i32, [vp, P(i32), i32]),
             "adg_graph_step_count": (i32, [vp, P(i64), P(i64)]),
             "adg_rk_step": (i32, [vp, P(Problem), i32, dbl]),
         }
@@ -614,6 +615,11 @@ class Solution:
     def stage(self, problem, scheme, stage, dt):
         self._check(lib().adg_stage(self._h, ctypes.byref(problem), scheme, stage, dt))
 
+    def set_class_variants(self, variants):
+        """Fixed mapping per kernel class (classes 0..3 FP64, 4..7 FP32; -1: autotuner)."""
+        arr = (ctypes.c_int * max(len(variants), 1))(*variants)
+        self._check(lib().adg_set_class_variants(self._h, arr, len(variants)))
+
     def kernel_tuning(self):
         """{(kernel class, grid-size class): variant} chosen by the autotuner so far."""
         n = ctypes.c_int()
@@ -647,11 +653,11 @@ class Solution:
         self._check(lib().adg_profile(self._h, int(enable)))
 
     def profile_get(self):
-        ms = np.zeros(5)
-        cnt = np.zeros(5, np.int64)
-        by = np.zeros(5)
+        ms = np.zeros(6)
+        cnt = np.zeros(6, np.int64)
+        by = np.zeros(6)
         self._check(lib().adg_profile_get(self._h, _dp(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _dp(by)))
-        names = ["sweep", "surplus", "flux", "rk", "other"]
+        names = ["sweep", "surplus", "flux", "rk", "other", "rhs_phase"]
         return {n: {"ms": float(ms[i]), "count": int(cnt[i]), "bytes": float(by[i])} for i, n in enumerate(names)}
 
     def profile_host(self):

@@ -20,6 +20,7 @@ one instance per host thread, for a bounded wall time.
 """
 import argparse
 import ctypes
+import glob
 import json
 import os
 import sys
@@ -202,6 +203,13 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"),
                     help="storage precision of the chosen form's fused stages (fp32: split FP32 tables, FP64 "
                          "accumulation, RK and refine/coarsen norms)")
+    ap.add_argument("--variant", type=int, default=-1, choices=range(-1, 10),
+                    help="chosen form: thread mapping of every fused kernel (-1: the autotuner picks per class)")
+    ap.add_argument("--mappings", default=None,
+                    help="chosen form: thread mapping per fused kernel class (eval-up phases, eval-up final, "
+                         "right-hand-side phases, right-hand-side final), e.g. 3,3,7,7, or 'auto' for the "
+                         "autotuner; default 3,3,7,7 on c4 (the best per class in a sweep of all ten mappings "
+                         "at the per-context size), auto elsewhere")
     ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
                     help="chosen form: record whole RK steps into HIP graphs and replay them (adg_set_graph_step)")
     ap.add_argument("--imex", action="store_true",
@@ -242,10 +250,18 @@ def main():
                 b.save_state(0)
                 bsols.append(b)
     prec = pyadg.PREC_FP32_SPLIT if args.precision == "fp32" else pyadg.PREC_FP64
+    if args.mappings is None:
+        args.mappings = "3,3,7,7" if args.workload == "c4" else "auto"
+    mappings = [] if args.mappings == "auto" else [int(v) for v in args.mappings.split(",")]
+    if mappings and prec == pyadg.PREC_FP32_SPLIT:  # classes 4..7 are the FP32 ones
+        mappings = [-1] * 4 + mappings
     chosen_sols = bsols if batched else ([] if w.linear else [workloads.make_solution(w)])
     for c in chosen_sols:
         c.set_precision(prec)
         c.set_graph_step(args.graph)
+        c.set_kernel_variant(args.variant)
+        if mappings:
+            c.set_class_variants(mappings)
         if c is not sol:
             c.save_state(0)
     if batched:
@@ -307,7 +323,17 @@ def main():
             for t in th:
                 t.join()
 
-        run_all(args.warmup)
+        # the first context warms up alone, so that the autotuner (whose
+        # decisions the contexts share) times its kernels without the others
+        if args.warmup > 0 and len(sols) > 1:
+            worker(0, args.warmup)
+            th = [threading.Thread(target=worker, args=(i, args.warmup)) for i in range(1, len(sols))]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+        else:
+            run_all(args.warmup)
         for s in sols:
             if not args.no_events:
                 s.profile(True)
@@ -370,27 +396,52 @@ def main():
 
     roofline = None
     prof = chosen["prof"]
-    names = {"sweep": "k_sweep" if chosen is plain else "k_phase", "flux": "k_up_final", "rk": "k_rhs_final"}
-    cat = max(("sweep", "flux", "rk") if chosen is not plain else ("sweep",), key=lambda c: prof[c]["ms"])
+    # the dominant kernel class of the chosen form, named by the mapping the
+    # autotuner chose for it (one kernel instance per launch in the phase
+    # classes; the final classes launch their tiles, penalties and combine)
+    cats = ("sweep",) if chosen is plain else ("sweep", "rhs_phase", "flux", "rk", "surplus")
+    cat = max(cats, key=lambda c: prof[c]["ms"])
+    kernel = {"sweep": "k_sweep", "surplus": "k_surplus<double>", "flux": "eval-up final", "rk": "rhs final"}.get(cat)
+    tuning = {}
+    if chosen is not plain and cat in ("sweep", "rhs_phase", "flux", "rk"):
+        cls = {"sweep": 0, "flux": 1, "rhs_phase": 2, "rk": 3}[cat] + (4 if prec == pyadg.PREC_FP32_SPLIT else 0)
+        fixed = args.variant if args.variant >= 0 else (mappings[cls] if cls < len(mappings) else -1)
+        if fixed >= 0:
+            tuning = {"all": fixed}
+        for c in (bsols if batched else chosen_sols or [sol]) if fixed < 0 else []:
+            for (kc, bucket), v in c.kernel_tuning().items():
+                if kc == cls:
+                    tuning[bucket] = v
+        if cat in ("sweep", "rhs_phase") and len(set(tuning.values())) == 1:
+            v = next(iter(tuning.values()))
+            pf, pt = (w.k + 1, w.m + 1) if cat == "sweep" else (w.m + 1, w.k + 1)
+            ty = "float" if prec == pyadg.PREC_FP32_SPLIT else "double"
+            kernel = ("k_tile<%d, %d, %s, %d, false>" % (pf, pt, ty, 4 if v >= 7 else 1) if v >= 4
+                      else "k_phase<%d, %d, %s, %d>" % (pf, pt, ty, v))
+        elif kernel is None:
+            kernel = "phase kernels (mappings %s)" % sorted(set(tuning.values()))
     pc = prof[cat]
     if pc["count"] > 0 and pc["ms"] > 0:
         avg_ms = pc["ms"] / pc["count"]
         avg_bytes = pc["bytes"] / pc["count"]
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
         # HBM bytes per launch: the traffic / algorithmic ratio measured by the FETCH_SIZE and
-        # WRITE_SIZE counter passes recorded under profiles/, applied to this run's launches
+        # WRITE_SIZE counter passes recorded under profiles/ for this kernel and workload,
+        # applied to this run's launches
         traffic, traffic_src = None, None
-        tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_k_rhs_final_c2.json")
-        if args.workload == "c2" and names[cat] == "k_rhs_final" and os.path.exists(tf):
+        here = os.path.dirname(os.path.abspath(__file__))
+        for tf in sorted(glob.glob(os.path.join(here, "profiles", "traffic_*.json"))):
             with open(tf) as f:
                 t = json.load(f)
-            traffic = t["traffic_over_algorithmic"] * avg_bytes
-            traffic_src = "profiles/traffic_k_rhs_final_c2.json (ratio %.2f)" % t["traffic_over_algorithmic"]
+            if t.get("workload") == args.workload and t.get("kernel") == kernel:
+                traffic = t["traffic_over_algorithmic"] * avg_bytes
+                traffic_src = "profiles/%s (ratio %.2f)" % (os.path.basename(tf), t["traffic_over_algorithmic"])
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
-                    "kernel": names[cat],
+                    "kernel": kernel, "kernel_class": cat, "tuned_mapping_by_grid_size_class": tuning or None,
                     "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes, "launches": pc["count"],
                     "kernel_ms_by_category": {k: round(v["ms"], 3) for k, v in prof.items()},
+                    "launches_by_category": {k: v["count"] for k, v in prof.items()},
                     "avg_algorithmic_bytes_by_category": {k: (v["bytes"] / v["count"] if v["count"] else 0.)
                                                           for k, v in prof.items()},
                     "host_ms_inclusive": {k: round(v, 3) for k, v in chosen["host"].items()}}
@@ -416,6 +467,7 @@ def main():
                              "concurrently" % (pname, args.batch, args.seed, args.seed + args.batch - 1, len(bsols)))
                        if batched else "chosen: fused %s stages" % pname,
                        launches_per_stage=chosen["launches_per_stage"], graph_step=args.graph,
+                       kernel_variant=args.variant, mappings=args.mappings,
                        graph_steps_recorded=chosen["graph_steps_recorded"],
                        graph_steps_replayed=chosen["graph_steps_replayed"],
                        precision=args.precision if not w.linear else "fp64",

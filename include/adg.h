@@ -476,6 +476,12 @@ int adg_graph_step_count(adg_ctx* c, int64_t* captures, int64_t* replays);
  * adg_rk_init followed by adg_stage for every stage) with step size dt. */
 int adg_rk_step(adg_ctx* c, const adg_problem* p, int scheme, double dt);
 
+/* Fixed thread mappings per kernel class (no reference counterpart): v[c] for
+ * class c = 0..n-1 (n <= 8; classes as in adg_kernel_tuning), -1 leaves the
+ * class to the autotuner.  adg_set_kernel_variant, when >= 0, overrides these.
+ * All mappings give identical values. */
+int adg_set_class_variants(adg_ctx* c, const int* v, int n);
+
 /* The autotuner's decisions so far (no reference counterpart): up to cap
  * (key, variant) pairs, key = kernel class * 64 + grid-size class (log2 of the
  * element count); classes 0..3 are the FP64 eval-up phases, eval-up final,
@@ -510,9 +516,11 @@ int adg_state_read(adg_ctx* c, const char* path);
 
 /* Kernel timing: HIP events around every launch on the context's stream
  * (enable = 1 starts a fresh record).  adg_profile_get fills, per category
- * (0 sweep, 1 surplus transform, 2 flux, 3 RK stage, 4 other), the summed
- * kernel milliseconds, launch count and algorithmic bytes (the bytes each
- * kernel must move: inputs read once plus outputs written once). */
+ * (0 sweeps: plain sweeps and the fused eval-up phases, 1 surplus transform,
+ * 2 flux / fused eval-up final, 3 RK stage / fused right-hand-side final,
+ * 4 other, 5 fused right-hand-side phases; arrays of 6), the summed kernel
+ * milliseconds, launch count and algorithmic bytes (the bytes each kernel
+ * must move: inputs read once plus outputs written once). */
 int adg_profile(adg_ctx* c, int enable);
 int adg_profile_get(adg_ctx* c, double* ms, int64_t* count, double* bytes);
 /* Host wall time (ms, inclusive) while profiling: [0] refine/coarsen
