@@ -235,10 +235,11 @@ def main():
     w = workload(args.seed)
     print(f"[bench] workload {w.name}: init", file=sys.stderr, flush=True)
     sol = workloads.make_solution(w)
-    # the single-instance fused form (a comparison point) runs the per-element
-    # mapping 3 throughout, so the chosen form's kernel instances are its own
-    # in kernel traces
-    sol.set_kernel_variant(3)
+    if batched:
+        # beside a batched chosen form, the single-instance fused form (a
+        # comparison point) runs the per-element mapping 3 throughout, so the
+        # chosen form's kernel instances are its own in kernel traces
+        sol.set_kernel_variant(3)
     print(f"[bench] init done: {sol.num_elem} elements; warmup {args.warmup} steps", file=sys.stderr, flush=True)
     sol.save_state(0)
     bsols = []
@@ -477,7 +478,8 @@ def main():
                        precision=args.precision if not w.linear else "fp64",
                        contexts=len(bsols) if batched else 1,
                        plain_form="plain FP64, one kernel per 1D sweep, one instance",
-                       fused_single_instance_form="fused FP64 stages, one instance, per-element mapping 3",
+                       fused_single_instance_form=("fused FP64 stages, one instance, per-element mapping 3"
+                                                   if batched else "fused FP64 stages, one instance (chosen)"),
                        plain_value=plain["value"], fused_single_instance_value=fused["value"],
                        chosen_value=chosen["value"], batch=args.batch if batched else 1,
                        speedup_vs_plain=chosen["value"] / plain["value"],
