@@ -69,3 +69,25 @@ def test_c4_single_instance_plain_and_fused_match_oracle():
         assert np.array_equal(ghk, hk) and np.array_equal(glv, lv), f"form {form}: element sets differ"
         a, c = s.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
         assert np.array_equal(a, c), f"form {form}: " + first_diff(a, c)
+
+
+def test_c4_batch_mixed_class_mappings_match_oracle():
+    """Fixed mappings per kernel class (the bench's 3,3,7,7 and a mix that
+    puts every kernel family in one stage) give the oracle's values, and the
+    autotuner's decisions are reported per class."""
+    seeds = [4, 8]
+    ws = [workloads.c4_advection_3d(seed=s, nmax=5, level_init=2) for s in seeds]
+    print(f"[case] c4 class mappings seeds={seeds}", flush=True)
+    refs = [_oracle(w, 3)[0] for w in ws]
+    for mappings in ([3, 3, 7, 7], [0, 5, 9, 1], [-1, 4, -1, 8]):
+        b = workloads.make_batch(ws)
+        b.set_class_variants(mappings)
+        prob = pyadg.Problem.from_buffer_copy(ws[0].problem)
+        prob.form, prob.max_steps = 1, 3
+        b.run_hyperbolic(prob)
+        for i, o in enumerate(refs):
+            a, c = b.instance_get(i), o.get(pyoracle.FIELD_UCOE)
+            assert np.array_equal(a, c), f"mappings {mappings} instance {i}: " + first_diff(a, c)
+        tuned = b.kernel_tuning()
+        assert all(cls in (0, 2) for cls, _ in tuned), tuned  # only the classes left to the autotuner
+        assert (-1 not in mappings) == (not tuned)
