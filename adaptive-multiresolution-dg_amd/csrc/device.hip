@@ -32,13 +32,14 @@ void hip_check(hipError_t e, const char* what) {
 }
 
 // Runs fn(i) for i in [0, n) on a persistent pool of up to 15 host threads
-// plus the caller (instances are independent).  The pool starts on first use;
-// one parallel_for runs at a time.
+// plus the caller (instances are independent).  Every calling host thread has
+// its own pool, started on its first use, so contexts driven from different
+// threads do not wait for each other's loops.
 namespace {
 class HostPool {
  public:
   static HostPool& get() {
-    static HostPool pool;
+    thread_local HostPool pool;
     return pool;
   }
   void run(int n, const std::function<void(int)>& fn) {
