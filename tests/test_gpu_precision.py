@@ -103,3 +103,37 @@ def test_fp32_runs_are_deterministic():
         outs.append(s)
     assert same_grid(outs[0], outs[1])
     assert np.array_equal(outs[0].get(pyadg.FIELD_UCOE), outs[1].get(pyadg.FIELD_UCOE))
+
+
+@pytest.mark.parametrize("workload", ["c3", "c4"])
+def test_fp32_on_benchmark_shapes(workload):
+    """The precision form on the 4D configuration's shape (sparse N = 6, 4
+    steps, fixed grid) and on a C4 batch (NMAX 5, 3 instances, adaptive, 4
+    steps): coefficients within 1e-5 relative of the FP64 form; C3's grid is
+    fixed, C4's element sets are reported (threshold decisions may flip)."""
+    print(f"[case] fp32 on {workload}", flush=True)
+    out = []
+    for prec in (pyadg.PREC_FP64, pyadg.PREC_FP32_SPLIT):
+        if workload == "c3":
+            w = workloads.c3_varcoef_4d(nmax=6)
+            s = workloads.make_solution(w)
+        else:
+            ws = [workloads.c4_advection_3d(seed=i, nmax=5, level_init=2) for i in (1, 2, 3)]
+            w = ws[0]
+            s = workloads.make_batch(ws)
+        s.set_precision(prec)
+        prob = pyadg.Problem.from_buffer_copy(w.problem)
+        prob.form, prob.max_steps = 1, 4
+        s.run_hyperbolic(prob)
+        if workload == "c3":
+            out.append((s.grid()[2], s.get(pyadg.FIELD_UCOE)))
+        else:
+            out.append((s.instance_grid(0)[2], s.instance_get(0)))
+    (k64, u64), (k32, u32) = out
+    if workload == "c3" or np.array_equal(k64, k32):
+        assert np.array_equal(k64, k32)
+        r = rel(u32, u64)
+        print(f"[fp32] {workload}: relative L2 difference {r:.3e}", flush=True)
+        assert r < 1e-5, r
+    else:
+        print(f"[fp32] {workload}: element sets differ by {abs(len(k64) - len(k32))} elements", flush=True)
