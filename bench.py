@@ -263,12 +263,15 @@ def main():
     chosen_sols = bsols if batched else ([] if w.linear else [workloads.make_solution(w)])
     for c in chosen_sols:
         c.set_precision(prec)
+        if c is not sol:
+            c.save_state(0)
+    # launch settings of the chosen form; without a batch the single-instance
+    # fused form measured on `sol` is the chosen form in FP64
+    for c in chosen_sols + ([] if batched else [sol]):
         c.set_graph_step(args.graph)
         c.set_kernel_variant(args.variant)
         if mappings:
             c.set_class_variants(mappings)
-        if c is not sol:
-            c.save_state(0)
     if batched:
         print(f"[bench] batch init done: {sum(b.num_elem for b in bsols)} elements", file=sys.stderr, flush=True)
     learned = None
