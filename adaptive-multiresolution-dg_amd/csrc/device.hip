@@ -493,8 +493,9 @@ void launch_sweep(const SweepArgs& a, int pf1, int pt1, hipStream_t s) {
   ADG_SW(2, 2) ADG_SW(2, 3) ADG_SW(2, 4) ADG_SW(3, 2) ADG_SW(3, 3) ADG_SW(3, 4)
   ADG_SW(4, 2) ADG_SW(4, 3) ADG_SW(4, 4)
   ADG_SW(2, 5) ADG_SW(3, 5) ADG_SW(4, 5) ADG_SW(5, 2) ADG_SW(5, 3) ADG_SW(5, 4)
+  ADG_SW(2, 6) ADG_SW(3, 6) ADG_SW(4, 6) ADG_SW(6, 2) ADG_SW(6, 3) ADG_SW(6, 4)
 #undef ADG_SW
-  throw std::runtime_error("sweep: unsupported basis sizes (Alpert degrees 1..3, Lagrange 1..4 are provided)");
+  throw std::runtime_error("sweep: unsupported basis sizes (Alpert degrees 1..3, Lagrange 1..5 are provided)");
 }
 
 // Dense level-ordered table (rows ns*pf1, cols ns*pt1) -> target-major cell blocks.
@@ -546,7 +547,7 @@ Context::Context(int dim, int k_, int m_, int nmax_)
     : D(dim), k(k_), m(m_), nmax(nmax_),
       alpt(BasisKind::Alpert, k_, nmax_), lagr(BasisKind::Lagrange, m_, nmax_) {
   if (D < 1 || D > kMaxDim) throw std::runtime_error("dimension must be 1..4");
-  if (k < 1 || k > 3 || m < 1 || m > 4) throw std::runtime_error("degrees must be k in 1..3 and m in 1..4");
+  if (k < 1 || k > 3 || m < 1 || m > 5) throw std::runtime_error("degrees must be k in 1..3 and m in 1..5");
   if (nmax < 1 || nmax > 13) throw std::runtime_error("maximum level must be 1..13");
   K = 1; P = 1;
   for (int d = 0; d < D; ++d) { K *= k + 1; P *= m + 1; }

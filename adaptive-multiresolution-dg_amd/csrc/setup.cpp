@@ -217,8 +217,28 @@ static double lagr_phi(double xt, int msh, int P, int p) {
       if (p == 3) return (x < 0) ? 0. : -13824. / 35. * (x - 2. / 3.) * (x - 1. / 3.) * (x - 7. / 24.) * (x - 1. / 6.);
       if (p == 4) return (x < 0) ? 0. : 192 * (x - 7. / 12.) * (x - 1. / 3.) * (x - 7. / 24.) * (x - 1. / 6.);
     }
+  } else if (P == 5) {
+    // source/LagrBasis.cpp:617-681 (msh_case 1, continuous: level-0 points at
+    // the cell ends, level-1 points on both sides of the midpoint)
+    if (msh == 0) {
+      if (p == 0) return -1.0 / 768. * (x - 1.) * (5.0 * x - 3.) * (5.0 * x + 3.) * (5.0 * x + 1.) * (5.0 * x - 1.);
+      if (p == 1) return 25.0 / 768. * (x - 1.) * (x + 1.) * (5.0 * x - 3.) * (5.0 * x + 1.) * (5.0 * x - 1.);
+      if (p == 2) return -25.0 / 384. * (x - 1.) * (x + 1.) * (5.0 * x - 3.) * (5.0 * x + 3.) * (5.0 * x - 1.);
+      if (p == 3) return 25.0 / 384. * (x - 1.) * (x + 1.) * (5.0 * x - 3.) * (5.0 * x + 3.) * (5.0 * x + 1.);
+      if (p == 4) return -25.0 / 768. * (x - 1.) * (x + 1.) * (5.0 * x + 3.) * (5.0 * x + 1.) * (5.0 * x - 1.);
+      if (p == 5) return 1.0 / 768. * (x + 1.) * (5.0 * x - 3.) * (5.0 * x + 3.) * (5.0 * x + 1.) * (5.0 * x - 1.);
+    } else {
+      if (p == 0) return (x <= 0) ? 25. / 24. * x * (x + 1.0) * (5.0 * x + 1.) * (5.0 * x + 2.) * (5.0 * x + 3.) : 0.;
+      if (p == 1) return (x <= 0) ? 25. / 12. * x * (x + 1.0) * (5.0 * x + 1.) * (5.0 * x + 4.) * (5.0 * x + 3.) : 0.;
+      if (p == 2)
+        return (x <= 0) ? 1. / 24. * (x + 1.0) * (5.0 * x + 1.) * (5.0 * x + 2.) * (5.0 * x + 3.) * (5.0 * x + 4.) : 0.;
+      if (p == 3)
+        return (x < 0) ? 0. : -1. / 24. * (x - 1.0) * (5.0 * x - 1.) * (5.0 * x - 2.) * (5.0 * x - 3.) * (5.0 * x - 4.);
+      if (p == 4) return (x < 0) ? 0. : -25. / 12. * x * (x - 1.0) * (5.0 * x - 1.) * (5.0 * x - 4.) * (5.0 * x - 3.);
+      if (p == 5) return (x < 0) ? 0. : -25. / 24. * x * (x - 1.0) * (5.0 * x - 1.) * (5.0 * x - 2.) * (5.0 * x - 3.);
+    }
   }
-  throw std::runtime_error("Lagrange basis: only degree 1..4, msh_case 1 are provided");
+  throw std::runtime_error("Lagrange basis: only degree 1..5, msh_case 1 are provided");
 }
 
 // ---------------------------------------------------------------------------
@@ -295,6 +315,10 @@ Basis1D::Basis1D(BasisKind k, int pmax_, int nmax_, int msh_case_)
     else if (pmax == 4) {
       msh0 = {-2. / 3., -5. / 12., -1. / 3., 1. / 6., 1. / 3.};
       msh1 = {-5. / 6., -17. / 24., 7. / 24., 7. / 12., 2. / 3.};
+    }
+    else if (pmax == 5) {
+      msh0 = {-1 + ROUND_OFF, -0.6, -0.2, 0.2, 0.6, 1 - ROUND_OFF};
+      msh1 = {-0.8, -0.4, 0.0 - ROUND_OFF, 0.0 + ROUND_OFF, 0.4, 0.8};
     }
     else throw std::runtime_error("Lagrange basis: degree out of range");
     for (int i = 0; i < pmax + 1; ++i) {

@@ -80,9 +80,12 @@ def test_sweep_bitexact_adaptive_grid():
             assert np.array_equal(a, b), f"table={table} lu={lu} flx={flx} d={d}: {first_diff(a, b)}"
 
 
-def test_init_and_operators_bitexact_burgers():
+@pytest.mark.parametrize("m", [3, 5])
+def test_init_and_operators_bitexact_burgers(m):
+    # m = 5: Lagrange P5 on the continuous mesh (points ROUND_OFF inside the
+    # cell ends and the midpoint), as the reference's linear-advection driver uses
     p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.07, (0.4, 0.6))])
-    g, o = adapted_pair(2, 2, 3, 6, 4, p, 1e-4)
+    g, o = adapted_pair(2, 2, m, 6, 4, p, 1e-4)
     a, b = g.get(pyadg.FIELD_UCOE), o.get(pyoracle.FIELD_UCOE)
     assert np.array_equal(a, b), "init ucoe: " + first_diff(a, b)
     g.eval_up(); o.eval_up()

@@ -83,7 +83,7 @@ def gl_fine(f, a, b, nsub=16, npt=12):
     return tot
 
 
-@pytest.mark.parametrize("m", [1, 2, 3, 4])
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5])
 def test_lagrange_mass_diagonal_independent_quadrature(m):
     nmax = 4
     o = pyoracle.Oracle(1, 2, m, nmax, False, 0)

@@ -189,24 +189,29 @@ def test_alpert_flux_tables_independent():
         assert abs(lft - ul[r, c]) < 1e-5 and abs(rgt - ur[r, c]) < 1e-5
 
 
-@pytest.mark.parametrize("m", [1, 2, 3, 4])
+@pytest.mark.parametrize("m", [1, 2, 3, 4, 5])
 def test_lagrange_basis_interpolation_property(m):
     nmax = 4
     o = pyoracle.Oracle(1, 1, m, nmax, False, 0)
     n = (1 << nmax) * (m + 1)
     pts = [o.lagr_point(i) for i in range(n)]
     lev = [0 if i // (m + 1) == 0 else int(math.log2(i // (m + 1))) + 1 for i in range(n)]
+    # degree 5 (the continuous mesh) puts points ROUND_OFF = 1e-13 inside the
+    # cell ends and on both sides of the midpoint, where the nodal functions are
+    # exactly 1 or 0: the values there are off by |phi'| * 1e-13 (measured up to
+    # 2e-11); a wrong coefficient or factor is off by O(1)
+    tol1, tol0 = (1e-13, 1e-12) if m < 5 else (1e-10, 1e-10)
     for i in range(n):
         for j in range(n):
             v = o.basis_val(1, i, pts[j])
             if j == i:
-                assert abs(v - 1.0) < 1e-13
+                assert abs(v - 1.0) < tol1, (i, v)
             elif lev[j] <= lev[i] and lev[j] >= 0 and (lev[j] < lev[i] or j // (m + 1) == i // (m + 1)):
                 # the basis vanishes at all its own other points and at every coarser point
-                assert abs(v) < 1e-12, (i, j, v)
+                assert abs(v) < tol0, (i, j, v)
 
 
-@pytest.mark.parametrize("dim,m", [(2, 3), (3, 2), (2, 4), (2, 1)])
+@pytest.mark.parametrize("dim,m", [(2, 3), (3, 2), (2, 4), (2, 1), (2, 5)])
 def test_surplus_vanishes_for_polynomials(dim, m):
     # interpolation of a polynomial of degree <= m in each variable is exact, so
     # every surplus on levels >= 1 vanishes (pins points, weights and the transform)
