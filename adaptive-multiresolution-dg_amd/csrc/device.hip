@@ -1671,6 +1671,9 @@ void Context::rk_step_eager(const StageProblem& p, int scheme, int nstage, doubl
 // memory: versions of the grid and of the tables built from it, the device
 // buffers (their addresses change when they grow), the problem, the launch
 // settings and, without per-instance steps, dt.
+// positions of the u / u_next addresses in a step key (a step swaps them)
+constexpr size_t kKeyU = 3, kKeyUNext = 4;
+
 std::vector<uintptr_t> Context::step_key(const StageProblem& p, int scheme, int nstage, double dt) const {
   auto bits = [](double v) {
     uint64_t b;
@@ -1689,6 +1692,7 @@ std::vector<uintptr_t> Context::step_key(const StageProblem& p, int scheme, int 
       static_cast<uintptr_t>(nstage), static_cast<uintptr_t>(p.flux_kind), bits(p.flux_param),
       static_cast<uintptr_t>(use_inst_dt), use_inst_dt ? ptr(d_inst_dt.p) : bits(dt), inst_alpha.size()};
   for (int d = 0; d < D; ++d) key.push_back(bits(p.alpha[d]));
+  if (key[kKeyU] != ptr(u.p) || key[kKeyUNext] != ptr(u_next.p)) throw std::logic_error("step key layout");
   return key;
 }
 
@@ -1711,7 +1715,7 @@ void Context::rk_step(const StageProblem& p, int scheme, int nstage, double dt) 
   const std::vector<uintptr_t> key = step_key(p, scheme, nstage, dt);
   // the key without the u / u_next pair, which every odd step swaps
   auto norm = [](std::vector<uintptr_t> k) {
-    k[3] = k[4] = 0;
+    k[kKeyU] = k[kKeyUNext] = 0;
     return k;
   };
   auto swaps = [&]() {
@@ -2289,27 +2293,31 @@ void Context::tuned_launch(int cls, const std::function<void(int)>& launch) {
   // tile sets), then its time is the least of three timed runs; the stage
   // kernels do not read their own outputs, so repeating a launch changes
   // no value
-  hipEvent_t a, b;
-  hip_check(hipEventCreate(&a), "hipEventCreate");
-  hip_check(hipEventCreate(&b), "hipEventCreate");
+  struct Events {  // released however the loop ends
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Events() {
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } ev;
+  hip_check(hipEventCreate(&ev.a), "hipEventCreate");
+  hip_check(hipEventCreate(&ev.b), "hipEventCreate");
   float best = 0.f;
   int bv = 0;
   for (int v = 0; v < kVariants; ++v) {
     if (!try_launch(v)) continue;
     float tv = 0.f;
     for (int r = 0; r < 3; ++r) {
-      hip_check(hipEventRecord(a, stream), "hipEventRecord");
+      hip_check(hipEventRecord(ev.a, stream), "hipEventRecord");
       launch(v);
-      hip_check(hipEventRecord(b, stream), "hipEventRecord");
-      hip_check(hipEventSynchronize(b), "hipEventSynchronize");
+      hip_check(hipEventRecord(ev.b, stream), "hipEventRecord");
+      hip_check(hipEventSynchronize(ev.b), "hipEventSynchronize");
       float ms = 0.f;
-      hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+      hip_check(hipEventElapsedTime(&ms, ev.a, ev.b), "hipEventElapsedTime");
       tv = r == 0 ? ms : std::min(tv, ms);
     }
     if (v == 0 || tv < best) { best = tv; bv = v; }
   }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   tuned[key] = bv;
   std::lock_guard<std::mutex> lock(g_tuned_mutex);
   g_tuned.emplace(gkey, bv);

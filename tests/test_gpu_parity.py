@@ -318,3 +318,15 @@ def test_point_values_and_quadrature_error():
         ea = g.error_no_separable_builtin(q, pts)
         eb = o.error_no_separable_c(pyadg.ic_eval_ptr(), ctypes.cast(ctypes.pointer(q), ctypes.c_void_p), pts)
         assert np.all(np.abs(ea - eb) <= 1e-12 * np.abs(eb)), (ea, eb)
+
+
+def test_fused_stage_refuses_lagrange_degree_5():
+    # the fused kernels are instantiated for Lagrange degrees 1..4; degree 5
+    # reports an error instead of launching (the per-sweep path serves it)
+    p = ic_params(2, base=0.5, ampl=0.25, bumps=[(0.15, 0.07, (0.4, 0.6))])
+    g = pyadg.Solution(2, 2, 5, 5, True, 3)
+    g.init_adaptive_intp_builtin(p, 1e-4)
+    prob = problem(flux_kind=0, alpha=[1.2, 1.2], wave_speed=[1.0, 1.0], cfl=0.2, t_final=1.0)
+    g.rk_init()
+    with pytest.raises(pyadg.AdgError, match="unsupported basis sizes"):
+        g.stage(prob, pyadg.RK3, 0, 1e-3)
