@@ -379,6 +379,21 @@ int adg_set_refine_mode(adg_ctx* h, int mode, double threshold) {
   });
 }
 
+int adg_set_judge_defer(adg_ctx* h, int defer) {
+  return guard(h, [&](adg::Context& c) {
+    if (defer != 0 && defer != 1) throw std::runtime_error("judge defer must be 0 or 1");
+    c.judge_defer = defer;
+  });
+}
+
+int adg_proposer_stats(adg_ctx* h, int64_t* proposed, int64_t* redone, int64_t* deferred) {
+  return guard(h, [&](adg::Context& c) {
+    if (proposed) *proposed = c.stat_proposed;
+    if (redone) *redone = c.stat_redo;
+    if (deferred) *deferred = c.stat_deferred;
+  });
+}
+
 int adg_proposer_init(adg_ctx* h, uint64_t seed) { return guard(h, [&](adg::Context& c) { c.proposer_init(seed); }); }
 
 int adg_proposer_set_params(adg_ctx* h, const float* params) {

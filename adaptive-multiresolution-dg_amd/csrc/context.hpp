@@ -378,6 +378,10 @@ class Context {
   // (the reference), 1 learned proposer + threshold judge, 2 current norms +
   // threshold judge
   int refine_mode = 0;
+  // judge of the proposer modes: 0 redo the instances it refines from u^n,
+  // 1 give the missed leaves their children after the step and go on
+  int judge_defer = 0;
+  int64_t stat_deferred = 0;  // leaves refined by a deferring judge
   double prop_threshold = 0.5;
   int64_t stat_redo = 0, stat_proposed = 0;
   // decision log of the predict-and-refine step: features of every leaf at

@@ -542,9 +542,16 @@ int64_t Context::proposer_step(const std::function<void(int, int, double)>& stag
       changed[i] = added[i] > 0;
       redo[i] = changed[i];
       tot += added[i];
-      redone += changed[i];
+      if (!judge_defer) redone += changed[i];
     }
     if (tot == 0) break;
+    if (judge_defer) {
+      // the step stands; the missed leaves get their children now (with zero
+      // coefficients, as refinement gives them) and the next step evolves them
+      set_grids(ns, true, &changed);
+      stat_deferred += tot;
+      break;
+    }
     set_grids(ns, true, &changed);
     // the instances the judge refined start again from u^n
     to_dev(inst_mask, redo.data(), nb, stream);

@@ -82,6 +82,8 @@ def lib():
             "adg_set_interpolation": (i32, [vp, i32]),
             "adg_set_precision": (i32, [vp, i32]),
             "adg_set_refine_mode": (i32, [vp, i32, dbl]),
+            "adg_set_judge_defer": (i32, [vp, i32]),
+            "adg_proposer_stats": (i32, [vp, P(i64), P(i64), P(i64)]),
             "adg_viscosity_elements": (i32, [vp, dbl, P(i64), I32P]),
             "adg_viscosity_flux": (i32, [vp, dbl, i32]),
             "adg_viscosity_operator": (i32, [vp, dbl, i32, ctypes.POINTER(i64)]),
@@ -319,6 +321,16 @@ class Solution:
         return it.value
 
     # ---- learned refinement proposer ----
+    def set_judge_defer(self, defer):
+        """0: the judge redoes the instances it refines; 1: it refines the missed leaves after the step."""
+        self._check(lib().adg_set_judge_defer(self._h, int(defer)))
+
+    def proposer_stats(self):
+        """(leaves proposed, instance steps redone, leaves refined by a deferring judge) so far."""
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._check(lib().adg_proposer_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
     def set_refine_mode(self, mode, threshold=0.5):
         """REFINE_PREDICT (forward-Euler prediction, the reference), REFINE_LEARNED
         (network proposer + threshold judge) or REFINE_CURRENT (current norms + judge)."""

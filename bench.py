@@ -198,6 +198,9 @@ def main():
                     help="refinement of the chosen form's adaptive steps: forward-Euler prediction (the reference) "
                          "or the learned proposer with the threshold judge")
     ap.add_argument("--threshold", type=float, default=0.01, help="proposer probability threshold")
+    ap.add_argument("--judge", default="redo", choices=("redo", "defer"),
+                    help="learned refinement: the judge redoes the instances it refines (redo) or gives the "
+                         "missed leaves their children after the step (defer)")
     ap.add_argument("--train-steps", type=int, default=40, help="time steps of each logged training run")
     ap.add_argument("--train-samples", type=int, default=2000000, help="cap on the logged decisions trained on")
     ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"),
@@ -295,9 +298,10 @@ def main():
         for c in chosen_sols:
             c.set_proposer_params(params)
             c.set_refine_mode(pyadg.REFINE_LEARNED, args.threshold)
+            c.set_judge_defer(1 if args.judge == "defer" else 0)
         learned = {"training_samples": int(len(y)), "refined_fraction": float(y.mean()), "final_loss": loss,
                    "training_seconds": time.perf_counter() - t0, "threshold": args.threshold,
-                   "training_steps_per_instance": args.train_steps}
+                   "training_steps_per_instance": args.train_steps, "judge": args.judge}
         print(f"[bench] proposer trained on {len(y)} logged decisions in {learned['training_seconds']:.1f} s",
               file=sys.stderr, flush=True)
 
@@ -399,6 +403,8 @@ def main():
     else:
         chosen = measure(chosen_sols, 1)
     value = chosen["value"]
+    if learned is not None:
+        learned["judge_deferred_leaves"] = int(sum(c.proposer_stats()[2] for c in chosen_sols))
     # instance 0 of the batch is the plain problem itself: equal L2 error means equal coefficients
     same_set = bool(np.array_equal(plain["keys"], chosen["keys"]))
     identical = same_set and bool(np.array_equal(plain["u"], chosen["u"]))

@@ -412,6 +412,15 @@ int adg_imex43_stage(adg_ctx* c, int stage, int* cg_iterations);
 #define ADG_PROPOSER_PARAMS 5057
 enum { ADG_REFINE_PREDICT = 0, ADG_REFINE_LEARNED = 1, ADG_REFINE_CURRENT = 2 };
 int adg_set_refine_mode(adg_ctx* c, int mode, double threshold);
+/* Judge of ADG_REFINE_LEARNED / ADG_REFINE_CURRENT (no reference counterpart):
+ * defer = 0 (default) redoes every instance the judge refines from u^n, so each
+ * accepted step satisfies the threshold rule on the grid it was computed on;
+ * defer = 1 keeps the step and gives the missed leaves their children after it
+ * (zero coefficients, evolved from the next step on), which never redoes a step
+ * but lets a missed leaf's children start one step late.
+ * adg_proposer_stats reports the leaves refined by a deferring judge. */
+int adg_set_judge_defer(adg_ctx* c, int defer);
+int adg_proposer_stats(adg_ctx* c, int64_t* proposed, int64_t* redone, int64_t* deferred);
 int adg_proposer_init(adg_ctx* c, uint64_t seed);
 int adg_proposer_set_params(adg_ctx* c, const float* params);
 int adg_proposer_get_params(adg_ctx* c, float* params);

@@ -224,3 +224,59 @@ def test_learned_proposer_gains_over_the_methods_without_it():
         assert cur[0].redone >= 8 and learned[0].redone * 4 <= cur[0].redone
         assert 3 + 3 * learned[0].redone / learned[0].steps < 3.5
         assert learned[1] <= 1.1 * plain[1]
+
+
+def test_deferred_judge_on_reduced_c5():
+    """The learned proposer with the deferred judge on the rotating wave (C5 at
+    NMAX 7): no step is redone, every instance of a batch equals its solve
+    alone bit for bit, and instance 0 ends at least as close to a run of the
+    reference's prediction at eps / 10 as the prediction at eps does (the
+    discontinuous data leaves grid-dependent oscillations, so every grid differs
+    from the finer one by tens of percent)."""
+    nmax = 7
+    mk = lambda s: workloads.c5_rotating(seed=s, nmax=nmax)
+    print("[case] deferred judge, C5 NMAX 7", flush=True)
+    train = workloads.make_batch([mk(20001 + i) for i in range(8)])
+    train.decision_log(True)
+    tp = pyadg.Problem.from_buffer_copy(mk(1).problem)
+    tp.form, tp.max_steps = 1, 20
+    train.run_hyperbolic(tp)
+    x, y = train.decision_log_get()
+    train.proposer_init(3)
+    train.proposer_train(x, y, epochs=8, batch=1024, lr=2e-3, seed=3)
+    params = train.proposer_params()
+    del train
+    xg = (np.arange(128) + 0.5) / 128
+
+    def run(sol, w, mode, eps_scale=1.0):
+        if mode:
+            sol.set_proposer_params(params)
+            sol.set_refine_mode(pyadg.REFINE_LEARNED, 0.01)
+            sol.set_judge_defer(1)
+        p = pyadg.Problem.from_buffer_copy(w.problem)
+        p.form, p.max_steps, p.t_final = 1, 0, 0.04
+        if eps_scale != 1.0:
+            p.eps *= eps_scale
+            p.eta *= eps_scale
+            sol.set_instance_thresholds([p.eps] * sol.num_instances, [p.eta] * sol.num_instances)
+        return sol.run_hyperbolic(p)
+
+    ws = [mk(s) for s in (1, 2, 3)]
+    b = workloads.make_batch(ws)
+    st = run(b, ws[0], True)
+    assert st.redone == 0 and b.proposer_stats()[1] == 0
+    for i, w in enumerate(ws):
+        s = workloads.make_solution(w)
+        run(s, w, True)
+        a, c = b.instance_get(i), s.get(pyadg.FIELD_UCOE)
+        assert np.array_equal(a, c), f"instance {i} differs from its solve alone"
+    u = {}
+    for name, mode, scale in (("learned", True, 1.0), ("predict", False, 1.0), ("fine", False, 0.1)):
+        s = workloads.make_solution(ws[0])
+        run(s, ws[0], mode, scale)
+        u[name] = s.eval_tensor(xg)
+    d_learned = np.linalg.norm(u["learned"] - u["fine"]) / np.linalg.norm(u["fine"])
+    d_predict = np.linalg.norm(u["predict"] - u["fine"]) / np.linalg.norm(u["fine"])
+    print(f"[deferred judge] relative L2 to the eps/10 run: learned {d_learned:.3e}, prediction {d_predict:.3e}",
+          flush=True)
+    assert d_learned <= 1.05 * d_predict
