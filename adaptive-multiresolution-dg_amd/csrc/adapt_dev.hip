@@ -1,0 +1,764 @@
+// Refine and coarsen of the union element table on the device.
+//
+// The rules are the reference's (DGAdapt::refine and DGAdapt::coarsen,
+// source/DGAdapt.cpp:371-404,675-688,982-1016,1250-1385) in the form the
+// host code states them (grid.cpp: refine_slots, coarsen_slots,
+// HostGrid::build):
+//   refine   every leaf (fewer children present than possible) whose norm
+//            exceeds eps gets its missing children; then every missing parent
+//            of a new element is added until no hole remains;
+//   coarsen  childless elements of positive maximum level whose norm is below
+//            eta are removed, all at once, repeatedly until none qualifies;
+//   table    elements ordered by (instance, Hash::hash_key); fibers of
+//            dimension d ordered by (instance, slots outside d), fiber f of
+//            the union at fbase = f << NMAX, fmaxl the deepest level in it.
+// Element sets, order and fiber tables equal the host build's (checked by
+// Context::check_device_grid).  Elements are identified by a 64-bit key:
+// instance above D fields of NMAX bits, field d = the slot in dimension d.
+//
+// The norm test takes sqrt of the sum of squares; within four ulps of the
+// threshold the reference's pow(s, 0.5) decides, which only the host has, so
+// such a norm makes the call return -1 before anything changes, and the host
+// path runs instead.  A full hash table or new-element list does the same.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <stdexcept>
+
+#include "context.hpp"
+
+namespace adg {
+namespace {
+
+constexpr int kAB = 256;
+inline unsigned ablocks(int64_t n) { return static_cast<unsigned>(std::max<int64_t>((n + kAB - 1) / kAB, 1)); }
+constexpr uint64_t kFree = ~uint64_t{0};
+// closure layers run as grid-stride loops over a list whose length the host
+// does not know
+constexpr unsigned kCloseBlocks = 256;
+
+// counter words: [0] new (refine) or removed (coarsen) elements, [1] flags,
+// [2] [3] the range of the current closure layer in the new-element list
+enum : int32_t { AF_BAND = 1, AF_FULL = 2, AF_DUP = 4, AF_OPEN = 8, AF_NEG = 16 };
+constexpr int kCtr = 4;
+
+__device__ inline int slot_level(int s) { return s == 0 ? 0 : 32 - __clz(s); }
+
+__device__ inline uint64_t amix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ inline int field(uint64_t key, int d, int nb) {
+  return static_cast<int>((key >> (nb * d)) & ((uint64_t{1} << nb) - 1));
+}
+
+__device__ inline uint64_t with_field(uint64_t key, int d, int nb, int v) {
+  const uint64_t m = ((uint64_t{1} << nb) - 1) << (nb * d);
+  return (key & ~m) | (static_cast<uint64_t>(v) << (nb * d));
+}
+
+// insert-if-absent with bounded probing: 1 inserted (slot in *at), 0 present,
+// -1 table full
+__device__ inline int tab_insert(uint64_t* hkey, uint64_t mask, uint64_t key, uint64_t* at) {
+  uint64_t h = amix(key) & mask;
+  for (uint64_t p = 0; p <= mask; ++p) {
+    const unsigned long long prev =
+        atomicCAS(reinterpret_cast<unsigned long long*>(hkey + h), static_cast<unsigned long long>(kFree),
+                  static_cast<unsigned long long>(key));
+    if (prev == kFree) { *at = h; return 1; }
+    if (prev == key) return 0;
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+__device__ inline int32_t tab_find(const uint64_t* hkey, const int32_t* hval, uint64_t mask, uint64_t key) {
+  uint64_t h = amix(key) & mask;
+  for (uint64_t p = 0; p <= mask; ++p) {
+    const uint64_t k = hkey[h];
+    if (k == key) return hval[h];
+    if (k == kFree) return -1;
+    h = (h + 1) & mask;
+  }
+  return -1;
+}
+
+// DGAdapt::indicator_norm's pow(s, 0.5) compared with a threshold: false
+// with r = sqrt(s) outside the band where sqrt and pow may round apart
+__device__ inline bool in_band(double s, double thr, double& r) {
+  r = sqrt(s);
+  const double a = fabs(thr);
+  const double band = 4.0 * (nextafter(a, __builtin_huge_val()) - a);
+  return fabs(r - thr) <= band;
+}
+
+struct AdArgs {
+  const int32_t* slot[kMaxDim];
+  const int32_t* fbase[kMaxDim];
+  const int32_t* fmap[kMaxDim];
+  const int32_t* inst;
+  const double* sumsq;
+  const double* thr;  // per instance
+  uint64_t* hkey;
+  int32_t* hval;
+  uint64_t mask;
+  uint64_t* newkey;
+  int32_t* ctr;
+  int32_t* nchild;
+  int32_t* zmark;
+  int32_t* dec;
+  int64_t E;
+  int D, nb, nmax;
+  int32_t cap_new;
+};
+
+__device__ inline uint64_t elem_key(const AdArgs& a, int64_t e) {
+  uint64_t k = static_cast<uint64_t>(a.inst[e]) << (a.D * a.nb);
+  for (int d = 0; d < a.D; ++d) k |= static_cast<uint64_t>(a.slot[d][e]) << (a.nb * d);
+  return k;
+}
+
+__device__ inline void add_new(const AdArgs& a, uint64_t key) {
+  uint64_t at;
+  const int r = tab_insert(a.hkey, a.mask, key, &at);
+  if (r < 0) { atomicOr(a.ctr + 1, AF_FULL); return; }
+  if (r == 0) return;
+  const int32_t idx = atomicAdd(a.ctr, 1);
+  if (idx < a.cap_new) a.newkey[idx] = key;
+  else atomicOr(a.ctr + 1, AF_FULL);
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_fill64(uint64_t* p, int64_t n, uint64_t v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_fill32(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_fillu32(uint32_t* p, int64_t n, uint32_t v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// the current elements into the hash table (keys are distinct)
+__global__ __launch_bounds__(kAB) void k_ad_table(AdArgs a) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= a.E) return;
+  uint64_t at;
+  const int r = tab_insert(a.hkey, a.mask, elem_key(a, e), &at);
+  if (r == 1) a.hval[at] = static_cast<int32_t>(e);
+  else atomicOr(a.ctr + 1, r == 0 ? AF_DUP : AF_FULL);
+}
+
+// refine: leaves above eps add their missing children.  The leaf test reads
+// the fiber maps of the current grid (children of e in dimension d lie in
+// e's d-fiber), so it sees the grid before this refinement, as the host's
+// does.
+__global__ __launch_bounds__(kAB) void k_ad_children(AdArgs a) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= a.E) return;
+  double r;
+  const double t = a.thr[a.inst[e]];
+  if (in_band(a.sumsq[e], t, r)) { atomicOr(a.ctr + 1, AF_BAND); return; }
+  if (!(r > t)) return;
+  int s[kMaxDim];
+  int total = 0, have = 0;
+  for (int d = 0; d < a.D; ++d) {
+    s[d] = a.slot[d][e];
+    const int l = slot_level(s[d]);
+    const int32_t* fm = a.fmap[d] + a.fbase[d][e];
+    if (l == 0) {
+      total += 1;
+      have += fm[1] >= 0;
+    } else if (l < a.nmax) {
+      total += 2;
+      have += (fm[2 * s[d]] >= 0) + (fm[2 * s[d] + 1] >= 0);
+    }
+  }
+  if (have >= total) return;
+  const uint64_t key = elem_key(a, e);
+  for (int d = 0; d < a.D; ++d) {
+    const int l = slot_level(s[d]);
+    if (l == 0) {
+      add_new(a, with_field(key, d, a.nb, 1));
+    } else if (l < a.nmax) {
+      add_new(a, with_field(key, d, a.nb, 2 * s[d]));
+      add_new(a, with_field(key, d, a.nb, 2 * s[d] + 1));
+    }
+  }
+}
+
+// one layer of the hole closure: the parents of the elements added by the
+// previous layer
+__global__ __launch_bounds__(kAB) void k_ad_close(AdArgs a) {
+  const int32_t b = a.ctr[2], en = a.ctr[3];
+  for (int64_t w = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; w < en;
+       w += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t key = a.newkey[w];
+    for (int d = 0; d < a.D; ++d) {
+      const int s = field(key, d, a.nb);
+      if (s == 0) continue;
+      add_new(a, with_field(key, d, a.nb, s >> 1));
+    }
+  }
+}
+
+__global__ void k_ad_advance(int32_t* ctr, int32_t cap_new) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  ctr[2] = ctr[3];
+  ctr[3] = min(ctr[0], cap_new);
+}
+
+__global__ void k_ad_closed(int32_t* ctr) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (ctr[2] != ctr[3]) atomicOr(ctr + 1, AF_OPEN);
+}
+
+// the union table after a refinement: old elements, then the new list
+struct UnionArgs {
+  const int32_t* slot[kMaxDim];
+  const int32_t* inst;
+  const uint64_t* newkey;
+  const int32_t* lvtab;  // level-vector part of the hash key
+  uint64_t* keys;
+  int32_t* vals;
+  int32_t* ctr;
+  int64_t E_old, N;
+  int D, nb;
+};
+
+__device__ inline void union_elem(const UnionArgs& a, int64_t q, int* s, int& in) {
+  if (q < a.E_old) {
+    for (int d = 0; d < a.D; ++d) s[d] = a.slot[d][q];
+    in = a.inst[q];
+  } else {
+    const uint64_t key = a.newkey[q - a.E_old];
+    for (int d = 0; d < a.D; ++d) s[d] = field(key, d, a.nb);
+    in = static_cast<int>(key >> (a.D * a.nb));
+  }
+}
+
+// sort key (instance, Hash::hash_key): the support part of the key in mixed
+// radix 2^level plus the level-vector part (source/Hash.cpp:55-114)
+__global__ __launch_bounds__(kAB) void k_ad_union_keys(UnionArgs a) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= a.N) return;
+  int s[kMaxDim], in;
+  union_elem(a, q, s, in);
+  int ind1 = 0, code = 0;
+  for (int t = 0; t < a.D; ++t) {
+    const int l = slot_level(s[t]);
+    ind1 = ind1 * (1 << l) + (l == 0 ? 0 : s[t] - (1 << (l - 1)));
+    code = code * 16 + l;
+  }
+  const int h = ind1 + a.lvtab[code];
+  if (h < 0) atomicOr(a.ctr + 1, AF_NEG);
+  a.keys[q] = (static_cast<uint64_t>(in) << 32) | static_cast<uint32_t>(h);
+  a.vals[q] = static_cast<int32_t>(q);
+}
+
+struct OutArgs {
+  int32_t* slot[kMaxDim];
+  int32_t* inst;
+  int32_t* lsum;
+  int32_t* remap;  // old index of every new element, -1 when new
+  int32_t* meta;   // [B] counts, [B * D] deepest levels, then per dimension fibers and largest fiber
+  int B;
+};
+
+__device__ inline void put_elem(const OutArgs& o, int64_t q, const int* s, int in, int32_t old, int D) {
+  int ls = 0;
+  for (int d = 0; d < D; ++d) {
+    o.slot[d][q] = s[d];
+    const int l = slot_level(s[d]);
+    ls += l;
+    atomicMax(o.meta + o.B + in * D + d, l);
+  }
+  o.inst[q] = in;
+  o.lsum[q] = ls;
+  o.remap[q] = old;
+  atomicAdd(o.meta + in, 1);
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_union_gather(UnionArgs a, const uint64_t* skeys, const int32_t* svals,
+                                                         OutArgs o) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= a.N) return;
+  if (q > 0 && skeys[q - 1] == skeys[q]) atomicOr(a.ctr + 1, AF_DUP);
+  const int32_t src = svals[q];
+  int s[kMaxDim], in;
+  union_elem(a, src, s, in);
+  put_elem(o, q, s, in, src < a.E_old ? src : -1, a.D);
+}
+
+// coarsen: children of every element, then removal passes
+__global__ __launch_bounds__(kAB) void k_ad_nchild(AdArgs a) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= a.E) return;
+  const uint64_t key = elem_key(a, e);
+  for (int d = 0; d < a.D; ++d) {
+    const int s = a.slot[d][e];
+    if (s == 0) continue;
+    const int32_t p = tab_find(a.hkey, a.hval, a.mask, with_field(key, d, a.nb, s >> 1));
+    if (p >= 0) atomicAdd(a.nchild + p, 1);
+  }
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_below(AdArgs a) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= a.E) return;
+  bool any = false;
+  for (int d = 0; d < a.D; ++d) any = any || a.slot[d][e] != 0;
+  double r;
+  const double t = a.thr[a.inst[e]];
+  const bool band = in_band(a.sumsq[e], t, r);
+  if (band && any) atomicOr(a.ctr + 1, AF_BAND);
+  a.dec[e] = (any && !band && r < t) ? 1 : 0;
+  a.zmark[e] = 0;
+}
+
+// pass `it` (1, 2, ...): every element alive, childless and below eta is
+// marked from the state after the previous pass
+__global__ __launch_bounds__(kAB) void k_ad_mark(AdArgs a, int it, int last) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= a.E) return;
+  if (a.zmark[e] != 0 || a.nchild[e] != 0 || !a.dec[e]) return;
+  a.zmark[e] = it;
+  atomicAdd(a.ctr, 1);
+  if (last) atomicOr(a.ctr + 1, AF_OPEN);
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_unlink(AdArgs a, int it) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= a.E || a.zmark[e] != it) return;
+  const uint64_t key = elem_key(a, e);
+  for (int d = 0; d < a.D; ++d) {
+    const int s = a.slot[d][e];
+    if (s == 0) continue;
+    const int32_t p = tab_find(a.hkey, a.hval, a.mask, with_field(key, d, a.nb, s >> 1));
+    if (p >= 0) atomicSub(a.nchild + p, 1);
+  }
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_keep(const int32_t* zmark, int32_t* keep, int64_t E) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < E) keep[e] = zmark[e] == 0 ? 1 : 0;
+}
+
+// survivors keep their order (the table stays sorted by instance and key)
+__global__ __launch_bounds__(kAB) void k_ad_compact(AdArgs a, const int32_t* pos, OutArgs o) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= a.E || a.zmark[e] != 0) return;
+  int s[kMaxDim];
+  for (int d = 0; d < a.D; ++d) s[d] = a.slot[d][e];
+  put_elem(o, pos[e], s, a.inst[e], static_cast<int32_t>(e), a.D);
+}
+
+// fiber tables of dimension dd
+struct FibArgs {
+  const int32_t* slot[kMaxDim];
+  const int32_t* inst;
+  uint64_t* keys;
+  int32_t* vals;
+  int64_t N;
+  int D, nb, dd;
+};
+
+__global__ __launch_bounds__(kAB) void k_ad_fib_keys(FibArgs a) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= a.N) return;
+  uint64_t k = static_cast<uint64_t>(a.inst[q]) << (a.D * a.nb);
+  for (int d = 0; d < a.D; ++d)
+    if (d != a.dd) k |= static_cast<uint64_t>(a.slot[d][q]) << (a.nb * d);
+  a.keys[q] = k;
+  a.vals[q] = static_cast<int32_t>(q);
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_run_flags(const uint64_t* skeys, int32_t* flags, int64_t N) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < N) flags[i] = (i == 0 || skeys[i] != skeys[i - 1]) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_runs(const int32_t* svals, const int32_t* run, const int32_t* slot_dd,
+                                                int32_t* runmax, int32_t* runlen, int32_t* fbase, int64_t N, int nmax,
+                                                int32_t* nfib) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int32_t e = svals[i];
+  const int32_t r = run[i] - 1;
+  fbase[e] = r << nmax;
+  atomicMax(runmax + r, slot_level(slot_dd[e]));
+  atomicAdd(runlen + r, 1);
+  if (i == N - 1) *nfib = r + 1;
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_run_levels(const int32_t* svals, const int32_t* run, const int32_t* runmax,
+                                                      const int32_t* runlen, int32_t* fmaxl, int64_t N,
+                                                      int32_t* maxfib) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int32_t r = run[i] - 1;
+  fmaxl[svals[i]] = runmax[r];
+  if (i == 0 || run[i] != run[i - 1]) atomicMax(maxfib, runlen[r]);
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_fmap(int32_t* fmap, uint32_t* fbits, const int32_t* fbase,
+                                                const int32_t* slot, int64_t N) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= N) return;
+  const int b = fbase[e] + slot[e];
+  fmap[b] = static_cast<int32_t>(e);
+  atomicOr(&fbits[b >> 5], 1u << (b & 31));
+}
+
+int bits_for(int64_t n) {  // bits to hold 0..n-1 (at least one)
+  int b = 1;
+  while ((int64_t{1} << b) < n) ++b;
+  return b;
+}
+
+template <class T>
+void to_device(DevBuf<T>& b, const std::vector<T>& h, hipStream_t s) {
+  b.ensure(std::max<size_t>(h.size(), 1));
+  if (!h.empty()) hip_check(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+
+}  // namespace
+
+bool Context::device_adapt_fits(int64_t n) const {
+  if (!device_adapt || refine_mode != 0 || recording || !up_host.empty() || n <= 0 || B() < 1) return false;
+  if (D * nmax + bits_for(B()) > 62) return false;
+  // fiber bases of up to n fibers and the int32 element indices
+  return (n << nmax) < (int64_t{1} << 31) - 1;
+}
+
+// Table of the current grid, per-instance thresholds and the norms (shared
+// set-up of refine and coarsen); returns the kernel arguments.
+static AdArgs adapt_args(Context& c, const std::vector<double>& thr, int32_t cap_new) {
+  const int64_t E = c.E();
+  if (c.ad_lvtab.cap == 0) to_device(c.ad_lvtab, c.hk.level_table(c.D), c.stream);
+  uint64_t tcap = 16;
+  while (tcap < 2 * static_cast<uint64_t>(E + cap_new)) tcap <<= 1;
+  c.ad_hkey.ensure(tcap);
+  c.ad_hval.ensure(tcap);
+  c.ad_newkey.ensure(static_cast<size_t>(std::max<int32_t>(cap_new, 1)));
+  c.ad_ctr.ensure(kCtr);
+  to_device(c.ad_thr, thr, c.stream);
+  c.indicator_sumsq_dev();
+  AdArgs a{};
+  for (int d = 0; d < c.D; ++d) {
+    a.slot[d] = c.g_slot[d].p;
+    a.fbase[d] = c.g_fbase[d].p;
+    a.fmap[d] = c.g_fmap[d].p;
+  }
+  a.inst = c.g_inst.p;
+  a.sumsq = c.sumsq.p;
+  a.thr = c.ad_thr.p;
+  a.hkey = c.ad_hkey.p;
+  a.hval = c.ad_hval.p;
+  a.mask = tcap - 1;
+  a.newkey = c.ad_newkey.p;
+  a.ctr = c.ad_ctr.p;
+  a.E = E;
+  a.D = c.D;
+  a.nb = c.nmax;
+  a.nmax = c.nmax;
+  a.cap_new = cap_new;
+  hipLaunchKernelGGL(k_ad_fill32, dim3(1), dim3(kAB), 0, c.stream, c.ad_ctr.p, int64_t{kCtr}, 0);
+  hipLaunchKernelGGL(k_ad_fill64, dim3(ablocks(static_cast<int64_t>(tcap))), dim3(kAB), 0, c.stream, c.ad_hkey.p,
+                     static_cast<int64_t>(tcap), kFree);
+  hipLaunchKernelGGL(k_ad_table, dim3(ablocks(E)), dim3(kAB), 0, c.stream, a);
+  return a;
+}
+
+static OutArgs out_args(Context& c, int64_t N) {
+  const size_t n = static_cast<size_t>(std::max<int64_t>(N, 1));
+  OutArgs o{};
+  for (int d = 0; d < c.D; ++d) {
+    c.ad_slot[d].ensure(n);
+    o.slot[d] = c.ad_slot[d].p;
+  }
+  c.ad_inst.ensure(n);
+  c.ad_lsum.ensure(n);
+  c.g_remap.ensure(n);
+  const int64_t nmeta = static_cast<int64_t>(c.B()) * (1 + c.D) + 2 * c.D;
+  c.ad_meta.ensure(static_cast<size_t>(nmeta));
+  hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(nmeta)), dim3(kAB), 0, c.stream, c.ad_meta.p, nmeta, 0);
+  o.inst = c.ad_inst.p;
+  o.lsum = c.ad_lsum.p;
+  o.remap = c.g_remap.p;
+  o.meta = c.ad_meta.p;
+  o.B = c.B();
+  return o;
+}
+
+static std::array<int32_t, 2> read_counters(Context& c) {
+  std::array<int32_t, 2> h{0, 0};
+  hip_check(hipMemcpyAsync(h.data(), c.ad_ctr.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream), "download");
+  c.sync();
+  return h;
+}
+
+int64_t Context::refine_device(const std::vector<double>& eps) {
+  const int64_t oldE = E();
+  if (!device_adapt_fits(oldE)) return -1;
+  const int32_t cap_new =
+      static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(2 * oldE, 4096), int64_t{1} << 28));
+  AdArgs a = adapt_args(*this, eps, cap_new);
+  hipLaunchKernelGGL(k_ad_children, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a);
+  hipLaunchKernelGGL(k_ad_advance, dim3(1), dim3(1), 0, stream, a.ctr, cap_new);
+  // every layer lowers the level sum by one: D * NMAX + 1 layers close any hole
+  for (int it = 0; it <= D * nmax; ++it) {
+    hipLaunchKernelGGL(k_ad_close, dim3(kCloseBlocks), dim3(kAB), 0, stream, a);
+    hipLaunchKernelGGL(k_ad_advance, dim3(1), dim3(1), 0, stream, a.ctr, cap_new);
+  }
+  hipLaunchKernelGGL(k_ad_closed, dim3(1), dim3(1), 0, stream, a.ctr);
+  const std::array<int32_t, 2> h = read_counters(*this);
+  if (h[1] & (AF_BAND | AF_FULL | AF_DUP | AF_OPEN) || h[0] > cap_new || !device_adapt_fits(oldE + h[0])) {
+    ++stat_dev_fallback;
+    return -1;
+  }
+  ++stat_dev_adapt;
+  if (h[0] == 0) return 0;
+  HostTimer tg(*this, HOST_GRID);
+  const int64_t N = oldE + h[0];
+  for (int i = 0; i < 2; ++i) {
+    ad_keys[i].ensure(static_cast<size_t>(N));
+    ad_vals[i].ensure(static_cast<size_t>(N));
+  }
+  UnionArgs ua{};
+  for (int d = 0; d < D; ++d) ua.slot[d] = g_slot[d].p;
+  ua.inst = g_inst.p;
+  ua.newkey = ad_newkey.p;
+  ua.lvtab = ad_lvtab.p;
+  ua.keys = ad_keys[0].p;
+  ua.vals = ad_vals[0].p;
+  ua.ctr = ad_ctr.p;
+  ua.E_old = oldE;
+  ua.N = N;
+  ua.D = D;
+  ua.nb = nmax;
+  hipLaunchKernelGGL(k_ad_union_keys, dim3(ablocks(N)), dim3(kAB), 0, stream, ua);
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p, ad_vals[1].p,
+                                           static_cast<int>(N), 0, 32 + bits_for(B()), stream);
+  ad_tmp.ensure(std::max<size_t>(bytes, 1));
+  hip_check(hipcub::DeviceRadixSort::SortPairs(ad_tmp.p, bytes, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p,
+                                               ad_vals[1].p, static_cast<int>(N), 0, 32 + bits_for(B()), stream),
+            "sort (element keys)");
+  const OutArgs o = out_args(*this, N);
+  hipLaunchKernelGGL(k_ad_union_gather, dim3(ablocks(N)), dim3(kAB), 0, stream, ua, ad_keys[1].p, ad_vals[1].p, o);
+  device_grid_finish(N, oldE);
+  return h[0];
+}
+
+int64_t Context::coarsen_device(const std::vector<double>& eta) {
+  const int64_t oldE = E();
+  if (!device_adapt_fits(oldE)) return -1;
+  AdArgs a = adapt_args(*this, eta, 0);
+  ad_nchild.ensure(static_cast<size_t>(oldE));
+  ad_zmark.ensure(static_cast<size_t>(oldE));
+  ad_dec.ensure(static_cast<size_t>(oldE));
+  a.nchild = ad_nchild.p;
+  a.zmark = ad_zmark.p;
+  a.dec = ad_dec.p;
+  hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(oldE)), dim3(kAB), 0, stream, ad_nchild.p, oldE, 0);
+  hipLaunchKernelGGL(k_ad_nchild, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a);
+  hipLaunchKernelGGL(k_ad_below, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a);
+  // an element removed in pass t has a chain of t - 1 removed descendants
+  // below it, so passes 1 .. D * NMAX remove everything and pass D * NMAX + 1
+  // finds nothing
+  const int passes = D * nmax + 1;
+  for (int it = 1; it <= passes; ++it) {
+    hipLaunchKernelGGL(k_ad_mark, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, it, it == passes ? 1 : 0);
+    if (it < passes) hipLaunchKernelGGL(k_ad_unlink, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, it);
+  }
+  const std::array<int32_t, 2> h = read_counters(*this);
+  if (h[1] & (AF_BAND | AF_FULL | AF_DUP | AF_OPEN) || h[0] >= oldE) {
+    ++stat_dev_fallback;
+    return -1;
+  }
+  ++stat_dev_adapt;
+  if (h[0] == 0) return 0;
+  HostTimer tg(*this, HOST_GRID);
+  const int64_t N = oldE - h[0];
+  ad_vals[0].ensure(static_cast<size_t>(oldE));
+  ad_vals[1].ensure(static_cast<size_t>(oldE));
+  hipLaunchKernelGGL(k_ad_keep, dim3(ablocks(oldE)), dim3(kAB), 0, stream, ad_zmark.p, ad_vals[0].p, oldE);
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ad_vals[0].p, ad_vals[1].p, static_cast<int>(oldE), stream);
+  ad_tmp.ensure(std::max<size_t>(bytes, 1));
+  hip_check(hipcub::DeviceScan::ExclusiveSum(ad_tmp.p, bytes, ad_vals[0].p, ad_vals[1].p, static_cast<int>(oldE),
+                                             stream),
+            "scan (survivors)");
+  const OutArgs o = out_args(*this, N);
+  hipLaunchKernelGGL(k_ad_compact, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, ad_vals[1].p, o);
+  device_grid_finish(N, oldE);
+  return h[0];
+}
+
+void Context::device_grid_finish(int64_t N, int64_t oldE) {
+  // fiber tables of every dimension: sort by (instance, slots outside d),
+  // runs of equal keys are the fibers
+  const int B_ = B();
+  int32_t* meta_nfib = ad_meta.p + static_cast<int64_t>(B_) * (1 + D);
+  int32_t* meta_maxfib = meta_nfib + D;
+  for (int i = 0; i < 2; ++i) {
+    ad_keys[i].ensure(static_cast<size_t>(N));
+    ad_vals[i].ensure(static_cast<size_t>(N));
+  }
+  ad_run.ensure(static_cast<size_t>(N));
+  ad_runmax.ensure(static_cast<size_t>(N));
+  ad_runlen.ensure(static_cast<size_t>(N));
+  const int endbit = D * nmax + bits_for(B_);
+  size_t sbytes = 0, cbytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sbytes, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p, ad_vals[1].p,
+                                           static_cast<int>(N), 0, endbit, stream);
+  (void)hipcub::DeviceScan::InclusiveSum(nullptr, cbytes, ad_run.p, ad_run.p, static_cast<int>(N), stream);
+  ad_tmp.ensure(std::max<size_t>(std::max(sbytes, cbytes), 1));
+  for (int dd = 0; dd < D; ++dd) {
+    FibArgs f{};
+    for (int d = 0; d < D; ++d) f.slot[d] = ad_slot[d].p;
+    f.inst = ad_inst.p;
+    f.keys = ad_keys[0].p;
+    f.vals = ad_vals[0].p;
+    f.N = N;
+    f.D = D;
+    f.nb = nmax;
+    f.dd = dd;
+    hipLaunchKernelGGL(k_ad_fib_keys, dim3(ablocks(N)), dim3(kAB), 0, stream, f);
+    size_t b1 = sbytes;
+    hip_check(hipcub::DeviceRadixSort::SortPairs(ad_tmp.p, b1, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p,
+                                                 ad_vals[1].p, static_cast<int>(N), 0, endbit, stream),
+              "sort (fiber keys)");
+    hipLaunchKernelGGL(k_ad_run_flags, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_keys[1].p, ad_vals[0].p, N);
+    size_t b2 = cbytes;
+    hip_check(hipcub::DeviceScan::InclusiveSum(ad_tmp.p, b2, ad_vals[0].p, ad_run.p, static_cast<int>(N), stream),
+              "scan (fiber runs)");
+    hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_runmax.p, N, 0);
+    hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_runlen.p, N, 0);
+    ad_fbase[dd].ensure(static_cast<size_t>(N));
+    ad_fmaxl[dd].ensure(static_cast<size_t>(N));
+    hipLaunchKernelGGL(k_ad_runs, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_vals[1].p, ad_run.p, ad_slot[dd].p,
+                       ad_runmax.p, ad_runlen.p, ad_fbase[dd].p, N, nmax, meta_nfib + dd);
+    hipLaunchKernelGGL(k_ad_run_levels, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_vals[1].p, ad_run.p, ad_runmax.p,
+                       ad_runlen.p, ad_fmaxl[dd].p, N, meta_maxfib + dd);
+  }
+  // counts per instance, deepest levels, fibers and largest fiber per dimension
+  const int64_t nmeta = static_cast<int64_t>(B_) * (1 + D) + 2 * D;
+  std::vector<int32_t> meta(static_cast<size_t>(nmeta));
+  std::array<int32_t, 2> h{0, 0};
+  hip_check(hipMemcpyAsync(meta.data(), ad_meta.p, nmeta * sizeof(int32_t), hipMemcpyDeviceToHost, stream),
+            "download");
+  hip_check(hipMemcpyAsync(h.data(), ad_ctr.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, stream), "download");
+  sync();
+  if (h[1] & AF_DUP) throw std::runtime_error("grid: duplicate hash key");
+  if (h[1] & AF_NEG) throw std::runtime_error("grid: negative hash key");
+  std::vector<int64_t> noff(B_ + 1, 0);
+  for (int i = 0; i < B_; ++i) noff[i + 1] = noff[i] + meta[i];
+  if (noff[B_] != N) throw std::logic_error("device refine/coarsen: instance counts do not add up");
+  // commit: fiber maps of the new table, then swap it in
+  for (int d = 0; d < D; ++d) {
+    const int64_t nfib = meta[static_cast<size_t>(B_) * (1 + D) + d];
+    const int64_t nmap = std::max<int64_t>(nfib << nmax, 1);
+    g_fmap[d].ensure(static_cast<size_t>(nmap));
+    hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(nmap)), dim3(kAB), 0, stream, g_fmap[d].p, nmap, -1);
+    const int64_t nwords = (nmap + 31) / 32;
+    g_fbits[d].ensure(static_cast<size_t>(nwords + 1));
+    hipLaunchKernelGGL(k_ad_fillu32, dim3(ablocks(nwords + 1)), dim3(kAB), 0, stream, g_fbits[d].p, nwords + 1, 0u);
+    fbits_words[d] = nwords;
+    hipLaunchKernelGGL(k_ad_fmap, dim3(ablocks(N)), dim3(kAB), 0, stream, g_fmap[d].p, g_fbits[d].p, ad_fbase[d].p,
+                       ad_slot[d].p, N);
+    g_slot[d].swap(ad_slot[d]);
+    g_fbase[d].swap(ad_fbase[d]);
+    g_fmaxl[d].swap(ad_fmaxl[d]);
+  }
+  g_inst.swap(ad_inst);
+  g_lsum.swap(ad_lsum);
+  maxfib = 1;
+  for (int d = 0; d < D; ++d) maxfib = std::max(maxfib, meta[static_cast<size_t>(B_) * (1 + D) + D + d]);
+  inst_lmax.assign(meta.begin() + B_, meta.begin() + static_cast<int64_t>(B_) * (1 + D));
+  off = noff;
+  remap_field_dev(u, K, N, oldE);
+  remap_field_dev(pred, K, N, oldE);
+  ++grid_version;
+  host_stale = true;
+  ensure_fields();
+  if (is_varcoef(flux_kind)) compute_vcoef();
+}
+
+void Context::ensure_host_grid() {
+  if (!host_stale) return;
+  const int64_t n = E();
+  std::vector<std::vector<int32_t>> dsl(D, std::vector<int32_t>(static_cast<size_t>(n)));
+  for (int d = 0; d < D; ++d)
+    if (n > 0)
+      hip_check(hipMemcpyAsync(dsl[d].data(), g_slot[d].p, n * sizeof(int32_t), hipMemcpyDeviceToHost, stream),
+                "download");
+  sync();
+  std::vector<HostGrid> ng(B());
+  parallel_for(B(), [&](int i) {
+    std::vector<int32_t> s(static_cast<size_t>(off[i + 1] - off[i]) * D);
+    for (int64_t e = off[i]; e < off[i + 1]; ++e)
+      for (int d = 0; d < D; ++d) s[(e - off[i]) * D + d] = dsl[d][e];
+    ng[i].build(D, nmax, s, hk);
+  });
+  grids = std::move(ng);
+  host_union();
+  host_stale = false;
+}
+
+int64_t Context::check_device_grid() {
+  ensure_host_grid();
+  const int64_t n = E();
+  int64_t bad = 0;
+  auto down = [&](const DevBuf<int32_t>& b, int64_t cnt) {
+    std::vector<int32_t> h(static_cast<size_t>(cnt));
+    if (cnt > 0)
+      hip_check(hipMemcpyAsync(h.data(), b.p, cnt * sizeof(int32_t), hipMemcpyDeviceToHost, stream), "download");
+    sync();
+    return h;
+  };
+  std::vector<int64_t> fib0(B() + 1, 0);
+  const std::vector<int32_t> inst = down(g_inst, n), lsum = down(g_lsum, n);
+  for (int64_t e = 0; e < n; ++e) {
+    bad += inst[e] != uinst[e];
+    int ls = 0;
+    for (int d = 0; d < D; ++d) ls += level(e, d);
+    bad += lsum[e] != ls;
+  }
+  int mf = 1;
+  for (int d = 0; d < D; ++d) {
+    for (int i = 0; i < B(); ++i) fib0[i + 1] = fib0[i] + grids[i].nfib[d];
+    const std::vector<int32_t> sl = down(g_slot[d], n), fb = down(g_fbase[d], n), fm = down(g_fmaxl[d], n);
+    const int64_t nmap = std::max<int64_t>(fib0[B()] << nmax, 1);
+    bad += fbits_words[d] != (nmap + 31) / 32;
+    const std::vector<int32_t> map = down(g_fmap[d], nmap);
+    std::vector<int32_t> want(static_cast<size_t>(nmap), -1);
+    for (int i = 0; i < B(); ++i) {
+      const HostGrid& g = grids[i];
+      for (int64_t e = 0; e < g.E; ++e) {
+        const int64_t u_ = off[i] + e;
+        const int32_t f = g.fbase[d][e] + static_cast<int32_t>(fib0[i] << nmax);
+        bad += sl[u_] != g.slot[e * D + d];
+        bad += fb[u_] != f;
+        bad += fm[u_] != g.fmaxl[d][e];
+        want[static_cast<size_t>(f) + g.slot[e * D + d]] = static_cast<int32_t>(u_);
+      }
+      mf = std::max(mf, g.maxfib[d]);
+    }
+    for (int64_t q = 0; q < nmap; ++q) bad += map[q] != want[q];
+  }
+  bad += maxfib != mf;
+  return bad;
+}
+
+}  // namespace adg

@@ -23,6 +23,8 @@ template <class F>
 int guard(adg_ctx* h, F&& f) {
   if (!h || !h->c) return ADG_E_ARG;
   try {
+    // a device refine / coarsen leaves the host copy of the tables stale
+    h->c->ensure_host_grid();
     f(*h->c);
     return ADG_OK;
   } catch (const adg::HipError& e) {
@@ -391,6 +393,27 @@ int adg_proposer_stats(adg_ctx* h, int64_t* proposed, int64_t* redone, int64_t* 
     if (proposed) *proposed = c.stat_proposed;
     if (redone) *redone = c.stat_redo;
     if (deferred) *deferred = c.stat_deferred;
+  });
+}
+
+int adg_set_device_adapt(adg_ctx* h, int on) {
+  return guard(h, [&](adg::Context& c) {
+    if (on != 0 && on != 1) throw std::runtime_error("device adapt must be 0 or 1");
+    c.device_adapt = on != 0;
+  });
+}
+
+int adg_device_adapt_stats(adg_ctx* h, int64_t* device_calls, int64_t* host_fallbacks) {
+  return guard(h, [&](adg::Context& c) {
+    if (device_calls) *device_calls = c.stat_dev_adapt;
+    if (host_fallbacks) *host_fallbacks = c.stat_dev_fallback;
+  });
+}
+
+int adg_grid_check(adg_ctx* h, int64_t* mismatches) {
+  return guard(h, [&](adg::Context& c) {
+    if (!mismatches) throw std::runtime_error("grid check: null output");
+    *mismatches = c.check_device_grid();
   });
 }
 
@@ -791,8 +814,7 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
         if (!active[i]) continue;
         double sum_c_dx = 0.;
         for (int d = 0; d < c.D; ++d) {
-          int lmax = 0;
-          for (int64_t e = c.off[i]; e < c.off[i + 1]; ++e) lmax = std::max(lmax, c.level(e, d));
+          const int lmax = c.inst_lmax[static_cast<size_t>(i) * c.D + d];
           const double ws = c.inst_wave.empty() ? p->wave_speed[d] : c.inst_wave[static_cast<size_t>(i) * c.D + d];
           sum_c_dx += std::abs(ws) * std::pow(2., lmax);
         }
@@ -858,6 +880,7 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
           adg::hip_check(hipMemcpyAsync(uall.data(), c.u.p, uall.size() * sizeof(double), hipMemcpyDeviceToHost,
                                         c.stream), "download");
         c.sync();
+        c.ensure_host_grid();
         std::vector<std::vector<int32_t>> ns(nb);
         for (int i = 0; i < nb; ++i) {
           if (!active[i] && !stashed[i]) {
@@ -880,6 +903,7 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
         adg::hip_check(hipMemcpyAsync(uall.data(), c.u.p, uall.size() * sizeof(double), hipMemcpyDeviceToHost,
                                       c.stream), "download");
       c.sync();
+      c.ensure_host_grid();
       std::vector<std::vector<int32_t>> ns(nb);
       std::vector<double> unew;
       for (int i = 0; i < nb; ++i) {

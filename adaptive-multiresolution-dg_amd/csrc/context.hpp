@@ -210,11 +210,37 @@ class Context {
   void copy_to_predict();
   void copy_predict_to_ucoe();
   void indicator_sumsq(double* host_out);
+  void indicator_sumsq_dev();  // sums of squares of every element into sumsq
   int64_t refine(double eps);
   int64_t coarsen(double eta);
   // one threshold per instance
   int64_t refine_instances(const std::vector<double>& eps);
   int64_t coarsen_instances(const std::vector<double>& eta);
+  // Refine / coarsen on the device (adapt_dev.hip): threshold tests, child
+  // generation, hole closure, hash keys, the sorted union table and its fiber
+  // tables are computed on the GPU and the host reads back counts only.  The
+  // host copy of the tables (grids, uslot, uhash, uinst) is then stale until
+  // ensure_host_grid() rebuilds it from the device slots.  A call returns -1
+  // with nothing changed when the host has to decide instead: a norm within
+  // a few ulps of its threshold (where the reference's pow decides), a full
+  // table, or sizes whose keys do not pack into 63 bits.
+  bool device_adapt = false;
+  bool host_stale = false;
+  int64_t stat_dev_adapt = 0, stat_dev_fallback = 0;
+  int64_t refine_device(const std::vector<double>& eps);
+  int64_t coarsen_device(const std::vector<double>& eta);
+  void ensure_host_grid();
+  // entries of the device tables (slots, instances, level sums, fiber bases,
+  // fiber levels, fiber maps) that differ from a host build of the same grid
+  int64_t check_device_grid();
+  std::vector<int32_t> inst_lmax;  // [B * D] deepest level per instance and dimension
+  void host_union();               // uslot, uhash, uinst and inst_lmax from grids
+  DevBuf<uint64_t> ad_hkey, ad_newkey, ad_keys[2];
+  DevBuf<int32_t> ad_hval, ad_ctr, ad_vals[2], ad_run, ad_runmax, ad_runlen, ad_meta, ad_nchild, ad_zmark, ad_dec,
+      ad_lvtab;
+  DevBuf<int32_t> ad_slot[kMaxDim], ad_fbase[kMaxDim], ad_fmaxl[kMaxDim], ad_inst, ad_lsum;
+  DevBuf<double> ad_thr;
+  DevBuf<uint8_t> ad_tmp;
   void init_adaptive_intp(double (*f)(const double*, int, void*), void* user, double eps);
   void init_adaptive_intp_batch(double (*f)(const double*, int, void*), void* const* users, double eps);
   void init_separable(double (*f)(double, int, int, void*), void* user, int nfunc, double eps);
@@ -431,6 +457,12 @@ class Context {
 
  private:
   void remap_field(DevBuf<double>& f, int per_elem, const std::vector<int32_t>& old_of_new, int64_t old_E);
+  // the same with old_of_new already in g_remap (new_E entries)
+  void remap_field_dev(DevBuf<double>& f, int per_elem, int64_t new_E, int64_t old_E);
+  // second half of a device refine / coarsen: fiber tables of the new union
+  // table in ad_*, read-back of counts, fiber maps, swap into g_*, field remap
+  void device_grid_finish(int64_t N, int64_t old_E);
+  bool device_adapt_fits(int64_t n) const;
   void surplus_transform(const double* in, double* out, double* scratch_a, double* scratch_b, int ncomp);
   void surplus_transform32(const float* in, float* out, float* scratch_a, float* scratch_b, int ncomp);
   void compute_vcoef();

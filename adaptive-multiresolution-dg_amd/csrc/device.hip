@@ -699,20 +699,32 @@ SweepTable Context::table(int id) {
   throw std::runtime_error("unknown table id");
 }
 
-void Context::upload_grid() {
-  HostTimer timer(*this, HOST_UPLOAD);
-  // union of the instance tables: element indices and fiber ids are shifted by
-  // the instance offsets, so fibers stay within their instance
+void Context::host_union() {
   const int64_t n = E();
   uslot.resize(n * D);
   uhash.resize(n);
   uinst.resize(n);
-  std::vector<int32_t> dsl(n), fb(n), fml(n), ls(n, 0);
+  inst_lmax.assign(static_cast<size_t>(B()) * D, 0);
   parallel_for(B(), [&](int i) {
     const HostGrid& g = grids[i];
     std::copy(g.slot.begin(), g.slot.end(), uslot.begin() + off[i] * D);
     std::copy(g.hash.begin(), g.hash.end(), uhash.begin() + off[i]);
     std::fill(uinst.begin() + off[i], uinst.begin() + off[i + 1], i);
+    for (int64_t e = 0; e < g.E; ++e)
+      for (int d = 0; d < D; ++d)
+        inst_lmax[static_cast<size_t>(i) * D + d] = std::max(inst_lmax[static_cast<size_t>(i) * D + d], g.level(e, d));
+  });
+}
+
+void Context::upload_grid() {
+  HostTimer timer(*this, HOST_UPLOAD);
+  // union of the instance tables: element indices and fiber ids are shifted by
+  // the instance offsets, so fibers stay within their instance
+  const int64_t n = E();
+  host_union();
+  std::vector<int32_t> dsl(n), fb(n), fml(n), ls(n, 0);
+  parallel_for(B(), [&](int i) {
+    const HostGrid& g = grids[i];
     for (int64_t e = 0; e < g.E; ++e)
       for (int d = 0; d < D; ++d) ls[off[i] + e] += level_of_slot(g.slot[e * D + d]);
   });
@@ -768,7 +780,10 @@ void Context::ensure_fields() {
 }
 
 void Context::remap_field(DevBuf<double>& f, int per_elem, const std::vector<int32_t>& old_of_new, int64_t old_E) {
-  const int64_t newE = static_cast<int64_t>(old_of_new.size());
+  remap_field_dev(f, per_elem, static_cast<int64_t>(old_of_new.size()), old_E);
+}
+
+void Context::remap_field_dev(DevBuf<double>& f, int per_elem, int64_t newE, int64_t old_E) {
   remap_tmp.ensure(std::max<int64_t>(old_E * per_elem, 1));
   if (old_E > 0)
     hipLaunchKernelGGL(k_copy, dim3(nblocks(old_E * per_elem)), dim3(kBlock), 0, stream, remap_tmp.p, f.p,
@@ -785,6 +800,7 @@ void Context::set_grid_slots(const std::vector<int32_t>& slots, bool keep_fields
 
 void Context::set_grids(const std::vector<std::vector<int32_t>>& slots, bool keep_fields,
                         const std::vector<char>* changed) {
+  ensure_host_grid();
   HostTimer timer(*this, HOST_GRID);
   const int nb = static_cast<int>(slots.size());
   // instances whose element set did not change keep their table
@@ -1801,6 +1817,7 @@ void Context::rk_step(const StageProblem& p, int scheme, int nstage, double dt) 
 
 void Context::save_state(int slot) {
   if (slot < 0 || slot >= 4) throw std::runtime_error("snapshot slot must be 0..3");
+  ensure_host_grid();
   Snapshot& sn = snaps[slot];
   sn.slots.clear();
   for (const HostGrid& g : grids) sn.slots.push_back(g.slot);
@@ -1813,6 +1830,7 @@ void Context::restore_state(int slot) {
   if (slot < 0 || slot >= 4) throw std::runtime_error("snapshot slot must be 0..3");
   Snapshot& sn = snaps[slot];
   if (sn.slots.empty()) throw std::runtime_error("snapshot slot is empty");
+  ensure_host_grid();
   bool same = static_cast<int>(sn.slots.size()) == B();
   for (int i = 0; same && i < B(); ++i) same = (sn.slots[i] == grids[i].slot);
   if (!same) set_grids(sn.slots, false);
@@ -1833,9 +1851,13 @@ void Context::check_error_flag() {
   }
 }
 
+void Context::indicator_sumsq_dev() {
+  if (E() > 0) hipLaunchKernelGGL(k_sumsq, dim3(nblocks(E())), dim3(kBlock), 0, stream, u.p, sumsq.p, E(), K);
+}
+
 void Context::indicator_sumsq(double* host_out) {
   if (E() == 0) return;
-  hipLaunchKernelGGL(k_sumsq, dim3(nblocks(E())), dim3(kBlock), 0, stream, u.p, sumsq.p, E(), K);
+  indicator_sumsq_dev();
   hip_check(hipMemcpyAsync(host_out, sumsq.p, E() * sizeof(double), hipMemcpyDeviceToHost, stream), "download");
   sync();
 }
@@ -1847,6 +1869,11 @@ int64_t Context::coarsen(double eta) { return coarsen_instances(std::vector<doub
 int64_t Context::refine_instances(const std::vector<double>& eps) {
   HostTimer timer(*this, HOST_ADAPT);
   if (static_cast<int>(eps.size()) != B()) throw std::runtime_error("refine: one threshold per instance expected");
+  if (device_adapt) {
+    const int64_t r = refine_device(eps);
+    if (r >= 0) return r;
+  }
+  ensure_host_grid();
   std::vector<double> s(E());
   indicator_sumsq(s.data());
   std::vector<std::vector<int32_t>> ns(B());
@@ -1866,6 +1893,11 @@ int64_t Context::refine_instances(const std::vector<double>& eps) {
 int64_t Context::coarsen_instances(const std::vector<double>& eta) {
   HostTimer timer(*this, HOST_ADAPT);
   if (static_cast<int>(eta.size()) != B()) throw std::runtime_error("coarsen: one threshold per instance expected");
+  if (device_adapt) {
+    const int64_t r = coarsen_device(eta);
+    if (r >= 0) return r;
+  }
+  ensure_host_grid();
   std::vector<double> s(E());
   indicator_sumsq(s.data());
   std::vector<std::vector<int32_t>> ns(B());
@@ -1888,6 +1920,7 @@ void Context::init_adaptive_intp(double (*f)(const double*, int, void*), void* u
 }
 
 void Context::init_adaptive_intp_batch(double (*f)(const double*, int, void*), void* const* users, double eps) {
+  ensure_host_grid();
   // initialisation is never restricted to a coarse grid
   struct CapGuard {
     int& c; int saved;
@@ -1962,6 +1995,7 @@ void Context::init_adaptive_intp_batch(double (*f)(const double*, int, void*), v
 // source/FastMultiplyLU.cpp:1617-1620; for the Hermite family
 // HermInterpolation::init_from_Herm_to_alpt, source/Interplation.cpp:5019-5030).
 void Context::init_from_points(const double* values) {
+  ensure_host_grid();
   std::vector<double> h(values, values + static_cast<size_t>(E()) * P);
   upload(up, h, stream);
   ensure_fields();
@@ -1997,6 +2031,7 @@ void Context::eval_der_up(int d0) {
 }
 
 int64_t Context::viscosity_elements(double kappa) {
+  ensure_host_grid();
   std::vector<double> s(E());
   indicator_sumsq(s.data());
   visc_elems.clear();
@@ -2034,6 +2069,7 @@ int64_t Context::viscosity_elements(double kappa) {
 }
 
 void Context::viscosity_flux(double coef, bool overlap) {
+  ensure_host_grid();
   LagrangeScope lagrange_scope(*this);
   const int p1 = m + 1;
   std::vector<double> w(static_cast<size_t>(E()) * P, 0.);
@@ -2103,6 +2139,7 @@ void Context::viscosity_flux(double coef, bool overlap) {
 // coefficient norm exceeds eps with separably initialised children
 // (refine_init_separable_scalar_sum, :850-894) until no leaf does.
 void Context::init_separable(double (*f)(double, int, int, void*), void* user, int nfunc, double eps) {
+  ensure_host_grid();
   if (B() != 1) throw std::runtime_error("separable initialisation: one instance expected");
   if (nfunc < 1) throw std::runtime_error("separable initialisation: at least one function expected");
   // DGSolution::seperable_project (source/DGSolution.cpp:150-167)
@@ -2156,6 +2193,7 @@ void Context::init_separable(double (*f)(double, int, int, void*), void* user, i
 // with level sum >= start is scaled by exp(-c (sum_d a_d 2^l_d)^2),
 // c = damp * dt^2/2 * pi^2; speed[e*D+d] is a_d for element e.
 void Context::filter(double damp, const double* speed, double dt, int start) {
+  ensure_host_grid();
   const double coefficient = damp * (dt * dt / 2.0) * (M_PI * M_PI);
   std::vector<double> fac(E(), 1.);
   std::vector<char> on(E(), 0);
@@ -2177,6 +2215,7 @@ void Context::filter(double damp, const double* speed, double dt, int start) {
 }
 
 void Context::eval_tensor(int nx, const double* x1d, double* out_host) {
+  ensure_host_grid();
   if (B() != 1) throw std::runtime_error("point evaluation: one instance expected");
   if (nx < 1) throw std::runtime_error("point evaluation: empty coordinate set");
   int64_t npts = 1;

@@ -305,6 +305,7 @@ inline bool meet1(int32_t sa, int32_t sb) {
 // ArtificialViscosity::assemble_matrix(coef, option) for the viscosity
 // elements of the last update, every instance.
 int64_t Context::visc_operator_assemble(double coef, int option) {
+  ensure_host_grid();
   if (option != 1 && option != 2) throw std::runtime_error("viscosity operator option must be 1 or 2");
   const int L = nmax + 1, kp = k + 1;
   const int64_t nE = E();

@@ -465,6 +465,7 @@ bool host_leaf(const HostGrid& g, int64_t e) {
 
 int64_t Context::proposer_step(const std::function<void(int, int, double)>& stage, const std::vector<double>& eps,
                                double* dt_inst) {
+  ensure_host_grid();
   const int nb = B();
   copy_to_predict();  // u^n
   // The judge holds the step to the rule of the prediction it replaces: every

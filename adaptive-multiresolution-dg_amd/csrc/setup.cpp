@@ -679,13 +679,8 @@ static int hk_pow(int x, int p) {
   return x * tmp * tmp;
 }
 
-int HashKey::key(const int* l, const int* j, int d) const {
-  if (d < 1 || d > 4) return key_direct(l, j, d);
-  int code = 0;
-  for (int t = 0; t < d; ++t) {
-    if (l[t] < 0 || l[t] > 15) return key_direct(l, j, d);
-    code = code * 16 + l[t];
-  }
+const std::vector<int>& HashKey::level_table(int d) const {
+  if (d < 1 || d > 4) throw std::runtime_error("hash key table: dimension must be 1..4");
   std::call_once(level_once_[d], [&]() {
     std::vector<int>& tab = level_part_[d];
     tab.assign(static_cast<size_t>(1) << (4 * d), 0);
@@ -697,9 +692,20 @@ int HashKey::key(const int* l, const int* j, int d) const {
       tab[c] = key_direct(lv.data(), ones.data(), d);
     }
   });
+  return level_part_[d];
+}
+
+int HashKey::key(const int* l, const int* j, int d) const {
+  if (d < 1 || d > 4) return key_direct(l, j, d);
+  int code = 0;
+  for (int t = 0; t < d; ++t) {
+    if (l[t] < 0 || l[t] > 15) return key_direct(l, j, d);
+    code = code * 16 + l[t];
+  }
+  const std::vector<int>& tab = level_table(d);
   int ind1 = 0;
   for (int t = 0; t < d; ++t) ind1 = ind1 * hk_pow(2, l[t]) + (j[t] - 1) / 2;
-  return ind1 + level_part_[d][code];
+  return ind1 + tab[code];
 }
 
 int HashKey::key_direct(const int* l, const int* j, int d) const {

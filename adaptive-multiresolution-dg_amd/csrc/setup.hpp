@@ -126,6 +126,10 @@ class HashKey {
  public:
   HashKey();
   int key(const int* level, const int* suppt, int dim) const;
+  // the level-vector part of the key for dimension dim (1..4): entry
+  // sum_t level[t] 16^(dim-1-t); the key is this part plus the support index
+  // sum over t of ((suppt[t] - 1) / 2) times the product of 2^level[s], s > t
+  const std::vector<int>& level_table(int dim) const;
  private:
   // the reference's arithmetic, term by term
   int key_direct(const int* level, const int* suppt, int dim) const;

@@ -119,6 +119,9 @@ def lib():
             "adg_indicator_sumsq": (i32, [vp, DP]),
             "adg_refine": (i32, [vp, dbl, P(i64)]),
             "adg_coarsen": (i32, [vp, dbl, P(i64)]),
+            "adg_set_device_adapt": (i32, [vp, i32]),
+            "adg_device_adapt_stats": (i32, [vp, P(i64), P(i64)]),
+            "adg_grid_check": (i32, [vp, P(i64)]),
             "adg_init_adaptive_intp": (i32, [vp, POINT_FN, vp, dbl]),
             "adg_ic_eval": (dbl, [DP, i32, vp]),
             "adg_run_linear": (i32, [vp, DP, dbl, i64, P(RunStats)]),
@@ -466,6 +469,22 @@ class Solution:
     def coarsen(self, eta):
         n = ctypes.c_int64()
         self._check(lib().adg_coarsen(self._h, eta, ctypes.byref(n)))
+        return n.value
+
+    def set_device_adapt(self, on):
+        """1: refine / coarsen and the element-table rebuild run on the GPU; 0: on the host."""
+        self._check(lib().adg_set_device_adapt(self._h, int(on)))
+
+    def device_adapt_stats(self):
+        """(refine / coarsen calls done on the device, calls handed to the host)."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(lib().adg_device_adapt_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def grid_check(self):
+        """Entries of the device element and fiber tables that differ from a host build."""
+        n = ctypes.c_int64()
+        self._check(lib().adg_grid_check(self._h, ctypes.byref(n)))
         return n.value
 
     def init_adaptive_intp(self, func, eps):

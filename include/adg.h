@@ -196,6 +196,23 @@ int adg_indicator_sumsq(adg_ctx* c, double* out);
  * New elements start with zero coefficients. */
 int adg_refine(adg_ctx* c, double eps, int64_t* n_added);
 int adg_coarsen(adg_ctx* c, double eta, int64_t* n_removed);
+/* Where refine and coarsen run (adg_refine, adg_coarsen and the adaptive steps
+ * of adg_run_hyperbolic with ADG_REFINE_PREDICT): on = 0 (default) the host
+ * decides from device-computed norms and rebuilds the element table
+ * (DGAdapt::refine / coarsen and the Hash-keyed element map of DGSolution,
+ * source/DGAdapt.cpp:371-404,675-688,982-1385, source/DGSolution.cpp:10-57);
+ * on = 1 the threshold tests, children, hole closure, hash keys, the sorted
+ * element table and its fiber tables are computed on the GPU and the host
+ * reads back counts only.  Both give the same element sets, order and
+ * coefficients.  A call whose norm lies within four ulps of its threshold
+ * (where the reference's pow decides) runs on the host instead.
+ * adg_device_adapt_stats: calls done on the device, calls handed to the host. */
+int adg_set_device_adapt(adg_ctx* c, int on);
+int adg_device_adapt_stats(adg_ctx* c, int64_t* device_calls, int64_t* host_fallbacks);
+/* Consistency check of the device element tables (no reference counterpart):
+ * entries of the slots, instances, level sums, fiber bases, fiber levels and
+ * fiber maps on the device that differ from a host build of the same grid. */
+int adg_grid_check(adg_ctx* c, int64_t* mismatches);
 
 /* DGAdaptIntp::init_adaptive_intp (source/DGAdaptIntp.cpp:35-45): adaptive
  * Lagrange interpolation of f with refinement threshold eps, then the
