@@ -39,11 +39,13 @@ constexpr uint64_t kFree = ~uint64_t{0};
 // closure layers run as grid-stride loops over a list whose length the host
 // does not know
 constexpr unsigned kCloseBlocks = 256;
+constexpr int64_t kMaxNew = int64_t{1} << 28;  // largest new-element list
 
 // counter words: [0] new (refine) or removed (coarsen) elements, [1] flags,
-// [2] [3] the range of the current closure layer in the new-element list
+// [2] [3] the range of the current closure layer in the new-element list,
+// [4 + t] elements removed by coarsen pass t
 enum : int32_t { AF_BAND = 1, AF_FULL = 2, AF_DUP = 4, AF_OPEN = 8, AF_NEG = 16 };
-constexpr int kCtr = 4;
+constexpr int kCtr = 4 + 4 * 15 + 2;
 
 __device__ inline int slot_level(int s) { return s == 0 ? 0 : 32 - __clz(s); }
 
@@ -63,10 +65,13 @@ __device__ inline uint64_t with_field(uint64_t key, int d, int nb, int v) {
 }
 
 // insert-if-absent with bounded probing: 1 inserted (slot in *at), 0 present,
-// -1 table full
+// -1 full.  At load <= 1/2 linear probing ends within a few steps; a table
+// filled past that (a refinement larger than the new-element list) gives up
+// after kMaxProbe steps and the call runs again with a larger table.
+constexpr uint64_t kMaxProbe = 512;
 __device__ inline int tab_insert(uint64_t* hkey, uint64_t mask, uint64_t key, uint64_t* at) {
   uint64_t h = amix(key) & mask;
-  for (uint64_t p = 0; p <= mask; ++p) {
+  for (uint64_t p = 0; p <= mask && p < kMaxProbe; ++p) {
     const unsigned long long prev =
         atomicCAS(reinterpret_cast<unsigned long long*>(hkey + h), static_cast<unsigned long long>(kFree),
                   static_cast<unsigned long long>(key));
@@ -77,15 +82,29 @@ __device__ inline int tab_insert(uint64_t* hkey, uint64_t mask, uint64_t key, ui
   return -1;
 }
 
-__device__ inline int32_t tab_find(const uint64_t* hkey, const int32_t* hval, uint64_t mask, uint64_t key) {
-  uint64_t h = amix(key) & mask;
-  for (uint64_t p = 0; p <= mask; ++p) {
-    const uint64_t k = hkey[h];
-    if (k == key) return hval[h];
-    if (k == kFree) return -1;
-    h = (h + 1) & mask;
+// sort key (instance, Hash::hash_key) of an element: the support part of the
+// key in mixed radix 2^level plus the level-vector part (source/Hash.cpp:55-114)
+__device__ inline uint64_t sort_key(const int* s, int in, int D, const int32_t* lvtab, int32_t* ctr) {
+  int ind1 = 0, code = 0;
+  for (int t = 0; t < D; ++t) {
+    const int l = slot_level(s[t]);
+    ind1 = ind1 * (1 << l) + (l == 0 ? 0 : s[t] - (1 << (l - 1)));
+    code = code * 16 + l;
   }
-  return -1;
+  const int h = ind1 + lvtab[code];
+  if (h < 0) atomicOr(ctr + 1, AF_NEG);
+  return (static_cast<uint64_t>(in) << 32) | static_cast<uint32_t>(h);
+}
+
+// whether the sorted keys of the current table hold k (binary search)
+__device__ inline bool old_has(const uint64_t* ukey, int64_t n, uint64_t k) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (ukey[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < n && ukey[lo] == k;
 }
 
 // DGAdapt::indicator_norm's pow(s, 0.5) compared with a threshold: false
@@ -104,8 +123,9 @@ struct AdArgs {
   const int32_t* inst;
   const double* sumsq;
   const double* thr;  // per instance
-  uint64_t* hkey;
-  int32_t* hval;
+  const uint64_t* ukey;   // sorted (instance, hash key) of the current table
+  const int32_t* lvtab;   // level-vector part of the hash key
+  uint64_t* hkey;         // insert-if-absent table of the new elements
   uint64_t mask;
   uint64_t* newkey;
   int32_t* ctr;
@@ -148,20 +168,10 @@ __global__ __launch_bounds__(kAB) void k_ad_fillu32(uint32_t* p, int64_t n, uint
   if (i < n) p[i] = v;
 }
 
-// the current elements into the hash table (keys are distinct)
-__global__ __launch_bounds__(kAB) void k_ad_table(AdArgs a) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= a.E) return;
-  uint64_t at;
-  const int r = tab_insert(a.hkey, a.mask, elem_key(a, e), &at);
-  if (r == 1) a.hval[at] = static_cast<int32_t>(e);
-  else atomicOr(a.ctr + 1, r == 0 ? AF_DUP : AF_FULL);
-}
-
-// refine: leaves above eps add their missing children.  The leaf test reads
-// the fiber maps of the current grid (children of e in dimension d lie in
-// e's d-fiber), so it sees the grid before this refinement, as the host's
-// does.
+// refine: leaves above eps add their missing children.  The leaf test and
+// the presence of each child read the fiber maps of the current grid
+// (children of e in dimension d lie in e's d-fiber), so they see the grid
+// before this refinement, as the host's do; the table dedups the new ones.
 __global__ __launch_bounds__(kAB) void k_ad_children(AdArgs a) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= a.E) return;
@@ -184,29 +194,37 @@ __global__ __launch_bounds__(kAB) void k_ad_children(AdArgs a) {
     }
   }
   if (have >= total) return;
+  // the missing ones (present children are in the fiber map)
   const uint64_t key = elem_key(a, e);
   for (int d = 0; d < a.D; ++d) {
     const int l = slot_level(s[d]);
+    const int32_t* fm = a.fmap[d] + a.fbase[d][e];
     if (l == 0) {
-      add_new(a, with_field(key, d, a.nb, 1));
+      if (fm[1] < 0) add_new(a, with_field(key, d, a.nb, 1));
     } else if (l < a.nmax) {
-      add_new(a, with_field(key, d, a.nb, 2 * s[d]));
-      add_new(a, with_field(key, d, a.nb, 2 * s[d] + 1));
+      if (fm[2 * s[d]] < 0) add_new(a, with_field(key, d, a.nb, 2 * s[d]));
+      if (fm[2 * s[d] + 1] < 0) add_new(a, with_field(key, d, a.nb, 2 * s[d] + 1));
     }
   }
 }
 
 // one layer of the hole closure: the parents of the elements added by the
-// previous layer
+// previous layer that the current table lacks
 __global__ __launch_bounds__(kAB) void k_ad_close(AdArgs a) {
   const int32_t b = a.ctr[2], en = a.ctr[3];
   for (int64_t w = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; w < en;
        w += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const uint64_t key = a.newkey[w];
+    int s[kMaxDim];
+    for (int d = 0; d < a.D; ++d) s[d] = field(key, d, a.nb);
+    const int in = static_cast<int>(key >> (a.D * a.nb));
     for (int d = 0; d < a.D; ++d) {
-      const int s = field(key, d, a.nb);
-      if (s == 0) continue;
-      add_new(a, with_field(key, d, a.nb, s >> 1));
+      if (s[d] == 0) continue;
+      const int sd = s[d];
+      s[d] = sd >> 1;
+      const bool have = old_has(a.ukey, a.E, sort_key(s, in, a.D, a.lvtab, a.ctr));
+      s[d] = sd;
+      if (!have) add_new(a, with_field(key, d, a.nb, sd >> 1));
     }
   }
 }
@@ -253,15 +271,7 @@ __global__ __launch_bounds__(kAB) void k_ad_union_keys(UnionArgs a) {
   if (q >= a.N) return;
   int s[kMaxDim], in;
   union_elem(a, q, s, in);
-  int ind1 = 0, code = 0;
-  for (int t = 0; t < a.D; ++t) {
-    const int l = slot_level(s[t]);
-    ind1 = ind1 * (1 << l) + (l == 0 ? 0 : s[t] - (1 << (l - 1)));
-    code = code * 16 + l;
-  }
-  const int h = ind1 + a.lvtab[code];
-  if (h < 0) atomicOr(a.ctr + 1, AF_NEG);
-  a.keys[q] = (static_cast<uint64_t>(in) << 32) | static_cast<uint32_t>(h);
+  a.keys[q] = sort_key(s, in, a.D, a.lvtab, a.ctr);
   a.vals[q] = static_cast<int32_t>(q);
 }
 
@@ -270,22 +280,20 @@ struct OutArgs {
   int32_t* inst;
   int32_t* lsum;
   int32_t* remap;  // old index of every new element, -1 when new
-  int32_t* meta;   // [B] counts, [B * D] deepest levels, then per dimension fibers and largest fiber
+  uint64_t* ukey;  // sort key of every element
   int B;
 };
 
-__device__ inline void put_elem(const OutArgs& o, int64_t q, const int* s, int in, int32_t old, int D) {
+__device__ inline void put_elem(const OutArgs& o, int64_t q, const int* s, int in, int32_t old, uint64_t key, int D) {
   int ls = 0;
   for (int d = 0; d < D; ++d) {
     o.slot[d][q] = s[d];
-    const int l = slot_level(s[d]);
-    ls += l;
-    atomicMax(o.meta + o.B + in * D + d, l);
+    ls += slot_level(s[d]);
   }
   o.inst[q] = in;
   o.lsum[q] = ls;
   o.remap[q] = old;
-  atomicAdd(o.meta + in, 1);
+  o.ukey[q] = key;
 }
 
 __global__ __launch_bounds__(kAB) void k_ad_union_gather(UnionArgs a, const uint64_t* skeys, const int32_t* svals,
@@ -296,18 +304,18 @@ __global__ __launch_bounds__(kAB) void k_ad_union_gather(UnionArgs a, const uint
   const int32_t src = svals[q];
   int s[kMaxDim], in;
   union_elem(a, src, s, in);
-  put_elem(o, q, s, in, src < a.E_old ? src : -1, a.D);
+  put_elem(o, q, s, in, src < a.E_old ? src : -1, skeys[q], a.D);
 }
 
 // coarsen: children of every element, then removal passes
 __global__ __launch_bounds__(kAB) void k_ad_nchild(AdArgs a) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= a.E) return;
-  const uint64_t key = elem_key(a, e);
+  // the parent in dimension d lies in e's d-fiber
   for (int d = 0; d < a.D; ++d) {
     const int s = a.slot[d][e];
     if (s == 0) continue;
-    const int32_t p = tab_find(a.hkey, a.hval, a.mask, with_field(key, d, a.nb, s >> 1));
+    const int32_t p = a.fmap[d][a.fbase[d][e] + (s >> 1)];
     if (p >= 0) atomicAdd(a.nchild + p, 1);
   }
 }
@@ -327,25 +335,31 @@ __global__ __launch_bounds__(kAB) void k_ad_below(AdArgs a) {
 
 // pass `it` (1, 2, ...): every element alive, childless and below eta is
 // marked from the state after the previous pass
+// (a pass after one that removed nothing changes nothing and exits)
 __global__ __launch_bounds__(kAB) void k_ad_mark(AdArgs a, int it, int last) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= a.E) return;
+  if (e >= a.E || (it > 1 && a.ctr[4 + it - 1] == 0)) return;
   if (a.zmark[e] != 0 || a.nchild[e] != 0 || !a.dec[e]) return;
   a.zmark[e] = it;
   atomicAdd(a.ctr, 1);
+  atomicAdd(a.ctr + 4 + it, 1);
   if (last) atomicOr(a.ctr + 1, AF_OPEN);
 }
 
 __global__ __launch_bounds__(kAB) void k_ad_unlink(AdArgs a, int it) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= a.E || a.zmark[e] != it) return;
-  const uint64_t key = elem_key(a, e);
+  if (e >= a.E || a.ctr[4 + it] == 0 || a.zmark[e] != it) return;
   for (int d = 0; d < a.D; ++d) {
     const int s = a.slot[d][e];
     if (s == 0) continue;
-    const int32_t p = tab_find(a.hkey, a.hval, a.mask, with_field(key, d, a.nb, s >> 1));
+    const int32_t p = a.fmap[d][a.fbase[d][e] + (s >> 1)];
     if (p >= 0) atomicSub(a.nchild + p, 1);
   }
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_drop(AdArgs a, const int32_t* drop) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < a.E) a.zmark[e] = drop[a.inst[e]] ? 1 : 0;
 }
 
 __global__ __launch_bounds__(kAB) void k_ad_keep(const int32_t* zmark, int32_t* keep, int64_t E) {
@@ -359,7 +373,7 @@ __global__ __launch_bounds__(kAB) void k_ad_compact(AdArgs a, const int32_t* pos
   if (e >= a.E || a.zmark[e] != 0) return;
   int s[kMaxDim];
   for (int d = 0; d < a.D; ++d) s[d] = a.slot[d][e];
-  put_elem(o, pos[e], s, a.inst[e], static_cast<int32_t>(e), a.D);
+  put_elem(o, pos[e], s, a.inst[e], static_cast<int32_t>(e), a.ukey[e], a.D);
 }
 
 // fiber tables of dimension dd
@@ -400,14 +414,29 @@ __global__ __launch_bounds__(kAB) void k_ad_runs(const int32_t* svals, const int
   if (i == N - 1) *nfib = r + 1;
 }
 
+// fiber levels; at the first element of every fiber, the largest fiber and
+// the instance's deepest level in dimension dd (one atomic per fiber)
 __global__ __launch_bounds__(kAB) void k_ad_run_levels(const int32_t* svals, const int32_t* run, const int32_t* runmax,
                                                       const int32_t* runlen, int32_t* fmaxl, int64_t N,
-                                                      int32_t* maxfib) {
+                                                      int32_t* maxfib, const int32_t* inst, int32_t* lmax, int D,
+                                                      int dd) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int32_t r = run[i] - 1;
   fmaxl[svals[i]] = runmax[r];
-  if (i == 0 || run[i] != run[i - 1]) atomicMax(maxfib, runlen[r]);
+  if (i == 0 || run[i] != run[i - 1]) {
+    atomicMax(maxfib, runlen[r]);
+    atomicMax(lmax + inst[svals[i]] * D + dd, runmax[r]);
+  }
+}
+
+// first and last element of every instance in the sorted table
+__global__ __launch_bounds__(kAB) void k_ad_bounds(const int32_t* inst, int64_t N, int32_t* first, int32_t* last) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= N) return;
+  const int32_t in = inst[q];
+  if (q == 0 || inst[q - 1] != in) first[in] = static_cast<int32_t>(q);
+  if (q == N - 1 || inst[q + 1] != in) last[in] = static_cast<int32_t>(q);
 }
 
 __global__ __launch_bounds__(kAB) void k_ad_fmap(int32_t* fmap, uint32_t* fbits, const int32_t* fbase,
@@ -435,9 +464,9 @@ void to_device(DevBuf<T>& b, const std::vector<T>& h, hipStream_t s) {
 
 bool Context::device_adapt_fits(int64_t n) const {
   if (!device_adapt || refine_mode != 0 || recording || !up_host.empty() || n <= 0 || B() < 1) return false;
-  if (D * nmax + bits_for(B()) > 62) return false;
-  // fiber bases of up to n fibers and the int32 element indices
-  return (n << nmax) < (int64_t{1} << 31) - 1;
+  // element keys in 63 bits, int32 element indices (the fiber bases are
+  // checked against int32 once the fibers are counted)
+  return D * nmax + bits_for(B()) <= 62 && n < (int64_t{1} << 31) - 1;
 }
 
 // Table of the current grid, per-instance thresholds and the norms (shared
@@ -446,9 +475,8 @@ static AdArgs adapt_args(Context& c, const std::vector<double>& thr, int32_t cap
   const int64_t E = c.E();
   if (c.ad_lvtab.cap == 0) to_device(c.ad_lvtab, c.hk.level_table(c.D), c.stream);
   uint64_t tcap = 16;
-  while (tcap < 2 * static_cast<uint64_t>(E + cap_new)) tcap <<= 1;
-  c.ad_hkey.ensure(tcap);
-  c.ad_hval.ensure(tcap);
+  while (tcap < 2 * static_cast<uint64_t>(cap_new)) tcap <<= 1;
+  if (cap_new > 0) c.ad_hkey.ensure(tcap);
   c.ad_newkey.ensure(static_cast<size_t>(std::max<int32_t>(cap_new, 1)));
   c.ad_ctr.ensure(kCtr);
   to_device(c.ad_thr, thr, c.stream);
@@ -462,8 +490,9 @@ static AdArgs adapt_args(Context& c, const std::vector<double>& thr, int32_t cap
   a.inst = c.g_inst.p;
   a.sumsq = c.sumsq.p;
   a.thr = c.ad_thr.p;
+  a.ukey = c.g_ukey.p;
+  a.lvtab = c.ad_lvtab.p;
   a.hkey = c.ad_hkey.p;
-  a.hval = c.ad_hval.p;
   a.mask = tcap - 1;
   a.newkey = c.ad_newkey.p;
   a.ctr = c.ad_ctr.p;
@@ -473,9 +502,9 @@ static AdArgs adapt_args(Context& c, const std::vector<double>& thr, int32_t cap
   a.nmax = c.nmax;
   a.cap_new = cap_new;
   hipLaunchKernelGGL(k_ad_fill32, dim3(1), dim3(kAB), 0, c.stream, c.ad_ctr.p, int64_t{kCtr}, 0);
-  hipLaunchKernelGGL(k_ad_fill64, dim3(ablocks(static_cast<int64_t>(tcap))), dim3(kAB), 0, c.stream, c.ad_hkey.p,
-                     static_cast<int64_t>(tcap), kFree);
-  hipLaunchKernelGGL(k_ad_table, dim3(ablocks(E)), dim3(kAB), 0, c.stream, a);
+  if (cap_new > 0)
+    hipLaunchKernelGGL(k_ad_fill64, dim3(ablocks(static_cast<int64_t>(tcap))), dim3(kAB), 0, c.stream, c.ad_hkey.p,
+                       static_cast<int64_t>(tcap), kFree);
   return a;
 }
 
@@ -489,13 +518,18 @@ static OutArgs out_args(Context& c, int64_t N) {
   c.ad_inst.ensure(n);
   c.ad_lsum.ensure(n);
   c.g_remap.ensure(n);
-  const int64_t nmeta = static_cast<int64_t>(c.B()) * (1 + c.D) + 2 * c.D;
+  c.ad_ukey.ensure(n);
+  // read-back words: [B] first and [B] last element of every instance (-1:
+  // none), [B * D] deepest levels, then per dimension fibers and largest fiber
+  const int64_t nmeta = static_cast<int64_t>(c.B()) * (2 + c.D) + 2 * c.D;
   c.ad_meta.ensure(static_cast<size_t>(nmeta));
   hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(nmeta)), dim3(kAB), 0, c.stream, c.ad_meta.p, nmeta, 0);
+  hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(2 * c.B())), dim3(kAB), 0, c.stream, c.ad_meta.p,
+                     int64_t{2} * c.B(), -1);
   o.inst = c.ad_inst.p;
   o.lsum = c.ad_lsum.p;
   o.remap = c.g_remap.p;
-  o.meta = c.ad_meta.p;
+  o.ukey = c.ad_ukey.p;
   o.B = c.B();
   return o;
 }
@@ -507,24 +541,54 @@ static std::array<int32_t, 2> read_counters(Context& c) {
   return h;
 }
 
+// elements with zmark == 0 keep their order; the new table then gets its
+// fiber tables (N survivors)
+static void compact_survivors(Context& c, const AdArgs& a, int64_t N) {
+  const int64_t oldE = a.E;
+  c.ad_vals[0].ensure(static_cast<size_t>(oldE));
+  c.ad_vals[1].ensure(static_cast<size_t>(oldE));
+  hipLaunchKernelGGL(k_ad_keep, dim3(ablocks(oldE)), dim3(kAB), 0, c.stream, c.ad_zmark.p, c.ad_vals[0].p, oldE);
+  size_t bytes = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, c.ad_vals[0].p, c.ad_vals[1].p, static_cast<int>(oldE),
+                                         c.stream);
+  c.ad_tmp.ensure(std::max<size_t>(bytes, 1));
+  hip_check(hipcub::DeviceScan::ExclusiveSum(c.ad_tmp.p, bytes, c.ad_vals[0].p, c.ad_vals[1].p,
+                                             static_cast<int>(oldE), c.stream),
+            "scan (survivors)");
+  const OutArgs o = out_args(c, N);
+  hipLaunchKernelGGL(k_ad_compact, dim3(ablocks(oldE)), dim3(kAB), 0, c.stream, a, c.ad_vals[1].p, o);
+  c.device_grid_finish(N, oldE);
+}
+
 int64_t Context::refine_device(const std::vector<double>& eps) {
   const int64_t oldE = E();
   if (!device_adapt_fits(oldE)) return -1;
-  const int32_t cap_new =
-      static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(2 * oldE, 4096), int64_t{1} << 28));
-  AdArgs a = adapt_args(*this, eps, cap_new);
-  hipLaunchKernelGGL(k_ad_children, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a);
-  hipLaunchKernelGGL(k_ad_advance, dim3(1), dim3(1), 0, stream, a.ctr, cap_new);
-  // every layer lowers the level sum by one: D * NMAX + 1 layers close any hole
-  for (int it = 0; it <= D * nmax; ++it) {
-    hipLaunchKernelGGL(k_ad_close, dim3(kCloseBlocks), dim3(kAB), 0, stream, a);
+  // new-element list of E/4 (at least 2^16); a refinement that adds more
+  // runs again with a list four times larger (nothing has changed yet)
+  int32_t cap_new = static_cast<int32_t>(std::min<int64_t>(std::max<int64_t>(oldE / 4, 65536), kMaxNew));
+  std::array<int32_t, 2> h{0, 0};
+  for (;;) {
+    AdArgs a = adapt_args(*this, eps, cap_new);
+    hipLaunchKernelGGL(k_ad_children, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a);
     hipLaunchKernelGGL(k_ad_advance, dim3(1), dim3(1), 0, stream, a.ctr, cap_new);
-  }
-  hipLaunchKernelGGL(k_ad_closed, dim3(1), dim3(1), 0, stream, a.ctr);
-  const std::array<int32_t, 2> h = read_counters(*this);
-  if (h[1] & (AF_BAND | AF_FULL | AF_DUP | AF_OPEN) || h[0] > cap_new || !device_adapt_fits(oldE + h[0])) {
-    ++stat_dev_fallback;
-    return -1;
+    // every layer lowers the level sum by one: D * NMAX + 1 layers close any hole
+    for (int it = 0; it <= D * nmax; ++it) {
+      hipLaunchKernelGGL(k_ad_close, dim3(kCloseBlocks), dim3(kAB), 0, stream, a);
+      hipLaunchKernelGGL(k_ad_advance, dim3(1), dim3(1), 0, stream, a.ctr, cap_new);
+    }
+    hipLaunchKernelGGL(k_ad_closed, dim3(1), dim3(1), 0, stream, a.ctr);
+    h = read_counters(*this);
+    const bool full = (h[1] & AF_FULL) || h[0] > cap_new;
+    if (full && !(h[1] & AF_BAND) && cap_new < kMaxNew) {
+      cap_new = static_cast<int32_t>(std::min<int64_t>(int64_t{cap_new} * 4, kMaxNew));
+      continue;
+    }
+    if (full || (h[1] & (AF_BAND | AF_DUP | AF_OPEN)) || !device_adapt_fits(oldE + h[0])) {
+      ++stat_dev_fallback;
+      dev_fallback_flags |= h[1] | (full ? AF_FULL : 0);
+      return -1;
+    }
+    break;
   }
   ++stat_dev_adapt;
   if (h[0] == 0) return 0;
@@ -584,33 +648,51 @@ int64_t Context::coarsen_device(const std::vector<double>& eta) {
   const std::array<int32_t, 2> h = read_counters(*this);
   if (h[1] & (AF_BAND | AF_FULL | AF_DUP | AF_OPEN) || h[0] >= oldE) {
     ++stat_dev_fallback;
+    dev_fallback_flags |= h[1];
     return -1;
   }
   ++stat_dev_adapt;
   if (h[0] == 0) return 0;
   HostTimer tg(*this, HOST_GRID);
-  const int64_t N = oldE - h[0];
-  ad_vals[0].ensure(static_cast<size_t>(oldE));
-  ad_vals[1].ensure(static_cast<size_t>(oldE));
-  hipLaunchKernelGGL(k_ad_keep, dim3(ablocks(oldE)), dim3(kAB), 0, stream, ad_zmark.p, ad_vals[0].p, oldE);
-  size_t bytes = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ad_vals[0].p, ad_vals[1].p, static_cast<int>(oldE), stream);
-  ad_tmp.ensure(std::max<size_t>(bytes, 1));
-  hip_check(hipcub::DeviceScan::ExclusiveSum(ad_tmp.p, bytes, ad_vals[0].p, ad_vals[1].p, static_cast<int>(oldE),
-                                             stream),
-            "scan (survivors)");
-  const OutArgs o = out_args(*this, N);
-  hipLaunchKernelGGL(k_ad_compact, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, ad_vals[1].p, o);
-  device_grid_finish(N, oldE);
+  compact_survivors(*this, a, oldE - h[0]);
   return h[0];
+}
+
+int64_t Context::drop_instances_device(const std::vector<char>& drop) {
+  const int64_t oldE = E();
+  if (static_cast<int>(drop.size()) != B()) throw std::runtime_error("drop instances: one flag per instance expected");
+  if (!device_adapt_fits(oldE)) return -1;
+  HostTimer tg(*this, HOST_GRID);
+  int64_t removed = 0;
+  std::vector<int32_t> flags(drop.size());
+  for (int i = 0; i < B(); ++i) {
+    flags[i] = drop[i] ? 1 : 0;
+    if (drop[i]) removed += off[i + 1] - off[i];
+  }
+  if (removed == 0) return 0;
+  ad_run.ensure(flags.size());
+  to_device(ad_run, flags, stream);
+  AdArgs a{};
+  for (int d = 0; d < D; ++d) a.slot[d] = g_slot[d].p;
+  a.inst = g_inst.p;
+  a.ukey = g_ukey.p;
+  ad_zmark.ensure(static_cast<size_t>(oldE));
+  a.zmark = ad_zmark.p;
+  a.E = oldE;
+  a.D = D;
+  hipLaunchKernelGGL(k_ad_drop, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, ad_run.p);
+  compact_survivors(*this, a, oldE - removed);  // (waits for the stream: the flag upload is done)
+  return removed;
 }
 
 void Context::device_grid_finish(int64_t N, int64_t oldE) {
   // fiber tables of every dimension: sort by (instance, slots outside d),
   // runs of equal keys are the fibers
   const int B_ = B();
-  int32_t* meta_nfib = ad_meta.p + static_cast<int64_t>(B_) * (1 + D);
+  int32_t* meta_lmax = ad_meta.p + 2 * static_cast<int64_t>(B_);
+  int32_t* meta_nfib = meta_lmax + static_cast<int64_t>(B_) * D;
   int32_t* meta_maxfib = meta_nfib + D;
+  hipLaunchKernelGGL(k_ad_bounds, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_inst.p, N, ad_meta.p, ad_meta.p + B_);
   for (int i = 0; i < 2; ++i) {
     ad_keys[i].ensure(static_cast<size_t>(N));
     ad_vals[i].ensure(static_cast<size_t>(N));
@@ -650,10 +732,11 @@ void Context::device_grid_finish(int64_t N, int64_t oldE) {
     hipLaunchKernelGGL(k_ad_runs, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_vals[1].p, ad_run.p, ad_slot[dd].p,
                        ad_runmax.p, ad_runlen.p, ad_fbase[dd].p, N, nmax, meta_nfib + dd);
     hipLaunchKernelGGL(k_ad_run_levels, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_vals[1].p, ad_run.p, ad_runmax.p,
-                       ad_runlen.p, ad_fmaxl[dd].p, N, meta_maxfib + dd);
+                       ad_runlen.p, ad_fmaxl[dd].p, N, meta_maxfib + dd, ad_inst.p, meta_lmax, D, dd);
   }
   // counts per instance, deepest levels, fibers and largest fiber per dimension
-  const int64_t nmeta = static_cast<int64_t>(B_) * (1 + D) + 2 * D;
+  const int64_t nmeta = static_cast<int64_t>(B_) * (2 + D) + 2 * D;
+  const size_t at_nfib = static_cast<size_t>(B_) * (2 + D), at_maxfib = at_nfib + D;
   std::vector<int32_t> meta(static_cast<size_t>(nmeta));
   std::array<int32_t, 2> h{0, 0};
   hip_check(hipMemcpyAsync(meta.data(), ad_meta.p, nmeta * sizeof(int32_t), hipMemcpyDeviceToHost, stream),
@@ -662,12 +745,19 @@ void Context::device_grid_finish(int64_t N, int64_t oldE) {
   sync();
   if (h[1] & AF_DUP) throw std::runtime_error("grid: duplicate hash key");
   if (h[1] & AF_NEG) throw std::runtime_error("grid: negative hash key");
+  for (int d = 0; d < D; ++d)
+    if ((int64_t{meta[at_nfib + d]} << nmax) > (int64_t{1} << 31) - 1)
+      throw std::runtime_error("fiber map exceeds int32 range");
   std::vector<int64_t> noff(B_ + 1, 0);
-  for (int i = 0; i < B_; ++i) noff[i + 1] = noff[i] + meta[i];
+  for (int i = 0; i < B_; ++i) {
+    const int32_t f = meta[i], l = meta[B_ + i];
+    if (f >= 0 && f != noff[i]) throw std::logic_error("device refine/coarsen: instances out of order");
+    noff[i + 1] = noff[i] + (f >= 0 ? l - f + 1 : 0);
+  }
   if (noff[B_] != N) throw std::logic_error("device refine/coarsen: instance counts do not add up");
   // commit: fiber maps of the new table, then swap it in
   for (int d = 0; d < D; ++d) {
-    const int64_t nfib = meta[static_cast<size_t>(B_) * (1 + D) + d];
+    const int64_t nfib = meta[at_nfib + d];
     const int64_t nmap = std::max<int64_t>(nfib << nmax, 1);
     g_fmap[d].ensure(static_cast<size_t>(nmap));
     hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(nmap)), dim3(kAB), 0, stream, g_fmap[d].p, nmap, -1);
@@ -683,9 +773,10 @@ void Context::device_grid_finish(int64_t N, int64_t oldE) {
   }
   g_inst.swap(ad_inst);
   g_lsum.swap(ad_lsum);
+  g_ukey.swap(ad_ukey);
   maxfib = 1;
-  for (int d = 0; d < D; ++d) maxfib = std::max(maxfib, meta[static_cast<size_t>(B_) * (1 + D) + D + d]);
-  inst_lmax.assign(meta.begin() + B_, meta.begin() + static_cast<int64_t>(B_) * (1 + D));
+  for (int d = 0; d < D; ++d) maxfib = std::max(maxfib, meta[at_maxfib + d]);
+  inst_lmax.assign(meta.begin() + 2 * static_cast<int64_t>(B_), meta.begin() + at_nfib);
   off = noff;
   remap_field_dev(u, K, N, oldE);
   remap_field_dev(pred, K, N, oldE);
@@ -716,6 +807,103 @@ void Context::ensure_host_grid() {
   host_stale = false;
 }
 
+bool Context::set_grids_fresh(const std::vector<std::vector<int32_t>>& slots) {
+  const int nb = static_cast<int>(slots.size());
+  int64_t N = 0;
+  for (const std::vector<int32_t>& v : slots) N += static_cast<int64_t>(v.size()) / D;
+  if (nb != B() || !up_host.empty() || !device_adapt_fits(N)) {
+    set_grids(slots, false);
+    return false;
+  }
+  // element keys of the new table, packed on the host
+  std::vector<uint64_t> keys(static_cast<size_t>(N));
+  int64_t q = 0;
+  for (int i = 0; i < nb; ++i)
+    for (size_t e = 0; e < slots[i].size() / D; ++e, ++q) {
+      uint64_t k = static_cast<uint64_t>(i) << (D * nmax);
+      for (int d = 0; d < D; ++d) {
+        const int32_t sl = slots[i][e * D + d];
+        if (sl < 0 || sl >= (1 << nmax)) throw std::runtime_error("grid: slot out of range");
+        k |= static_cast<uint64_t>(sl) << (nmax * d);
+      }
+      keys[q] = k;
+    }
+  HostTimer tg(*this, HOST_GRID);
+  if (ad_lvtab.cap == 0) to_device(ad_lvtab, hk.level_table(D), stream);
+  ad_ctr.ensure(kCtr);
+  hipLaunchKernelGGL(k_ad_fill32, dim3(1), dim3(kAB), 0, stream, ad_ctr.p, int64_t{kCtr}, 0);
+  to_device(ad_newkey, keys, stream);
+  for (int i = 0; i < 2; ++i) {
+    ad_keys[i].ensure(static_cast<size_t>(N));
+    ad_vals[i].ensure(static_cast<size_t>(N));
+  }
+  UnionArgs ua{};
+  for (int d = 0; d < D; ++d) ua.slot[d] = g_slot[d].p;
+  ua.inst = g_inst.p;
+  ua.newkey = ad_newkey.p;
+  ua.lvtab = ad_lvtab.p;
+  ua.keys = ad_keys[0].p;
+  ua.vals = ad_vals[0].p;
+  ua.ctr = ad_ctr.p;
+  ua.E_old = 0;
+  ua.N = N;
+  ua.D = D;
+  ua.nb = nmax;
+  hipLaunchKernelGGL(k_ad_union_keys, dim3(ablocks(N)), dim3(kAB), 0, stream, ua);
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p, ad_vals[1].p,
+                                           static_cast<int>(N), 0, 32 + bits_for(B()), stream);
+  ad_tmp.ensure(std::max<size_t>(bytes, 1));
+  hip_check(hipcub::DeviceRadixSort::SortPairs(ad_tmp.p, bytes, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p,
+                                               ad_vals[1].p, static_cast<int>(N), 0, 32 + bits_for(B()), stream),
+            "sort (element keys)");
+  const OutArgs o = out_args(*this, N);
+  hipLaunchKernelGGL(k_ad_union_gather, dim3(ablocks(N)), dim3(kAB), 0, stream, ua, ad_keys[1].p, ad_vals[1].p, o);
+  // (every element new: u and the predict copy come back zero)
+  device_grid_finish(N, E());
+  new_add.assign(static_cast<size_t>(N), 1);
+  for (DevBuf<double>* b : {&tn, &rhs})
+    hipLaunchKernelGGL(k_ad_fill64, dim3(ablocks(N * K)), dim3(kAB), 0, stream, reinterpret_cast<uint64_t*>(b->p),
+                       N * K, uint64_t{0});
+  return true;
+}
+
+std::vector<std::vector<int32_t>> Context::all_instance_slots() {
+  std::vector<std::vector<int32_t>> out(B());
+  if (!host_stale) {
+    for (int i = 0; i < B(); ++i) out[i] = grids[i].slot;
+    return out;
+  }
+  const int64_t n = E();
+  std::vector<std::vector<int32_t>> col(D, std::vector<int32_t>(static_cast<size_t>(n)));
+  for (int d = 0; d < D; ++d)
+    if (n > 0)
+      hip_check(hipMemcpyAsync(col[d].data(), g_slot[d].p, n * sizeof(int32_t), hipMemcpyDeviceToHost, stream),
+                "download");
+  sync();
+  for (int i = 0; i < B(); ++i) {
+    out[i].resize(static_cast<size_t>(off[i + 1] - off[i]) * D);
+    for (int64_t e = off[i]; e < off[i + 1]; ++e)
+      for (int d = 0; d < D; ++d) out[i][(e - off[i]) * D + d] = col[d][e];
+  }
+  return out;
+}
+
+std::vector<int32_t> Context::instance_slots(int i) {
+  if (i < 0 || i >= B()) throw std::runtime_error("instance index out of range");
+  if (!host_stale) return grids[i].slot;
+  const int64_t n = off[i + 1] - off[i];
+  std::vector<int32_t> s(static_cast<size_t>(n) * D), col(static_cast<size_t>(n));
+  for (int d = 0; d < D; ++d) {
+    if (n > 0)
+      hip_check(hipMemcpyAsync(col.data(), g_slot[d].p + off[i], n * sizeof(int32_t), hipMemcpyDeviceToHost, stream),
+                "download");
+    sync();
+    for (int64_t e = 0; e < n; ++e) s[e * D + d] = col[e];
+  }
+  return s;
+}
+
 int64_t Context::check_device_grid() {
   ensure_host_grid();
   const int64_t n = E();
@@ -729,8 +917,13 @@ int64_t Context::check_device_grid() {
   };
   std::vector<int64_t> fib0(B() + 1, 0);
   const std::vector<int32_t> inst = down(g_inst, n), lsum = down(g_lsum, n);
+  std::vector<uint64_t> ukey(static_cast<size_t>(n));
+  if (n > 0)
+    hip_check(hipMemcpyAsync(ukey.data(), g_ukey.p, n * sizeof(uint64_t), hipMemcpyDeviceToHost, stream), "download");
+  sync();
   for (int64_t e = 0; e < n; ++e) {
     bad += inst[e] != uinst[e];
+    bad += ukey[e] != ((static_cast<uint64_t>(uinst[e]) << 32) | static_cast<uint32_t>(uhash[e]));
     int ls = 0;
     for (int d = 0; d < D; ++d) ls += level(e, d);
     bad += lsum[e] != ls;

@@ -19,12 +19,16 @@ struct adg_ctx {
 
 namespace {
 
+// Calls that read the host copy of the element tables get it rebuilt first
+// (a device refine / coarsen leaves it stale); the ones marked kDeviceTables
+// work from the device tables and leave it as it is.
+enum GuardKind { kHostTables, kDeviceTables };
+
 template <class F>
-int guard(adg_ctx* h, F&& f) {
+int guard(adg_ctx* h, F&& f, GuardKind kind = kHostTables) {
   if (!h || !h->c) return ADG_E_ARG;
   try {
-    // a device refine / coarsen leaves the host copy of the tables stale
-    h->c->ensure_host_grid();
+    if (kind == kHostTables) h->c->ensure_host_grid();
     f(*h->c);
     return ADG_OK;
   } catch (const adg::HipError& e) {
@@ -403,10 +407,11 @@ int adg_set_device_adapt(adg_ctx* h, int on) {
   });
 }
 
-int adg_device_adapt_stats(adg_ctx* h, int64_t* device_calls, int64_t* host_fallbacks) {
+int adg_device_adapt_stats(adg_ctx* h, int64_t* device_calls, int64_t* host_fallbacks, int32_t* reasons) {
   return guard(h, [&](adg::Context& c) {
     if (device_calls) *device_calls = c.stat_dev_adapt;
     if (host_fallbacks) *host_fallbacks = c.stat_dev_fallback;
+    if (reasons) *reasons = c.dev_fallback_flags;
   });
 }
 
@@ -875,23 +880,26 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
       }
       // instances that reached t_final leave the batch while others continue
       if (finished && any_active()) {
-        std::vector<double> uall(static_cast<size_t>(c.E()) * c.K);
-        if (c.E() > 0)
-          adg::hip_check(hipMemcpyAsync(uall.data(), c.u.p, uall.size() * sizeof(double), hipMemcpyDeviceToHost,
-                                        c.stream), "download");
-        c.sync();
-        c.ensure_host_grid();
-        std::vector<std::vector<int32_t>> ns(nb);
+        // keep the finished instances' elements and coefficients on the host
+        std::vector<char> drop(nb, 0);
         for (int i = 0; i < nb; ++i) {
-          if (!active[i] && !stashed[i]) {
-            fin_slots[i] = c.grids[i].slot;
-            fin_u[i].assign(uall.begin() + c.off[i] * c.K, uall.begin() + c.off[i + 1] * c.K);
-            stashed[i] = 1;
-          } else if (active[i]) {
-            ns[i] = c.grids[i].slot;
-          }
+          if (active[i] || stashed[i]) continue;
+          drop[i] = 1;
+          stashed[i] = 1;
+          fin_slots[i] = c.instance_slots(i);
+          fin_u[i].resize(static_cast<size_t>(c.off[i + 1] - c.off[i]) * c.K);
+          if (!fin_u[i].empty())
+            adg::hip_check(hipMemcpyAsync(fin_u[i].data(), c.u.p + c.off[i] * c.K, fin_u[i].size() * sizeof(double),
+                                          hipMemcpyDeviceToHost, c.stream), "download");
         }
-        c.set_grids(ns, true);
+        c.sync();
+        if (c.drop_instances_device(drop) < 0) {
+          c.ensure_host_grid();
+          std::vector<std::vector<int32_t>> ns(nb);
+          for (int i = 0; i < nb; ++i)
+            if (active[i]) ns[i] = c.grids[i].slot;
+          c.set_grids(ns, true);
+        }
       }
     }
     // put every instance back, at its final state
@@ -903,19 +911,17 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
         adg::hip_check(hipMemcpyAsync(uall.data(), c.u.p, uall.size() * sizeof(double), hipMemcpyDeviceToHost,
                                       c.stream), "download");
       c.sync();
-      c.ensure_host_grid();
-      std::vector<std::vector<int32_t>> ns(nb);
+      std::vector<std::vector<int32_t>> ns = c.all_instance_slots();
       std::vector<double> unew;
       for (int i = 0; i < nb; ++i) {
         if (stashed[i]) {
           ns[i] = fin_slots[i];
           unew.insert(unew.end(), fin_u[i].begin(), fin_u[i].end());
         } else {
-          ns[i] = c.grids[i].slot;
           unew.insert(unew.end(), uall.begin() + c.off[i] * c.K, uall.begin() + c.off[i + 1] * c.K);
         }
       }
-      c.set_grids(ns, false);
+      c.set_grids_fresh(ns);
       if (!unew.empty())
         adg::hip_check(hipMemcpyAsync(c.u.p, unew.data(), unew.size() * sizeof(double), hipMemcpyHostToDevice,
                                       c.stream), "upload");
@@ -935,7 +941,7 @@ int adg_run_hyperbolic(adg_ctx* h, const adg_problem* p, adg_run_stats* st) {
       st->redone = c.stat_redo - redo0;
       st->proposed = c.stat_proposed - proposed0;
     }
-  });
+  }, kDeviceTables);
 }
 
 int adg_grid_init_batch(adg_ctx* h, int ninst, int sparse, int level_init) {
@@ -1171,7 +1177,9 @@ int adg_state_read(adg_ctx* h, const char* path) {
 }
 
 int adg_state_save(adg_ctx* h, int slot) { return guard(h, [&](adg::Context& c) { c.save_state(slot); }); }
-int adg_state_restore(adg_ctx* h, int slot) { return guard(h, [&](adg::Context& c) { c.restore_state(slot); }); }
+int adg_state_restore(adg_ctx* h, int slot) {
+  return guard(h, [&](adg::Context& c) { c.restore_state(slot); }, kDeviceTables);
+}
 
 int adg_profile(adg_ctx* h, int enable) {
   return guard(h, [&](adg::Context& c) {

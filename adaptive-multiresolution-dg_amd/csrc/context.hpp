@@ -227,16 +227,34 @@ class Context {
   bool device_adapt = false;
   bool host_stale = false;
   int64_t stat_dev_adapt = 0, stat_dev_fallback = 0;
+  int32_t dev_fallback_flags = 0;  // why calls went to the host: 1 norm at a threshold, 2 table full
   int64_t refine_device(const std::vector<double>& eps);
   int64_t coarsen_device(const std::vector<double>& eta);
   void ensure_host_grid();
+  // removes every element of the flagged instances on the device (they stay
+  // in the batch with no elements); -1 when the host has to do it
+  int64_t drop_instances_device(const std::vector<char>& drop);
+  // second half of a device refine / coarsen / drop: fiber tables of the new
+  // union table in ad_*, read-back of its counts (the call waits for the
+  // stream), fiber maps, swap into g_*, field remap
+  void device_grid_finish(int64_t N, int64_t old_E);
+  bool device_adapt_fits(int64_t n) const;
+  // element slots of instance i in table order (from the device when the
+  // host copy is stale)
+  std::vector<int32_t> instance_slots(int i);
+  std::vector<std::vector<int32_t>> all_instance_slots();
+  // set_grids(slots, false) with the table built on the device when device
+  // adapt is on (fields zero; the host copy is left stale); false when it
+  // fell back to set_grids
+  bool set_grids_fresh(const std::vector<std::vector<int32_t>>& slots);
   // entries of the device tables (slots, instances, level sums, fiber bases,
   // fiber levels, fiber maps) that differ from a host build of the same grid
   int64_t check_device_grid();
   std::vector<int32_t> inst_lmax;  // [B * D] deepest level per instance and dimension
   void host_union();               // uslot, uhash, uinst and inst_lmax from grids
-  DevBuf<uint64_t> ad_hkey, ad_newkey, ad_keys[2];
-  DevBuf<int32_t> ad_hval, ad_ctr, ad_vals[2], ad_run, ad_runmax, ad_runlen, ad_meta, ad_nchild, ad_zmark, ad_dec,
+  DevBuf<uint64_t> g_ukey;  // (instance << 32 | hash key) of every element, ascending
+  DevBuf<uint64_t> ad_hkey, ad_newkey, ad_keys[2], ad_ukey;
+  DevBuf<int32_t> ad_ctr, ad_vals[2], ad_run, ad_runmax, ad_runlen, ad_meta, ad_nchild, ad_zmark, ad_dec,
       ad_lvtab;
   DevBuf<int32_t> ad_slot[kMaxDim], ad_fbase[kMaxDim], ad_fmaxl[kMaxDim], ad_inst, ad_lsum;
   DevBuf<double> ad_thr;
@@ -459,10 +477,7 @@ class Context {
   void remap_field(DevBuf<double>& f, int per_elem, const std::vector<int32_t>& old_of_new, int64_t old_E);
   // the same with old_of_new already in g_remap (new_E entries)
   void remap_field_dev(DevBuf<double>& f, int per_elem, int64_t new_E, int64_t old_E);
-  // second half of a device refine / coarsen: fiber tables of the new union
-  // table in ad_*, read-back of counts, fiber maps, swap into g_*, field remap
-  void device_grid_finish(int64_t N, int64_t old_E);
-  bool device_adapt_fits(int64_t n) const;
+
   void surplus_transform(const double* in, double* out, double* scratch_a, double* scratch_b, int ncomp);
   void surplus_transform32(const float* in, float* out, float* scratch_a, float* scratch_b, int ncomp);
   void compute_vcoef();

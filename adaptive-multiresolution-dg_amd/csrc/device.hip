@@ -761,6 +761,10 @@ void Context::upload_grid() {
   }
   upload(g_inst, uinst, stream);
   upload(g_lsum, ls, stream);
+  std::vector<uint64_t> uk(n);
+  for (int64_t e = 0; e < n; ++e)
+    uk[e] = (static_cast<uint64_t>(uinst[e]) << 32) | static_cast<uint32_t>(uhash[e]);
+  upload(g_ukey, uk, stream);
   maxfib = 1;
   for (const HostGrid& g : grids)
     for (int dd = 0; dd < D; ++dd) maxfib = std::max(maxfib, g.maxfib[dd]);
@@ -1830,10 +1834,13 @@ void Context::restore_state(int slot) {
   if (slot < 0 || slot >= 4) throw std::runtime_error("snapshot slot must be 0..3");
   Snapshot& sn = snaps[slot];
   if (sn.slots.empty()) throw std::runtime_error("snapshot slot is empty");
-  ensure_host_grid();
-  bool same = static_cast<int>(sn.slots.size()) == B();
+  // (with a stale host copy the grid is rebuilt on the device without comparing)
+  bool same = !host_stale && static_cast<int>(sn.slots.size()) == B();
   for (int i = 0; same && i < B(); ++i) same = (sn.slots[i] == grids[i].slot);
-  if (!same) set_grids(sn.slots, false);
+  if (!same) {
+    if (device_adapt && static_cast<int>(sn.slots.size()) == B()) set_grids_fresh(sn.slots);
+    else set_grids(sn.slots, false);
+  }
   if (E() > 0) hipLaunchKernelGGL(k_copy, dim3(nblocks(E() * K)), dim3(kBlock), 0, stream, u.p, sn.u.p, E() * K);
 }
 

@@ -120,7 +120,7 @@ def lib():
             "adg_refine": (i32, [vp, dbl, P(i64)]),
             "adg_coarsen": (i32, [vp, dbl, P(i64)]),
             "adg_set_device_adapt": (i32, [vp, i32]),
-            "adg_device_adapt_stats": (i32, [vp, P(i64), P(i64)]),
+            "adg_device_adapt_stats": (i32, [vp, P(i64), P(i64), P(i32)]),
             "adg_grid_check": (i32, [vp, P(i64)]),
             "adg_init_adaptive_intp": (i32, [vp, POINT_FN, vp, dbl]),
             "adg_ic_eval": (dbl, [DP, i32, vp]),
@@ -475,11 +475,12 @@ class Solution:
         """1: refine / coarsen and the element-table rebuild run on the GPU; 0: on the host."""
         self._check(lib().adg_set_device_adapt(self._h, int(on)))
 
-    def device_adapt_stats(self):
-        """(refine / coarsen calls done on the device, calls handed to the host)."""
-        a, b = ctypes.c_int64(), ctypes.c_int64()
-        self._check(lib().adg_device_adapt_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+    def device_adapt_stats(self, reasons=False):
+        """(refine / coarsen calls done on the device, calls handed to the host[, bits of why:
+        1 a norm at a threshold, 2 new elements beyond the table])."""
+        a, b, r = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        self._check(lib().adg_device_adapt_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(r)))
+        return (a.value, b.value, r.value) if reasons else (a.value, b.value)
 
     def grid_check(self):
         """Entries of the device element and fiber tables that differ from a host build."""

@@ -215,6 +215,9 @@ def main():
                          "at the per-context size), auto elsewhere")
     ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
                     help="chosen form: record whole RK steps into HIP graphs and replay them (adg_set_graph_step)")
+    ap.add_argument("--adapt", default="host", choices=("host", "device"),
+                    help="refine / coarsen and the element-table rebuild on the host (decisions from device norms) "
+                         "or on the GPU (adg_set_device_adapt)")
     ap.add_argument("--imex", action="store_true",
                     help="instead of the main measurement: IMEX43 with the assembled artificial viscosity, GPU "
                          "against the CPU oracle on one fixed grid")
@@ -272,6 +275,7 @@ def main():
     # fused form measured on `sol` is the chosen form in FP64
     for c in chosen_sols + ([] if batched else [sol]):
         c.set_graph_step(args.graph)
+        c.set_device_adapt(1 if args.adapt == "device" else 0)
         c.set_kernel_variant(args.variant)
         if mappings:
             c.set_class_variants(mappings)
@@ -403,6 +407,7 @@ def main():
     else:
         chosen = measure(chosen_sols, 1)
     value = chosen["value"]
+    dstats = [c.device_adapt_stats(reasons=True) for c in (bsols if batched else chosen_sols or [sol])]
     if learned is not None:
         learned["judge_deferred_leaves"] = int(sum(c.proposer_stats()[2] for c in chosen_sols))
     # instance 0 of the batch is the plain problem itself: equal L2 error means equal coefficients
@@ -507,6 +512,11 @@ def main():
                        refine=args.refine if learned else "predict", learned_proposer=learned,
                        chosen_time_steps_per_solve=chosen["time_steps"] // max(args.steps, 1),
                        judge_redone_instance_steps=int(chosen["redone"]),
+                       adapt=args.adapt, device_adapt_calls=int(sum(d[0] for d in dstats)),
+                       device_adapt_host_fallbacks=int(sum(d[1] for d in dstats)),
+                       device_adapt_fallback_reasons=int(np.bitwise_or.reduce([d[2] for d in dstats] or [0])),
+                       host_grid_share_per_context=(chosen["host"].get("grid", 0.) * 1e-3
+                                                    / (chosen["sec"] * (len(bsols) if batched else 1))),
                        sim_time=chosen["sim_time"]),
         "roofline": roofline, "cpu_baseline": cpu,
     }

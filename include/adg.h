@@ -206,9 +206,10 @@ int adg_coarsen(adg_ctx* c, double eta, int64_t* n_removed);
  * reads back counts only.  Both give the same element sets, order and
  * coefficients.  A call whose norm lies within four ulps of its threshold
  * (where the reference's pow decides) runs on the host instead.
- * adg_device_adapt_stats: calls done on the device, calls handed to the host. */
+ * adg_device_adapt_stats: calls done on the device, calls handed to the host
+ * and why (bits: 1 a norm at a threshold, 2 new elements beyond the table). */
 int adg_set_device_adapt(adg_ctx* c, int on);
-int adg_device_adapt_stats(adg_ctx* c, int64_t* device_calls, int64_t* host_fallbacks);
+int adg_device_adapt_stats(adg_ctx* c, int64_t* device_calls, int64_t* host_fallbacks, int32_t* reasons);
 /* Consistency check of the device element tables (no reference counterpart):
  * entries of the slots, instances, level sums, fiber bases, fiber levels and
  * fiber maps on the device that differ from a host build of the same grid. */

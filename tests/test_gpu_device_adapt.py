@@ -143,3 +143,57 @@ def test_batch_run_with_device_adapt_matches_host(config):
     print(f"[device adapt] calls on the device {dev}, handed to the host {host}", flush=True)
     assert dev > 0
     assert b.grid_check() == 0
+
+
+def test_refinement_larger_than_the_new_element_list():
+    """A refinement that adds more elements than the first new-element list
+    holds (a quarter of the table, at least 2^16) runs again on the device
+    with a larger list and still equals the host path."""
+    print("[case] device adapt large refinement", flush=True)
+    sols = []
+    for dev in (0, 1):
+        s = pyadg.Solution(2, 1, 1, 9, False, 6)
+        s.init_batch(12, sparse=False, level_init=6)  # 8,192 new elements per instance
+        s.set_device_adapt(dev)
+        rng = np.random.default_rng(11)
+        s.set(pyadg.FIELD_UCOE, rng.uniform(0.5, 1.0, s.num_elem * s.K))
+        sols.append(s)
+    a, b = sols
+    e0 = b.num_elem
+    na, nb = a.refine(1e-12), b.refine(1e-12)
+    print(f"[device adapt] {e0} elements, {nb} added", flush=True)
+    assert na == nb and nb > max(e0 // 4, 65536)
+    _same(a, b, "large refinement")
+    assert b.grid_check() == 0
+    assert b.device_adapt_stats() == (1, 0)
+
+
+def test_restore_and_rerun_with_device_tables():
+    """Restoring a snapshot after a run with device adapt rebuilds the saved
+    grid on the device (fields as saved), and a second run repeats the first
+    exactly."""
+    ws = [workloads.c2_burgers(seed=s, nmax=6, level_init=4, t_final=0.02) for s in (4, 9)]
+    print("[case] device adapt restore and rerun", flush=True)
+    b = workloads.make_batch(ws)
+    b.set_device_adapt(1)
+    b.save_state(0)
+    start = [(b.instance_grid(i), b.instance_get(i)) for i in range(len(ws))]
+    prob = pyadg.Problem.from_buffer_copy(ws[0].problem)
+    prob.form = 1
+    runs = []
+    for r in range(2):
+        if r:
+            b.restore_state(0)
+            assert b.grid_check() == 0
+            for i, (g, u) in enumerate(start):
+                for x, y in zip(b.instance_grid(i), g):
+                    assert np.array_equal(x, y), f"instance {i}: restored grid differs"
+                assert np.array_equal(b.instance_get(i), u), f"instance {i}: restored coefficients differ"
+        st = b.run_hyperbolic(prob)
+        runs.append(((st.steps, st.dof_stages), [(b.instance_grid(i), b.instance_get(i)) for i in range(len(ws))]))
+    assert runs[0][0] == runs[1][0]
+    for i in range(len(ws)):
+        (g1, u1), (g2, u2) = runs[0][1][i], runs[1][1][i]
+        for x, y in zip(g1, g2):
+            assert np.array_equal(x, y), f"instance {i}: second run's grid differs"
+        assert np.array_equal(u1, u2), f"instance {i}: " + first_diff(u1, u2)
