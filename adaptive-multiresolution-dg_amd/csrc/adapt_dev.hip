@@ -36,13 +36,9 @@ namespace {
 constexpr int kAB = 256;
 inline unsigned ablocks(int64_t n) { return static_cast<unsigned>(std::max<int64_t>((n + kAB - 1) / kAB, 1)); }
 constexpr uint64_t kFree = ~uint64_t{0};
-// closure layers run as grid-stride loops over a list whose length the host
-// does not know
-constexpr unsigned kCloseBlocks = 256;
 constexpr int64_t kMaxNew = int64_t{1} << 28;  // largest new-element list
 
 // counter words: [0] new (refine) or removed (coarsen) elements, [1] flags,
-// [2] [3] the range of the current closure layer in the new-element list,
 // [4 + t] elements removed by coarsen pass t
 enum : int32_t { AF_BAND = 1, AF_FULL = 2, AF_DUP = 4, AF_OPEN = 8, AF_NEG = 16 };
 constexpr int kCtr = 4 + 4 * 15 + 2;
@@ -208,36 +204,44 @@ __global__ __launch_bounds__(kAB) void k_ad_children(AdArgs a) {
   }
 }
 
-// one layer of the hole closure: the parents of the elements added by the
-// previous layer that the current table lacks
-__global__ __launch_bounds__(kAB) void k_ad_close(AdArgs a) {
-  const int32_t b = a.ctr[2], en = a.ctr[3];
-  for (int64_t w = b + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; w < en;
-       w += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const uint64_t key = a.newkey[w];
-    int s[kMaxDim];
-    for (int d = 0; d < a.D; ++d) s[d] = field(key, d, a.nb);
-    const int in = static_cast<int>(key >> (a.D * a.nb));
-    for (int d = 0; d < a.D; ++d) {
-      if (s[d] == 0) continue;
-      const int sd = s[d];
-      s[d] = sd >> 1;
-      const bool have = old_has(a.ukey, a.E, sort_key(s, in, a.D, a.lvtab, a.ctr));
-      s[d] = sd;
-      if (!have) add_new(a, with_field(key, d, a.nb, sd >> 1));
-    }
+// The hole closure in one workgroup: layer after layer, the parents (that
+// the current table lacks) of the elements the previous layer added, the
+// first layer over the children.  Every layer lowers the level sum by one, so
+// `layers` = D * NMAX + 1 always suffices; a last layer that still added
+// something is flagged.
+constexpr int kCloseThreads = 1024;
+__global__ __launch_bounds__(kCloseThreads) void k_ad_close(AdArgs a, int layers) {
+  __shared__ int32_t range[2];
+  if (threadIdx.x == 0) {
+    range[0] = 0;
+    range[1] = min(atomicAdd(a.ctr, 0), a.cap_new);
   }
-}
-
-__global__ void k_ad_advance(int32_t* ctr, int32_t cap_new) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  ctr[2] = ctr[3];
-  ctr[3] = min(ctr[0], cap_new);
-}
-
-__global__ void k_ad_closed(int32_t* ctr) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  if (ctr[2] != ctr[3]) atomicOr(ctr + 1, AF_OPEN);
+  __syncthreads();
+  for (int it = 0; it < layers; ++it) {
+    const int32_t b = range[0], en = range[1];
+    if (b >= en) break;
+    for (int32_t w = b + static_cast<int32_t>(threadIdx.x); w < en; w += kCloseThreads) {
+      const uint64_t key = a.newkey[w];
+      int s[kMaxDim];
+      for (int d = 0; d < a.D; ++d) s[d] = field(key, d, a.nb);
+      const int in = static_cast<int>(key >> (a.D * a.nb));
+      for (int d = 0; d < a.D; ++d) {
+        if (s[d] == 0) continue;
+        const int sd = s[d];
+        s[d] = sd >> 1;
+        const bool have = old_has(a.ukey, a.E, sort_key(s, in, a.D, a.lvtab, a.ctr));
+        s[d] = sd;
+        if (!have) add_new(a, with_field(key, d, a.nb, sd >> 1));
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      range[0] = en;
+      range[1] = min(atomicAdd(a.ctr, 0), a.cap_new);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && range[0] != range[1]) atomicOr(a.ctr + 1, AF_OPEN);
 }
 
 // the union table after a refinement: old elements, then the new list
@@ -360,6 +364,45 @@ __global__ __launch_bounds__(kAB) void k_ad_unlink(AdArgs a, int it) {
 __global__ __launch_bounds__(kAB) void k_ad_drop(AdArgs a, const int32_t* drop) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e < a.E) a.zmark[e] = drop[a.inst[e]] ? 1 : 0;
+}
+
+// All removal passes in one workgroup (tables of up to kSmallTable
+// elements): the same passes as k_ad_mark / k_ad_unlink, with barriers
+// between them; stops after a pass that removes nothing.
+constexpr int64_t kSmallTable = int64_t{1} << 17;
+__global__ __launch_bounds__(kCloseThreads) void k_ad_passes(AdArgs a, int passes) {
+  __shared__ int32_t removed;
+  for (int it = 1; it <= passes; ++it) {
+    if (threadIdx.x == 0) removed = 0;
+    __syncthreads();
+    int32_t mine = 0;
+    // (child counts change by atomics, which the L1 does not see: atomic loads)
+    for (int64_t e = threadIdx.x; e < a.E; e += kCloseThreads)
+      if (a.zmark[e] == 0 && a.dec[e] &&
+          __hip_atomic_load(a.nchild + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        a.zmark[e] = it;
+        ++mine;
+      }
+    if (mine) atomicAdd(&removed, mine);
+    __syncthreads();
+    const int32_t r = removed;
+    if (r == 0) break;
+    if (threadIdx.x == 0) {
+      atomicAdd(a.ctr, r);
+      if (it == passes) atomicOr(a.ctr + 1, AF_OPEN);
+    }
+    if (it == passes) break;
+    for (int64_t e = threadIdx.x; e < a.E; e += kCloseThreads) {
+      if (a.zmark[e] != it) continue;
+      for (int d = 0; d < a.D; ++d) {
+        const int s = a.slot[d][e];
+        if (s == 0) continue;
+        const int32_t p = a.fmap[d][a.fbase[d][e] + (s >> 1)];
+        if (p >= 0) atomicSub(a.nchild + p, 1);
+      }
+    }
+    __syncthreads();
+  }
 }
 
 __global__ __launch_bounds__(kAB) void k_ad_keep(const int32_t* zmark, int32_t* keep, int64_t E) {
@@ -570,13 +613,7 @@ int64_t Context::refine_device(const std::vector<double>& eps) {
   for (;;) {
     AdArgs a = adapt_args(*this, eps, cap_new);
     hipLaunchKernelGGL(k_ad_children, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a);
-    hipLaunchKernelGGL(k_ad_advance, dim3(1), dim3(1), 0, stream, a.ctr, cap_new);
-    // every layer lowers the level sum by one: D * NMAX + 1 layers close any hole
-    for (int it = 0; it <= D * nmax; ++it) {
-      hipLaunchKernelGGL(k_ad_close, dim3(kCloseBlocks), dim3(kAB), 0, stream, a);
-      hipLaunchKernelGGL(k_ad_advance, dim3(1), dim3(1), 0, stream, a.ctr, cap_new);
-    }
-    hipLaunchKernelGGL(k_ad_closed, dim3(1), dim3(1), 0, stream, a.ctr);
+    hipLaunchKernelGGL(k_ad_close, dim3(1), dim3(kCloseThreads), 0, stream, a, D * nmax + 1);
     h = read_counters(*this);
     const bool full = (h[1] & AF_FULL) || h[0] > cap_new;
     if (full && !(h[1] & AF_BAND) && cap_new < kMaxNew) {
@@ -641,9 +678,13 @@ int64_t Context::coarsen_device(const std::vector<double>& eta) {
   // below it, so passes 1 .. D * NMAX remove everything and pass D * NMAX + 1
   // finds nothing
   const int passes = D * nmax + 1;
-  for (int it = 1; it <= passes; ++it) {
-    hipLaunchKernelGGL(k_ad_mark, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, it, it == passes ? 1 : 0);
-    if (it < passes) hipLaunchKernelGGL(k_ad_unlink, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, it);
+  if (oldE <= kSmallTable) {
+    hipLaunchKernelGGL(k_ad_passes, dim3(1), dim3(kCloseThreads), 0, stream, a, passes);
+  } else {
+    for (int it = 1; it <= passes; ++it) {
+      hipLaunchKernelGGL(k_ad_mark, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, it, it == passes ? 1 : 0);
+      if (it < passes) hipLaunchKernelGGL(k_ad_unlink, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, it);
+    }
   }
   const std::array<int32_t, 2> h = read_counters(*this);
   if (h[1] & (AF_BAND | AF_FULL | AF_DUP | AF_OPEN) || h[0] >= oldE) {

@@ -215,9 +215,9 @@ def main():
                          "at the per-context size), auto elsewhere")
     ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
                     help="chosen form: record whole RK steps into HIP graphs and replay them (adg_set_graph_step)")
-    ap.add_argument("--adapt", default="host", choices=("host", "device"),
-                    help="refine / coarsen and the element-table rebuild on the host (decisions from device norms) "
-                         "or on the GPU (adg_set_device_adapt)")
+    ap.add_argument("--adapt", default="device", choices=("host", "device"),
+                    help="chosen form: refine / coarsen and the element-table rebuild on the GPU "
+                         "(adg_set_device_adapt, default) or on the host (decisions from device norms)")
     ap.add_argument("--imex", action="store_true",
                     help="instead of the main measurement: IMEX43 with the assembled artificial viscosity, GPU "
                          "against the CPU oracle on one fixed grid")
