@@ -480,6 +480,7 @@ def main():
         one = cpu_baseline(cws, args.cpu_seconds / 4, 1)
         cpu["one_thread_value"] = one["value"]
 
+    where = "GPU" if args.adapt == "device" else "host"
     pname = ("FP32 (split FP32 tables; FP64 right-hand-side accumulation, RK and norms)"
              if prec == pyadg.PREC_FP32_SPLIT and not w.linear else "FP64")
     line = {
@@ -489,8 +490,9 @@ def main():
         "data": "synthetic",
         "config": dict(w.config(), step="one full solve from the saved initial state",
                        form=("chosen: fused %s stages, %d seeded instances (seeds %d..%d) in %d context(s) solved "
-                             "concurrently" % (pname, args.batch, args.seed, args.seed + args.batch - 1, len(bsols)))
-                       if batched else "chosen: fused %s stages" % pname,
+                             "concurrently, refine/coarsen on the %s"
+                             % (pname, args.batch, args.seed, args.seed + args.batch - 1, len(bsols), where))
+                       if batched else "chosen: fused %s stages, refine/coarsen on the %s" % (pname, where),
                        launches_per_stage=chosen["launches_per_stage"], graph_step=args.graph,
                        kernel_variant=args.variant, mappings=args.mappings,
                        graph_steps_recorded=chosen["graph_steps_recorded"],
