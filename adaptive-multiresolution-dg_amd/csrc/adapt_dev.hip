@@ -39,7 +39,8 @@ constexpr uint64_t kFree = ~uint64_t{0};
 constexpr int64_t kMaxNew = int64_t{1} << 28;  // largest new-element list
 
 // counter words: [0] new (refine) or removed (coarsen) elements, [1] flags,
-// [4 + t] elements removed by coarsen pass t
+// [2] coarsen candidates, [3] first closure layer, [4 + t] elements removed
+// by coarsen pass t
 enum : int32_t { AF_BAND = 1, AF_FULL = 2, AF_DUP = 4, AF_OPEN = 8, AF_NEG = 16 };
 constexpr int kCtr = 4 + 4 * 15 + 2;
 
@@ -128,6 +129,7 @@ struct AdArgs {
   int32_t* nchild;
   int32_t* zmark;
   int32_t* dec;
+  int32_t* cand;  // coarsen candidates (below eta, positive maximum level)
   int64_t E;
   int D, nb, nmax;
   int32_t cap_new;
@@ -159,9 +161,11 @@ __global__ __launch_bounds__(kAB) void k_ad_fill32(int32_t* p, int64_t n, int32_
   if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(kAB) void k_ad_fillu32(uint32_t* p, int64_t n, uint32_t v) {
+// an empty fiber map: every entry -1, every occupancy word 0
+__global__ __launch_bounds__(kAB) void k_ad_clear_map(int32_t* fmap, int64_t nmap, uint32_t* fbits, int64_t nwords) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  if (i < nmap) fmap[i] = -1;
+  if (i < nwords) fbits[i] = 0u;
 }
 
 // refine: leaves above eps add their missing children.  The leaf test and
@@ -204,36 +208,50 @@ __global__ __launch_bounds__(kAB) void k_ad_children(AdArgs a) {
   }
 }
 
-// The hole closure in one workgroup: layer after layer, the parents (that
-// the current table lacks) of the elements the previous layer added, the
-// first layer over the children.  Every layer lowers the level sum by one, so
-// `layers` = D * NMAX + 1 always suffices; a last layer that still added
-// something is flagged.
+// One element of a hole-closure layer: its parents that neither the current
+// table (binary search in its sorted keys) nor the new-key table holds.
+__device__ inline void close_one(const AdArgs& a, int32_t w) {
+  const uint64_t key = a.newkey[w];
+  int s[kMaxDim];
+  for (int d = 0; d < a.D; ++d) s[d] = field(key, d, a.nb);
+  const int in = static_cast<int>(key >> (a.D * a.nb));
+  for (int d = 0; d < a.D; ++d) {
+    if (s[d] == 0) continue;
+    const int sd = s[d];
+    s[d] = sd >> 1;
+    const bool have = old_has(a.ukey, a.E, sort_key(s, in, a.D, a.lvtab, a.ctr));
+    s[d] = sd;
+    if (!have) add_new(a, with_field(key, d, a.nb, sd >> 1));
+  }
+}
+
+// ctr[3] = the children's count, the extent of the first closure layer
+__global__ void k_ad_snap(int32_t* ctr, int32_t cap_new) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr[3] = min(ctr[0], cap_new);
+}
+
+// first closure layer (the parents of the children, the largest), grid-stride
+__global__ __launch_bounds__(kAB) void k_ad_close_first(AdArgs a) {
+  const int32_t n = a.ctr[3];
+  for (int32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) close_one(a, w);
+}
+
+// The remaining layers in one workgroup: layer after layer, the parents of the
+// elements the previous layer added.  Every layer lowers the level sum by one,
+// so `layers` = D * NMAX layers after the first always suffice; a last layer
+// that still added something is flagged.
 constexpr int kCloseThreads = 1024;
 __global__ __launch_bounds__(kCloseThreads) void k_ad_close(AdArgs a, int layers) {
   __shared__ int32_t range[2];
   if (threadIdx.x == 0) {
-    range[0] = 0;
+    range[0] = a.ctr[3];
     range[1] = min(atomicAdd(a.ctr, 0), a.cap_new);
   }
   __syncthreads();
   for (int it = 0; it < layers; ++it) {
     const int32_t b = range[0], en = range[1];
     if (b >= en) break;
-    for (int32_t w = b + static_cast<int32_t>(threadIdx.x); w < en; w += kCloseThreads) {
-      const uint64_t key = a.newkey[w];
-      int s[kMaxDim];
-      for (int d = 0; d < a.D; ++d) s[d] = field(key, d, a.nb);
-      const int in = static_cast<int>(key >> (a.D * a.nb));
-      for (int d = 0; d < a.D; ++d) {
-        if (s[d] == 0) continue;
-        const int sd = s[d];
-        s[d] = sd >> 1;
-        const bool have = old_has(a.ukey, a.E, sort_key(s, in, a.D, a.lvtab, a.ctr));
-        s[d] = sd;
-        if (!have) add_new(a, with_field(key, d, a.nb, sd >> 1));
-      }
-    }
+    for (int32_t w = b + static_cast<int32_t>(threadIdx.x); w < en; w += kCloseThreads) close_one(a, w);
     __syncthreads();
     if (threadIdx.x == 0) {
       range[0] = en;
@@ -324,7 +342,9 @@ __global__ __launch_bounds__(kAB) void k_ad_nchild(AdArgs a) {
   }
 }
 
-__global__ __launch_bounds__(kAB) void k_ad_below(AdArgs a) {
+// (first_pass: also mark pass 1, the childless candidates, into list0 with
+// its count in ctr[3]; the child counts are final here)
+__global__ __launch_bounds__(kAB) void k_ad_below(AdArgs a, int32_t* list0) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= a.E) return;
   bool any = false;
@@ -333,31 +353,48 @@ __global__ __launch_bounds__(kAB) void k_ad_below(AdArgs a) {
   const double t = a.thr[a.inst[e]];
   const bool band = in_band(a.sumsq[e], t, r);
   if (band && any) atomicOr(a.ctr + 1, AF_BAND);
-  a.dec[e] = (any && !band && r < t) ? 1 : 0;
+  const bool below = any && !band && r < t;
+  a.dec[e] = below ? 1 : 0;
   a.zmark[e] = 0;
+  // only candidates can be removed: the passes walk this list (its order
+  // does not matter, every pass decides on the state before it)
+  if (below) a.cand[atomicAdd(a.ctr + 2, 1)] = static_cast<int32_t>(e);
+  if (below && list0 && a.nchild[e] == 0) {
+    a.zmark[e] = 1;
+    list0[atomicAdd(a.ctr + 3, 1)] = static_cast<int32_t>(e);
+  }
 }
 
 // pass `it` (1, 2, ...): every element alive, childless and below eta is
 // marked from the state after the previous pass
 // (a pass after one that removed nothing changes nothing and exits)
 __global__ __launch_bounds__(kAB) void k_ad_mark(AdArgs a, int it, int last) {
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= a.E || (it > 1 && a.ctr[4 + it - 1] == 0)) return;
-  if (a.zmark[e] != 0 || a.nchild[e] != 0 || !a.dec[e]) return;
-  a.zmark[e] = it;
-  atomicAdd(a.ctr, 1);
-  atomicAdd(a.ctr + 4 + it, 1);
-  if (last) atomicOr(a.ctr + 1, AF_OPEN);
+  if (it > 1 && a.ctr[4 + it - 1] == 0) return;
+  const int32_t n = a.ctr[2];
+  for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < n;
+       j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t e = a.cand[j];
+    if (a.zmark[e] != 0 || a.nchild[e] != 0) continue;
+    a.zmark[e] = it;
+    atomicAdd(a.ctr, 1);
+    atomicAdd(a.ctr + 4 + it, 1);
+    if (last) atomicOr(a.ctr + 1, AF_OPEN);
+  }
 }
 
 __global__ __launch_bounds__(kAB) void k_ad_unlink(AdArgs a, int it) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= a.E || a.ctr[4 + it] == 0 || a.zmark[e] != it) return;
-  for (int d = 0; d < a.D; ++d) {
-    const int s = a.slot[d][e];
-    if (s == 0) continue;
-    const int32_t p = a.fmap[d][a.fbase[d][e] + (s >> 1)];
-    if (p >= 0) atomicSub(a.nchild + p, 1);
+  if (a.ctr[4 + it] == 0) return;
+  const int32_t n = a.ctr[2];
+  for (int64_t j = e; j < n; j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t c = a.cand[j];
+    if (a.zmark[c] != it) continue;
+    for (int d = 0; d < a.D; ++d) {
+      const int s = a.slot[d][c];
+      if (s == 0) continue;
+      const int32_t p = a.fmap[d][a.fbase[d][c] + (s >> 1)];
+      if (p >= 0) atomicSub(a.nchild + p, 1);
+    }
   }
 }
 
@@ -367,41 +404,48 @@ __global__ __launch_bounds__(kAB) void k_ad_drop(AdArgs a, const int32_t* drop) 
 }
 
 // All removal passes in one workgroup (tables of up to kSmallTable
-// elements): the same passes as k_ad_mark / k_ad_unlink, with barriers
-// between them; stops after a pass that removes nothing.
+// elements), the passes of k_ad_mark / k_ad_unlink with barriers between
+// them.  Pass 1 is the childless candidates; after it an element can only become
+// removable when its last child goes, so the thread whose decrement takes a
+// parent's count to zero puts that parent (if it is a candidate) on the next
+// pass's list.  Stops after a pass that removes nothing.
 constexpr int64_t kSmallTable = int64_t{1} << 17;
-__global__ __launch_bounds__(kCloseThreads) void k_ad_passes(AdArgs a, int passes) {
-  __shared__ int32_t removed;
+__global__ __launch_bounds__(kCloseThreads) void k_ad_passes(AdArgs a, int passes, int32_t* list0, int32_t* list1) {
+  __shared__ int32_t count[2];
+  // pass 1 was marked by k_ad_below (list0, ctr[3] elements)
+  if (threadIdx.x == 0) {
+    count[0] = a.ctr[3];
+    count[1] = 0;
+  }
+  __syncthreads();
+  int32_t* cur = list0;
+  int32_t* nxt = list1;
   for (int it = 1; it <= passes; ++it) {
-    if (threadIdx.x == 0) removed = 0;
-    __syncthreads();
-    int32_t mine = 0;
-    // (child counts change by atomics, which the L1 does not see: atomic loads)
-    for (int64_t e = threadIdx.x; e < a.E; e += kCloseThreads)
-      if (a.zmark[e] == 0 && a.dec[e] &&
-          __hip_atomic_load(a.nchild + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-        a.zmark[e] = it;
-        ++mine;
-      }
-    if (mine) atomicAdd(&removed, mine);
-    __syncthreads();
-    const int32_t r = removed;
-    if (r == 0) break;
+    const int32_t m = count[(it - 1) & 1];
+    if (m == 0) break;
     if (threadIdx.x == 0) {
-      atomicAdd(a.ctr, r);
+      atomicAdd(a.ctr, m);
       if (it == passes) atomicOr(a.ctr + 1, AF_OPEN);
+      count[it & 1] = 0;
     }
     if (it == passes) break;
-    for (int64_t e = threadIdx.x; e < a.E; e += kCloseThreads) {
-      if (a.zmark[e] != it) continue;
+    __syncthreads();
+    for (int32_t j = threadIdx.x; j < m; j += kCloseThreads) {
+      const int32_t e = cur[j];
       for (int d = 0; d < a.D; ++d) {
         const int s = a.slot[d][e];
         if (s == 0) continue;
         const int32_t p = a.fmap[d][a.fbase[d][e] + (s >> 1)];
-        if (p >= 0) atomicSub(a.nchild + p, 1);
+        if (p >= 0 && atomicSub(a.nchild + p, 1) == 1 && a.dec[p] && a.zmark[p] == 0) {
+          a.zmark[p] = it + 1;
+          nxt[atomicAdd(&count[it & 1], 1)] = p;
+        }
       }
     }
     __syncthreads();
+    int32_t* t = cur;
+    cur = nxt;
+    nxt = t;
   }
 }
 
@@ -439,9 +483,14 @@ __global__ __launch_bounds__(kAB) void k_ad_fib_keys(FibArgs a) {
   a.vals[q] = static_cast<int32_t>(q);
 }
 
-__global__ __launch_bounds__(kAB) void k_ad_run_flags(const uint64_t* skeys, int32_t* flags, int64_t N) {
+// run starts of the sorted fiber keys; zeroes the per-run accumulators
+__global__ __launch_bounds__(kAB) void k_ad_run_flags(const uint64_t* skeys, int32_t* flags, int64_t N, int32_t* runmax,
+                                                     int32_t* runlen) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < N) flags[i] = (i == 0 || skeys[i] != skeys[i - 1]) ? 1 : 0;
+  if (i >= N) return;
+  flags[i] = (i == 0 || skeys[i] != skeys[i - 1]) ? 1 : 0;
+  runmax[i] = 0;
+  runlen[i] = 0;
 }
 
 __global__ __launch_bounds__(kAB) void k_ad_runs(const int32_t* svals, const int32_t* run, const int32_t* slot_dd,
@@ -613,7 +662,9 @@ int64_t Context::refine_device(const std::vector<double>& eps) {
   for (;;) {
     AdArgs a = adapt_args(*this, eps, cap_new);
     hipLaunchKernelGGL(k_ad_children, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a);
-    hipLaunchKernelGGL(k_ad_close, dim3(1), dim3(kCloseThreads), 0, stream, a, D * nmax + 1);
+    hipLaunchKernelGGL(k_ad_snap, dim3(1), dim3(1), 0, stream, a.ctr, cap_new);
+    hipLaunchKernelGGL(k_ad_close_first, dim3(std::min<unsigned>(ablocks(oldE), 1024)), dim3(kAB), 0, stream, a);
+    hipLaunchKernelGGL(k_ad_close, dim3(1), dim3(kCloseThreads), 0, stream, a, D * nmax);
     h = read_counters(*this);
     const bool full = (h[1] & AF_FULL) || h[0] > cap_new;
     if (full && !(h[1] & AF_BAND) && cap_new < kMaxNew) {
@@ -668,22 +719,30 @@ int64_t Context::coarsen_device(const std::vector<double>& eta) {
   ad_nchild.ensure(static_cast<size_t>(oldE));
   ad_zmark.ensure(static_cast<size_t>(oldE));
   ad_dec.ensure(static_cast<size_t>(oldE));
+  ad_cand.ensure(static_cast<size_t>(oldE));
   a.nchild = ad_nchild.p;
   a.zmark = ad_zmark.p;
   a.dec = ad_dec.p;
+  a.cand = ad_cand.p;
   hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(oldE)), dim3(kAB), 0, stream, ad_nchild.p, oldE, 0);
   hipLaunchKernelGGL(k_ad_nchild, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a);
-  hipLaunchKernelGGL(k_ad_below, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a);
+  const bool small = oldE <= kSmallTable;
+  if (small) {
+    ad_vals[0].ensure(static_cast<size_t>(oldE));
+    ad_vals[1].ensure(static_cast<size_t>(oldE));
+  }
+  hipLaunchKernelGGL(k_ad_below, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, small ? ad_vals[0].p : nullptr);
   // an element removed in pass t has a chain of t - 1 removed descendants
   // below it, so passes 1 .. D * NMAX remove everything and pass D * NMAX + 1
   // finds nothing
   const int passes = D * nmax + 1;
-  if (oldE <= kSmallTable) {
-    hipLaunchKernelGGL(k_ad_passes, dim3(1), dim3(kCloseThreads), 0, stream, a, passes);
+  if (small) {
+    hipLaunchKernelGGL(k_ad_passes, dim3(1), dim3(kCloseThreads), 0, stream, a, passes, ad_vals[0].p, ad_vals[1].p);
   } else {
+    const unsigned g = std::min<unsigned>(ablocks(oldE), 1024);
     for (int it = 1; it <= passes; ++it) {
-      hipLaunchKernelGGL(k_ad_mark, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, it, it == passes ? 1 : 0);
-      if (it < passes) hipLaunchKernelGGL(k_ad_unlink, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a, it);
+      hipLaunchKernelGGL(k_ad_mark, dim3(g), dim3(kAB), 0, stream, a, it, it == passes ? 1 : 0);
+      if (it < passes) hipLaunchKernelGGL(k_ad_unlink, dim3(g), dim3(kAB), 0, stream, a, it);
     }
   }
   const std::array<int32_t, 2> h = read_counters(*this);
@@ -762,12 +821,11 @@ void Context::device_grid_finish(int64_t N, int64_t oldE) {
     hip_check(hipcub::DeviceRadixSort::SortPairs(ad_tmp.p, b1, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p,
                                                  ad_vals[1].p, static_cast<int>(N), 0, endbit, stream),
               "sort (fiber keys)");
-    hipLaunchKernelGGL(k_ad_run_flags, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_keys[1].p, ad_vals[0].p, N);
+    hipLaunchKernelGGL(k_ad_run_flags, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_keys[1].p, ad_vals[0].p, N,
+                       ad_runmax.p, ad_runlen.p);
     size_t b2 = cbytes;
     hip_check(hipcub::DeviceScan::InclusiveSum(ad_tmp.p, b2, ad_vals[0].p, ad_run.p, static_cast<int>(N), stream),
               "scan (fiber runs)");
-    hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_runmax.p, N, 0);
-    hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_runlen.p, N, 0);
     ad_fbase[dd].ensure(static_cast<size_t>(N));
     ad_fmaxl[dd].ensure(static_cast<size_t>(N));
     hipLaunchKernelGGL(k_ad_runs, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_vals[1].p, ad_run.p, ad_slot[dd].p,
@@ -801,10 +859,10 @@ void Context::device_grid_finish(int64_t N, int64_t oldE) {
     const int64_t nfib = meta[at_nfib + d];
     const int64_t nmap = std::max<int64_t>(nfib << nmax, 1);
     g_fmap[d].ensure(static_cast<size_t>(nmap));
-    hipLaunchKernelGGL(k_ad_fill32, dim3(ablocks(nmap)), dim3(kAB), 0, stream, g_fmap[d].p, nmap, -1);
     const int64_t nwords = (nmap + 31) / 32;
     g_fbits[d].ensure(static_cast<size_t>(nwords + 1));
-    hipLaunchKernelGGL(k_ad_fillu32, dim3(ablocks(nwords + 1)), dim3(kAB), 0, stream, g_fbits[d].p, nwords + 1, 0u);
+    hipLaunchKernelGGL(k_ad_clear_map, dim3(ablocks(std::max(nmap, nwords + 1))), dim3(kAB), 0, stream, g_fmap[d].p,
+                       nmap, g_fbits[d].p, nwords + 1);
     fbits_words[d] = nwords;
     hipLaunchKernelGGL(k_ad_fmap, dim3(ablocks(N)), dim3(kAB), 0, stream, g_fmap[d].p, g_fbits[d].p, ad_fbase[d].p,
                        ad_slot[d].p, N);

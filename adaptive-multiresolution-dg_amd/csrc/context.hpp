@@ -254,7 +254,7 @@ class Context {
   void host_union();               // uslot, uhash, uinst and inst_lmax from grids
   DevBuf<uint64_t> g_ukey;  // (instance << 32 | hash key) of every element, ascending
   DevBuf<uint64_t> ad_hkey, ad_newkey, ad_keys[2], ad_ukey;
-  DevBuf<int32_t> ad_ctr, ad_vals[2], ad_run, ad_runmax, ad_runlen, ad_meta, ad_nchild, ad_zmark, ad_dec,
+  DevBuf<int32_t> ad_ctr, ad_vals[2], ad_run, ad_runmax, ad_runlen, ad_meta, ad_nchild, ad_zmark, ad_dec, ad_cand,
       ad_lvtab;
   DevBuf<int32_t> ad_slot[kMaxDim], ad_fbase[kMaxDim], ad_fmaxl[kMaxDim], ad_inst, ad_lsum;
   DevBuf<double> ad_thr;
