@@ -51,11 +51,17 @@ def test_c1_full_run_bitexact_and_error():
     assert err[1] < 1e-4
 
 
+# refine / coarsen on the host (0) or on the device (1, adg_set_device_adapt)
+ADAPT = [pytest.param(0, id="host-adapt"), pytest.param(1, id="device-adapt")]
+
+
+@pytest.mark.parametrize("adapt", ADAPT)
 @pytest.mark.parametrize("seed", [1, 2])
-def test_c2_full_solve_bitexact(seed):
+def test_c2_full_solve_bitexact(seed, adapt):
     w = workloads.c2_burgers(seed=seed)
-    print(f"[case] c2: seed {seed}, NMAX 8, N0 6, t=0.1", flush=True)
+    print(f"[case] c2: seed {seed}, NMAX 8, N0 6, t=0.1, adapt {adapt}", flush=True)
     g = workloads.make_solution(w)
+    g.set_device_adapt(adapt)
     o = oracle_for(w)
     assert same_set(g, o)
     w.problem.form = 1
@@ -81,13 +87,16 @@ def test_c3_full_grid_steps_bitexact():
     assert np.array_equal(a, b), first_diff(a, b)
 
 
-def test_c4_full_size_instances_bitexact():
+@pytest.mark.parametrize("adapt", ADAPT)
+def test_c4_full_size_instances_bitexact(adapt):
     ws = [workloads.c4_advection_3d(seed=s) for s in (1, 3, 5, 8)]
-    print("[case] c4: 4 instances at NMAX 7, 5 adaptive steps", flush=True)
+    print(f"[case] c4: 4 instances at NMAX 7, 5 adaptive steps, adapt {adapt}", flush=True)
     b = workloads.make_batch(ws)
+    b.set_device_adapt(adapt)
     prob = pyadg.Problem.from_buffer_copy(ws[0].problem)
     prob.form, prob.max_steps = 1, 5
     b.run_hyperbolic(prob)
+    assert b.device_adapt_stats() == ((10, 0) if adapt else (0, 0))
     for i, w in enumerate(ws):
         o = oracle_for(w)
         p = w.oracle_problem()
@@ -100,10 +109,12 @@ def test_c4_full_size_instances_bitexact():
         assert np.array_equal(a, c), f"instance {i}: " + first_diff(a, c)
 
 
-def test_c5_full_size_adaptive_steps():
+@pytest.mark.parametrize("adapt", ADAPT)
+def test_c5_full_size_adaptive_steps(adapt):
     w = workloads.c5_rotating(seed=1)
-    print("[case] c5: NMAX 9, 5 adaptive steps", flush=True)
+    print(f"[case] c5: NMAX 9, 5 adaptive steps, adapt {adapt}", flush=True)
     g = workloads.make_solution(w)
+    g.set_device_adapt(adapt)
     o = oracle_for(w)
     assert same_set(g, o)
     w.problem.form, w.problem.max_steps = 1, 5
