@@ -120,6 +120,7 @@ struct AdArgs {
   const int32_t* inst;
   const double* sumsq;
   const double* thr;  // per instance
+  const int32_t* flags;  // refine: elements to refine (if set; else the norm test against thr)
   const uint64_t* ukey;   // sorted (instance, hash key) of the current table
   const int32_t* lvtab;   // level-vector part of the hash key
   uint64_t* hkey;         // insert-if-absent table of the new elements
@@ -175,10 +176,14 @@ __global__ __launch_bounds__(kAB) void k_ad_clear_map(int32_t* fmap, int64_t nma
 __global__ __launch_bounds__(kAB) void k_ad_children(AdArgs a) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= a.E) return;
-  double r;
-  const double t = a.thr[a.inst[e]];
-  if (in_band(a.sumsq[e], t, r)) { atomicOr(a.ctr + 1, AF_BAND); return; }
-  if (!(r > t)) return;
+  if (a.flags) {
+    if (!a.flags[e]) return;
+  } else {
+    double r;
+    const double t = a.thr[a.inst[e]];
+    if (in_band(a.sumsq[e], t, r)) { atomicOr(a.ctr + 1, AF_BAND); return; }
+    if (!(r > t)) return;
+  }
   int s[kMaxDim];
   int total = 0, have = 0;
   for (int d = 0; d < a.D; ++d) {
@@ -540,6 +545,61 @@ __global__ __launch_bounds__(kAB) void k_ad_fmap(int32_t* fmap, uint32_t* fbits,
   atomicOr(&fbits[b >> 5], 1u << (b & 31));
 }
 
+// whether each element is a leaf (fewer children present than possible)
+__global__ __launch_bounds__(kAB) void k_ad_leaf(AdArgs a, int32_t* leaf) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= a.E) return;
+  int total = 0, have = 0;
+  for (int d = 0; d < a.D; ++d) {
+    const int s = a.slot[d][e];
+    const int l = slot_level(s);
+    const int32_t* fm = a.fmap[d] + a.fbase[d][e];
+    if (l == 0) {
+      total += 1;
+      have += fm[1] >= 0;
+    } else if (l < a.nmax) {
+      total += 2;
+      have += (fm[2 * s] >= 0) + (fm[2 * s + 1] >= 0);
+    }
+  }
+  leaf[e] = have < total ? 1 : 0;
+}
+
+// a per-element flag carried to the table after a refinement (new: 0)
+__global__ __launch_bounds__(kAB) void k_ad_carry(const int32_t* remap, const int32_t* old, int32_t* out, int64_t N) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q < N) out[q] = remap[q] >= 0 ? old[remap[q]] : 0;
+}
+
+__global__ __launch_bounds__(kAB) void k_ad_prob_flags(const float* prob, double thr, int32_t* flags, int64_t E) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < E) flags[e] = static_cast<double>(prob[e]) > thr ? 1 : 0;
+}
+
+// flags[e] = (mask[e] if a mask) and norm above the instance's threshold;
+// norms within the band where pow decides are listed for the host
+__global__ __launch_bounds__(kAB) void k_ad_norm_flags(AdArgs a, const int32_t* mask, int32_t* flags, int32_t* band) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= a.E) return;
+  if (mask && !mask[e]) {
+    flags[e] = 0;
+    return;
+  }
+  double r;
+  const double t = a.thr[a.inst[e]];
+  if (in_band(a.sumsq[e], t, r)) {
+    flags[e] = 0;
+    band[atomicAdd(a.ctr + 2, 1)] = static_cast<int32_t>(e);
+    return;
+  }
+  flags[e] = r > t ? 1 : 0;
+}
+
+__global__ void k_ad_set_flags(const int32_t* ids, const int32_t* vals, int32_t n, int32_t* flags) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flags[ids[i]] = vals[i];
+}
+
 int bits_for(int64_t n) {  // bits to hold 0..n-1 (at least one)
   int b = 1;
   while ((int64_t{1} << b) < n) ++b;
@@ -555,7 +615,7 @@ void to_device(DevBuf<T>& b, const std::vector<T>& h, hipStream_t s) {
 }  // namespace
 
 bool Context::device_adapt_fits(int64_t n) const {
-  if (!device_adapt || refine_mode != 0 || recording || !up_host.empty() || n <= 0 || B() < 1) return false;
+  if (!device_adapt || recording || !up_host.empty() || n <= 0 || B() < 1) return false;
   // element keys in 63 bits, int32 element indices (the fiber bases are
   // checked against int32 once the fibers are counted)
   return D * nmax + bits_for(B()) <= 62 && n < (int64_t{1} << 31) - 1;
@@ -563,7 +623,7 @@ bool Context::device_adapt_fits(int64_t n) const {
 
 // Table of the current grid, per-instance thresholds and the norms (shared
 // set-up of refine and coarsen); returns the kernel arguments.
-static AdArgs adapt_args(Context& c, const std::vector<double>& thr, int32_t cap_new) {
+static AdArgs adapt_args(Context& c, const std::vector<double>* thr, int32_t cap_new) {
   const int64_t E = c.E();
   if (c.ad_lvtab.cap == 0) to_device(c.ad_lvtab, c.hk.level_table(c.D), c.stream);
   uint64_t tcap = 16;
@@ -571,8 +631,10 @@ static AdArgs adapt_args(Context& c, const std::vector<double>& thr, int32_t cap
   if (cap_new > 0) c.ad_hkey.ensure(tcap);
   c.ad_newkey.ensure(static_cast<size_t>(std::max<int32_t>(cap_new, 1)));
   c.ad_ctr.ensure(kCtr);
-  to_device(c.ad_thr, thr, c.stream);
-  c.indicator_sumsq_dev();
+  if (thr) {  // the norm test (else the caller's flags decide)
+    to_device(c.ad_thr, *thr, c.stream);
+    c.indicator_sumsq_dev();
+  }
   AdArgs a{};
   for (int d = 0; d < c.D; ++d) {
     a.slot[d] = c.g_slot[d].p;
@@ -652,7 +714,11 @@ static void compact_survivors(Context& c, const AdArgs& a, int64_t N) {
   c.device_grid_finish(N, oldE);
 }
 
-int64_t Context::refine_device(const std::vector<double>& eps) {
+int64_t Context::refine_device(const std::vector<double>& eps) { return refine_device_impl(&eps, nullptr); }
+
+int64_t Context::refine_device_flags(const int32_t* flags) { return refine_device_impl(nullptr, flags); }
+
+int64_t Context::refine_device_impl(const std::vector<double>* eps, const int32_t* flags) {
   const int64_t oldE = E();
   if (!device_adapt_fits(oldE)) return -1;
   // new-element list of E/4 (at least 2^16); a refinement that adds more
@@ -661,6 +727,7 @@ int64_t Context::refine_device(const std::vector<double>& eps) {
   std::array<int32_t, 2> h{0, 0};
   for (;;) {
     AdArgs a = adapt_args(*this, eps, cap_new);
+    a.flags = flags;
     hipLaunchKernelGGL(k_ad_children, dim3(ablocks(oldE)), dim3(kAB), 0, stream, a);
     hipLaunchKernelGGL(k_ad_snap, dim3(1), dim3(1), 0, stream, a.ctr, cap_new);
     hipLaunchKernelGGL(k_ad_close_first, dim3(std::min<unsigned>(ablocks(oldE), 1024)), dim3(kAB), 0, stream, a);
@@ -715,7 +782,7 @@ int64_t Context::refine_device(const std::vector<double>& eps) {
 int64_t Context::coarsen_device(const std::vector<double>& eta) {
   const int64_t oldE = E();
   if (!device_adapt_fits(oldE)) return -1;
-  AdArgs a = adapt_args(*this, eta, 0);
+  AdArgs a = adapt_args(*this, &eta, 0);
   ad_nchild.ensure(static_cast<size_t>(oldE));
   ad_zmark.ensure(static_cast<size_t>(oldE));
   ad_dec.ensure(static_cast<size_t>(oldE));
@@ -965,6 +1032,81 @@ bool Context::set_grids_fresh(const std::vector<std::vector<int32_t>>& slots) {
     hipLaunchKernelGGL(k_ad_fill64, dim3(ablocks(N * K)), dim3(kAB), 0, stream, reinterpret_cast<uint64_t*>(b->p),
                        N * K, uint64_t{0});
   return true;
+}
+
+void Context::norm_flags_dev(const std::vector<double>& thr, const int32_t* mask, int32_t* flags) {
+  const int64_t n = E();
+  AdArgs a = adapt_args(*this, &thr, 0);
+  ad_band.ensure(static_cast<size_t>(std::max<int64_t>(n, 1)));
+  hipLaunchKernelGGL(k_ad_norm_flags, dim3(ablocks(n)), dim3(kAB), 0, stream, a, mask, flags, ad_band.p);
+  int32_t nband = 0;
+  hip_check(hipMemcpyAsync(&nband, ad_ctr.p + 2, sizeof(int32_t), hipMemcpyDeviceToHost, stream), "download");
+  sync();
+  if (nband == 0) return;
+  // norms within a few ulps of the threshold: the reference's pow decides
+  std::vector<int32_t> ids(static_cast<size_t>(nband)), vals(static_cast<size_t>(nband));
+  std::vector<double> s(static_cast<size_t>(n));
+  hip_check(hipMemcpyAsync(ids.data(), ad_band.p, nband * sizeof(int32_t), hipMemcpyDeviceToHost, stream), "download");
+  hip_check(hipMemcpyAsync(s.data(), sumsq.p, n * sizeof(double), hipMemcpyDeviceToHost, stream), "download");
+  sync();
+  for (int32_t q = 0; q < nband; ++q) {
+    const int64_t e = ids[q];
+    const int i = static_cast<int>(std::upper_bound(off.begin(), off.end(), e) - off.begin()) - 1;
+    vals[q] = threshold_norm(s[e], thr[i]) > thr[i] ? 1 : 0;
+  }
+  DevBuf<int32_t> dids, dvals;
+  to_device(dids, ids, stream);
+  to_device(dvals, vals, stream);
+  hipLaunchKernelGGL(k_ad_set_flags, dim3(ablocks(nband)), dim3(kAB), 0, stream, dids.p, dvals.p, nband, flags);
+  sync();  // (the id and value buffers are freed on return)
+}
+
+int64_t Context::proposer_step_device(const std::function<void(int, int, double)>& stage,
+                                      const std::vector<double>& eps) {
+  const int64_t E0 = E();
+  // leaves of the grid at the start of the step (the judge's candidates)
+  for (DevBuf<int32_t>& b : ad_leaf) b.ensure(static_cast<size_t>(E0));
+  ad_flag.ensure(static_cast<size_t>(E0));
+  {
+    AdArgs a = adapt_args(*this, nullptr, 0);
+    hipLaunchKernelGGL(k_ad_leaf, dim3(ablocks(E0)), dim3(kAB), 0, stream, a, ad_leaf[0].p);
+  }
+  // proposals: leaves with a probability above the threshold (learned), or
+  // with a current norm above eps, get their children
+  {
+    HostTimer timer(*this, HOST_ADAPT);
+    if (refine_mode == 1) {
+      prop_feats.ensure(static_cast<size_t>(std::max<int64_t>(E0, 1)) * kPropIn);
+      prop_leaf.ensure(std::max<int64_t>(E0, 1));
+      prop_prob.ensure(std::max<int64_t>(E0, 1));
+      proposer_features(eps, prop_feats.p, prop_leaf.p);
+      proposer_predict(prop_feats.p, E0, prop_prob.p);
+      hipLaunchKernelGGL(k_ad_prob_flags, dim3(ablocks(E0)), dim3(kAB), 0, stream, prop_prob.p, prop_threshold,
+                         ad_flag.p, E0);
+    } else {
+      norm_flags_dev(eps, nullptr, ad_flag.p);
+    }
+    const int64_t added = refine_device_flags(ad_flag.p);
+    if (added < 0) throw std::runtime_error("device refinement of the proposals failed");
+    stat_proposed += added;
+    if (added > 0) {
+      ad_leaf[1].ensure(static_cast<size_t>(E()));
+      hipLaunchKernelGGL(k_ad_carry, dim3(ablocks(E())), dim3(kAB), 0, stream, g_remap.p, ad_leaf[0].p,
+                         ad_leaf[1].p, E());
+      ad_leaf[0].swap(ad_leaf[1]);
+    }
+  }
+  rk_init();
+  for (int st = 0; st < 3; ++st) stage(3, st, 0.);
+  // the deferring judge: start-of-step leaves whose norm now exceeds eps get
+  // their children after the step
+  HostTimer timer(*this, HOST_ADAPT);
+  ad_flag.ensure(static_cast<size_t>(E()));
+  norm_flags_dev(eps, ad_leaf[0].p, ad_flag.p);
+  const int64_t judged = refine_device_flags(ad_flag.p);
+  if (judged < 0) throw std::runtime_error("device refinement of the judge failed");
+  stat_deferred += judged;
+  return 0;
 }
 
 std::vector<std::vector<int32_t>> Context::all_instance_slots() {

@@ -229,6 +229,15 @@ class Context {
   int64_t stat_dev_adapt = 0, stat_dev_fallback = 0;
   int32_t dev_fallback_flags = 0;  // why calls went to the host: 1 norm at a threshold, 2 table full
   int64_t refine_device(const std::vector<double>& eps);
+  // refine on the device the leaves flagged in flags (device array, one per element)
+  int64_t refine_device_flags(const int32_t* flags);
+  int64_t refine_device_impl(const std::vector<double>* eps, const int32_t* flags);
+  // flags[e] = (mask[e] if mask) && norm of e above thr of its instance, on
+  // the device; norms within the band where pow decides are decided on the host
+  void norm_flags_dev(const std::vector<double>& thr, const int32_t* mask, int32_t* flags);
+  // proposer_step with a deferring judge, all on the device
+  int64_t proposer_step_device(const std::function<void(int, int, double)>& stage, const std::vector<double>& eps);
+  DevBuf<int32_t> ad_leaf[2], ad_flag, ad_band;
   int64_t coarsen_device(const std::vector<double>& eta);
   void ensure_host_grid();
   // removes every element of the flagged instances on the device (they stay

@@ -465,6 +465,12 @@ bool host_leaf(const HostGrid& g, int64_t e) {
 
 int64_t Context::proposer_step(const std::function<void(int, int, double)>& stage, const std::vector<double>& eps,
                                double* dt_inst) {
+  // with a deferring judge and refine/coarsen on the device, the whole step
+  // runs there (same proposals, step and judge)
+  if (judge_defer && device_adapt_fits(E())) {
+    copy_to_predict();
+    return proposer_step_device(stage, eps);
+  }
   ensure_host_grid();
   const int nb = B();
   copy_to_predict();  // u^n

@@ -204,3 +204,37 @@ def test_restore_and_rerun_with_device_tables():
         for x, y in zip(g1, g2):
             assert np.array_equal(x, y), f"instance {i}: second run's grid differs"
         assert np.array_equal(u1, u2), f"instance {i}: " + first_diff(u1, u2)
+
+
+@pytest.mark.parametrize("mode", ["current", "learned"])
+def test_proposer_with_deferred_judge_on_the_device_matches_host(mode):
+    """Proposals, step and deferring judge on the device (refine modes with
+    judge_defer = 1) give the host path's grids, values and counts."""
+    ws = [workloads.c2_burgers(seed=s, nmax=6, level_init=4, t_final=0.02) for s in (3, 8)]
+    print(f"[case] device adapt proposer mode={mode}", flush=True)
+    res = []
+    for dev in (0, 1):
+        b = workloads.make_batch(ws)
+        b.set_device_adapt(dev)
+        if mode == "learned":
+            b.proposer_init(7)  # an untrained network proposes something; both paths must agree
+            b.set_refine_mode(pyadg.REFINE_LEARNED, 0.5)
+        else:
+            b.set_refine_mode(pyadg.REFINE_CURRENT, 0.5)
+        b.set_judge_defer(1)
+        prob = pyadg.Problem.from_buffer_copy(ws[0].problem)
+        prob.form = 1
+        st = b.run_hyperbolic(prob)
+        res.append((b, (st.steps, st.dof_stages, st.max_elem), b.proposer_stats()))
+    (a, sa, pa), (b, sb, pb) = res
+    print(f"[device adapt] proposed / redone / deferred: host {pa}, device {pb}", flush=True)
+    assert sa == sb and pa == pb
+    assert pb[0] > 0
+    for i in range(len(ws)):
+        for x, y in zip(a.instance_grid(i), b.instance_grid(i)):
+            assert np.array_equal(x, y), f"instance {i}: element sets differ"
+        x, y = a.instance_get(i), b.instance_get(i)
+        assert np.array_equal(x, y), f"instance {i}: " + first_diff(x, y)
+    dev, host = b.device_adapt_stats()
+    assert dev > 0 and host == 0
+    assert b.grid_check() == 0
