@@ -13,13 +13,18 @@
 //            dimension d ordered by (instance, slots outside d), fiber f of
 //            the union at fbase = f << NMAX, fmaxl the deepest level in it.
 // Element sets, order and fiber tables equal the host build's (checked by
-// Context::check_device_grid).  Elements are identified by a 64-bit key:
+// Context::check_device_grid).  New elements are identified by a 64-bit key:
 // instance above D fields of NMAX bits, field d = the slot in dimension d.
+// Lookups in the current table go through its fiber maps (children and
+// parents lie in an element's fibers) or a binary search in its sorted
+// (instance << 32 | hash key) array; only new keys enter a hash table.
 //
 // The norm test takes sqrt of the sum of squares; within four ulps of the
 // threshold the reference's pow(s, 0.5) decides, which only the host has, so
-// such a norm makes the call return -1 before anything changes, and the host
-// path runs instead.  A full hash table or new-element list does the same.
+// such a norm makes refine_device / coarsen_device return -1 before anything
+// changes, and the host path runs instead (norm_flags_dev instead sends just
+// those elements to the host).  A refinement larger than its new-element list
+// runs again with a larger list.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
