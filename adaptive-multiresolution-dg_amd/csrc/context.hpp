@@ -224,7 +224,7 @@ class Context {
   // with nothing changed when the host has to decide instead: a norm within
   // a few ulps of its threshold (where the reference's pow decides), a full
   // table, or sizes whose keys do not pack into 63 bits.
-  bool device_adapt = false;
+  bool device_adapt = true;  // (adg_set_device_adapt; the host form with 0)
   bool host_stale = false;
   int64_t stat_dev_adapt = 0, stat_dev_fallback = 0;
   int32_t dev_fallback_flags = 0;  // why calls went to the host: 1 norm at a threshold, 2 table full

@@ -197,13 +197,13 @@ int adg_indicator_sumsq(adg_ctx* c, double* out);
 int adg_refine(adg_ctx* c, double eps, int64_t* n_added);
 int adg_coarsen(adg_ctx* c, double eta, int64_t* n_removed);
 /* Where refine and coarsen run (adg_refine, adg_coarsen and the adaptive steps
- * of adg_run_hyperbolic with ADG_REFINE_PREDICT): on = 0 (default) the host
- * decides from device-computed norms and rebuilds the element table
- * (DGAdapt::refine / coarsen and the Hash-keyed element map of DGSolution,
- * source/DGAdapt.cpp:371-404,675-688,982-1385, source/DGSolution.cpp:10-57);
- * on = 1 the threshold tests, children, hole closure, hash keys, the sorted
- * element table and its fiber tables are computed on the GPU and the host
- * reads back counts only.  Both give the same element sets, order and
+ * of adg_run_hyperbolic, and the deferring judge of the proposer modes):
+ * on = 1 (default) the threshold tests, children, hole closure, hash keys,
+ * the sorted element table and its fiber tables are computed on the GPU and
+ * the host reads back counts only; on = 0 the host decides from
+ * device-computed norms and rebuilds the element table (DGAdapt::refine /
+ * coarsen and the Hash-keyed element map of DGSolution,
+ * source/DGAdapt.cpp:371-404,675-688,982-1385, source/DGSolution.cpp:10-57).  Both give the same element sets, order and
  * coefficients.  A call whose norm lies within four ulps of its threshold
  * (where the reference's pow decides) runs on the host instead.
  * adg_device_adapt_stats: calls done on the device, calls handed to the host
