@@ -244,8 +244,10 @@ def main():
     if batched:
         # beside a batched chosen form, the single-instance fused form (a
         # comparison point) runs the per-element mapping 3 throughout, so the
-        # chosen form's kernel instances are its own in kernel traces
+        # chosen form's kernel instances are its own in kernel traces; the
+        # comparison forms adapt on the host, as the reference does
         sol.set_kernel_variant(3)
+        sol.set_device_adapt(0)
     print(f"[bench] init done: {sol.num_elem} elements; warmup {args.warmup} steps", file=sys.stderr, flush=True)
     sol.save_state(0)
     bsols = []
@@ -499,8 +501,10 @@ def main():
                        graph_steps_replayed=chosen["graph_steps_replayed"],
                        precision=args.precision if not w.linear else "fp64",
                        contexts=len(bsols) if batched else 1,
-                       plain_form="plain FP64, one kernel per 1D sweep, one instance",
-                       fused_single_instance_form=("fused FP64 stages, one instance, per-element mapping 3"
+                       plain_form="plain FP64, one kernel per 1D sweep, one instance, refine/coarsen on the %s"
+                       % ("host" if batched else where),
+                       fused_single_instance_form=("fused FP64 stages, one instance, per-element mapping 3, "
+                                                   "refine/coarsen on the host"
                                                    if batched else "fused FP64 stages, one instance (chosen)"),
                        plain_value=plain["value"], fused_single_instance_value=fused["value"],
                        chosen_value=chosen["value"], batch=args.batch if batched else 1,
