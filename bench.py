@@ -48,11 +48,12 @@ def _oracle_for(w):
 
 
 def cpu_baseline(ws, max_seconds, threads):
-    """Times the CPU oracle on the workload's own instances: `threads` host
-    threads, each solving instances (one OpenMP thread per oracle) from t=0 one
-    time step per call, until max_seconds of wall time have passed (the
-    oracle's setup is outside the timed windows).  For the linear workload:
-    time steps of the one instance on `threads` OpenMP threads."""
+    """Times the CPU oracle on the workload's own instances: min(threads,
+    instances) host threads, each solving instances from t=0 one time step per
+    call with threads / (host threads) OpenMP threads per oracle (all of them
+    for a single instance), until max_seconds of wall time have passed.  For
+    the linear workload: time steps of the one instance on `threads` OpenMP
+    threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     import runs
@@ -75,8 +76,11 @@ def cpu_baseline(ws, max_seconds, threads):
     state = {"next": 0, "dof": 0, "steps": 0, "solves": 0, "partial": 0}
     deadline = [None]
 
+    nworkers = max(1, min(threads, len(ws)))
+    omp = max(1, threads // nworkers)
+
     def worker():
-        pyoracle.set_threads(1)
+        pyoracle.set_threads(omp)
         while True:
             with lock:
                 i = state["next"]
@@ -101,7 +105,7 @@ def cpu_baseline(ws, max_seconds, threads):
                 state["solves" if done else "partial"] += 1
             o.close()
 
-    th = [threading.Thread(target=worker) for _ in range(threads)]
+    th = [threading.Thread(target=worker) for _ in range(nworkers)]
     t0 = time.perf_counter()
     deadline[0] = t0 + max_seconds
     for t in th:
@@ -110,11 +114,11 @@ def cpu_baseline(ws, max_seconds, threads):
         t.join()
     sec = time.perf_counter() - t0
     return {"value": state["dof"] / sec, "unit": "DoF-stage updates/s", "cores": threads, "kind": "port",
-            "sample": "CPU oracle on the workload's instances (%d threads, one instance and one OpenMP thread each, "
-                      "reference data structures, g++ -O3 -fopenmp): %d whole solves and %d stopped at the %.0f s "
-                      "wall limit, %d time steps, %d DoF-stage updates in %.2f s wall (oracle setup included in "
-                      "the wall time)" % (threads, state["solves"], state["partial"], max_seconds, state["steps"],
-                                          state["dof"], sec)}
+            "sample": "CPU oracle on the workload's instances (%d host threads, one instance at a time each on %d "
+                      "OpenMP threads, reference data structures, g++ -O3 -fopenmp): %d whole solves and %d stopped "
+                      "at the %.0f s wall limit, %d time steps, %d DoF-stage updates in %.2f s wall (oracle setup "
+                      "included in the wall time)" % (nworkers, omp, state["solves"], state["partial"], max_seconds,
+                                                      state["steps"], state["dof"], sec)}
 
 
 def imex_sample(args):
@@ -435,7 +439,8 @@ def main():
         fixed = args.variant if args.variant >= 0 else (mappings[cls] if cls < len(mappings) else -1)
         if fixed >= 0:
             tuning = {"all": fixed}
-        for c in (bsols if batched else chosen_sols or [sol]) if fixed < 0 else []:
+        tune_src = bsols if batched else ([sol] if chosen is fused else chosen_sols)
+        for c in tune_src if fixed < 0 else []:
             for (kc, bucket), v in c.kernel_tuning().items():
                 if kc == cls:
                     tuning[bucket] = v
