@@ -179,7 +179,7 @@ int adg_sync(adg_ctx* h) {
   return guard(h, [](adg::Context& c) {
     c.sync();
     c.check_error_flag();
-  });
+  }, kDeviceTables);
 }
 
 int adg_grid_init(adg_ctx* h, int sparse, int level_init) {
@@ -242,7 +242,7 @@ int adg_field_set(adg_ctx* h, int field, const double* host) {
     adg::hip_check(hipMemcpyAsync(field_ptr(c, field), host, n * sizeof(double), hipMemcpyHostToDevice, c.stream),
                    "field upload");
     c.sync();
-  });
+  }, kDeviceTables);
 }
 
 int adg_field_get(adg_ctx* h, int field, double* host) {
@@ -251,7 +251,7 @@ int adg_field_get(adg_ctx* h, int field, double* host) {
     adg::hip_check(hipMemcpyAsync(host, field_ptr(c, field), n * sizeof(double), hipMemcpyDeviceToHost, c.stream),
                    "field download");
     c.sync();
-  });
+  }, kDeviceTables);
 }
 
 namespace {
@@ -397,14 +397,14 @@ int adg_proposer_stats(adg_ctx* h, int64_t* proposed, int64_t* redone, int64_t* 
     if (proposed) *proposed = c.stat_proposed;
     if (redone) *redone = c.stat_redo;
     if (deferred) *deferred = c.stat_deferred;
-  });
+  }, kDeviceTables);
 }
 
 int adg_set_device_adapt(adg_ctx* h, int on) {
   return guard(h, [&](adg::Context& c) {
     if (on != 0 && on != 1) throw std::runtime_error("device adapt must be 0 or 1");
     c.device_adapt = on != 0;
-  });
+  }, kDeviceTables);
 }
 
 int adg_device_adapt_stats(adg_ctx* h, int64_t* device_calls, int64_t* host_fallbacks, int32_t* reasons) {
@@ -412,7 +412,7 @@ int adg_device_adapt_stats(adg_ctx* h, int64_t* device_calls, int64_t* host_fall
     if (device_calls) *device_calls = c.stat_dev_adapt;
     if (host_fallbacks) *host_fallbacks = c.stat_dev_fallback;
     if (reasons) *reasons = c.dev_fallback_flags;
-  });
+  }, kDeviceTables);
 }
 
 int adg_grid_check(adg_ctx* h, int64_t* mismatches) {
@@ -535,7 +535,7 @@ int adg_set_interpolation(adg_ctx* h, int family) {
 int adg_set_level_sum_cap(adg_ctx* h, int cap) {
   return guard(h, [&](adg::Context& c) { c.lsum_cap = cap < 0 ? -1 : cap; });
 }
-int adg_eval_up(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.eval_up(); }); }
+int adg_eval_up(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.eval_up(); }, kDeviceTables); }
 int adg_flux(adg_ctx* h, int kind, const double* params) {
   return guard(h, [&](adg::Context& c) {
     if (kind < 0 || kind > 2) throw std::runtime_error("unknown flux kind");
@@ -555,36 +555,36 @@ int adg_fp_to_coe(adg_ctx* h) {
   return guard(h, [](adg::Context& c) {
     c.fp_to_coe();
     c.check_error_flag();  // a missing ancestor (hole in the grid)
-  });
+  }, kDeviceTables);
 }
-int adg_rhs_zero(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_zero(); }); }
-int adg_rhs_lagr_vol(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_lagr(false); }); }
-int adg_rhs_lagr_flx(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_lagr(true); }); }
-int adg_rhs_penalty(adg_ctx* h, const double* alpha) { return guard(h, [&](adg::Context& c) { c.rhs_penalty(alpha); }); }
+int adg_rhs_zero(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_zero(); }, kDeviceTables); }
+int adg_rhs_lagr_vol(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_lagr(false); }, kDeviceTables); }
+int adg_rhs_lagr_flx(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rhs_lagr(true); }, kDeviceTables); }
+int adg_rhs_penalty(adg_ctx* h, const double* alpha) { return guard(h, [&](adg::Context& c) { c.rhs_penalty(alpha); }, kDeviceTables); }
 int adg_rhs_linear(adg_ctx* h, const double* coef) { return guard(h, [&](adg::Context& c) { c.rhs_linear(coef); }); }
-int adg_rk_init(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rk_init(); }); }
+int adg_rk_init(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.rk_init(); }, kDeviceTables); }
 int adg_rk_stage(adg_ctx* h, int scheme, int stage, double dt) {
   return guard(h, [&](adg::Context& c) {
     if (!((scheme == ADG_RK_EULER && stage == 0) || (scheme == ADG_RK2 && stage >= 0 && stage <= 1) ||
           (scheme == ADG_RK3 && stage >= 0 && stage <= 2)))
       throw std::runtime_error("invalid RK scheme/stage");
     c.rk_stage(scheme, stage, dt);
-  });
+  }, kDeviceTables);
 }
-int adg_copy_to_predict(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.copy_to_predict(); }); }
-int adg_copy_predict_to_ucoe(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.copy_predict_to_ucoe(); }); }
-int adg_indicator_sumsq(adg_ctx* h, double* out) { return guard(h, [&](adg::Context& c) { c.indicator_sumsq(out); }); }
+int adg_copy_to_predict(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.copy_to_predict(); }, kDeviceTables); }
+int adg_copy_predict_to_ucoe(adg_ctx* h) { return guard(h, [](adg::Context& c) { c.copy_predict_to_ucoe(); }, kDeviceTables); }
+int adg_indicator_sumsq(adg_ctx* h, double* out) { return guard(h, [&](adg::Context& c) { c.indicator_sumsq(out); }, kDeviceTables); }
 int adg_refine(adg_ctx* h, double eps, int64_t* n_added) {
   return guard(h, [&](adg::Context& c) {
     const int64_t a = c.refine(eps);
     if (n_added) *n_added = a;
-  });
+  }, kDeviceTables);
 }
 int adg_coarsen(adg_ctx* h, double eta, int64_t* n_removed) {
   return guard(h, [&](adg::Context& c) {
     const int64_t r = c.coarsen(eta);
     if (n_removed) *n_removed = r;
-  });
+  }, kDeviceTables);
 }
 int adg_init_adaptive_intp(adg_ctx* h, adg_point_fn f, void* user, double eps) {
   return guard(h, [&](adg::Context& c) {
@@ -970,7 +970,7 @@ int adg_instance_range(adg_ctx* h, int inst, int64_t* first, int64_t* count) {
     if (inst < 0 || inst >= c.B()) throw std::runtime_error("instance out of range");
     if (first) *first = c.off[inst];
     if (count) *count = c.off[inst + 1] - c.off[inst];
-  });
+  }, kDeviceTables);
 }
 
 int adg_set_instance_thresholds(adg_ctx* h, const double* eps, const double* eta) {
@@ -1021,7 +1021,7 @@ int adg_stage_launch_count(adg_ctx* h, int64_t* launches, int64_t* stages) {
   return guard(h, [&](adg::Context& c) {
     if (launches) *launches = c.stage_launches;
     if (stages) *stages = c.stage_count;
-  });
+  }, kDeviceTables);
 }
 
 int adg_set_class_variants(adg_ctx* h, const int* v, int n) {
@@ -1059,7 +1059,7 @@ int adg_graph_step_count(adg_ctx* h, int64_t* captures, int64_t* replays) {
   return guard(h, [&](adg::Context& c) {
     if (captures) *captures = c.graph_captures;
     if (replays) *replays = c.graph_replays;
-  });
+  }, kDeviceTables);
 }
 
 int adg_rk_step(adg_ctx* h, const adg_problem* p, int scheme, double dt) {
@@ -1068,7 +1068,7 @@ int adg_rk_step(adg_ctx* h, const adg_problem* p, int scheme, double dt) {
     const int nstage = scheme == ADG_RK_EULER ? 1 : scheme == ADG_RK2 ? 2 : scheme == ADG_RK3 ? 3 : 0;
     if (nstage == 0) throw std::runtime_error("invalid RK scheme");
     c.rk_step(stage_problem(p, c.D), scheme, nstage, dt);
-  });
+  }, kDeviceTables);
 }
 
 int adg_stage(adg_ctx* h, const adg_problem* p, int scheme, int stage, double dt) {
@@ -1078,7 +1078,7 @@ int adg_stage(adg_ctx* h, const adg_problem* p, int scheme, int stage, double dt
           (scheme == ADG_RK3 && stage >= 0 && stage <= 2)))
       throw std::runtime_error("invalid RK scheme/stage");
     c.stage_fused(stage_problem(p, c.D), scheme, stage, dt);
-  });
+  }, kDeviceTables);
 }
 
 
@@ -1185,14 +1185,14 @@ int adg_profile(adg_ctx* h, int enable) {
   return guard(h, [&](adg::Context& c) {
     c.prof_reset();
     c.prof_on = enable != 0;
-  });
+  }, kDeviceTables);
 }
 
 int adg_profile_host(adg_ctx* h, double* ms) {
   return guard(h, [&](adg::Context& c) {
     for (int i = 0; i < adg::Context::HOST_NCAT; ++i)
       if (ms) ms[i] = c.host_ms[i];
-  });
+  }, kDeviceTables);
 }
 
 int adg_profile_get(adg_ctx* h, double* ms, int64_t* count, double* bytes) {
@@ -1203,7 +1203,7 @@ int adg_profile_get(adg_ctx* h, double* ms, int64_t* count, double* bytes) {
       if (count) count[i] = c.prof_count[i];
       if (bytes) bytes[i] = c.prof_bytes[i];
     }
-  });
+  }, kDeviceTables);
 }
 
 }  // extern "C"
