@@ -626,6 +626,24 @@ bool Context::device_adapt_fits(int64_t n) const {
   return D * nmax + bits_for(B()) <= 62 && n < (int64_t{1} << 31) - 1;
 }
 
+// Radix sort of n (key, value) pairs between the two buffers of each array
+// (the double-buffer form: no copy back); afterwards k / v point to the
+// sorted pairs and ka / va to the free buffers.
+static void sort_pairs(Context& c, int64_t n, int endbit, uint64_t*& k, uint64_t*& ka, int32_t*& v, int32_t*& va,
+                       const char* what) {
+  hipcub::DoubleBuffer<uint64_t> dk(k, ka);
+  hipcub::DoubleBuffer<int32_t> dv(v, va);
+  size_t bytes = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, dk, dv, static_cast<int>(n), 0, endbit, c.stream);
+  c.ad_tmp.ensure(std::max<size_t>(bytes, 1));
+  hip_check(hipcub::DeviceRadixSort::SortPairs(c.ad_tmp.p, bytes, dk, dv, static_cast<int>(n), 0, endbit, c.stream),
+            what);
+  k = dk.Current();
+  ka = dk.Alternate();
+  v = dv.Current();
+  va = dv.Alternate();
+}
+
 // Table of the current grid, per-instance thresholds and the norms (shared
 // set-up of refine and coarsen); returns the kernel arguments.
 static AdArgs adapt_args(Context& c, const std::vector<double>* thr, int32_t cap_new) {
@@ -771,15 +789,11 @@ int64_t Context::refine_device_impl(const std::vector<double>* eps, const int32_
   ua.D = D;
   ua.nb = nmax;
   hipLaunchKernelGGL(k_ad_union_keys, dim3(ablocks(N)), dim3(kAB), 0, stream, ua);
-  size_t bytes = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p, ad_vals[1].p,
-                                           static_cast<int>(N), 0, 32 + bits_for(B()), stream);
-  ad_tmp.ensure(std::max<size_t>(bytes, 1));
-  hip_check(hipcub::DeviceRadixSort::SortPairs(ad_tmp.p, bytes, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p,
-                                               ad_vals[1].p, static_cast<int>(N), 0, 32 + bits_for(B()), stream),
-            "sort (element keys)");
+  uint64_t *sk = ad_keys[0].p, *ska = ad_keys[1].p;
+  int32_t *sv = ad_vals[0].p, *sva = ad_vals[1].p;
+  sort_pairs(*this, N, 32 + bits_for(B()), sk, ska, sv, sva, "sort (element keys)");
   const OutArgs o = out_args(*this, N);
-  hipLaunchKernelGGL(k_ad_union_gather, dim3(ablocks(N)), dim3(kAB), 0, stream, ua, ad_keys[1].p, ad_vals[1].p, o);
+  hipLaunchKernelGGL(k_ad_union_gather, dim3(ablocks(N)), dim3(kAB), 0, stream, ua, sk, sv, o);
   device_grid_finish(N, oldE);
   return h[0];
 }
@@ -873,11 +887,8 @@ void Context::device_grid_finish(int64_t N, int64_t oldE) {
   ad_runmax.ensure(static_cast<size_t>(N));
   ad_runlen.ensure(static_cast<size_t>(N));
   const int endbit = D * nmax + bits_for(B_);
-  size_t sbytes = 0, cbytes = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sbytes, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p, ad_vals[1].p,
-                                           static_cast<int>(N), 0, endbit, stream);
+  size_t cbytes = 0;
   (void)hipcub::DeviceScan::InclusiveSum(nullptr, cbytes, ad_run.p, ad_run.p, static_cast<int>(N), stream);
-  ad_tmp.ensure(std::max<size_t>(std::max(sbytes, cbytes), 1));
   for (int dd = 0; dd < D; ++dd) {
     FibArgs f{};
     for (int d = 0; d < D; ++d) f.slot[d] = ad_slot[d].p;
@@ -889,20 +900,20 @@ void Context::device_grid_finish(int64_t N, int64_t oldE) {
     f.nb = nmax;
     f.dd = dd;
     hipLaunchKernelGGL(k_ad_fib_keys, dim3(ablocks(N)), dim3(kAB), 0, stream, f);
-    size_t b1 = sbytes;
-    hip_check(hipcub::DeviceRadixSort::SortPairs(ad_tmp.p, b1, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p,
-                                                 ad_vals[1].p, static_cast<int>(N), 0, endbit, stream),
-              "sort (fiber keys)");
-    hipLaunchKernelGGL(k_ad_run_flags, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_keys[1].p, ad_vals[0].p, N,
-                       ad_runmax.p, ad_runlen.p);
+    uint64_t *sk = ad_keys[0].p, *ska = ad_keys[1].p;
+    int32_t *sv = ad_vals[0].p, *sva = ad_vals[1].p;
+    sort_pairs(*this, N, endbit, sk, ska, sv, sva, "sort (fiber keys)");
+    // run starts into the free value buffer, then their inclusive sum
+    hipLaunchKernelGGL(k_ad_run_flags, dim3(ablocks(N)), dim3(kAB), 0, stream, sk, sva, N, ad_runmax.p, ad_runlen.p);
+    ad_tmp.ensure(std::max<size_t>(cbytes, 1));
     size_t b2 = cbytes;
-    hip_check(hipcub::DeviceScan::InclusiveSum(ad_tmp.p, b2, ad_vals[0].p, ad_run.p, static_cast<int>(N), stream),
+    hip_check(hipcub::DeviceScan::InclusiveSum(ad_tmp.p, b2, sva, ad_run.p, static_cast<int>(N), stream),
               "scan (fiber runs)");
     ad_fbase[dd].ensure(static_cast<size_t>(N));
     ad_fmaxl[dd].ensure(static_cast<size_t>(N));
-    hipLaunchKernelGGL(k_ad_runs, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_vals[1].p, ad_run.p, ad_slot[dd].p,
+    hipLaunchKernelGGL(k_ad_runs, dim3(ablocks(N)), dim3(kAB), 0, stream, sv, ad_run.p, ad_slot[dd].p,
                        ad_runmax.p, ad_runlen.p, ad_fbase[dd].p, N, nmax, meta_nfib + dd);
-    hipLaunchKernelGGL(k_ad_run_levels, dim3(ablocks(N)), dim3(kAB), 0, stream, ad_vals[1].p, ad_run.p, ad_runmax.p,
+    hipLaunchKernelGGL(k_ad_run_levels, dim3(ablocks(N)), dim3(kAB), 0, stream, sv, ad_run.p, ad_runmax.p,
                        ad_runlen.p, ad_fmaxl[dd].p, N, meta_maxfib + dd, ad_inst.p, meta_lmax, D, dd);
   }
   // counts per instance, deepest levels, fibers and largest fiber per dimension
@@ -1021,15 +1032,11 @@ bool Context::set_grids_fresh(const std::vector<std::vector<int32_t>>& slots) {
   ua.D = D;
   ua.nb = nmax;
   hipLaunchKernelGGL(k_ad_union_keys, dim3(ablocks(N)), dim3(kAB), 0, stream, ua);
-  size_t bytes = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p, ad_vals[1].p,
-                                           static_cast<int>(N), 0, 32 + bits_for(B()), stream);
-  ad_tmp.ensure(std::max<size_t>(bytes, 1));
-  hip_check(hipcub::DeviceRadixSort::SortPairs(ad_tmp.p, bytes, ad_keys[0].p, ad_keys[1].p, ad_vals[0].p,
-                                               ad_vals[1].p, static_cast<int>(N), 0, 32 + bits_for(B()), stream),
-            "sort (element keys)");
+  uint64_t *sk = ad_keys[0].p, *ska = ad_keys[1].p;
+  int32_t *sv = ad_vals[0].p, *sva = ad_vals[1].p;
+  sort_pairs(*this, N, 32 + bits_for(B()), sk, ska, sv, sva, "sort (element keys)");
   const OutArgs o = out_args(*this, N);
-  hipLaunchKernelGGL(k_ad_union_gather, dim3(ablocks(N)), dim3(kAB), 0, stream, ua, ad_keys[1].p, ad_vals[1].p, o);
+  hipLaunchKernelGGL(k_ad_union_gather, dim3(ablocks(N)), dim3(kAB), 0, stream, ua, sk, sv, o);
   // (every element new: u and the predict copy come back zero)
   device_grid_finish(N, E());
   new_add.assign(static_cast<size_t>(N), 1);
