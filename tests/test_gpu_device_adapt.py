@@ -238,3 +238,27 @@ def test_proposer_with_deferred_judge_on_the_device_matches_host(mode):
     dev, host = b.device_adapt_stats()
     assert dev > 0 and host == 0
     assert b.grid_check() == 0
+
+
+def test_deferred_judge_norm_at_eps_decided_on_host():
+    """The device proposer step sends norms within a few ulps of eps to the
+    host element by element (the reference's pow decides them): with eps
+    equal to an element's norm, the run equals the host path's."""
+    w = workloads.c2_burgers(seed=5, nmax=6, level_init=4, t_final=0.02)
+    print("[case] device adapt proposer, eps at a norm", flush=True)
+    res = []
+    for dev in (0, 1):
+        s = workloads.make_solution(w)
+        s.set_device_adapt(dev)
+        sumsq = s.indicator_sumsq()
+        eps = math.sqrt(float(sumsq[int(np.argmax(sumsq))]))
+        s.set_refine_mode(pyadg.REFINE_CURRENT, 0.5)
+        s.set_judge_defer(1)
+        prob = pyadg.Problem.from_buffer_copy(w.problem)
+        prob.form, prob.eps, prob.eta, prob.max_steps = 1, eps, eps / 10, 3
+        st = s.run_hyperbolic(prob)
+        res.append((s, (st.steps, st.dof_stages), s.proposer_stats()))
+    (a, sa, pa), (b, sb, pb) = res
+    assert sa == sb and pa == pb
+    _same(a, b, "eps at a norm")
+    assert b.grid_check() == 0
