@@ -197,7 +197,7 @@ def main():
                     help="contexts (HIP streams, host threads) the batch is split over in the chosen form")
     ap.add_argument("--batch", type=int, default=None,
                     help="independent seeded instances solved together in the chosen form (c4: 256, the "
-                         "configuration's own batch; c2, c5: 1024 seeds of the configuration)")
+                         "configuration's own batch; c5: 256 and c2: 1024 seeds of the configuration)")
     ap.add_argument("--refine", default="predict", choices=("predict", "learned"),
                     help="refinement of the chosen form's adaptive steps: forward-Euler prediction (the reference) "
                          "or the learned proposer with the threshold judge")
@@ -236,7 +236,8 @@ def main():
     make = workloads.WORKLOADS[args.workload]
     seeded = args.workload in ("c2", "c4", "c5")
     if args.batch is None:
-        args.batch = 256 if args.workload == "c4" else (1024 if seeded else 1)
+        # C5's grids hold ~45 k elements per instance at NMAX 9: 256 instances
+        args.batch = 256 if args.workload in ("c4", "c5") else (1024 if seeded else 1)
     batched = seeded and args.batch > 1
 
     def workload(seed):
